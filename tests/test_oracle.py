@@ -1,0 +1,466 @@
+"""Pins the CPU oracle (oracle/mtl4_oracle.c) before anything is checked
+against it: the known-answer values MTL4's own tests assert
+(tests/golden/mtl4_kats.json), closed forms, textbook factorizations and
+hand-derived cases where the MTL4 summation order gives a distinct value.
+CPU only."""
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import orc
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mtl4_kats.json")))
+
+
+def dense(A):
+    D = np.zeros((A.n, A.ncols))
+    for r in range(A.n):
+        for j in range(A.rowptr[r], A.rowptr[r + 1]):
+            D[r, A.col[j]] += A.val[j]
+    return D
+
+
+def from_dense(D):
+    rp, ci, va = [0], [], []
+    for r in range(D.shape[0]):
+        for c in range(D.shape[1]):
+            if D[r, c] != 0:
+                ci.append(c)
+                va.append(D[r, c])
+        rp.append(len(ci))
+    return orc.Matrix(rp, ci, va, ncols=D.shape[1])
+
+
+def from_entries(n, entries):
+    D = np.zeros((n, n))
+    for r, c, v in entries:
+        D[r, c] += v
+    return from_dense(D)
+
+
+# ---------------------------------------------------------------- generators
+@pytest.mark.parametrize("m,n", KATS["laplacian_setup"]["grids"])
+def test_laplacian_setup_kat(m, n):
+    A = orc.laplacian_setup(m, n)
+    D = dense(A)
+    assert A.nnz == 5 * m * n - 2 * m - 2 * n
+    if m > 1 and n > 1:      # laplacian_setup_test.cpp:36-42
+        assert D[0, 0] == 4 and D[0, 1] == -1 and D[0, n] == -1 and D[1, 0] == -1 and D[n, 0] == -1
+        if n > 2:
+            assert D[0, 2] == 0 and D[2, 0] == 0
+    # closed form: kron(I_m, T_n) + kron(T_m, I_n) with T = tridiag(-1, 2, -1)
+    T = lambda k: 2 * np.eye(k) - np.eye(k, k=1) - np.eye(k, k=-1)
+    assert np.array_equal(D, np.kron(np.eye(m), T(n)) + np.kron(T(m), np.eye(n)))
+    for r in range(A.n):
+        cols = A.col[A.rowptr[r]:A.rowptr[r + 1]]
+        assert np.all(np.diff(cols) > 0)
+
+
+def test_config1_size():
+    A = orc.laplacian_setup(64, 64)
+    assert (A.n, A.nnz) == (4096, 20224)
+
+
+# ---------------------------------------------------------------- SpMV
+def test_matrix_vector_product_kat():
+    k = KATS["matrix_vector_product"]
+    A = orc.laplacian_setup(*k["grid"])
+    v = dense(A)[k["row"]].copy()
+    w = orc.spmv(A, v)
+    for i, e in k["assign"].items():
+        assert w[int(i)] == e
+    orc.spmv(A, v, w, orc.PLUS)
+    for i, e in k["plus"].items():
+        assert w[int(i)] == e
+    orc.spmv(A, v, w, orc.MINUS)
+    for i, e in k["minus"].items():
+        assert w[int(i)] == e
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_spmv_vs_dense(seed):
+    rng = np.random.default_rng(seed)
+    D = rng.uniform(-1, 1, (37, 37)) * (rng.random((37, 37)) < 0.2)
+    A = from_dense(D)
+    x = rng.uniform(-1, 1, 37)
+    assert np.allclose(orc.spmv(A, x), D @ x, rtol=1e-14, atol=1e-14)
+    y0 = rng.uniform(-1, 1, 37)
+    y = y0.copy()
+    orc.spmv(A, x, y, orc.MINUS)
+    assert np.allclose(y, y0 - D @ x, rtol=1e-14, atol=1e-14)
+
+
+def test_spmv_row_order():
+    # one row holding 1e16, 1, -1e16, 1: left-to-right sum in column order is
+    # ((1e16+1)-1e16)+1 = 1 (the first +1 is absorbed)
+    A = orc.Matrix([0, 4], [0, 1, 2, 3], [1e16, 1.0, -1e16, 1.0], ncols=4)
+    assert orc.spmv(A, np.ones(4))[0] == 1.0
+
+
+# ---------------------------------------------------------------- reductions
+@pytest.mark.parametrize("unroll", KATS["dot"]["unrolls"])
+def test_dot_kat(unroll):
+    u = np.arange(1, KATS["dot"]["size"] + 1, dtype=np.float64)
+    assert orc.dot(u, u, unroll) == KATS["dot"]["expected"]
+
+
+@pytest.mark.parametrize("unroll", KATS["unary_dot"]["unrolls"])
+def test_unary_dot_kat(unroll):
+    u = np.arange(1, KATS["unary_dot"]["size"] + 1, dtype=np.float64)
+    assert orc.unary_dot(u, unroll) == KATS["unary_dot"]["expected"]
+    assert abs(orc.two_norm(u) ** 2 - 285.0) < 1e-3
+
+
+def test_dot4_grouping():
+    # dot<4>: ((t0+t1)+(t2+t3)): (1e16+1) + (-1e16+1) = 1e16 - 1e16 = 0,
+    # whereas a left-to-right sum gives 1 and dot<1> (dot_simple) gives 1.
+    a = np.array([1e16, 1.0, -1e16, 1.0])
+    assert orc.dot(a, np.ones(4), 4) == 0.0
+    assert orc.dot(a, np.ones(4), 1) == 1.0
+    # remainder goes into t0: n=5, a4 joins t0 = 1e16 -> absorbed
+    a5 = np.array([1e16, 0.0, -1e16, 0.0, 1.0])
+    assert orc.dot(a5, np.ones(5), 4) == 0.0
+
+
+def test_unary_dot_right_nested():
+    # reduction<8>::finish nests to the right: 1e16 + (1+(1+...)) = 1e16 + 7
+    # rounds to 1e16+8; any left-to-right order absorbs each +1 (ties to even).
+    x = np.array([1e8, 1, 1, 1, 1, 1, 1, 1.0])
+    assert orc.unary_dot(x, 8) == 1e16 + 8
+    assert orc.unary_dot(x, 1) == 1e16
+    # two_norm<4>: t0 + (t1 + (t2 + t3)) on [1e16, 1, 1, 1] (squares) = 1e16+4 (exact: 1e16+3 ties->+4)
+    y = np.array([1e8, 1, 1, 1.0])
+    assert orc.two_norm(y) == math.sqrt(1e16 + 4)
+
+
+def test_fused_kats():
+    k = KATS["fuse"]
+    n = k["n"]
+    B = from_dense(2.0 * np.eye(n))
+    v = np.ones(n)
+    w, d = orc.fused_spmv_dot(B, v)
+    assert abs(d - k["spmv_dot"]) < 1e-12 and np.all(w == 2.0)
+    r = np.full(n, 6.0)
+    q = np.full(n, 2.0)
+    rho = orc.fused_axpy_unary_dot(k["alpha"], q, r)
+    assert abs(rho - k["axpy_unary_dot"]) < 1e-9
+    x7 = np.full(n, 7.0)
+    assert orc.unary_dot(x7, 4) == k["unary_dot_7"]
+    assert abs(orc.two_norm(x7) - k["two_norm_7"]) < 1e-4
+    assert abs(orc.dot(r, np.full(n, 2.0)) - k["dot_r_x2"]) < 1e-9
+    r2 = k["alpha"] * q
+    assert abs(orc.dot(r2, q) - k["dot_alpha_q"]) < 1e-9
+
+
+def test_fused_dot_order():
+    # fused lazy_dot sums tmp0+tmp1+tmp2+tmp3 from the left:
+    # ((1e16+1)+(-1e16))+1 = 1, unlike dot<4>'s grouping (which gives 0)
+    A = from_dense(np.eye(4))
+    p = np.array([1e8, 1.0, 1.0, 1.0])
+    # p*q = p^2: 1e16, 1, 1, 1 -> ((1e16+1)+1)+1 = 1e16 (ties to even each time)
+    q, d = orc.fused_spmv_dot(A, p)
+    assert d == 1e16
+    # fused unary_dot: (t0+t1)+(t2+t3) = (1e16+1)+2 = 1e16+2
+    r = np.array([1e8, 1.0, 1.0, 1.0])
+    assert orc.fused_axpy_unary_dot(0.0, np.zeros(4), r) == 1e16 + 2
+
+
+def test_reductions_vs_fsum():
+    rng = np.random.default_rng(5)
+    for n in (0, 1, 3, 4, 7, 1000, 1001):
+        a, b = rng.standard_normal(n), rng.standard_normal(n)
+        exact = math.fsum(a * b)
+        assert abs(orc.dot(a, b) - exact) <= 1e-13 * max(1.0, np.abs(a * b).sum())
+        assert abs(orc.unary_dot(a) - math.fsum(a * a)) <= 1e-13 * max(1.0, (a * a).sum())
+
+
+# ---------------------------------------------------------------- trisolves / PCs
+def test_inverse_trisolve_kat():
+    k = KATS["inverse_trisolve"]
+    n = k["n"]
+    U = from_entries(n, k["upper_entries"])
+    x = np.array(k["x"])
+    b = dense(U) @ x
+    Ud = dense(U)
+    Uinv = Ud.copy()
+    np.fill_diagonal(Uinv, 1.0 / np.diag(Ud))      # invert_diagonal(A)
+    y = np.zeros(n)
+    orc.lib().orc_upper_invdiag_first(from_dense(Uinv).ref(), orc._p(b), orc._p(y))
+    assert abs(y[k["check_index"]] - k["check_value"]) < k["tol"]
+    assert np.allclose(y, x, rtol=1e-14)
+    Lt = from_dense(Uinv.T.copy())                   # B = trans(A), diagonal last
+    bt = Ud.T @ x
+    z = np.zeros(n)
+    assert orc.lib().orc_lower_invdiag_last(Lt.ref(), orc._p(bt), orc._p(z)) == 0
+    assert abs(z[k["check_index"]] - k["check_value"]) < k["tol"]
+    assert np.allclose(z, x, rtol=1e-14)
+
+
+def test_trisolve_vs_scipy():
+    rng = np.random.default_rng(3)
+    n = 40
+    Lw = np.tril(rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < 0.3), -1)
+    v = rng.standard_normal(n)
+    w = np.zeros(n)
+    orc.lib().orc_lower_unit(from_dense(Lw).ref(), orc._p(v), orc._p(w))
+    assert np.allclose(w, sla.solve_triangular(Lw + np.eye(n), v, lower=True, unit_diagonal=True))
+    dia = rng.uniform(1, 2, n)
+    Uw = np.triu(rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < 0.3), 1)
+    Us = Uw + np.diag(1.0 / dia)                     # stored inverse diagonal
+    w2 = np.zeros(n)
+    orc.lib().orc_upper_invdiag_first(from_dense(Us).ref(), orc._p(v), orc._p(w2))
+    assert np.allclose(w2, sla.solve_triangular(Uw + np.diag(dia), v, lower=False))
+
+
+def test_ic0_golden_vector():
+    k = KATS["ic_0_solve"]
+    A = orc.laplacian_setup(*k["grid"])
+    P = orc.Preconditioner("ic_0", A, k["value_bytes"])
+    y = P.solve(np.array(k["x"], dtype=np.float64))
+    assert np.linalg.norm(y - np.array(k["y"])) <= k["tol"]
+    assert np.linalg.norm(y - np.array(k["y"])) <= 2e-5   # printed to 6 significant digits
+
+
+def dense_ic0(A):
+    """Textbook IC(0) of ic_0_cg_test.cpp:18-40 (U with the true diagonal)."""
+    U = np.triu(A).copy()
+    n = len(U)
+    for k in range(n):
+        dia = U[k, k] = math.sqrt(U[k, k])
+        for i in range(k + 1, n):
+            U[k, i] /= dia
+            d = U[k, i]
+            for j in range(k + 1, i + 1):
+                if U[j, i] != 0.0:
+                    U[j, i] -= d * U[k, j]
+    return U
+
+
+@pytest.mark.parametrize("grid", [(3, 3), (4, 5), (6, 6)])
+def test_ic0_vs_dense(grid):
+    A = orc.laplacian_setup(*grid)
+    P = orc.Preconditioner("ic_0", A, 8)
+    Ud = dense_ic0(dense(A))
+    Us = dense(P.factor("U"))
+    # off-diagonals equal the textbook factor; stored diagonal is inverted
+    assert np.allclose(np.triu(Us, 1), np.triu(Ud, 1), atol=1e-14)
+    assert np.allclose(np.diag(Us), 1.0 / np.diag(Ud), rtol=1e-14)
+    k = KATS["ic_0_cg"]
+    if grid == (3, 3):
+        for (i, j) in k["u_checks"]:
+            ref = Ud[i, j] if i != j else 1.0 / Ud[i, j]
+            assert abs(Us[i, j] - ref) < k["tol"]
+    # L = trans(U)
+    assert np.array_equal(dense(P.factor("L")), Us.T)
+
+
+def test_ic0_tridiagonal_is_exact():
+    # no fill-in is dropped for a tridiagonal SPD matrix: IC(0) is Cholesky
+    n = 30
+    D = 3 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    A = from_dense(D)
+    P = orc.Preconditioner("ic_0", A)
+    x = np.random.default_rng(1).standard_normal(n)
+    assert np.allclose(P.solve(D @ x), x, rtol=1e-12)
+
+
+def glibc_rand_matrix(n):
+    """ilu_0_test.cpp:16-28 with glibc rand() from its default seed."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    D = np.eye(n)
+    pattern = np.eye(n, dtype=bool)
+    for _ in range(2 * n):
+        r = libc.rand() % n
+        c = libc.rand() % n
+        D[r, c] += -1
+        D[r, r] += 1
+        pattern[r, c] = True
+    rp, ci, va = [0], [], []
+    for r in range(n):
+        for c in range(n):
+            if pattern[r, c]:
+                ci.append(c)
+                va.append(D[r, c])
+        rp.append(len(ci))
+    return orc.Matrix(rp, ci, va), D
+
+
+def dense_ilu0(D):
+    """Textbook ILU(0) of ilu_0_test.cpp:31-50."""
+    LU = D.copy()
+    n = len(LU)
+    for i in range(1, n):
+        for k in range(i):
+            LU[i, k] /= LU[k, k]
+            for j in range(k + 1, n):
+                if LU[i, j] != 0:
+                    LU[i, j] -= LU[i, k] * LU[k, j]
+    return LU
+
+
+def test_ilu0_rand_kat():
+    k = KATS["ilu_0_rand"]
+    A, D = glibc_rand_matrix(k["n"])
+    P = orc.Preconditioner("ilu_0", A)
+    LU = dense_ilu0(D)
+    Ls, Us = dense(P.factor("L")), dense(P.factor("U"))
+    assert abs(LU[2, 1] - Ls[2, 1]) <= k["dense_tol"]
+    assert abs(LU[2, 2] - 1.0 / Us[2, 2]) <= k["dense_tol"]
+    n = k["n"]
+    x = np.full(n, 3.0)
+    Ut = np.triu(Us, 1) + np.diag(1.0 / np.diag(Us))    # U with its true diagonal
+    Px = (Ls + np.eye(n)) @ (Ut @ x)
+    assert np.linalg.norm(P.solve(Px) - x) <= k["tol"]
+    PTx = Ut.T @ ((Ls + np.eye(n)).T @ x)
+    assert np.linalg.norm(P.adjoint_solve(PTx) - x) <= k["tol"]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_ilu0_vs_dense_random(seed):
+    rng = np.random.default_rng(seed)
+    n = 25
+    D = rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < 0.2)
+    np.fill_diagonal(D, np.abs(D).sum(1) + 1.0)
+    A = from_dense(D)
+    P = orc.Preconditioner("ilu_0", A)
+    LU = dense_ilu0(D)
+    Ls, Us = dense(P.factor("L")), dense(P.factor("U"))
+    assert np.allclose(np.tril(LU, -1), Ls, atol=1e-12)
+    assert np.allclose(np.triu(LU, 1), np.triu(Us, 1), atol=1e-12)
+    assert np.allclose(np.diag(LU), 1.0 / np.diag(Us), rtol=1e-12)
+    # solve is (L+I)^-1 then U^-1 of those factors
+    v = rng.standard_normal(n)
+    Ut = np.triu(Us, 1) + np.diag(1.0 / np.diag(Us))
+    assert np.allclose(P.solve((Ls + np.eye(n)) @ (Ut @ v)), v, rtol=1e-10)
+    assert np.allclose(P.adjoint_solve(Ut.T @ ((Ls + np.eye(n)).T @ v)), v, rtol=1e-10)
+
+
+def test_ilu0_banded_is_exact():
+    n = 20
+    D = 4 * np.eye(n) - np.eye(n, k=1) - 2 * np.eye(n, k=-1)
+    P = orc.Preconditioner("ilu_0", from_dense(D))
+    x = np.random.default_rng(2).standard_normal(n)
+    assert np.allclose(P.solve(D @ x), x, rtol=1e-12)
+
+
+def test_diagonal_pc():
+    A = orc.laplacian_setup(4, 5)
+    P = orc.Preconditioner("diagonal", A)
+    x = np.arange(20.0)
+    assert np.array_equal(P.solve(x), (1.0 / 4.0) * x)
+
+
+def test_missing_diagonal():
+    A = orc.Matrix([0, 1, 2], [1, 0], [1.0, 1.0])
+    with pytest.raises(RuntimeError):
+        orc.Preconditioner("ic_0", A)
+    with pytest.raises(RuntimeError):
+        orc.Preconditioner("ilu_0", A)
+
+
+# ---------------------------------------------------------------- solvers
+def test_config1_cg_diagonal():
+    A = orc.laplacian_setup(64, 64)
+    xt = np.random.default_rng(0).uniform(-1, 1, A.n)
+    b = orc.spmv(A, xt)
+    P = orc.Preconditioner("diagonal", A)
+    x, st, it, _ = orc.cg(A, b, P, max_iter=2000, rtol=1e-10)
+    assert st == 0 and it.error == 0
+    # independent restatement of the same recurrences: 201 iterations
+    assert abs(it.i - 201) <= 2
+    assert it.resid <= 1e-10 * np.linalg.norm(b)
+    true_res = np.linalg.norm(b - dense_mul(A, x)) / np.linalg.norm(b)
+    assert true_res < 1e-9
+    assert np.linalg.norm(x - xt) / np.linalg.norm(xt) < 1e-7
+
+
+def dense_mul(A, x):
+    return orc.spmv(A, x)
+
+
+def test_cg_identity_smoke_and_exact():
+    k = KATS["cg_identity"]
+    A = orc.laplacian_setup(*k["grid"])
+    b = orc.spmv(A, np.ones(A.n))
+    x, st, it, _ = orc.cg(A, b, None, max_iter=k["max_iter"], rtol=k["rtol"])
+    assert st == 0 and np.allclose(x, 1.0, rtol=1e-9)
+    assert np.allclose(x, np.linalg.solve(dense(A), b), rtol=1e-9)
+
+
+def test_ilu0_cg_kat():
+    k = KATS["ilu_0_cg"]
+    A = orc.laplacian_setup(*k["grid"])
+    b = orc.spmv(A, np.ones(A.n))
+    P = orc.Preconditioner("ilu_0", A)
+    x, st, it, _ = orc.cg(A, b, P, max_iter=k["max_iter"], rtol=k["rtol"])
+    assert st == 0 and it.i <= k["max_iterations_allowed"]
+
+
+def test_ic0_cg_kat():
+    k = KATS["ic_0_cg"]
+    A = orc.laplacian_setup(*k["grid"])
+    b = orc.spmv(A, np.ones(A.n))
+    P = orc.Preconditioner("ic_0", A, 4)
+    x, st, it, _ = orc.cg(A, b, P, max_iter=k["max_iter"], rtol=k["rtol"])
+    assert st == 0 and np.allclose(x, 1.0, atol=1e-4)
+
+
+def test_absolute_termination_kat():
+    k = KATS["absolute_termination"]
+    A = orc.laplacian_setup(*k["grid"])
+    b = orc.spmv(A, np.ones(A.n))
+    P = orc.Preconditioner("diagonal", A)
+    x, st, it, _ = orc.cg(A, b, P, max_iter=k["max_iter"], rtol=0.0, atol=k["atol"])
+    assert st == 0
+    assert np.linalg.norm(b - orc.spmv(A, x)) <= k["atol"]
+
+
+def test_bicgstab_diagonal_kat():
+    k = KATS["bicgstab_diagonal"]
+    A = orc.laplacian_setup(*k["grid"])
+    b = orc.spmv(A, np.ones(A.n))
+    P = orc.Preconditioner("diagonal", A)
+    x, st, it, _ = orc.bicgstab(A, b, P, max_iter=k["max_iter"], rtol=k["rtol"])
+    assert st == 0 and it.resid <= k["rtol"] * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("pc", ["identity", "diagonal", "ilu_0"])
+def test_bicgstab_nonsymmetric(pc):
+    rng = np.random.default_rng(11)
+    n = 60
+    D = rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < 0.1)
+    np.fill_diagonal(D, np.abs(D).sum(1) + 1.0)
+    A = from_dense(D)
+    xt = rng.standard_normal(n)
+    b = D @ xt
+    P = None if pc == "identity" else orc.Preconditioner(pc, A)
+    x, st, it, _ = orc.bicgstab(A, b, P, max_iter=200, rtol=1e-12)
+    assert st == 0 and np.allclose(x, xt, rtol=1e-8, atol=1e-9)
+
+
+def test_iteration_semantics():
+    A = orc.laplacian_setup(8, 8)
+    P = orc.Preconditioner("diagonal", A)
+    b = orc.spmv(A, np.ones(A.n))
+    # max_iter reached -> error 1 ("Too many iterations", basic_iteration.hpp:43-48)
+    x, st, it, _ = orc.cg(A, b, P, max_iter=3, rtol=1e-12)
+    assert st == 1 and it.error == 1 and it.i == 3
+    # b = 0: ||r0|| = 0 and resid 0 <= atol 0 -> converged at 0 iterations (:90-91)
+    x, st, it, _ = orc.cg(A, np.zeros(A.n), P, max_iter=10, rtol=1e-10)
+    assert st == 0 and it.i == 0
+    # history holds one residual per finished() call
+    x, st, it, h = orc.cg(A, b, P, max_iter=100, rtol=1e-8, hist_cap=200)
+    assert len(h) == it.i + 1 and h[0] == pytest.approx(np.linalg.norm(b))
+
+
+def test_bicgstab_breakdown():
+    # A = 0 off a zero diagonal gives gamma = dot(rtilde, A*phat) = 0
+    A = orc.Matrix([0, 1, 2], [0, 1], [0.0, 0.0])
+    x, st, it, _ = orc.bicgstab(A, np.ones(2), None, max_iter=10, rtol=1e-8)
+    assert st == orc.BREAKDOWN
