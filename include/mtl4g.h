@@ -1,0 +1,205 @@
+/*
+ * mtl4g.h -- C ABI of the MI355X (gfx950) sparse Krylov library.
+ *
+ * This is the boundary between host code written against MTL4's types
+ * (mtl::compressed2D, mtl::dense_vector, itl::cg, itl::bicgstab,
+ * itl::pc::{identity,diagonal,ilu_0,ic_0}, itl::basic_iteration) and the
+ * hand-written HIP kernels.  Plain C: opaque handles, plain pointers and
+ * sizes, int status codes, no exceptions across the ABI.
+ *
+ * Reference interfaces are cited as path:line in the MTL4 tree.
+ *
+ * Memory:  "d_" arguments are device pointers (hipMalloc'd or any device
+ * allocation of the current device); "h_" arguments are host pointers.
+ * Vectors are dense, contiguous, FP64 unless the name says f32.
+ * Streams: every call takes a hipStream_t as void* (NULL = default stream)
+ * and is stream-ordered; calls that return a scalar to the host synchronize
+ * that stream.
+ * Threading: every call is reentrant.  Handles keep their scratch inside
+ * the handle; concurrent calls on distinct handles are safe, one handle is
+ * used by one thread at a time (MTL4's ilu/ic_0 solves are not reentrant
+ * at all: they keep a function-local static, itl/pc/ilu.hpp:76,111,
+ * itl/pc/ic_0.hpp:72).
+ */
+#ifndef MTL4G_H
+#define MTL4G_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+/* >= 0 mirrors basic_iteration::error (itl/iteration/basic_iteration.hpp:
+ * 43-48: 0 converged, 1 "Too many iterations"); < 0 are the conditions MTL4
+ * throws for (mtl/utility/exception.hpp:95-236, itl/utility/exception.hpp). */
+#define MTL4G_OK                 0
+#define MTL4G_MAX_ITER           1   /* basic_iteration error 1 */
+#define MTL4G_ERR_INVALID      (-1)  /* null handle / bad size / bad enum */
+#define MTL4G_ERR_NOT_SQUARE   (-2)  /* mtl::matrix_not_square */
+#define MTL4G_ERR_SIZE         (-3)  /* mtl::incompatible_size */
+#define MTL4G_ERR_MISSING_DIAG (-4)  /* mtl::missing_diagonal */
+#define MTL4G_ERR_BREAKDOWN    (-5)  /* itl::unexpected_orthogonality */
+#define MTL4G_ERR_NONFINITE    (-6)  /* NaN/Inf reached a scalar of the recurrence */
+#define MTL4G_ERR_OOM          (-7)  /* device allocation failed */
+#define MTL4G_ERR_HIP          (-8)  /* other HIP runtime error */
+#define MTL4G_ERR_UNSUPPORTED  (-9)  /* index range or format not supported */
+#define MTL4G_ERR_TIMEOUT     (-10)  /* a bounded device-side wait expired */
+
+/* assign modes of w (=,+=,-=) A*v: mtl/operation/assign_mode.hpp:22-131 */
+#define MTL4G_ASSIGN 0
+#define MTL4G_PLUS   1
+#define MTL4G_MINUS  2
+
+/* preconditioner kinds: itl/pc/{identity,diagonal,ilu_0,ic_0}.hpp */
+#define MTL4G_PC_IDENTITY 0
+#define MTL4G_PC_DIAGONAL 1
+#define MTL4G_PC_ILU0     2
+#define MTL4G_PC_IC0      3
+
+/* value formats (the Value template argument of ilu_0/ic_0, e.g.
+ * ic_0<Matrix,float>, itl/pc/ic_0.hpp:38, and inner-solve storage) */
+#define MTL4G_F64  0
+#define MTL4G_F32  1
+#define MTL4G_BF16 2   /* values stored as bfloat16, arithmetic in FP32 */
+
+/* Krylov methods */
+#define MTL4G_CG       0   /* itl::cg,       itl/krylov/cg.hpp:38-111 */
+#define MTL4G_BICGSTAB 1   /* itl::bicgstab, itl/krylov/bicgstab.hpp:27-78 */
+
+typedef struct mtl4g_matrix_s* mtl4g_matrix;
+typedef struct mtl4g_pc_s* mtl4g_pc;
+typedef struct mtl4g_solver_s* mtl4g_solver;
+
+/* State of itl::basic_iteration<double> (basic_iteration.hpp:23-164):
+ * in: norm_r0, max_iter, rtol, atol, i (start count); out: i, error, resid.
+ * A solve stops when resid <= rtol*norm_r0 or resid <= atol (when
+ * norm_r0 == 0: resid <= atol only), or with error 1 when i >= max_iter. */
+typedef struct {
+    double norm_r0;
+    int32_t max_iter;
+    double rtol;
+    double atol;
+    int32_t i;
+    int32_t error;
+    double resid;
+} mtl4g_iteration;
+
+/* ---- library -------------------------------------------------------- */
+const char* mtl4g_status_string(int status);
+/* Message of the last failing call on this thread ("" if none). */
+const char* mtl4g_last_error(void);
+int mtl4g_version(void);
+/* Sets the device used by later calls on this thread. */
+int mtl4g_set_device(int device);
+
+/* ---- compressed2D --------------------------------------------------- */
+/* mtl::compressed2D<double>(m, n, nnz, Major, Minor, Entries): copies the
+ * arrays (matrix/compressed2D.hpp:355-364).  int32 row pointers and column
+ * indices (mtl::mat::unsigned_parameters, matrix/parameter.hpp:47), rows
+ * sorted by column, no duplicates.  where: 0 = host arrays, 1 = device. */
+int mtl4g_csr_create(mtl4g_matrix* A, int32_t nrows, int32_t ncols, int64_t nnz,
+                     const int32_t* rowptr, const int32_t* col, const double* val,
+                     int where);
+int mtl4g_csr_destroy(mtl4g_matrix A);
+/* num_rows(A), num_cols(A), A.nnz() (compressed2D.hpp:610-620) */
+int mtl4g_csr_info(mtl4g_matrix A, int32_t* nrows, int32_t* ncols, int64_t* nnz);
+/* Overwrite the values (same pattern), e.g. after an inserter with
+ * update_store on existing entries (matrix/inserter.hpp:30-48). */
+int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
+/* Row-block partition used by the SpMV kernels: at most rows_per_block
+ * rows and nnz_per_block non-zeros per workgroup (0 = library default). */
+int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
+
+/* w (=,+=,-=) A*v: smat_cvec_mult, operation/mat_vec_mult.hpp:613-683,
+ * dispatched from vector/crtp_base_vector.hpp:93-104,191-200,268-277.
+ * Each row is summed in column order, as MTL4 does. */
+int mtl4g_spmv(mtl4g_matrix A, const double* d_v, double* d_w, int assign, void* stream);
+/* The same product on the FP32 or BF16 value copy with FP32 vectors
+ * (inner solves of mixed-precision refinement; no MTL4 counterpart). */
+int mtl4g_spmv_f32(mtl4g_matrix A, int value_format, const float* d_v, float* d_w,
+                   int assign, void* stream);
+/* (lazy(q)= A*p) || (lazy(alpha)= lazy_dot(p,q)), itl/krylov/cg.hpp:103 */
+int mtl4g_spmv_dot(mtl4g_matrix A, const double* d_p, double* d_q, double* h_dot, void* stream);
+
+/* ---- dense_vector reductions and updates ----------------------------- */
+/* dot(v, w), operation/dot.hpp:226-232 */
+int mtl4g_dot(int64_t n, const double* d_v, const double* d_w, double* h_result, void* stream);
+/* unary_dot(v), operation/unary_dot.hpp:56-65 */
+int mtl4g_unary_dot(int64_t n, const double* d_v, double* h_result, void* stream);
+/* two_norm(v), operation/two_norm.hpp:52-57 */
+int mtl4g_two_norm(int64_t n, const double* d_v, double* h_result, void* stream);
+/* w += alpha*v (vector/vec_vec_aop_expr.hpp:61-180) */
+int mtl4g_axpy(int64_t n, double alpha, const double* d_v, double* d_w, void* stream);
+/* (lazy(r) -= alpha*q) || (lazy(rr)= lazy_unary_dot(r)), cg.hpp:108 */
+int mtl4g_axpy_unary_dot(int64_t n, double alpha, const double* d_q, double* d_r,
+                         double* h_rr, void* stream);
+
+/* ---- preconditioners ------------------------------------------------ */
+/* itl::pc::identity(A) (identity.hpp:32), diagonal(A) (diagonal.hpp:37-45),
+ * ilu_0<Matrix,Value>(A) (ilu_0.hpp:34-92 with ilu.hpp:54-56), ic_0<Matrix,
+ * Value>(A) (ic_0.hpp:57,135-186).  Factorization runs on the host in FP64
+ * in MTL4's loop order; value_format selects the stored factor precision.
+ * Sweeps run level-scheduled on the device. */
+int mtl4g_pc_create(mtl4g_pc* P, mtl4g_matrix A, int kind, int value_format);
+int mtl4g_pc_destroy(mtl4g_pc P);
+/* solve(P, x) into y: P.solve(x, y) (pc/solver.hpp:23-52) */
+int mtl4g_pc_solve(mtl4g_pc P, const double* d_x, double* d_y, void* stream);
+/* adjoint_solve(P, x) into y (diagonal.hpp:76-83, ilu.hpp:105-115) */
+int mtl4g_pc_adjoint_solve(mtl4g_pc P, const double* d_x, double* d_y, void* stream);
+/* Copy a factor to the host, as stored (inverted diagonals, FP64 values):
+ * which 0 = inverse diagonal (n values), 1 = L, 2 = U (CSR, natural row
+ * order).  Pass NULL arrays to query *n_out and *nnz_out first. */
+int mtl4g_pc_factor(mtl4g_pc P, int which, int32_t* n_out, int64_t* nnz_out,
+                    int32_t* h_rowptr, int32_t* h_col, double* h_val);
+/* Number of dependency levels of the lower / upper sweep. */
+int mtl4g_pc_levels(mtl4g_pc P, int32_t* lower_levels, int32_t* upper_levels);
+
+/* ---- Krylov solvers ------------------------------------------------- */
+/* itl::cg(A, x, b, P, iter) (cg.hpp:74-111; P == NULL or identity takes
+ * the unpreconditioned form cg.hpp:38-71 as is_identity does, :80-81) and
+ * itl::bicgstab(A, x, b, P, iter) (bicgstab.hpp:27-78).  x holds the start
+ * vector and receives the result.  The whole loop runs on the device;
+ * h_hist (may be NULL) receives the residual of each finished() call, up
+ * to hist_cap values.  Returns it->error (0 or 1) or a negative status. */
+int mtl4g_cg(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
+             mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
+int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
+                   mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
+
+/* Reusable solver: the workspace (MTL4 allocates it per call, cg.hpp:90)
+ * stays with the handle, so repeated solves allocate nothing. */
+int mtl4g_solver_create(mtl4g_solver* S, mtl4g_matrix A, mtl4g_pc P, int method);
+int mtl4g_solver_destroy(mtl4g_solver S);
+int mtl4g_solver_solve(mtl4g_solver S, double* d_x, const double* d_b, mtl4g_iteration* it,
+                       double* h_hist, int32_t hist_cap, void* stream);
+
+/* ---- mixed-precision iterative refinement (no MTL4 counterpart) ------ */
+/* Outer loop in FP64 on the device: r = b - A*x, stop when
+ * ||r|| <= rtol*norm_r0 (or atol); otherwise solve A*d = r with the inner
+ * method in FP32 arithmetic on FP32 or BF16 stored values (inner stop:
+ * inner_rtol relative to ||r||), then x += d in FP64. */
+typedef struct {
+    int32_t method;          /* MTL4G_CG or MTL4G_BICGSTAB */
+    int32_t value_format;    /* MTL4G_F32 or MTL4G_BF16 for the inner matrix */
+    double inner_rtol;       /* inner relative tolerance, e.g. 1e-5 */
+    int32_t inner_max_iter;  /* per inner solve */
+    int32_t max_outer;       /* refinement steps */
+} mtl4g_ir_params;
+
+typedef struct {
+    int32_t outer;           /* refinement steps taken */
+    int32_t inner_total;     /* inner iterations summed over all steps */
+    double relres;           /* final FP64 ||b - A x|| / norm_r0 */
+} mtl4g_ir_stats;
+
+/* P32 (may be NULL = identity) must be created with value_format F32. */
+int mtl4g_ir_solve(mtl4g_matrix A, mtl4g_pc P32, double* d_x, const double* d_b,
+                   mtl4g_iteration* it, const mtl4g_ir_params* prm, mtl4g_ir_stats* st,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MTL4G_H */

@@ -1,0 +1,453 @@
+"""mtl4_amd -- MTL4's sparse Krylov path on MI355X (gfx950).
+
+Python mirror of MTL4's interface for this path, over the C ABI of
+include/mtl4g.h (libmtl4g.so, hand-written HIP kernels):
+
+    A = compressed2D(n, n, rowptr, col, val)      # mtl::compressed2D<double>
+    x = dense_vector(n); b = A * xt               # mtl::dense_vector<double>, w = A*v
+    P = pc.diagonal(A)                            # itl::pc::{identity,diagonal,ilu_0,ic_0}
+    it = basic_iteration(b, 2000, 1e-10)          # itl::basic_iteration<double>
+    err = cg(A, x, b, P, it)                      # itl::cg / itl::bicgstab
+
+Device memory and streams come from torch (plumbing only); every operation
+runs in libmtl4g.so and fails loudly when it is missing.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, PLUS, MtlError, check,
+                   incompatible_size, logic_error, matrix_not_square, missing_diagonal,
+                   nonfinite_error, unexpected_orthogonality)
+
+__all__ = ["compressed2D", "dense_vector", "pc", "solve", "adjoint_solve", "basic_iteration",
+           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "dot", "unary_dot",
+           "two_norm", "num_rows", "num_cols", "size", "laplacian_setup", "ir_solve",
+           "MtlError", "matrix_not_square", "incompatible_size", "missing_diagonal",
+           "unexpected_orthogonality", "nonfinite_error", "logic_error"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _L():
+    return _lib.lib()
+
+
+# ---------------------------------------------------------------- containers
+class dense_vector:
+    """mtl::dense_vector<double> (vector/dense_vector.hpp:50-269), device resident.
+
+    dense_vector(n), dense_vector(n, value), dense_vector(array_or_tensor)."""
+
+    def __init__(self, n_or_data, value=0.0, dtype=None):
+        torch = _torch()
+        dt = dtype or torch.float64
+        if isinstance(n_or_data, (int, np.integer)):
+            self.t = torch.full((int(n_or_data),), float(value), dtype=dt, device="cuda")
+        elif isinstance(n_or_data, torch.Tensor):
+            self.t = n_or_data.to(device="cuda", dtype=dt).contiguous()
+        else:
+            self.t = torch.as_tensor(np.ascontiguousarray(n_or_data), dtype=dt).to("cuda")
+
+    @property
+    def ptr(self):
+        return C.c_void_p(self.t.data_ptr())
+
+    def size(self):
+        return self.t.numel()
+
+    def __len__(self):
+        return self.t.numel()
+
+    def numpy(self):
+        return self.t.cpu().numpy()
+
+    def __getitem__(self, i):
+        return float(self.t[i])
+
+    def __setitem__(self, i, v):
+        self.t[i] = v
+
+    def assign(self, other):
+        """v = w (copy)."""
+        self.t.copy_(other.t if isinstance(other, dense_vector) else _torch().as_tensor(other))
+        return self
+
+
+def size(v):
+    return v.size()
+
+
+class compressed2D:
+    """mtl::compressed2D<double> with int32 indices (matrix/compressed2D.hpp:258-653).
+
+    compressed2D(m, n, rowptr, col, val) copies the CSR arrays to the device,
+    as the raw-pointer constructor (compressed2D.hpp:355-364) copies them."""
+
+    def __init__(self, nrows, ncols=None, rowptr=None, col=None, val=None):
+        ncols = nrows if ncols is None else ncols
+        if rowptr is None:
+            rowptr = np.zeros(nrows + 1, np.int32)
+            col = np.zeros(0, np.int32)
+            val = np.zeros(0)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        if len(self.rowptr) != nrows + 1:
+            raise incompatible_size("rowptr must have nrows+1 entries")
+        h = C.c_void_p()
+        check(_L().mtl4g_csr_create(C.byref(h), nrows, ncols, len(self.col),
+                                    self.rowptr.ctypes.data_as(C.c_void_p),
+                                    self.col.ctypes.data_as(C.c_void_p),
+                                    self.val.ctypes.data_as(C.c_void_p), 0), "compressed2D")
+        self.h = h
+        self._n, self._m = nrows, ncols
+
+    @classmethod
+    def from_csr(cls, rowptr, col, val, ncols=None):
+        n = len(rowptr) - 1
+        return cls(n, n if ncols is None else ncols, rowptr, col, val)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.mtl4g_csr_destroy(h)
+            self.h = None
+
+    def num_rows(self):
+        return self._n
+
+    def num_cols(self):
+        return self._m
+
+    def nnz(self):
+        return len(self.col)
+
+    def set_values(self, val):
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        check(_L().mtl4g_csr_set_values(self.h, self.val.ctypes.data_as(C.c_void_p), 0))
+
+    def tune(self, rows_per_block=0, nnz_per_block=0):
+        check(_L().mtl4g_csr_tune(self.h, rows_per_block, nnz_per_block), "tune")
+
+    def mult(self, v, w, assign=ASSIGN):
+        """w (=,+=,-=) A*v  (operation/mat_vec_mult.hpp:613-683)."""
+        if v.size() != self._m or w.size() != self._n:
+            raise incompatible_size("A*v: sizes %d x %d, v %d, w %d" % (self._n, self._m, v.size(), w.size()))
+        check(_L().mtl4g_spmv(self.h, v.ptr, w.ptr, assign, _stream()), "A*v")
+        return w
+
+    def __mul__(self, v):
+        w = dense_vector(self._n)
+        return self.mult(v, w)
+
+
+def num_rows(A):
+    return A.num_rows()
+
+
+def num_cols(A):
+    return A.num_cols()
+
+
+def laplacian_setup(m, n):
+    """mtl::mat::laplacian_setup(A, m, n) (matrix/laplacian_setup.hpp:25-44)."""
+    from .generators import laplacian_setup as gen
+    rp, ci, va = gen(m, n)
+    return compressed2D(m * n, m * n, rp, ci, va)
+
+
+# ---------------------------------------------------------------- reductions
+def dot(v, w):
+    """dot(v, w) (operation/dot.hpp:226-232)."""
+    if v.size() != w.size():
+        raise incompatible_size("dot")
+    r = C.c_double()
+    check(_L().mtl4g_dot(v.size(), v.ptr, w.ptr, C.byref(r), _stream()), "dot")
+    return r.value
+
+
+def unary_dot(v):
+    r = C.c_double()
+    check(_L().mtl4g_unary_dot(v.size(), v.ptr, C.byref(r), _stream()), "unary_dot")
+    return r.value
+
+
+def two_norm(v):
+    r = C.c_double()
+    check(_L().mtl4g_two_norm(v.size(), v.ptr, C.byref(r), _stream()), "two_norm")
+    return r.value
+
+
+# ---------------------------------------------------------------- preconditioners
+class _PC:
+    kind = None
+
+    def __init__(self, A, value=F64):
+        if value in (np.float32, "float", "float32"):
+            value = F32
+        if value in (np.float64, "double", "float64"):
+            value = F64
+        h = C.c_void_p()
+        check(_L().mtl4g_pc_create(C.byref(h), A.h, self.kind, value), type(self).__name__)
+        self.h, self.A, self.value = h, A, value
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.mtl4g_pc_destroy(h)
+            self.h = None
+
+    def solve(self, x, y=None):
+        y = dense_vector(x.size()) if y is None else y
+        check(_L().mtl4g_pc_solve(self.h, x.ptr, y.ptr, _stream()), "solve")
+        return y
+
+    def adjoint_solve(self, x, y=None):
+        y = dense_vector(x.size()) if y is None else y
+        check(_L().mtl4g_pc_adjoint_solve(self.h, x.ptr, y.ptr, _stream()), "adjoint_solve")
+        return y
+
+    def _factor(self, which):
+        n, nnz = C.c_int32(), C.c_int64()
+        check(_L().mtl4g_pc_factor(self.h, which, C.byref(n), C.byref(nnz), None, None, None))
+        if nnz.value < 0:
+            raise logic_error("factor not present")
+        if which == 0:
+            val = np.zeros(nnz.value)
+            check(_L().mtl4g_pc_factor(self.h, 0, None, None, None, None,
+                                       val.ctypes.data_as(C.c_void_p)))
+            return val
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        va = np.zeros(nnz.value)
+        check(_L().mtl4g_pc_factor(self.h, which, None, None, rp.ctypes.data_as(C.c_void_p),
+                                   ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p)))
+        return rp, ci, va
+
+    def get_L(self):
+        return self._factor(1)
+
+    def get_U(self):
+        return self._factor(2)
+
+    def inv_diag(self):
+        return self._factor(0)
+
+    def levels(self):
+        lo, up = C.c_int32(), C.c_int32()
+        check(_L().mtl4g_pc_levels(self.h, C.byref(lo), C.byref(up)))
+        return lo.value, up.value
+
+
+class _PCNamespace:
+    class identity(_PC):
+        """itl::pc::identity (itl/pc/identity.hpp:24-71)."""
+        kind = _lib.PC_IDENTITY
+
+    class diagonal(_PC):
+        """itl::pc::diagonal (itl/pc/diagonal.hpp:28-103)."""
+        kind = _lib.PC_DIAGONAL
+
+    class ilu_0(_PC):
+        """itl::pc::ilu_0<Matrix, Value> (itl/pc/ilu_0.hpp:34-92)."""
+        kind = _lib.PC_ILU0
+
+    class ic_0(_PC):
+        """itl::pc::ic_0<Matrix, Value> (itl/pc/ic_0.hpp:39-194)."""
+        kind = _lib.PC_IC0
+
+
+pc = _PCNamespace
+
+
+def solve(P, x):
+    """solve(P, x) (itl/pc/solver.hpp:23-52)."""
+    return P.solve(x)
+
+
+def adjoint_solve(P, x):
+    return P.adjoint_solve(x)
+
+
+# ---------------------------------------------------------------- iteration control
+class basic_iteration:
+    """itl::basic_iteration<double> (itl/iteration/basic_iteration.hpp:23-164).
+
+    basic_iteration(r0_vector_or_norm, max_iter, rtol, atol=0)."""
+
+    def __init__(self, r0, max_iter, rtol, atol=0.0):
+        self.norm_r0_ = abs(two_norm(r0)) if isinstance(r0, dense_vector) else float(r0)
+        self.max_iter = int(max_iter)
+        self.rtol_, self.atol_ = float(rtol), float(atol)
+        self.i, self.error, self.resid_ = 0, 0, 0.0
+        self.is_finished = False
+        self.quite, self.suppress = False, False
+        self.history = []
+
+    # accessors of basic_iteration
+    def iterations(self): return self.i
+    def resid(self): return self.resid_
+    def relresid(self): return self.resid_ / self.norm_r0_ if self.norm_r0_ else float("nan")
+    def norm_r0(self): return self.norm_r0_
+    def set_norm_r0(self, v): self.norm_r0_ = float(v)
+    def tol(self): return self.rtol_
+    def atol(self): return self.atol_
+    def max_iterations(self): return self.max_iter
+    def set_max_iterations(self, m): self.max_iter = int(m)
+    def restart(self): self.i = 0
+    def first(self): return self.i <= 1
+    def is_converged(self): return self.is_finished and self.error == 0
+    def set_quite(self, q): self.quite = q
+    def suppress_resume(self, s): self.suppress = s
+    def error_code(self): return self.error
+    def __int__(self): return self.error_code()
+    def fail(self, code, msg=""):
+        self.error = code
+        return self.error_code()
+
+    def _state(self):
+        return _lib.Iteration(self.norm_r0_, self.max_iter, self.rtol_, self.atol_, self.i, 0, 0.0)
+
+    def _update(self, st, hist, method):
+        self.i, self.error, self.resid_ = st.i, st.error, st.resid
+        self.is_finished = True
+        self.history = list(hist) if hist is not None else []
+
+    def _wants_history(self):
+        return 0
+
+
+class cyclic_iteration(basic_iteration):
+    """itl::cyclic_iteration (itl/iteration/cyclic_iteration.hpp:23-100): prints
+    'iteration i: resid r' every `cycle` iterations and a final resume."""
+
+    def __init__(self, r0, max_iter, rtol, atol=0.0, cycle=100, out=None):
+        super().__init__(r0, max_iter, rtol, atol)
+        self.cycle = int(cycle)
+        self.out = out or sys.stdout
+        self._i0 = 0
+
+    def _wants_history(self):
+        return 2 * max(self.max_iter - self.i, 0) + 4
+
+    def _state(self):
+        self._i0 = self.i
+        return super()._state()
+
+    def _update(self, st, hist, method):
+        super()._update(st, hist, method)
+        if self.quite:
+            return
+        # finished() calls: CG one per iteration; BiCGSTAB r at the loop head,
+        # then s (after ++iter) and r again per iteration
+        counts, i = [], self._i0
+        for k in range(len(self.history)):
+            if method == _lib.BICGSTAB and k > 0 and k % 2 == 1:
+                i += 1
+            elif method == _lib.CG and k > 0:
+                i += 1
+            counts.append(i)
+        last = -1
+        for i, r in zip(counts, self.history):
+            if i % self.cycle == 0 and i != last:
+                print("iteration %d: resid %g" % (i, r), file=self.out)
+                last = i
+
+    def error_code(self):
+        if not self.suppress:
+            it = max(self.i, 1)
+            rel = self.relresid()
+            conv = rel ** (1.0 / it) if rel == rel and rel > 0 else float("nan")
+            print("finished! error code = %d\n%d iterations\n%g is actual final residual. \n"
+                  "%g is actual relative tolerance achieved. \nRelative tol: %g  Absolute tol: %g\n"
+                  "Convergence:  %g" % (self.error, self.i, self.resid_, rel, self.rtol_,
+                                        self.atol_, conv), file=self.out)
+        return self.error
+
+
+class noisy_iteration(cyclic_iteration):
+    """itl::noisy_iteration (itl/iteration/noisy_iteration.hpp:21-30): cycle 1."""
+
+    def __init__(self, r0, max_iter, rtol, atol=0.0, out=None):
+        super().__init__(r0, max_iter, rtol, atol, 1, out)
+
+
+# ---------------------------------------------------------------- solvers
+def _krylov(fn, method, A, x, b, P, it):
+    if A.num_rows() != A.num_cols():
+        raise matrix_not_square("Krylov solve")
+    if x.size() != A.num_cols() or b.size() != A.num_rows():
+        raise incompatible_size("Krylov solve")
+    st = it._state()
+    cap = it._wants_history()
+    hist = np.full(max(cap, 1), np.nan)
+    ph = P.h if P is not None else None
+    status = fn(A.h, ph, x.ptr, b.ptr, C.byref(st),
+                hist.ctypes.data_as(C.c_void_p) if cap else None, cap, _stream())
+    h = hist[:cap]
+    h = h[~np.isnan(h)] if cap else None
+    it._update(st, h, method)
+    check(status, "cg" if method == CG else "bicgstab")
+    return it.error_code()
+
+
+def cg(A, x, b, P=None, it=None):
+    """itl::cg(A, x, b, P, iter) or cg(A, x, b, iter) (itl/krylov/cg.hpp:38-111)."""
+    if isinstance(P, basic_iteration):
+        P, it = None, P
+    return _krylov(_L().mtl4g_cg, CG, A, x, b, P, it)
+
+
+def bicgstab(A, x, b, P=None, it=None):
+    """itl::bicgstab(A, x, b, P, iter) (itl/krylov/bicgstab.hpp:27-78)."""
+    if isinstance(P, basic_iteration):
+        P, it = None, P
+    return _krylov(_L().mtl4g_bicgstab, BICGSTAB, A, x, b, P, it)
+
+
+class Solver:
+    """Reusable device workspace for repeated solves with one (A, P)."""
+
+    def __init__(self, A, P=None, method=CG):
+        h = C.c_void_p()
+        check(_L().mtl4g_solver_create(C.byref(h), A.h, P.h if P is not None else None, method))
+        self.h, self.A, self.P, self.method = h, A, P, method
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.mtl4g_solver_destroy(h)
+            self.h = None
+
+    def solve(self, x, b, it):
+        st = it._state()
+        status = _L().mtl4g_solver_solve(self.h, x.ptr, b.ptr, C.byref(st), None, 0, _stream())
+        it._update(st, None, self.method)
+        check(status, "solve")
+        return it.error
+
+
+def ir_solve(A, x, b, it, P32=None, method=CG, value_format=F32, inner_rtol=1e-5,
+             inner_max_iter=10000, max_outer=50):
+    """Mixed-precision iterative refinement: FP64 residual/correction/norm on
+    the device, inner Krylov solve in FP32 arithmetic (FP32 or BF16 values).
+    Returns (error_code, stats dict)."""
+    prm = _lib.IRParams(method, value_format, inner_rtol, inner_max_iter, max_outer)
+    stats = _lib.IRStats()
+    st = it._state()
+    status = _L().mtl4g_ir_solve(A.h, P32.h if P32 is not None else None, x.ptr, b.ptr,
+                                 C.byref(st), C.byref(prm), C.byref(stats), _stream())
+    it._update(st, None, method)
+    check(status, "ir_solve")
+    return it.error, {"outer": stats.outer, "inner_total": stats.inner_total,
+                      "relres": stats.relres}

@@ -1,0 +1,238 @@
+// internal.hpp -- shared host/device pieces of the MI355X sparse Krylov library.
+//
+// Layout in HBM (one GPU):
+//   CSR: int32 rowptr[n+1], int32 col[nnz], double val[nnz] (plus optional
+//        float / bfloat16 copies of val for the inner solves of mixed
+//        precision), rows sorted by column as in mtl::compressed2D.
+//   Row blocks: int32 blk[nb+1] -- the SpMV partition; block b covers rows
+//        [blk[b], blk[b+1]) with at most rows_per_block rows and at most
+//        nnz_per_block non-zeros, or a single longer row.
+//   Solver control: one Ctl struct (scalars of the recurrence, iteration
+//        counter, stop flag, reduction tickets) and a partials array.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mtl4g.h"
+
+namespace mtl4g {
+
+// ----------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+int hip_status(hipError_t e, const char* what, const char* file, int line);
+
+#define M4_HIP(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) return ::mtl4g::hip_status(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define M4_TRY(expr)                \
+    do {                            \
+        int s_ = (expr);            \
+        if (s_ != MTL4G_OK) return s_; \
+    } while (0)
+
+constexpr int kWave = 64;      // CDNA wavefront
+constexpr int kXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round robin
+constexpr int kMaxPartials = 1 << 16;
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Device buffer owned by a handle.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        free();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) { p = nullptr; return hip_status(e, "hipMalloc", __FILE__, __LINE__); }
+        n = count;
+        return MTL4G_OK;
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// ----------------------------------------------------------------- device helpers
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+    return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+
+template <typename MT>
+__device__ __forceinline__ double val_f64(MT v) { return static_cast<double>(v); }
+
+template <typename AT, typename MT>
+__device__ __forceinline__ AT load_val(const MT* p, int64_t j) { return static_cast<AT>(p[j]); }
+template <>
+__device__ __forceinline__ float load_val<float, uint16_t>(const uint16_t* p, int64_t j) {
+    return bf16_to_f32(p[j]);
+}
+
+// Agent-scope loads/stores for values exchanged between workgroups
+// inside one kernel (partials, level flags); they bypass non-coherent
+// per-XCD cache lines.
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Butterfly sum over the 64 lanes: every lane ends with the same value and
+// the association order is fixed, so results do not depend on timing.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Block sum with a fixed order; result valid in every thread.
+template <int BS, typename T>
+__device__ __forceinline__ T block_sum(T v, T* sh /* BS/64 */) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    T r = sh[0];
+#pragma unroll
+    for (int i = 1; i < BS / kWave; i++) r += sh[i];
+    return r;
+}
+
+// Last-block pattern: every block stores its partials, then the block that
+// arrives last (ticket nblocks-1) reduces them in index order and resets
+// the ticket for the next launch (no memset needed, graph-replay safe).
+template <int BS>
+__device__ __forceinline__ bool arrive_last(unsigned* ticket, unsigned nblocks) {
+    __shared__ int am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned prev = atomicAdd(ticket, 1u);
+        am_last = (prev == nblocks - 1);
+    }
+    __syncthreads();
+    if (am_last) {
+        __threadfence();
+        if (threadIdx.x == 0) st_agent(ticket, 0u);
+    }
+    return am_last != 0;
+}
+
+// Sum of partials[0..n) in a fixed order, valid in all threads.
+template <int BS, typename T>
+__device__ __forceinline__ T reduce_partials(const T* partials, int n, int stride, T* sh) {
+    T s = 0;
+    for (int i = threadIdx.x; i < n; i += BS) s += ld_agent(partials + (size_t)i * stride);
+    return block_sum<BS>(s, sh);
+}
+
+// XCD-aware block order: hardware deals workgroup g to XCD g % 8; this maps
+// it to logical block (g % 8) * per + g / 8 so each XCD walks a contiguous
+// range of rows and the gathers of neighbouring rows share its L2.
+__device__ __forceinline__ int xcd_logical_block(int g, int per_xcd) {
+    return (g % kXcd) * per_xcd + g / kXcd;
+}
+
+// ----------------------------------------------------------------- CSR core
+// One workgroup, rows [r0, r1) of a row block whose non-zeros fit the LDS
+// tile: all lanes stream val/col coalesced and form the products into LDS,
+// then lane t sums row r0+t in column order (the order of
+// mat_vec_mult.hpp:656-682, so FP64 results match MTL4 bit for bit).
+// Long single rows (nnz > TILE) are summed by the whole workgroup.
+template <int BS, int TILE, typename AT, typename MT, typename XT>
+__device__ __forceinline__ AT csr_block_rows(const int32_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ col,
+                                             const MT* __restrict__ val,
+                                             const XT* __restrict__ x, int r0, int r1,
+                                             AT* prod, AT* sh, int& row) {
+    const int t = threadIdx.x;
+    row = r0 + t;
+    const int j0 = rowptr[r0];
+    const int j1 = rowptr[r1];
+    const int nz = j1 - j0;
+    int a = 0, b = 0;
+    if (row < r1) {
+        a = rowptr[row] - j0;
+        b = rowptr[row + 1] - j0;
+    }
+    if (nz > TILE) {  // one long row: strided partial sums, fixed-order combine
+        AT s = 0;
+        for (int k = t; k < nz; k += BS)
+            s += load_val<AT>(val, (int64_t)j0 + k) * static_cast<AT>(x[col[j0 + k]]);
+        s = block_sum<BS>(s, sh);
+        return s;   // valid for lane 0 == the row
+    }
+    constexpr int U = 4;
+    for (int base = t; base < nz; base += BS * U) {
+        int c[U];
+        AT v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int k = base + u * BS;
+            c[u] = k < nz ? col[j0 + k] : 0;
+            v[u] = k < nz ? load_val<AT>(val, (int64_t)j0 + k) : AT(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int k = base + u * BS;
+            if (k < nz) prod[k] = v[u] * static_cast<AT>(x[c[u]]);
+        }
+    }
+    __syncthreads();
+    AT s = 0;
+    for (int k = a; k < b; k++) s += prod[k];
+    return s;
+}
+
+}  // namespace mtl4g
+
+// ----------------------------------------------------------------- handles
+struct mtl4g_matrix_s {
+    int device = 0;
+    int32_t n = 0, m = 0;
+    int64_t nnz = 0;
+    mtl4g::DevBuf<int32_t> rowptr, col;
+    mtl4g::DevBuf<double> val;
+    mtl4g::DevBuf<float> val32;       // FP32 copy (created on demand)
+    mtl4g::DevBuf<uint16_t> valbf;    // BF16 copy (created on demand)
+    mtl4g::DevBuf<double> diag;       // A[i][i] (0 where absent)
+    mtl4g::DevBuf<int32_t> blk;       // row-block partition
+    int nb = 0;                       // number of row blocks
+    int rows_per_block = 0, nnz_per_block = 0;
+    int32_t max_row = 0;
+    std::vector<int32_t> h_rowptr, h_col;   // host pattern (factorizations)
+    std::vector<double> h_val;
+};
+
+namespace mtl4g {
+// Row-block partition (host).
+int build_row_blocks(mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block);
+int ensure_lowp_values(mtl4g_matrix_s* A, int value_format, hipStream_t s);
+
+// SpMV launchers used by several translation units.
+int launch_spmv(const mtl4g_matrix_s* A, const double* x, double* y, int assign, hipStream_t s);
+int launch_spmv_f32(const mtl4g_matrix_s* A, int fmt, const float* x, float* y, int assign,
+                    hipStream_t s);
+int blocks_for(int64_t n);   // grid size of the vector kernels
+}  // namespace mtl4g

@@ -1,0 +1,126 @@
+// kernels.hpp -- the two kernel shapes every operation of the library is
+// built from, each parameterized by an epilogue functor:
+//
+//   k_csr: one workgroup per CSR row block (SpMV and everything fused into
+//          it: residuals, dots of the product with another vector);
+//   k_vec: one workgroup per contiguous chunk of a vector (axpy-type
+//          updates fused with their reductions).
+//
+// Epilogue concept:
+//   static constexpr int NP;                       // partial sums produced
+//   bool skip() const;                             // uniform early exit
+//   void prepare();                                // load scalars once per thread
+//   void operator()(int row_or_i, [AT s,] AT* part) const;
+//   void finish(const AT* totals) const;           // last workgroup, lane 0
+// Partial sums are combined in block-index order by the last workgroup to
+// arrive, so every reduction is deterministic and needs no memset.
+#pragma once
+
+#include "internal.hpp"
+
+namespace mtl4g {
+
+constexpr int kMaxTile = 4096;
+
+template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
+                                            const int32_t* __restrict__ col,
+                                            const MT* __restrict__ val,
+                                            const XT* __restrict__ x,
+                                            const int32_t* __restrict__ blk, int nb, int per_xcd,
+                                            Epi epi, AT* partials, unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    __shared__ AT prod[TILE];
+    __shared__ AT sh[BS / kWave];
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    const int b = xcd_logical_block(blockIdx.x, per_xcd);
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+    if (b < nb) {
+        const int r0 = blk[b], r1 = blk[b + 1];
+        int row;
+        AT s = csr_block_rows<BS, TILE, AT>(rowptr, col, val, x, r0, r1, prod, sh, row);
+        if (row < r1) epi(row, s, part);
+    }
+    if constexpr (Epi::NP > 0) {
+        const unsigned G = gridDim.x;
+#pragma unroll
+        for (int k = 0; k < Epi::NP; k++) {
+            AT v = block_sum<BS>(part[k], sh);
+            if (threadIdx.x == 0) st_agent(partials + (size_t)k * G + b, v);
+        }
+        if (arrive_last<BS>(ticket, G)) {
+            AT tot[Epi::NP];
+#pragma unroll
+            for (int k = 0; k < Epi::NP; k++) tot[k] = reduce_partials<BS>(partials + (size_t)k * G, G, 1, sh);
+            if (threadIdx.x == 0) epi.finish(tot);
+        }
+    }
+}
+
+template <int BS, typename AT, typename Epi>
+__global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    __shared__ AT sh[BS / kWave];
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    const unsigned G = gridDim.x;
+    const int b = blockIdx.x;
+    int64_t chunk = (n + G - 1) / G;
+    chunk = (chunk + BS - 1) / BS * BS;
+    const int64_t lo = (int64_t)b * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += BS) epi(i, part);
+    if constexpr (Epi::NP > 0) {
+#pragma unroll
+        for (int k = 0; k < Epi::NP; k++) {
+            AT v = block_sum<BS>(part[k], sh);
+            if (threadIdx.x == 0) st_agent(partials + (size_t)k * G + b, v);
+        }
+        if (arrive_last<BS>(ticket, G)) {
+            AT tot[Epi::NP];
+#pragma unroll
+            for (int k = 0; k < Epi::NP; k++) tot[k] = reduce_partials<BS>(partials + (size_t)k * G, G, 1, sh);
+            if (threadIdx.x == 0) epi.finish(tot);
+        }
+    }
+}
+
+// ---- host launchers
+template <typename AT, typename MT, typename XT, typename Epi>
+int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& epi,
+               AT* partials, unsigned* ticket, hipStream_t s) {
+    if (A->nb == 0) return MTL4G_OK;
+    const int per = (A->nb + kXcd - 1) / kXcd;
+    const int grid = per * kXcd;
+    if (Epi::NP > 0 && grid > kMaxPartials) {
+        set_error("matrix too large for one reduction pass (%d blocks)", grid);
+        return MTL4G_ERR_UNSUPPORTED;
+    }
+    const int tile = A->nnz_per_block;
+#define M4_LAUNCH(T)                                                                          \
+    hipLaunchKernelGGL((k_csr<256, T, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,         \
+                       A->rowptr.p, A->col.p, val, x, A->blk.p, A->nb, per, epi, partials, ticket)
+    if (tile <= 1024) M4_LAUNCH(1024);
+    else if (tile <= 2048) M4_LAUNCH(2048);
+    else M4_LAUNCH(4096);
+#undef M4_LAUNCH
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
+template <typename AT, typename Epi>
+int launch_vec(int64_t n, const Epi& epi, AT* partials, unsigned* ticket, hipStream_t s) {
+    if (n <= 0) return MTL4G_OK;
+    hipLaunchKernelGGL((k_vec<256, AT, Epi>), dim3(blocks_for(n)), dim3(256), 0, s, n, epi,
+                       partials, ticket);
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
+}  // namespace mtl4g

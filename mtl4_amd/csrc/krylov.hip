@@ -1,0 +1,602 @@
+// krylov.hip -- itl::cg and itl::bicgstab with the whole loop on the device.
+//
+// Every scalar of the recurrence (rho, rho_1, alpha, beta, omega, ...) and
+// the state of basic_iteration live in a device Ctl block.  Each reduction
+// is fused into the kernel that produces its operands, and the last
+// workgroup of that kernel combines the partials and takes the decisions
+// MTL4 takes on the host (finished(), ++iter, first(), the breakdown
+// throws).  Once Ctl::done is set every later kernel exits at entry, so the
+// host only enqueues chunks of iterations and polls Ctl::done between
+// chunks; no host round trip sits inside an iteration.
+//
+// CG + diagonal (cg.hpp:74-111) runs three kernels per iteration:
+//   P : p = z + beta*p            with z = inv_diag .* r formed on the fly
+//   Q : q = A*p, p.q              (SpMV + dot, cg.hpp:103)      -> alpha
+//   XR: x += alpha*p, r -= alpha*q, r.r, r.(inv_diag .* r)      -> stop?, beta
+// i.e. B_spmv + 11 N V bytes per iteration; z is never stored.
+// General preconditioners add pc_apply(r -> z) and a dot for rho.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.hpp"
+#include "krylov.hpp"
+
+namespace mtl4g {
+
+enum { PCM_NONE = 0, PCM_DIAG = 1, PCM_GENERAL = 2 };
+
+// basic_iteration::finished(Real) (basic_iteration.hpp:60-65,85-93,43-48)
+// followed by ++iter when the loop goes on.  NaN/Inf residuals stop the
+// solve with MTL4G_ERR_NONFINITE instead of running to max_iter.
+__device__ __forceinline__ void iter_finished(Ctl* c, double resid, double* hist) {
+    if (c->hist_n < c->hist_cap) hist[c->hist_n] = resid;
+    c->hist_n++;
+    c->resid = resid;
+    const bool conv = c->norm_r0 == 0 ? (resid <= c->atol)
+                                      : (resid <= c->rtol * c->norm_r0 || resid <= c->atol);
+    if (conv) { c->done = 1; return; }
+    if (c->i >= c->max_iter) { c->error = 1; c->done = 1; return; }
+    if (!isfinite(resid)) { c->status = MTL4G_ERR_NONFINITE; c->done = 1; return; }
+    c->i++;
+    c->first = c->i <= 1;
+}
+
+// ---------------------------------------------------------------- CG epilogues
+// r = b - A*x; r.r (and r.(d.*r) for the diagonal PC)  (cg.hpp:92-94)
+template <typename VT, typename DT, int PCM>
+struct EpiCgInit {
+    static constexpr int NP = PCM == PCM_DIAG ? 2 : 1;
+    const VT* b;
+    VT* r;
+    const DT* d;
+    Ctl* c;
+    double* hist;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, VT s, VT* part) const {
+        const VT v = b[row] - s;
+        r[row] = v;
+        part[0] += v * v;
+        if (PCM == PCM_DIAG) {
+            const VT z = static_cast<VT>(d[row]) * v;
+            part[1] += v * z;
+        }
+    }
+    __device__ void finish(const VT* tot) const {
+        c->rr = (double)tot[0];
+        iter_finished(c, (double)std::sqrt(std::fabs(tot[0])), hist);
+        if (!c->done && PCM != PCM_GENERAL) {
+            const VT rho = PCM == PCM_DIAG ? tot[1] : tot[0];
+            c->beta = c->first ? 0.0 : (double)(rho / static_cast<VT>(c->rho_1));
+            c->rho = (double)rho;
+        }
+    }
+};
+
+// rho = dot(r, z) after z = solve(P, r)  (cg.hpp:96)
+template <typename VT>
+struct EpiCgRho {
+    static constexpr int NP = 1;
+    const VT *r, *z;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, VT* part) const { part[0] += r[i] * z[i]; }
+    __device__ void finish(const VT* tot) const {
+        c->beta = c->first ? 0.0 : (double)(tot[0] / static_cast<VT>(c->rho_1));
+        c->rho = (double)tot[0];
+    }
+};
+
+// p = z  or  p = z + (rho/rho_1)*p  (cg.hpp:98-101)
+template <typename VT, typename DT, int PCM>
+struct EpiCgP {
+    static constexpr int NP = 0;
+    const VT* r;
+    const DT* d;
+    const VT* z;
+    VT* p;
+    Ctl* c;
+    VT beta;
+    int first;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        beta = static_cast<VT>(c->beta);
+        first = c->first;
+    }
+    __device__ void operator()(int64_t i, VT*) const {
+        VT zi;
+        if (PCM == PCM_NONE) zi = r[i];
+        else if (PCM == PCM_DIAG) zi = static_cast<VT>(d[i]) * r[i];
+        else zi = z[i];
+        p[i] = first ? zi : zi + beta * p[i];
+    }
+    __device__ void finish(const VT*) const {}
+};
+
+// (lazy(q)= A*p) || (lazy(alpha_1)= lazy_dot(p,q)); alpha = rho/alpha_1 (cg.hpp:103-107)
+template <typename VT>
+struct EpiCgQ {
+    static constexpr int NP = 1;
+    const VT* p;
+    VT* q;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, VT s, VT* part) const {
+        q[row] = s;
+        part[0] += p[row] * s;
+    }
+    __device__ void finish(const VT* tot) const {
+        c->alpha_1 = (double)tot[0];
+        c->alpha = (double)(static_cast<VT>(c->rho) / tot[0]);
+        c->rho_1 = c->rho;
+    }
+};
+
+// x += alpha*p; (lazy(r) -= alpha*q) || (lazy(rr)= lazy_unary_dot(r)); then
+// the loop head: finished(sqrt(|rr|)), ++iter and, for the fused PCs, the next
+// rho and beta  (cg.hpp:106-108, 94-101)
+template <typename VT, typename DT, int PCM>
+struct EpiCgXR {
+    static constexpr int NP = PCM == PCM_DIAG ? 2 : 1;
+    VT* x;
+    VT* r;
+    const VT *p, *q;
+    const DT* d;
+    Ctl* c;
+    double* hist;
+    VT alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
+    __device__ void operator()(int64_t i, VT* part) const {
+        x[i] = x[i] + alpha * p[i];
+        const VT v = r[i] - alpha * q[i];
+        r[i] = v;
+        part[0] += v * v;
+        if (PCM == PCM_DIAG) {
+            const VT z = static_cast<VT>(d[i]) * v;
+            part[1] += v * z;
+        }
+    }
+    __device__ void finish(const VT* tot) const {
+        c->rr = (double)tot[0];
+        iter_finished(c, (double)std::sqrt(std::fabs(tot[0])), hist);
+        if (!c->done && PCM != PCM_GENERAL) {
+            const VT rho = PCM == PCM_DIAG ? tot[1] : tot[0];
+            c->beta = (double)(rho / static_cast<VT>(c->rho_1));
+            c->rho = (double)rho;
+        }
+    }
+};
+
+// ---------------------------------------------------------------- BiCGSTAB epilogues
+// r = b - A*x; rtilde = r; finished(two_norm(r))  (bicgstab.hpp:40-43)
+template <typename VT>
+struct EpiBiInit {
+    static constexpr int NP = 1;
+    const VT* b;
+    VT *r, *rt;
+    Ctl* c;
+    double* hist;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, VT s, VT* part) const {
+        const VT v = b[row] - s;
+        r[row] = v;
+        rt[row] = v;
+        part[0] += v * v;
+    }
+    __device__ void finish(const VT* tot) const {
+        iter_finished(c, (double)std::sqrt(tot[0]), hist);
+    }
+};
+
+// rho_1 = dot(rtilde, r); breakdown checks; beta  (bicgstab.hpp:45-53)
+template <typename VT>
+struct EpiBiRho {
+    static constexpr int NP = 1;
+    const VT *rt, *r;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, VT* part) const { part[0] += rt[i] * r[i]; }
+    __device__ void finish(const VT* tot) const {
+        const VT rho_1 = tot[0];
+        c->rho_1 = (double)rho_1;
+        if (rho_1 == VT(0)) { c->status = MTL4G_ERR_BREAKDOWN; c->done = 1; return; }
+        if (!c->first) {
+            if (c->omega == 0.0) { c->status = MTL4G_ERR_BREAKDOWN; c->done = 1; return; }
+            c->beta = (double)((rho_1 / static_cast<VT>(c->rho_2)) *
+                               (static_cast<VT>(c->alpha) / static_cast<VT>(c->omega)));
+        }
+    }
+};
+
+// p = r  or  p = r + beta*(p - omega*v)  (bicgstab.hpp:48-53)
+template <typename VT>
+struct EpiBiP {
+    static constexpr int NP = 0;
+    const VT *r, *v;
+    VT* p;
+    Ctl* c;
+    VT beta, omega;
+    int first;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        beta = static_cast<VT>(c->beta);
+        omega = static_cast<VT>(c->omega);
+        first = c->first;
+    }
+    __device__ void operator()(int64_t i, VT*) const {
+        p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+    }
+    __device__ void finish(const VT*) const {}
+};
+
+// v = A*phat; gamma = dot(rtilde, v); alpha = rho_1/gamma  (bicgstab.hpp:56-60)
+template <typename VT>
+struct EpiBiV {
+    static constexpr int NP = 1;
+    const VT* rt;
+    VT* v;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, VT s, VT* part) const {
+        v[row] = s;
+        part[0] += rt[row] * s;
+    }
+    __device__ void finish(const VT* tot) const {
+        c->gamma = (double)tot[0];
+        if (tot[0] == VT(0)) { c->status = MTL4G_ERR_BREAKDOWN; c->done = 1; return; }
+        c->alpha = (double)(static_cast<VT>(c->rho_1) / tot[0]);
+    }
+};
+
+// s = r - alpha*v; half-step exit if finished(s)  (bicgstab.hpp:61-67)
+template <typename VT>
+struct EpiBiS {
+    static constexpr int NP = 1;
+    const VT *r, *v;
+    VT* s;
+    Ctl* c;
+    double* hist;
+    VT alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
+    __device__ void operator()(int64_t i, VT* part) const {
+        const VT t = r[i] - alpha * v[i];
+        s[i] = t;
+        part[0] += t * t;
+    }
+    __device__ void finish(const VT* tot) const {
+        const int i_before = c->i;
+        iter_finished(c, (double)std::sqrt(tot[0]), hist);
+        if (c->done) c->half = 1;
+        else c->i = i_before;   // finished(s) does not count an iteration
+    }
+};
+
+// x += alpha*phat after a half-step exit; runs once (half 1 -> 2)
+template <typename VT>
+struct EpiBiHalf {
+    static constexpr int NP = 1;
+    const VT* phat;
+    VT* x;
+    Ctl* c;
+    VT alpha;
+    __device__ bool skip() const { return !(c->done && c->half == 1); }
+    __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
+    __device__ void operator()(int64_t i, VT*) const { x[i] = x[i] + alpha * phat[i]; }
+    __device__ void finish(const VT*) const { c->half = 2; }
+};
+
+// t = A*shat; omega = dot(t,s)/dot(t,t)  (bicgstab.hpp:68-70)
+template <typename VT>
+struct EpiBiT {
+    static constexpr int NP = 2;
+    const VT* s;
+    VT* t;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, VT v, VT* part) const {
+        t[row] = v;
+        part[0] += v * s[row];
+        part[1] += v * v;
+    }
+    __device__ void finish(const VT* tot) const { c->omega = (double)(tot[0] / tot[1]); }
+};
+
+// x += omega*shat + alpha*phat; r = s - omega*t; rho_2 = rho_1; loop head
+// finished(r)  (bicgstab.hpp:72-75, 43)
+template <typename VT>
+struct EpiBiXR {
+    static constexpr int NP = 1;
+    VT *x, *r;
+    const VT *shat, *phat, *s, *t;
+    Ctl* c;
+    double* hist;
+    VT alpha, omega;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        alpha = static_cast<VT>(c->alpha);
+        omega = static_cast<VT>(c->omega);
+    }
+    __device__ void operator()(int64_t i, VT* part) const {
+        x[i] = x[i] + (omega * shat[i] + alpha * phat[i]);
+        const VT v = s[i] - omega * t[i];
+        r[i] = v;
+        part[0] += v * v;
+    }
+    __device__ void finish(const VT* tot) const {
+        c->rho_2 = c->rho_1;
+        iter_finished(c, (double)std::sqrt(tot[0]), hist);
+    }
+};
+
+__global__ void k_ctl_init(Ctl* c, Ctl init) { *c = init; }
+
+// ---------------------------------------------------------------- host driver
+template <typename VT>
+static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_pc_s* P,
+                            int32_t hist_cap) {
+    if (W.n != n || W.vecs.n < (size_t)n * nvec) {
+        M4_TRY(W.vecs.alloc((size_t)std::max(n, 1) * nvec));
+        M4_HIP(hipMemset(W.vecs.p, 0, W.vecs.n * sizeof(VT)));
+        W.n = n;
+    }
+    if (!W.partials.p) M4_TRY(W.partials.alloc((size_t)2 * kMaxPartials));
+    if (!W.ctl.p) {
+        M4_TRY(W.ctl.alloc(1));
+        M4_HIP(hipMemset(W.ctl.p, 0, sizeof(Ctl)));
+    }
+    if (!W.h_ctl) M4_HIP(hipHostMalloc((void**)&W.h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    if (hist_cap > 0 && W.hist.n < (size_t)hist_cap) M4_TRY(W.hist.alloc(hist_cap));
+    if (!W.hist.p) M4_TRY(W.hist.alloc(1));
+    size_t words = P && (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) ? pc_counter_words(P) : 1;
+    if (W.pc_counters.n < words) {
+        M4_TRY(W.pc_counters.alloc(words));
+        M4_HIP(hipMemset(W.pc_counters.p, 0, words * sizeof(unsigned)));
+    }
+    return MTL4G_OK;
+}
+
+// One iteration of CG.
+template <typename VT, typename MT, typename DT, int PCM>
+static int cg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P, const DT* d,
+                        Workspace<VT>& W, VT* x, hipStream_t s) {
+    const int64_t n = A->n;
+    VT* p = W.vecs.p;
+    VT* q = p + n;
+    VT* r = q + n;
+    VT* z = r + n;
+    VT* y0 = z + n;
+    Ctl* c = W.ctl.p;
+    VT* part = reinterpret_cast<VT*>(W.partials.p);
+    unsigned* tk = &c->ticket[0];
+    if (PCM == PCM_GENERAL) {
+        M4_TRY(pc_apply<VT>(P, r, z, y0, &c->done, W.pc_counters.p, s));
+        M4_TRY(launch_vec<VT>(n, EpiCgRho<VT>{r, z, c}, part, tk, s));
+    }
+    M4_TRY(launch_vec<VT>(n, EpiCgP<VT, DT, PCM>{r, d, z, p, c, VT(0), 0}, part, tk, s));
+    M4_TRY(launch_csr<VT>(A, val, p, EpiCgQ<VT>{p, q, c}, part, tk, s));
+    M4_TRY(launch_vec<VT>(n, EpiCgXR<VT, DT, PCM>{x, r, p, q, d, c, W.hist.p, VT(0)}, part, tk, s));
+    return MTL4G_OK;
+}
+
+template <typename VT, typename MT, typename DT, int PCM>
+static int cg_init(const mtl4g_matrix_s* A, const MT* val, const DT* d, Workspace<VT>& W,
+                   const VT* x, const VT* b, hipStream_t s) {
+    const int64_t n = A->n;
+    VT* r = W.vecs.p + 2 * n;
+    Ctl* c = W.ctl.p;
+    return launch_csr<VT>(A, val, x, EpiCgInit<VT, DT, PCM>{b, r, d, c, W.hist.p},
+                          reinterpret_cast<VT*>(W.partials.p), &c->ticket[0], s);
+}
+
+template <typename VT, typename MT>
+static int bicg_init(const mtl4g_matrix_s* A, const MT* val, Workspace<VT>& W, const VT* x,
+                     const VT* b, hipStream_t s) {
+    const int64_t n = A->n;
+    VT* r = W.vecs.p + 6 * n;
+    VT* rt = W.vecs.p + 7 * n;
+    Ctl* c = W.ctl.p;
+    return launch_csr<VT>(A, val, x, EpiBiInit<VT>{b, r, rt, c, W.hist.p},
+                          reinterpret_cast<VT*>(W.partials.p), &c->ticket[0], s);
+}
+
+template <typename VT, typename MT>
+static int bicg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P,
+                          Workspace<VT>& W, VT* x, hipStream_t s) {
+    const int64_t n = A->n;
+    VT* base = W.vecs.p;
+    VT *p = base, *phat = base + n, *sv = base + 2 * n, *shat = base + 3 * n, *t = base + 4 * n,
+       *v = base + 5 * n, *r = base + 6 * n, *rt = base + 7 * n, *y0 = base + 8 * n;
+    const bool ident = !P || P->kind == MTL4G_PC_IDENTITY;
+    if (ident) {   // solve(identity, p) is a copy: use p and s directly
+        phat = p;
+        shat = sv;
+    }
+    Ctl* c = W.ctl.p;
+    VT* part = reinterpret_cast<VT*>(W.partials.p);
+    unsigned* tk = &c->ticket[0];
+    M4_TRY(launch_vec<VT>(n, EpiBiRho<VT>{rt, r, c}, part, tk, s));
+    M4_TRY(launch_vec<VT>(n, EpiBiP<VT>{r, v, p, c, VT(0), VT(0), 0}, part, tk, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s));
+    M4_TRY(launch_csr<VT>(A, val, phat, EpiBiV<VT>{rt, v, c}, part, tk, s));
+    M4_TRY(launch_vec<VT>(n, EpiBiS<VT>{r, v, sv, c, W.hist.p, VT(0)}, part, tk, s));
+    M4_TRY(launch_vec<VT>(n, EpiBiHalf<VT>{phat, x, c, VT(0)}, part, tk, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, sv, shat, y0, &c->done, W.pc_counters.p, s));
+    M4_TRY(launch_csr<VT>(A, val, shat, EpiBiT<VT>{sv, t, c}, part, tk, s));
+    M4_TRY(launch_vec<VT>(n, EpiBiXR<VT>{x, r, shat, phat, sv, t, c, W.hist.p, VT(0), VT(0)}, part, tk, s));
+    return MTL4G_OK;
+}
+
+template <typename VT, typename MT>
+static int run_typed(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P, int method,
+                     Workspace<VT>& W, VT* x, const VT* b, hipStream_t s, int32_t budget) {
+    const int pcm = (!P || P->kind == MTL4G_PC_IDENTITY) ? PCM_NONE
+                    : P->kind == MTL4G_PC_DIAGONAL          ? PCM_DIAG
+                                                             : PCM_GENERAL;
+    // inverse diagonal in the vector type of the iteration
+    const VT* d = nullptr;
+    if (pcm == PCM_DIAG) {
+        if (sizeof(VT) == 8 && P->fmt == MTL4G_F64) d = (const VT*)P->inv_diag64.p;
+        else if (sizeof(VT) == 4 && P->fmt == MTL4G_F32) d = (const VT*)P->inv_diag32.p;
+    }
+    auto iteration = [&]() -> int {
+        if (method == MTL4G_BICGSTAB) return bicg_iteration<VT, MT>(A, val, P, W, x, s);
+        if (pcm == PCM_NONE) return cg_iteration<VT, MT, VT, PCM_NONE>(A, val, P, d, W, x, s);
+        if (pcm == PCM_DIAG) {
+            if (d) return cg_iteration<VT, MT, VT, PCM_DIAG>(A, val, P, d, W, x, s);
+            if (P->fmt == MTL4G_F32)
+                return cg_iteration<VT, MT, float, PCM_DIAG>(A, val, P, P->inv_diag32.p, W, x, s);
+            return cg_iteration<VT, MT, double, PCM_DIAG>(A, val, P, P->inv_diag64.p, W, x, s);
+        }
+        return cg_iteration<VT, MT, VT, PCM_GENERAL>(A, val, P, d, W, x, s);
+    };
+    if (method == MTL4G_BICGSTAB) {
+        M4_TRY((bicg_init<VT, MT>(A, val, W, x, b, s)));
+    } else if (pcm == PCM_NONE) {
+        M4_TRY((cg_init<VT, MT, VT, PCM_NONE>(A, val, d, W, x, b, s)));
+    } else if (pcm == PCM_DIAG) {
+        if (d) M4_TRY((cg_init<VT, MT, VT, PCM_DIAG>(A, val, d, W, x, b, s)));
+        else if (P->fmt == MTL4G_F32) M4_TRY((cg_init<VT, MT, float, PCM_DIAG>(A, val, P->inv_diag32.p, W, x, b, s)));
+        else M4_TRY((cg_init<VT, MT, double, PCM_DIAG>(A, val, P->inv_diag64.p, W, x, b, s)));
+    } else {
+        M4_TRY((cg_init<VT, MT, VT, PCM_GENERAL>(A, val, d, W, x, b, s)));
+    }
+    // Enqueue chunks of iterations and poll the stop flag between chunks.
+    int32_t launched = 0, chunk = 2;
+    for (;;) {
+        M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        M4_HIP(hipStreamSynchronize(s));
+        if (W.h_ctl->done) break;
+        if (launched > budget) {
+            set_error("solver did not stop after %d launched iterations", launched);
+            return MTL4G_ERR_INVALID;
+        }
+        for (int k = 0; k < chunk; k++) M4_TRY(iteration());
+        launched += chunk;
+        chunk = std::min(chunk * 2, 32);
+    }
+    return MTL4G_OK;
+}
+
+template <typename VT>
+int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt,
+               Workspace<VT>& W, VT* x, const VT* b, mtl4g_iteration* it, double* h_hist,
+               int32_t hist_cap, hipStream_t s) {
+    if (!A || !it || (A->n && (!x || !b))) return MTL4G_ERR_INVALID;
+    if (A->n != A->m) {
+        set_error("Krylov solve needs a square matrix");
+        return MTL4G_ERR_NOT_SQUARE;
+    }
+    if (P && P->n != A->n) {
+        set_error("preconditioner size %d != matrix size %d", P->n, A->n);
+        return MTL4G_ERR_SIZE;
+    }
+    if (method != MTL4G_CG && method != MTL4G_BICGSTAB) return MTL4G_ERR_INVALID;
+    if (A->n == 0) {   // r is empty: finished(0) holds at once (basic_iteration.hpp:85-93)
+        it->resid = 0.0;
+        it->error = (it->norm_r0 == 0 || 0.0 <= it->rtol * it->norm_r0) ? 0 : 1;
+        if (h_hist && hist_cap > 0) h_hist[0] = 0.0;
+        return it->error;
+    }
+    const int nvec = method == MTL4G_CG ? 5 : 9;
+    M4_TRY(ensure_workspace(W, A->n, nvec, P, hist_cap));
+    Ctl init;
+    std::memset(&init, 0, sizeof init);
+    init.norm_r0 = it->norm_r0;
+    init.rtol = it->rtol;
+    init.atol = it->atol;
+    init.max_iter = it->max_iter;
+    init.i = it->i;
+    init.hist_cap = h_hist ? hist_cap : 0;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
+    M4_HIP(hipGetLastError());
+    const int32_t budget = std::max(0, it->max_iter - it->i) + 8;
+    int st;
+    if (sizeof(VT) == 8) {
+        st = run_typed<VT, double>(A, A->val.p, P, method, W, x, b, s, budget);
+    } else {
+        M4_TRY(ensure_lowp_values(const_cast<mtl4g_matrix_s*>(A), fmt, s));
+        if (fmt == MTL4G_BF16)
+            st = run_typed<VT, uint16_t>(A, A->valbf.p, P, method, W, x, b, s, budget);
+        else
+            st = run_typed<VT, float>(A, A->val32.p, P, method, W, x, b, s, budget);
+    }
+    M4_TRY(st);
+    if (P) M4_TRY(pc_check_error(P, s));
+    M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    M4_HIP(hipStreamSynchronize(s));
+    const Ctl& c = *W.h_ctl;
+    it->i = c.i;
+    it->error = c.error;
+    it->resid = c.resid;
+    if (h_hist && hist_cap > 0) {
+        int32_t nh = std::min(c.hist_n, hist_cap);
+        if (nh > 0) M4_HIP(hipMemcpy(h_hist, W.hist.p, (size_t)nh * sizeof(double), hipMemcpyDeviceToHost));
+        for (int32_t k = nh; k < hist_cap; k++) h_hist[k] = NAN;
+    }
+    if (c.status != 0) {
+        set_error(c.status == MTL4G_ERR_BREAKDOWN ? "unexpected orthogonality (breakdown)"
+                                                   : "non-finite residual");
+        return c.status;
+    }
+    return c.error;
+}
+
+template int krylov_run<double>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<double>&,
+                                double*, const double*, mtl4g_iteration*, double*, int32_t, hipStream_t);
+template int krylov_run<float>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<float>&,
+                               float*, const float*, mtl4g_iteration*, double*, int32_t, hipStream_t);
+
+}  // namespace mtl4g
+
+using namespace mtl4g;
+
+extern "C" {
+
+int mtl4g_solver_create(mtl4g_solver* out, mtl4g_matrix A, mtl4g_pc P, int method) {
+    if (!out || !A || (method != MTL4G_CG && method != MTL4G_BICGSTAB)) return MTL4G_ERR_INVALID;
+    mtl4g_solver_s* S_ = new (std::nothrow) mtl4g_solver_s;
+    if (!S_) return MTL4G_ERR_OOM;
+    S_->A = A;
+    S_->P = P;
+    S_->method = method;
+    *out = S_;
+    return MTL4G_OK;
+}
+
+int mtl4g_solver_destroy(mtl4g_solver S_) {
+    delete S_;
+    return MTL4G_OK;
+}
+
+int mtl4g_solver_solve(mtl4g_solver S_, double* x, const double* b, mtl4g_iteration* it,
+                       double* h_hist, int32_t hist_cap, void* stream) {
+    if (!S_) return MTL4G_ERR_INVALID;
+    return krylov_run<double>(S_->A, S_->P, S_->method, MTL4G_F64, S_->W, x, b, it, h_hist,
+                              hist_cap, S(stream));
+}
+
+int mtl4g_cg(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,
+             double* h_hist, int32_t hist_cap, void* stream) {
+    Workspace<double> W;
+    return krylov_run<double>(A, P, MTL4G_CG, MTL4G_F64, W, x, b, it, h_hist, hist_cap, S(stream));
+}
+
+int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,
+                   double* h_hist, int32_t hist_cap, void* stream) {
+    Workspace<double> W;
+    return krylov_run<double>(A, P, MTL4G_BICGSTAB, MTL4G_F64, W, x, b, it, h_hist, hist_cap,
+                              S(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// krylov.hpp -- device-resident itl::cg / itl::bicgstab.
+#pragma once
+
+#include "internal.hpp"
+#include "precond.hpp"
+
+namespace mtl4g {
+
+// Scalars of the recurrences and of basic_iteration, kept on the device.
+struct Ctl {
+    double rho, rho_1, rr, alpha, alpha_1, beta, omega, gamma, rho_2;
+    double norm_r0, rtol, atol, resid;
+    double inner_scale;
+    int32_t i, max_iter, error, done, first, half, status;
+    int32_t hist_n, hist_cap;
+    int32_t pad;
+    unsigned ticket[4];
+};
+
+// Element type of the vectors (VT) and of the stored matrix values (MT).
+template <typename VT>
+struct Workspace {
+    int32_t n = 0;
+    DevBuf<VT> vecs;          // method-specific vectors
+    DevBuf<double> partials;  // 2 x kMaxPartials
+    DevBuf<Ctl> ctl;
+    DevBuf<double> hist;
+    DevBuf<unsigned> pc_counters;
+    Ctl* h_ctl = nullptr;     // pinned mirror for polling
+    ~Workspace() {
+        if (h_ctl) (void)hipHostFree(h_ctl);
+    }
+};
+
+// Runs a whole Krylov solve on the device.  x and b are device vectors of
+// VT; value_format selects the stored matrix values for VT = float.
+template <typename VT>
+int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int value_format,
+               Workspace<VT>& W, VT* x, const VT* b, mtl4g_iteration* it, double* h_hist,
+               int32_t hist_cap, hipStream_t s);
+
+}  // namespace mtl4g
+
+struct mtl4g_solver_s {
+    mtl4g_matrix_s* A = nullptr;
+    mtl4g_pc_s* P = nullptr;
+    int method = MTL4G_CG;
+    mtl4g::Workspace<double> W;
+};

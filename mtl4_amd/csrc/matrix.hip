@@ -1,0 +1,447 @@
+// matrix.hip -- compressed2D on the device, CSR x dense_vector kernels and
+// the dense_vector reductions/updates of the C ABI.
+//
+// SpMV (mat_vec_mult.hpp:613-683) is HBM bound: per non-zero it streams
+// 4 B of column index and V B of value and gathers x[col] (mostly from
+// L2/MALL).  Each workgroup owns a row block sized to an LDS tile of
+// products: every lane streams val/col coalesced (4 independent loads in
+// flight per lane), forms val*x[col] into LDS, then lane t sums its row in
+// column order.  Row lengths are binned by construction (a block holds as
+// many rows as fit the tile), and blocks are mapped XCD-contiguously.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "kernels.hpp"
+
+namespace mtl4g {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+int hip_status(hipError_t e, const char* what, const char* file, int line) {
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? MTL4G_ERR_OOM : MTL4G_ERR_HIP;
+}
+
+int blocks_for(int64_t n) {
+    int64_t g = (n + 256 * 8 - 1) / (256 * 8);
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return (int)g;
+}
+
+// ------------------------------------------------------------------ partition
+int build_row_blocks(mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block) {
+    if (rows_per_block <= 0) rows_per_block = 256;
+    if (nnz_per_block <= 0) nnz_per_block = 2048;
+    if (rows_per_block > 256 || nnz_per_block > kMaxTile) {
+        set_error("row block limits: rows_per_block <= 256, nnz_per_block <= %d", kMaxTile);
+        return MTL4G_ERR_INVALID;
+    }
+    std::vector<int32_t> blk;
+    blk.reserve(A->n / 64 + 2);
+    const std::vector<int32_t>& rp = A->h_rowptr;
+    int32_t r = 0;
+    blk.push_back(0);
+    while (r < A->n) {
+        int32_t r0 = r;
+        int64_t nz = 0;
+        while (r < A->n && r - r0 < rows_per_block && nz + (rp[r + 1] - rp[r]) <= nnz_per_block) {
+            nz += rp[r + 1] - rp[r];
+            r++;
+        }
+        if (r == r0) r++;   // a row longer than the tile gets its own block
+        blk.push_back(r);
+    }
+    A->nb = (int)blk.size() - 1;
+    A->rows_per_block = rows_per_block;
+    A->nnz_per_block = nnz_per_block;
+    M4_TRY(A->blk.alloc(blk.size()));
+    M4_HIP(hipMemcpy(A->blk.p, blk.data(), blk.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    return MTL4G_OK;
+}
+
+static uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);   // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                           // nearest even
+    return (uint16_t)(u >> 16);
+}
+
+int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
+    (void)s;
+    if (fmt == MTL4G_F32 && !A->val32.p) {
+        std::vector<float> v(A->h_val.begin(), A->h_val.end());
+        M4_TRY(A->val32.alloc(v.size()));
+        M4_HIP(hipMemcpy(A->val32.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (fmt == MTL4G_BF16 && !A->valbf.p) {
+        std::vector<uint16_t> v(A->h_val.size());
+        for (size_t i = 0; i < v.size(); i++) v[i] = f32_to_bf16((float)A->h_val[i]);
+        M4_TRY(A->valbf.alloc(v.size()));
+        M4_HIP(hipMemcpy(A->valbf.p, v.data(), v.size() * 2, hipMemcpyHostToDevice));
+    }
+    return MTL4G_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+template <typename AT, int MODE>
+struct EpiAssign {
+    static constexpr int NP = 0;
+    AT* y;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, AT s, AT*) const {
+        if (MODE == MTL4G_ASSIGN) y[row] = s;
+        else if (MODE == MTL4G_PLUS) y[row] = y[row] + s;
+        else y[row] = y[row] - s;
+    }
+    __device__ void finish(const AT*) const {}
+};
+
+// (lazy(q)= A*p) || (lazy(alpha)= lazy_dot(p,q)), result to *out
+struct EpiSpmvDot {
+    static constexpr int NP = 1;
+    const double* p;
+    double* q;
+    double* out;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, double s, double* part) const {
+        q[row] = s;
+        part[0] += p[row] * s;
+    }
+    __device__ void finish(const double* tot) const { *out = tot[0]; }
+};
+
+int launch_spmv(const mtl4g_matrix_s* A, const double* x, double* y, int assign, hipStream_t s) {
+    switch (assign) {
+    case MTL4G_ASSIGN: return launch_csr<double>(A, A->val.p, x, EpiAssign<double, 0>{y}, (double*)nullptr, nullptr, s);
+    case MTL4G_PLUS: return launch_csr<double>(A, A->val.p, x, EpiAssign<double, 1>{y}, (double*)nullptr, nullptr, s);
+    case MTL4G_MINUS: return launch_csr<double>(A, A->val.p, x, EpiAssign<double, 2>{y}, (double*)nullptr, nullptr, s);
+    }
+    set_error("assign mode %d", assign);
+    return MTL4G_ERR_INVALID;
+}
+
+template <int MODE>
+static int spmv32_mode(const mtl4g_matrix_s* A, int fmt, const float* x, float* y, hipStream_t s) {
+    if (fmt == MTL4G_F32)
+        return launch_csr<float>(A, A->val32.p, x, EpiAssign<float, MODE>{y}, (float*)nullptr, nullptr, s);
+    return launch_csr<float>(A, A->valbf.p, x, EpiAssign<float, MODE>{y}, (float*)nullptr, nullptr, s);
+}
+
+int launch_spmv_f32(const mtl4g_matrix_s* A, int fmt, const float* x, float* y, int assign,
+                    hipStream_t s) {
+    switch (assign) {
+    case MTL4G_ASSIGN: return spmv32_mode<0>(A, fmt, x, y, s);
+    case MTL4G_PLUS: return spmv32_mode<1>(A, fmt, x, y, s);
+    case MTL4G_MINUS: return spmv32_mode<2>(A, fmt, x, y, s);
+    }
+    set_error("assign mode %d", assign);
+    return MTL4G_ERR_INVALID;
+}
+
+// ---- vector kernels: block b owns a contiguous chunk; fixed partial layout
+struct EpiDot {
+    static constexpr int NP = 1;
+    const double *a, *b;
+    double* out;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double* part) const { part[0] += a[i] * b[i]; }
+    __device__ void finish(const double* tot) const { *out = tot[0]; }
+};
+
+struct EpiUnaryDot {
+    static constexpr int NP = 1;
+    const double* a;
+    double* out;
+    int sqrt_result;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double* part) const { part[0] += a[i] * a[i]; }
+    __device__ void finish(const double* tot) const { *out = sqrt_result ? sqrt(tot[0]) : tot[0]; }
+};
+
+struct EpiAxpy {
+    static constexpr int NP = 0;
+    double alpha;
+    const double* v;
+    double* w;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double*) const { w[i] = w[i] + alpha * v[i]; }
+    __device__ void finish(const double*) const {}
+};
+
+struct EpiAxpyUnary {
+    static constexpr int NP = 1;
+    double alpha;
+    const double* q;
+    double* r;
+    double* out;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double* part) const {
+        double v = r[i] - alpha * q[i];
+        r[i] = v;
+        part[0] += v * v;
+    }
+    __device__ void finish(const double* tot) const { *out = tot[0]; }
+};
+
+// Stream-ordered scratch for one reduction: partials, ticket and result.
+struct Scratch {
+    char* base = nullptr;
+    hipStream_t s = nullptr;
+    double* partials() { return reinterpret_cast<double*>(base + 256); }
+    unsigned* ticket() { return reinterpret_cast<unsigned*>(base); }
+    double* result() { return reinterpret_cast<double*>(base + 64); }
+    int get(hipStream_t st) {
+        s = st;
+        M4_HIP(hipMallocAsync((void**)&base, 256 + sizeof(double) * 4 * kMaxPartials, st));
+        M4_HIP(hipMemsetAsync(base, 0, 256, st));
+        return MTL4G_OK;
+    }
+    ~Scratch() {
+        if (base) (void)hipFreeAsync(base, s);
+    }
+};
+
+template <typename Epi>
+static int run_vec(int64_t n, const Epi& epi, Scratch* sc, hipStream_t s) {
+    return launch_vec<double>(n, epi, sc ? sc->partials() : nullptr, sc ? sc->ticket() : nullptr, s);
+}
+
+static int fetch(Scratch& sc, double* h, hipStream_t s) {
+    M4_HIP(hipMemcpyAsync(h, sc.result(), sizeof(double), hipMemcpyDeviceToHost, s));
+    M4_HIP(hipStreamSynchronize(s));
+    return MTL4G_OK;
+}
+
+}  // namespace mtl4g
+
+using namespace mtl4g;
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char* mtl4g_status_string(int st) {
+    switch (st) {
+    case MTL4G_OK: return "ok";
+    case MTL4G_MAX_ITER: return "too many iterations";
+    case MTL4G_ERR_INVALID: return "invalid argument";
+    case MTL4G_ERR_NOT_SQUARE: return "matrix not square";
+    case MTL4G_ERR_SIZE: return "incompatible size";
+    case MTL4G_ERR_MISSING_DIAG: return "missing diagonal";
+    case MTL4G_ERR_BREAKDOWN: return "unexpected orthogonality";
+    case MTL4G_ERR_NONFINITE: return "non-finite value";
+    case MTL4G_ERR_OOM: return "out of device memory";
+    case MTL4G_ERR_HIP: return "HIP runtime error";
+    case MTL4G_ERR_UNSUPPORTED: return "unsupported";
+    case MTL4G_ERR_TIMEOUT: return "device wait timed out";
+    }
+    return "unknown status";
+}
+
+const char* mtl4g_last_error(void) { return g_last_error.c_str(); }
+
+int mtl4g_version(void) { return 100; }
+
+int mtl4g_set_device(int device) {
+    M4_HIP(hipSetDevice(device));
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_create(mtl4g_matrix* out, int32_t nrows, int32_t ncols, int64_t nnz,
+                     const int32_t* rowptr, const int32_t* col, const double* val, int where) {
+    if (!out || nrows < 0 || ncols < 0 || nnz < 0 || (nrows > 0 && !rowptr) ||
+        (nnz > 0 && (!col || !val))) {
+        set_error("mtl4g_csr_create: bad arguments");
+        return MTL4G_ERR_INVALID;
+    }
+    if (nnz >= (int64_t)INT32_MAX) {
+        set_error("mtl4g_csr_create: nnz %lld exceeds int32 indexing", (long long)nnz);
+        return MTL4G_ERR_UNSUPPORTED;
+    }
+    *out = nullptr;
+    mtl4g_matrix_s* A = new (std::nothrow) mtl4g_matrix_s;
+    if (!A) return MTL4G_ERR_OOM;
+    auto fail = [&](int st) { delete A; return st; };
+    (void)hipGetDevice(&A->device);
+    A->n = nrows;
+    A->m = ncols;
+    A->nnz = nnz;
+    A->h_rowptr.resize((size_t)nrows + 1);
+    A->h_col.resize((size_t)nnz);
+    A->h_val.resize((size_t)nnz);
+    if (where == 1) {
+        hipError_t e = hipMemcpy(A->h_rowptr.data(), rowptr, ((size_t)nrows + 1) * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && nnz)
+            e = hipMemcpy(A->h_col.data(), col, (size_t)nnz * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && nnz)
+            e = hipMemcpy(A->h_val.data(), val, (size_t)nnz * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail(hip_status(e, "hipMemcpy", __FILE__, __LINE__));
+    } else {
+        std::memcpy(A->h_rowptr.data(), rowptr, ((size_t)nrows + 1) * 4);
+        if (nnz) {
+            std::memcpy(A->h_col.data(), col, (size_t)nnz * 4);
+            std::memcpy(A->h_val.data(), val, (size_t)nnz * 8);
+        }
+    }
+    // validate: monotone row pointers, sorted in-range columns
+    if (A->h_rowptr[0] != 0 || A->h_rowptr[nrows] != nnz) {
+        set_error("mtl4g_csr_create: rowptr[0] must be 0 and rowptr[n] == nnz");
+        return fail(MTL4G_ERR_INVALID);
+    }
+    std::vector<double> diag((size_t)nrows, 0.0);
+    for (int32_t r = 0; r < nrows; r++) {
+        int32_t a = A->h_rowptr[r], b = A->h_rowptr[r + 1];
+        if (b < a) {
+            set_error("mtl4g_csr_create: rowptr not monotone at row %d", r);
+            return fail(MTL4G_ERR_INVALID);
+        }
+        A->max_row = std::max(A->max_row, b - a);
+        for (int32_t j = a; j < b; j++) {
+            int32_t c = A->h_col[j];
+            if (c < 0 || c >= ncols || (j > a && c <= A->h_col[j - 1])) {
+                set_error("mtl4g_csr_create: column %d of row %d out of range or unsorted", c, r);
+                return fail(MTL4G_ERR_INVALID);
+            }
+            if (c == r) diag[r] = A->h_val[j];
+        }
+    }
+    int st;
+    if ((st = A->rowptr.alloc((size_t)nrows + 1)) || (st = A->col.alloc((size_t)nnz)) ||
+        (st = A->val.alloc((size_t)nnz)) || (st = A->diag.alloc((size_t)nrows)))
+        return fail(st);
+    hipError_t e = hipMemcpy(A->rowptr.p, A->h_rowptr.data(), ((size_t)nrows + 1) * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz) e = hipMemcpy(A->col.p, A->h_col.data(), (size_t)nnz * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz) e = hipMemcpy(A->val.p, A->h_val.data(), (size_t)nnz * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nrows) e = hipMemcpy(A->diag.p, diag.data(), (size_t)nrows * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(hip_status(e, "hipMemcpy", __FILE__, __LINE__));
+    if ((st = build_row_blocks(A, 0, 0))) return fail(st);
+    *out = A;
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_destroy(mtl4g_matrix A) {
+    delete A;
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_info(mtl4g_matrix A, int32_t* nrows, int32_t* ncols, int64_t* nnz) {
+    if (!A) return MTL4G_ERR_INVALID;
+    if (nrows) *nrows = A->n;
+    if (ncols) *ncols = A->m;
+    if (nnz) *nnz = A->nnz;
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where) {
+    if (!A || (A->nnz && !val)) return MTL4G_ERR_INVALID;
+    if (where == 1) {
+        M4_HIP(hipMemcpy(A->h_val.data(), val, (size_t)A->nnz * 8, hipMemcpyDeviceToHost));
+    } else if (A->nnz) {
+        std::memcpy(A->h_val.data(), val, (size_t)A->nnz * 8);
+    }
+    std::vector<double> diag((size_t)A->n, 0.0);
+    for (int32_t r = 0; r < A->n; r++)
+        for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++)
+            if (A->h_col[j] == r) diag[r] = A->h_val[j];
+    if (A->nnz) M4_HIP(hipMemcpy(A->val.p, A->h_val.data(), (size_t)A->nnz * 8, hipMemcpyHostToDevice));
+    if (A->n) M4_HIP(hipMemcpy(A->diag.p, diag.data(), (size_t)A->n * 8, hipMemcpyHostToDevice));
+    A->val32.free();
+    A->valbf.free();
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block) {
+    if (!A) return MTL4G_ERR_INVALID;
+    return build_row_blocks(A, rows_per_block, nnz_per_block);
+}
+
+int mtl4g_spmv(mtl4g_matrix A, const double* v, double* w, int assign, void* stream) {
+    if (!A || (A->n && !w) || (A->m && !v)) return MTL4G_ERR_INVALID;
+    return launch_spmv(A, v, w, assign, S(stream));
+}
+
+int mtl4g_spmv_f32(mtl4g_matrix A, int fmt, const float* v, float* w, int assign, void* stream) {
+    if (!A || (A->n && !w) || (A->m && !v) || (fmt != MTL4G_F32 && fmt != MTL4G_BF16))
+        return MTL4G_ERR_INVALID;
+    M4_TRY(ensure_lowp_values(A, fmt, S(stream)));
+    return launch_spmv_f32(A, fmt, v, w, assign, S(stream));
+}
+
+int mtl4g_spmv_dot(mtl4g_matrix A, const double* p, double* q, double* h_dot, void* stream) {
+    if (!A || !h_dot || A->n != A->m) return A && A->n != A->m ? MTL4G_ERR_NOT_SQUARE : MTL4G_ERR_INVALID;
+    hipStream_t s = S(stream);
+    if (A->n == 0) { *h_dot = 0; return MTL4G_OK; }
+    Scratch sc;
+    M4_TRY(sc.get(s));
+    M4_TRY(launch_csr<double>(A, A->val.p, p, EpiSpmvDot{p, q, sc.result()}, sc.partials(), sc.ticket(), s));
+    return fetch(sc, h_dot, s);
+}
+
+int mtl4g_dot(int64_t n, const double* v, const double* w, double* h, void* stream) {
+    if (!h || n < 0) return MTL4G_ERR_INVALID;
+    if (n == 0) { *h = 0; return MTL4G_OK; }
+    hipStream_t s = S(stream);
+    Scratch sc;
+    M4_TRY(sc.get(s));
+    M4_TRY(run_vec(n, EpiDot{v, w, sc.result()}, &sc, s));
+    return fetch(sc, h, s);
+}
+
+int mtl4g_unary_dot(int64_t n, const double* v, double* h, void* stream) {
+    if (!h || n < 0) return MTL4G_ERR_INVALID;
+    if (n == 0) { *h = 0; return MTL4G_OK; }
+    hipStream_t s = S(stream);
+    Scratch sc;
+    M4_TRY(sc.get(s));
+    M4_TRY(run_vec(n, EpiUnaryDot{v, sc.result(), 0}, &sc, s));
+    return fetch(sc, h, s);
+}
+
+int mtl4g_two_norm(int64_t n, const double* v, double* h, void* stream) {
+    if (!h || n < 0) return MTL4G_ERR_INVALID;
+    if (n == 0) { *h = 0; return MTL4G_OK; }
+    hipStream_t s = S(stream);
+    Scratch sc;
+    M4_TRY(sc.get(s));
+    M4_TRY(run_vec(n, EpiUnaryDot{v, sc.result(), 1}, &sc, s));
+    return fetch(sc, h, s);
+}
+
+int mtl4g_axpy(int64_t n, double alpha, const double* v, double* w, void* stream) {
+    if (n < 0) return MTL4G_ERR_INVALID;
+    return run_vec(n, EpiAxpy{alpha, v, w}, nullptr, S(stream));
+}
+
+int mtl4g_axpy_unary_dot(int64_t n, double alpha, const double* q, double* r, double* h,
+                         void* stream) {
+    if (!h || n < 0) return MTL4G_ERR_INVALID;
+    if (n == 0) { *h = 0; return MTL4G_OK; }
+    hipStream_t s = S(stream);
+    Scratch sc;
+    M4_TRY(sc.get(s));
+    M4_TRY(run_vec(n, EpiAxpyUnary{alpha, q, r, sc.result()}, &sc, s));
+    return fetch(sc, h, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,557 @@
+// precond.hip -- itl::pc::identity / diagonal / ilu_0 / ic_0 on the device.
+//
+// Factorizations run on the host in FP64 in MTL4's loop order (so the
+// stored factors are the ones MTL4 computes); the triangular sweeps run on
+// the GPU, level scheduled.  A sweep is one persistent kernel: workgroups
+// take tickets in order, each owns up to kSweepBS rows of one dependency
+// level, waits until every workgroup of the previous level has published
+// (bounded wait, abort flag), solves its rows with MTL4's per-row
+// subtraction order (FP64 results equal MTL4's bit for bit) and publishes.
+// The last workgroup of the last level resets the counters, so a sweep
+// needs no memset between launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.hpp"
+#include "precond.hpp"
+
+namespace mtl4g {
+
+constexpr int kSweepBS = 128;
+constexpr unsigned kSpinLimit = 1u << 20;   // polls before a wait gives up
+
+// ------------------------------------------------------------------ device
+template <typename MT, typename VT>
+struct SweepView {
+    const int32_t *perm, *rowptr, *col, *lev_ptr, *lev_blk, *blk_level;
+    const MT *val, *dia;
+    int32_t nlev;
+    int unit;
+};
+
+template <int BS, typename MT, typename VT>
+__global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __restrict__ v,
+                                              VT* w, unsigned* counters, int* err,
+                                              const int* skip) {
+    if (skip && *skip) return;
+    __shared__ int s_b, s_ok, s_final;
+    if (threadIdx.x == 0) s_b = (int)atomicAdd(&counters[0], 1u);
+    __syncthreads();
+    const int b = s_b;
+    const int L = S.blk_level[b];
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        if (L > 0) {
+            const unsigned need = (unsigned)(S.lev_blk[L] - S.lev_blk[L - 1]);
+            unsigned it = 0;
+            while (ld_agent(&counters[2 + L - 1]) < need) {
+                if (ld_agent(&counters[1]) != 0u) { ok = 0; break; }
+                if (++it > kSpinLimit) { st_agent(&counters[1], 1u); ok = 0; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    if (s_ok) {
+        __threadfence();
+        const int k = S.lev_ptr[L] + (b - S.lev_blk[L]) * BS + (int)threadIdx.x;
+        if (k < S.lev_ptr[L + 1]) {
+            const int r = S.perm[k];
+            VT rr = v[r];
+            for (int j = S.rowptr[k]; j < S.rowptr[k + 1]; j++)
+                rr -= static_cast<VT>(S.val[j]) * ld_agent(w + S.col[j]);
+            if (!S.unit) rr = rr * static_cast<VT>(S.dia[k]);
+            st_agent(w + r, rr);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned need = (unsigned)(S.lev_blk[L + 1] - S.lev_blk[L]);
+        const unsigned done = atomicAdd(&counters[2 + L], 1u) + 1u;
+        s_final = (L == S.nlev - 1 && done == need);
+    }
+    __syncthreads();
+    if (s_final) {   // every other workgroup has published: reset for the next launch
+        __threadfence();
+        if (threadIdx.x == 0 && ld_agent(&counters[1]) != 0u) atomicOr(err, 1);
+        for (int i = threadIdx.x; i < S.nlev + 2; i += BS) st_agent(&counters[i], 0u);
+    }
+}
+
+template <typename VT, typename MT>
+__global__ void k_diag_apply(int64_t n, const MT* __restrict__ d, const VT* __restrict__ r,
+                             VT* __restrict__ z, const int* skip) {
+    if (skip && *skip) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        z[i] = static_cast<VT>(d[i]) * r[i];
+}
+
+template <typename VT>
+__global__ void k_copy(int64_t n, const VT* __restrict__ a, VT* __restrict__ b, const int* skip) {
+    if (skip && *skip) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// ------------------------------------------------------------------ host
+static int64_t find_entry(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                          int32_t r, int32_t c) {
+    auto b = ci.begin() + rp[r], e = ci.begin() + rp[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    return (it != e && *it == c) ? (int64_t)(it - ci.begin()) : -1;
+}
+
+static float round_f32(double v) { return (float)v; }
+
+int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
+    const int32_t n = R.n;
+    // dependency levels
+    std::vector<int32_t> lev(n, 0);
+    int32_t nlev = n ? 1 : 0;
+    auto level_of = [&](int32_t r) {
+        int32_t l = 0;
+        for (int32_t j = R.rowptr[r]; j < R.rowptr[r + 1]; j++) l = std::max(l, lev[R.col[j]] + 1);
+        lev[r] = l;
+        nlev = std::max(nlev, l + 1);
+    };
+    if (R.lower)
+        for (int32_t r = 0; r < n; r++) level_of(r);
+    else
+        for (int32_t r = n; r-- > 0;) level_of(r);
+    // rows ordered by level, ascending row inside a level
+    std::vector<int32_t> lev_ptr(nlev + 1, 0), perm(n);
+    for (int32_t r = 0; r < n; r++) lev_ptr[lev[r] + 1]++;
+    for (int32_t l = 0; l < nlev; l++) lev_ptr[l + 1] += lev_ptr[l];
+    {
+        std::vector<int32_t> fill(lev_ptr.begin(), lev_ptr.end() - 1);
+        for (int32_t r = 0; r < n; r++) perm[fill[lev[r]]++] = r;
+    }
+    std::vector<int32_t> rowptr(n + 1, 0);
+    int64_t nnz = R.rowptr[n];
+    std::vector<int32_t> col(nnz);
+    std::vector<double> val(nnz), dia(n, 1.0);
+    for (int32_t k = 0; k < n; k++) {
+        int32_t r = perm[k];
+        int32_t len = R.rowptr[r + 1] - R.rowptr[r];
+        rowptr[k + 1] = rowptr[k] + len;
+        for (int32_t t = 0; t < len; t++) {
+            col[rowptr[k] + t] = R.col[R.rowptr[r] + t];
+            val[rowptr[k] + t] = R.val[R.rowptr[r] + t];
+        }
+        if (!R.unit) dia[k] = R.dia[r];
+    }
+    // workgroups: ceil(rows / kSweepBS) per level
+    std::vector<int32_t> lev_blk(nlev + 1, 0), blk_level;
+    for (int32_t l = 0; l < nlev; l++) {
+        int32_t rows = lev_ptr[l + 1] - lev_ptr[l];
+        int32_t nb = (rows + kSweepBS - 1) / kSweepBS;
+        lev_blk[l + 1] = lev_blk[l] + nb;
+        for (int32_t i = 0; i < nb; i++) blk_level.push_back(l);
+    }
+    S->n = n;
+    S->nlev = nlev;
+    S->nblocks = lev_blk[nlev];
+    S->unit = R.unit;
+    S->h_lev_ptr = lev_ptr;
+    auto up = [](auto& buf, const auto& vec) -> int {
+        using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+        M4_TRY(buf.alloc(vec.size()));
+        if (!vec.empty()) M4_HIP(hipMemcpy(buf.p, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice));
+        return MTL4G_OK;
+    };
+    M4_TRY(up(S->perm, perm));
+    M4_TRY(up(S->rowptr, rowptr));
+    M4_TRY(up(S->col, col));
+    M4_TRY(up(S->lev_ptr, lev_ptr));
+    M4_TRY(up(S->lev_blk, lev_blk));
+    M4_TRY(up(S->blk_level, blk_level));
+    if (fmt == MTL4G_F64) {
+        M4_TRY(up(S->val64, val));
+        M4_TRY(up(S->dia64, dia));
+    } else {
+        std::vector<float> v32(val.size()), d32(dia.size());
+        for (size_t i = 0; i < val.size(); i++) v32[i] = round_f32(val[i]);
+        for (size_t i = 0; i < dia.size(); i++) d32[i] = round_f32(dia[i]);
+        M4_TRY(up(S->val32, v32));
+        M4_TRY(up(S->dia32, d32));
+    }
+    M4_TRY(S->counters.alloc((size_t)nlev + 2));
+    M4_HIP(hipMemset(S->counters.p, 0, ((size_t)nlev + 2) * sizeof(unsigned)));
+    return MTL4G_OK;
+}
+
+template <typename VT>
+int launch_sweep(const Sweep& S, int fmt, const VT* v, VT* w, const int* skip, int* err,
+                 unsigned* counters, hipStream_t s) {
+    if (S.n == 0) return MTL4G_OK;
+    if (!counters) counters = S.counters.p;
+    if (fmt == MTL4G_F64) {
+        SweepView<double, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
+                                S.blk_level.p, S.val64.p, S.dia64.p, S.nlev, S.unit ? 1 : 0};
+        hipLaunchKernelGGL((k_sweep<kSweepBS, double, VT>), dim3(S.nblocks), dim3(kSweepBS), 0, s,
+                           V, v, w, counters, err, skip);
+    } else {
+        SweepView<float, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
+                               S.blk_level.p, S.val32.p, S.dia32.p, S.nlev, S.unit ? 1 : 0};
+        hipLaunchKernelGGL((k_sweep<kSweepBS, float, VT>), dim3(S.nblocks), dim3(kSweepBS), 0, s,
+                           V, v, w, counters, err, skip);
+    }
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
+template int launch_sweep<double>(const Sweep&, int, const double*, double*, const int*, int*, unsigned*, hipStream_t);
+template int launch_sweep<float>(const Sweep&, int, const float*, float*, const int*, int*, unsigned*, hipStream_t);
+
+static int grid_for(int64_t n) {
+    int64_t g = (n + 1023) / 1024;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
+size_t pc_counter_words(const mtl4g_pc_s* P) {
+    return (size_t)P->lower.nlev + 2 + (size_t)P->upper.nlev + 2;
+}
+
+template <typename VT>
+int pc_apply(const mtl4g_pc_s* P, const VT* r, VT* z, VT* y0, const int* skip,
+             unsigned* counters, hipStream_t s) {
+    const int64_t n = P->n;
+    if (n == 0) return MTL4G_OK;
+    switch (P->kind) {
+    case MTL4G_PC_IDENTITY:
+        hipLaunchKernelGGL((k_copy<VT>), dim3(grid_for(n)), dim3(256), 0, s, n, r, z, skip);
+        break;
+    case MTL4G_PC_DIAGONAL:
+        if (P->fmt == MTL4G_F64)
+            hipLaunchKernelGGL((k_diag_apply<VT, double>), dim3(grid_for(n)), dim3(256), 0, s, n,
+                               P->inv_diag64.p, r, z, skip);
+        else
+            hipLaunchKernelGGL((k_diag_apply<VT, float>), dim3(grid_for(n)), dim3(256), 0, s, n,
+                               P->inv_diag32.p, r, z, skip);
+        break;
+    default:
+        M4_TRY(launch_sweep<VT>(P->lower, P->fmt, r, y0, skip, P->err.p, counters, s));
+        M4_TRY(launch_sweep<VT>(P->upper, P->fmt, y0, z, skip, P->err.p,
+                                counters ? counters + P->lower.nlev + 2 : nullptr, s));
+        return MTL4G_OK;
+    }
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
+template int pc_apply<double>(const mtl4g_pc_s*, const double*, double*, double*, const int*, unsigned*, hipStream_t);
+template int pc_apply<float>(const mtl4g_pc_s*, const float*, float*, float*, const int*, unsigned*, hipStream_t);
+
+int pc_check_error(const mtl4g_pc_s* P, hipStream_t s) {
+    if (!P || !P->err.p) return MTL4G_OK;
+    int h = 0;
+    M4_HIP(hipMemcpyAsync(&h, P->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    M4_HIP(hipStreamSynchronize(s));
+    if (h) {
+        M4_HIP(hipMemsetAsync(P->err.p, 0, sizeof(int), s));
+        set_error("triangular sweep: a bounded level wait expired");
+        return MTL4G_ERR_TIMEOUT;
+    }
+    return MTL4G_OK;
+}
+
+// ---- factorizations (host, FP64, MTL4's loop order)
+
+// ilu_0_factorizer::factorize, itl/pc/ilu_0.hpp:45-82: IKJ elimination in
+// place on a copy of A, entries of row k looked up by column (a missing
+// entry reads 0), then invert_diagonal and the strict_lower / upper split.
+static int factor_ilu0(const mtl4g_matrix_s* A, mtl4g_pc_s* P) {
+    const int32_t n = A->n;
+    std::vector<double> lu = A->h_val;
+    const auto& rp = A->h_rowptr;
+    const auto& ci = A->h_col;
+    std::vector<double> inv_dia(n, 0.0);
+    std::vector<int64_t> dpos(n);
+    for (int32_t i = 0; i < n; i++) {
+        dpos[i] = find_entry(rp, ci, i, i);
+        if (dpos[i] < 0) {
+            set_error("ilu_0: row %d has no diagonal entry", i);
+            return MTL4G_ERR_MISSING_DIAG;
+        }
+        for (int32_t kc = rp[i]; kc < rp[i + 1]; kc++) {
+            const int32_t k = ci[kc];
+            if (k == i) break;
+            const double aik = lu[kc] * inv_dia[k];
+            lu[kc] = aik;
+            for (int32_t jc = kc + 1; jc < rp[i + 1]; jc++) {
+                const int64_t kj = find_entry(rp, ci, k, ci[jc]);
+                lu[jc] = lu[jc] - aik * (kj >= 0 ? lu[kj] : 0.0);
+            }
+        }
+        inv_dia[i] = 1.0 / lu[dpos[i]];
+    }
+    for (int32_t i = 0; i < n; i++) lu[dpos[i]] = 1.0 / lu[dpos[i]];
+    const bool f32 = P->fmt != MTL4G_F64;
+    auto store = [&](double v) { return f32 ? (double)(float)v : v; };
+    P->L_rowptr.assign(n + 1, 0);
+    P->U_rowptr.assign(n + 1, 0);
+    for (int32_t r = 0; r < n; r++) {
+        for (int32_t j = rp[r]; j < rp[r + 1]; j++) {
+            if (ci[j] < r) { P->L_col.push_back(ci[j]); P->L_val.push_back(store(lu[j])); }
+            else { P->U_col.push_back(ci[j]); P->U_val.push_back(store(lu[j])); }
+        }
+        P->L_rowptr[r + 1] = (int32_t)P->L_col.size();
+        P->U_rowptr[r + 1] = (int32_t)P->U_col.size();
+    }
+    return MTL4G_OK;
+}
+
+// ic_0::factorizer::factorize, itl/pc/ic_0.hpp:135-186, on U = upper(A):
+// row k: U[k][k] <- 1/sqrt(U[k][k]); each later entry U[k][i] <- d =
+// U[k][i]*U[k][k]; then entries of row i with column in (k, i] get
+// U(j,i) -= d*U[k][j] -- in an upper factor only j = i can match.
+// Rounded to float afterwards for ic_0<Matrix,float> (ic_0.hpp:122-130).
+static int factor_ic0(const mtl4g_matrix_s* A, mtl4g_pc_s* P) {
+    const int32_t n = A->n;
+    auto& urp = P->U_rowptr;
+    auto& uci = P->U_col;
+    auto& uva = P->U_val;
+    urp.assign(n + 1, 0);
+    for (int32_t r = 0; r < n; r++) {
+        for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++)
+            if (A->h_col[j] >= r) { uci.push_back(A->h_col[j]); uva.push_back(A->h_val[j]); }
+        urp[r + 1] = (int32_t)uci.size();
+    }
+    for (int32_t k = 0; k < n; k++) {
+        int32_t ic = urp[k];
+        const int32_t iend = urp[k + 1];
+        if (ic == iend || uci[ic] != k) {
+            set_error("ic_0: row %d has no diagonal entry", k);
+            return MTL4G_ERR_MISSING_DIAG;
+        }
+        const double inv_dia = 1.0 / std::sqrt(uva[ic]);
+        uva[ic] = inv_dia;
+        for (++ic; ic < iend; ++ic) {
+            const double d = uva[ic] * inv_dia;
+            uva[ic] = d;
+            const int32_t i = uci[ic];
+            int32_t jc = urp[i], jend = urp[i + 1];
+            while (jc < jend && uci[jc] <= k) ++jc;
+            while (jend > jc && uci[jend - 1] > i) --jend;
+            for (; jc < jend; ++jc) {
+                const int32_t j = uci[jc];
+                const int64_t ji = find_entry(urp, uci, j, i);
+                const int64_t kj = find_entry(urp, uci, k, j);
+                if (ji >= 0) uva[ji] -= d * (kj >= 0 ? uva[kj] : 0.0);
+            }
+        }
+    }
+    if (P->fmt != MTL4G_F64)
+        for (auto& v : uva) v = (double)(float)v;
+    // L = trans(U) (ic_0.hpp:57): row r of L lists columns ascending, the
+    // (inverse) diagonal last.
+    auto& lrp = P->L_rowptr;
+    lrp.assign(n + 1, 0);
+    for (int32_t c : uci) lrp[c + 1]++;
+    for (int32_t r = 0; r < n; r++) lrp[r + 1] += lrp[r];
+    P->L_col.resize(uci.size());
+    P->L_val.resize(uci.size());
+    std::vector<int32_t> fill(lrp.begin(), lrp.end() - 1);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t j = urp[r]; j < urp[r + 1]; j++) {
+            int32_t d = fill[uci[j]]++;
+            P->L_col[d] = r;
+            P->L_val[d] = uva[j];
+        }
+    return MTL4G_OK;
+}
+
+// Sweep rows from a stored factor.
+//  kind 0: unit lower on strict L           (lower_trisolve v5, :174-189)
+//  kind 1: lower, inverse diagonal last      (lower_trisolve v6, :193-209)
+//  kind 2: upper, inverse diagonal first     (upper_trisolve v3, :161-176)
+static SweepRows rows_from(int32_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                           const std::vector<double>& va, int kind) {
+    SweepRows R;
+    R.n = n;
+    R.lower = kind != 2;
+    R.unit = kind == 0;
+    R.rowptr.assign(n + 1, 0);
+    R.dia.assign(n, 1.0);
+    for (int32_t r = 0; r < n; r++) {
+        int32_t a = rp[r], b = rp[r + 1];
+        if (kind == 1) { R.dia[r] = va[b - 1]; b--; }
+        if (kind == 2) { R.dia[r] = va[a]; a++; }
+        for (int32_t j = a; j < b; j++) { R.col.push_back(ci[j]); R.val.push_back(va[j]); }
+        R.rowptr[r + 1] = (int32_t)R.col.size();
+    }
+    return R;
+}
+
+// Adjoint sweeps of ilu (ilu.hpp:105-115) on the transposed views: the
+// column-oriented generic sweeps (lower_trisolve.hpp:154-170 on trans(U),
+// upper_trisolve.hpp:128-147 on trans(L)) subtract in the order the columns
+// are visited: ascending for the lower, descending for the upper one.
+static int build_adjoint(mtl4g_pc_s* P) {
+    const int32_t n = P->n;
+    auto transpose = [n](const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                         const std::vector<double>& va, std::vector<int32_t>& trp,
+                         std::vector<int32_t>& tci, std::vector<double>& tva) {
+        trp.assign(n + 1, 0);
+        for (int32_t c : ci) trp[c + 1]++;
+        for (int32_t r = 0; r < n; r++) trp[r + 1] += trp[r];
+        tci.resize(ci.size());
+        tva.resize(ci.size());
+        std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+        for (int32_t r = 0; r < n; r++)
+            for (int32_t j = rp[r]; j < rp[r + 1]; j++) {
+                int32_t d = fill[ci[j]]++;
+                tci[d] = r;
+                tva[d] = va[j];
+            }
+    };
+    std::vector<int32_t> trp, tci;
+    std::vector<double> tva;
+    transpose(P->U_rowptr, P->U_col, P->U_val, trp, tci, tva);   // trans(U): diagonal last
+    M4_TRY(upload_sweep(rows_from(n, trp, tci, tva, 1), P->fmt, &P->adj_lower));
+    transpose(P->L_rowptr, P->L_col, P->L_val, trp, tci, tva);   // trans(L): strictly upper
+    SweepRows R = rows_from(n, trp, tci, tva, 0);
+    R.lower = false;
+    for (int32_t r = 0; r < n; r++) {   // descending subtraction order
+        std::reverse(R.col.begin() + R.rowptr[r], R.col.begin() + R.rowptr[r + 1]);
+        std::reverse(R.val.begin() + R.rowptr[r], R.val.begin() + R.rowptr[r + 1]);
+    }
+    M4_TRY(upload_sweep(R, P->fmt, &P->adj_upper));
+    P->has_adjoint = true;
+    return MTL4G_OK;
+}
+
+}  // namespace mtl4g
+
+using namespace mtl4g;
+
+extern "C" {
+
+int mtl4g_pc_create(mtl4g_pc* out, mtl4g_matrix A, int kind, int fmt) {
+    if (!out || !A || kind < MTL4G_PC_IDENTITY || kind > MTL4G_PC_IC0 ||
+        (fmt != MTL4G_F64 && fmt != MTL4G_F32)) {
+        set_error("mtl4g_pc_create: bad arguments");
+        return MTL4G_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (A->n != A->m) {
+        set_error("preconditioner needs a square matrix");
+        return MTL4G_ERR_NOT_SQUARE;
+    }
+    mtl4g_pc_s* P = new (std::nothrow) mtl4g_pc_s;
+    if (!P) return MTL4G_ERR_OOM;
+    P->kind = kind;
+    P->fmt = fmt;
+    P->n = A->n;
+    P->A = A;
+    int st = P->err.alloc(1);
+    if (!st && hipMemset(P->err.p, 0, sizeof(int)) != hipSuccess) st = MTL4G_ERR_HIP;
+    if (!st && kind == MTL4G_PC_DIAGONAL) {
+        // diagonal(A): inv_diag[i] = reciprocal(A[i][i]) (diagonal.hpp:37-45)
+        P->h_inv_diag.assign(A->n, 0.0);
+        for (int32_t r = 0; r < A->n; r++) {
+            int64_t d = find_entry(A->h_rowptr, A->h_col, r, r);
+            P->h_inv_diag[r] = 1.0 / (d >= 0 ? A->h_val[d] : 0.0);
+        }
+        if (fmt == MTL4G_F64) {
+            st = P->inv_diag64.alloc(A->n);
+            if (!st && A->n)
+                st = hipMemcpy(P->inv_diag64.p, P->h_inv_diag.data(), A->n * 8, hipMemcpyHostToDevice) ? MTL4G_ERR_HIP : 0;
+        } else {
+            std::vector<float> f(P->h_inv_diag.begin(), P->h_inv_diag.end());
+            for (size_t i = 0; i < f.size(); i++) P->h_inv_diag[i] = f[i];
+            st = P->inv_diag32.alloc(A->n);
+            if (!st && A->n)
+                st = hipMemcpy(P->inv_diag32.p, f.data(), A->n * 4, hipMemcpyHostToDevice) ? MTL4G_ERR_HIP : 0;
+        }
+    } else if (!st && kind == MTL4G_PC_ILU0) {
+        st = factor_ilu0(A, P);
+        if (!st) st = upload_sweep(rows_from(A->n, P->L_rowptr, P->L_col, P->L_val, 0), fmt, &P->lower);
+        if (!st) st = upload_sweep(rows_from(A->n, P->U_rowptr, P->U_col, P->U_val, 2), fmt, &P->upper);
+    } else if (!st && kind == MTL4G_PC_IC0) {
+        st = factor_ic0(A, P);
+        if (!st) st = upload_sweep(rows_from(A->n, P->L_rowptr, P->L_col, P->L_val, 1), fmt, &P->lower);
+        if (!st) st = upload_sweep(rows_from(A->n, P->U_rowptr, P->U_col, P->U_val, 2), fmt, &P->upper);
+    }
+    if (st) {
+        delete P;
+        return st;
+    }
+    *out = P;
+    return MTL4G_OK;
+}
+
+int mtl4g_pc_destroy(mtl4g_pc P) {
+    delete P;
+    return MTL4G_OK;
+}
+
+static int pc_solve_impl(mtl4g_pc P, const double* x, double* y, void* stream, bool adjoint) {
+    if (!P || (P->n && (!x || !y))) return MTL4G_ERR_INVALID;
+    hipStream_t s = S(stream);
+    if (P->n == 0) return MTL4G_OK;
+    if (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) {
+        if (adjoint && P->kind == MTL4G_PC_ILU0 && !P->has_adjoint) M4_TRY(build_adjoint(P));
+        double* y0 = nullptr;
+        M4_HIP(hipMallocAsync((void**)&y0, (size_t)P->n * sizeof(double), s));
+        int st;
+        if (adjoint && P->kind == MTL4G_PC_ILU0) {
+            st = launch_sweep<double>(P->adj_lower, P->fmt, x, y0, nullptr, P->err.p, nullptr, s);
+            if (!st) st = launch_sweep<double>(P->adj_upper, P->fmt, y0, y, nullptr, P->err.p, nullptr, s);
+        } else {
+            st = pc_apply<double>(P, x, y, y0, nullptr, nullptr, s);
+        }
+        (void)hipFreeAsync(y0, s);
+        M4_TRY(st);
+        return pc_check_error(P, s);
+    }
+    return pc_apply<double>(P, x, y, nullptr, nullptr, nullptr, s);
+}
+
+int mtl4g_pc_solve(mtl4g_pc P, const double* x, double* y, void* stream) {
+    return pc_solve_impl(P, x, y, stream, false);
+}
+
+int mtl4g_pc_adjoint_solve(mtl4g_pc P, const double* x, double* y, void* stream) {
+    return pc_solve_impl(P, x, y, stream, true);
+}
+
+int mtl4g_pc_factor(mtl4g_pc P, int which, int32_t* n_out, int64_t* nnz_out, int32_t* rowptr,
+                    int32_t* col, double* val) {
+    if (!P || which < 0 || which > 2) return MTL4G_ERR_INVALID;
+    if (n_out) *n_out = P->n;
+    if (which == 0) {
+        if (nnz_out) *nnz_out = (int64_t)P->h_inv_diag.size();
+        if (val && !P->h_inv_diag.empty())
+            std::memcpy(val, P->h_inv_diag.data(), P->h_inv_diag.size() * 8);
+        return MTL4G_OK;
+    }
+    const auto& rp = which == 1 ? P->L_rowptr : P->U_rowptr;
+    const auto& ci = which == 1 ? P->L_col : P->U_col;
+    const auto& va = which == 1 ? P->L_val : P->U_val;
+    if (rp.empty()) {
+        if (nnz_out) *nnz_out = -1;
+        return MTL4G_OK;
+    }
+    if (nnz_out) *nnz_out = (int64_t)ci.size();
+    if (rowptr) std::memcpy(rowptr, rp.data(), rp.size() * 4);
+    if (col && !ci.empty()) std::memcpy(col, ci.data(), ci.size() * 4);
+    if (val && !va.empty()) std::memcpy(val, va.data(), va.size() * 8);
+    return MTL4G_OK;
+}
+
+int mtl4g_pc_levels(mtl4g_pc P, int32_t* lower_levels, int32_t* upper_levels) {
+    if (!P) return MTL4G_ERR_INVALID;
+    if (lower_levels) *lower_levels = P->lower.nlev;
+    if (upper_levels) *upper_levels = P->upper.nlev;
+    return MTL4G_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// precond.hpp -- preconditioner handle and the level-scheduled sweep.
+#pragma once
+
+#include "internal.hpp"
+
+namespace mtl4g {
+
+// One triangular sweep  w[r] = (v[r] - sum_j T[r][c_j] * w[c_j]) * dia[r]
+// stored in level order: position k of the sweep handles row perm[k]; the
+// off-diagonal entries of that row follow in the order MTL4 subtracts them.
+struct Sweep {
+    int32_t n = 0, nlev = 0, nblocks = 0;
+    bool unit = false;
+    DevBuf<int32_t> perm, rowptr, col, lev_ptr, lev_blk, blk_level;
+    DevBuf<double> val64, dia64;
+    DevBuf<float> val32, dia32;
+    DevBuf<unsigned> counters;   // default counters: [0] ticket, [1] abort, [2..2+nlev) level done counts
+    std::vector<int32_t> h_lev_ptr;
+};
+
+// Host description of a sweep before upload: rows in natural order.
+struct SweepRows {
+    int32_t n = 0;
+    bool lower = true;     // dependencies point to smaller (true) or larger rows
+    bool unit = false;
+    std::vector<int32_t> rowptr, col;   // off-diagonal entries in subtraction order
+    std::vector<double> val;
+    std::vector<double> dia;            // stored (inverse) diagonal, unused if unit
+};
+
+int upload_sweep(const SweepRows& rows, int value_format, Sweep* S);
+
+// counters: nlev+2 zeroed words owned by the caller (NULL = the sweep's own);
+// each launch leaves them zeroed again.
+template <typename VT>
+int launch_sweep(const Sweep& S, int value_format, const VT* v, VT* w, const int* skip, int* err,
+                 unsigned* counters, hipStream_t s);
+
+}  // namespace mtl4g
+
+struct mtl4g_pc_s {
+    int kind = MTL4G_PC_IDENTITY;
+    int fmt = MTL4G_F64;
+    int32_t n = 0;
+    const mtl4g_matrix_s* A = nullptr;
+    mtl4g::DevBuf<double> inv_diag64;
+    mtl4g::DevBuf<float> inv_diag32;
+    mtl4g::Sweep lower, upper;            // solve: lower then upper
+    mtl4g::Sweep adj_lower, adj_upper;    // adjoint solve (built on demand)
+    bool has_adjoint = false;
+    // host copies of the factors as MTL4 stores them (natural row order)
+    std::vector<double> h_inv_diag;
+    std::vector<int32_t> L_rowptr, L_col, U_rowptr, U_col;
+    std::vector<double> L_val, U_val;
+    mtl4g::DevBuf<int> err;               // device error word of the sweeps
+};
+
+namespace mtl4g {
+// z = P^{-1} r on the device.  y0 is caller scratch of n values (ilu/ic
+// only); skip (may be null) makes every kernel a no-op when *skip != 0;
+// counters (may be null) are pc_counter_words(P) zeroed words of the caller.
+template <typename VT>
+int pc_apply(const mtl4g_pc_s* P, const VT* r, VT* z, VT* y0, const int* skip,
+             unsigned* counters, hipStream_t s);
+size_t pc_counter_words(const mtl4g_pc_s* P);
+int pc_check_error(const mtl4g_pc_s* P, hipStream_t s);
+}  // namespace mtl4g

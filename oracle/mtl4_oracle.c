@@ -325,8 +325,9 @@ struct orc_pc {
     double* y0;           /* workspace (MTL keeps a function-local static) */
 };
 
-/* diagonal(A): inv_diag[i]= reciprocal(A[i][i]), itl/pc/diagonal.hpp:37-45 */
-static int setup_diagonal(orc_pc* P, const orc_csr* A)
+/* diagonal(A): inv_diag[i]= reciprocal(A[i][i]), itl/pc/diagonal.hpp:37-45;
+ * with Value = float the reciprocal is stored in a dense_vector<float>.  */
+static int setup_diagonal(orc_pc* P, const orc_csr* A, int value_bytes)
 {
     P->inv_diag = (double*)malloc((size_t)A->n * sizeof(double));
     if (!P->inv_diag) return ORC_OOM;
@@ -334,6 +335,7 @@ static int setup_diagonal(orc_pc* P, const orc_csr* A)
         int64_t k = find_entry(A->rowptr, A->col, i, i);
         double a = k >= 0 ? A->val[k] : 0.0;
         P->inv_diag[i] = 1.0 / a;
+        if (value_bytes == 4) P->inv_diag[i] = (double)(float)P->inv_diag[i];
     }
     return ORC_OK;
 }
@@ -427,7 +429,7 @@ orc_pc* orc_pc_create(int kind, const orc_csr* A, int value_bytes, int* status)
     P->kind = kind;
     P->n = A->n;
     if (A->n != A->ncols) st = ORC_NOT_SQUARE;
-    else if (kind == ORC_PC_DIAGONAL) st = setup_diagonal(P, A);
+    else if (kind == ORC_PC_DIAGONAL) st = setup_diagonal(P, A, value_bytes);
     else if (kind == ORC_PC_ILU0) st = setup_ilu0(P, A, value_bytes);
     else if (kind == ORC_PC_IC0) st = setup_ic0(P, A, value_bytes);
     if (st == ORC_OK && (kind == ORC_PC_ILU0 || kind == ORC_PC_IC0)) {

@@ -1,0 +1,52 @@
+"""CPU checks of the C-ABI library: it loads, exports every function that
+include/mtl4g.h declares, and its pure-host entry points answer.  No compute
+calls (there is no GPU here)."""
+import ctypes
+import os
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    from mtl4_amd import _lib
+    lib = _lib.lib()
+    names = _lib.declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_symbols_are_c_linkage():
+    lib = ctypes.CDLL(os.path.join(ROOT, "mtl4_amd", "libmtl4g.so"))
+    for n in ("mtl4g_csr_create", "mtl4g_spmv", "mtl4g_cg", "mtl4g_bicgstab", "mtl4g_pc_create",
+              "mtl4g_ir_solve"):
+        assert getattr(lib, n) is not None
+
+
+def test_status_strings():
+    from mtl4_amd import _lib
+    lib = _lib.lib()
+    assert lib.mtl4g_status_string(0) == b"ok"
+    assert lib.mtl4g_status_string(_lib.ERR_BREAKDOWN) == b"unexpected orthogonality"
+    assert lib.mtl4g_status_string(_lib.ERR_MISSING_DIAG) == b"missing diagonal"
+    assert lib.mtl4g_status_string(12345) == b"unknown status"
+    assert lib.mtl4g_version() >= 100
+
+
+def test_exception_mapping():
+    from mtl4_amd import _lib
+    with pytest.raises(_lib.unexpected_orthogonality):
+        _lib.check(_lib.ERR_BREAKDOWN, "x")
+    with pytest.raises(_lib.missing_diagonal):
+        _lib.check(_lib.ERR_MISSING_DIAG, "x")
+    assert _lib.check(1) == 1
+
+
+def test_null_handles_rejected():
+    from mtl4_amd import _lib
+    lib = _lib.lib()
+    assert lib.mtl4g_csr_info(None, None, None, None) == _lib.ERR_INVALID
+    assert lib.mtl4g_pc_levels(None, None, None) == _lib.ERR_INVALID
+    assert lib.mtl4g_csr_create(None, 1, 1, 0, None, None, None, 0) == _lib.ERR_INVALID

@@ -1,0 +1,358 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: bit-exact where MTL4's per-row order is kept (SpMV rows, element-wise
+updates, triangular sweeps, factor values); 1e-13 relative for the tree
+reductions (dot, norms), whose summation order differs from MTL4's unrolled
+partial sums; 1e-12 relative on the solution and +-2 iterations for CG with
+the diagonal preconditioner at config 1, as the MTL4 path requires."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from helpers import FAMILIES, family, pair
+
+pytestmark = pytest.mark.gpu
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mtl4_kats.json")))
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mtl4_amd
+    return mtl4_amd
+
+
+def dv(m, a):
+    return m.dense_vector(np.asarray(a, dtype=np.float64))
+
+
+# ---------------------------------------------------------------- SpMV
+@pytest.mark.parametrize("fam", FAMILIES)
+def test_spmv_bitwise(m, fam):
+    rp, ci, va = family(fam, seed=3)
+    O, A = pair(rp, ci, va)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, A.num_cols())
+    y0 = rng.uniform(-1, 1, A.num_rows())
+    ref = orc.spmv(O, x)
+    got = (A * dv(m, x)).numpy()
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    for mode in (m.PLUS, m.MINUS):
+        w = dv(m, y0)
+        A.mult(dv(m, x), w, mode)
+        yr = y0.copy()
+        orc.spmv(O, x, yr, mode)
+        assert np.array_equal(w.numpy(), yr)
+
+
+@pytest.mark.parametrize("tile", [(64, 1024), (256, 2048), (256, 4096), (32, 512)])
+def test_spmv_partitions(m, tile):
+    rp, ci, va = family("random_3000", seed=1)
+    O, A = pair(rp, ci, va)
+    A.tune(*tile)
+    x = np.random.default_rng(2).uniform(-1, 1, A.num_cols())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+
+
+def test_spmv_kat(m):
+    k = KATS["matrix_vector_product"]
+    A = m.laplacian_setup(*k["grid"])
+    D = np.zeros((25, 25))
+    for r in range(25):
+        for j in range(A.rowptr[r], A.rowptr[r + 1]):
+            D[r, A.col[j]] = A.val[j]
+    v = dv(m, D[k["row"]])
+    w = A * v
+    for i, e in k["assign"].items():
+        assert w[int(i)] == e
+    A.mult(v, w, m.PLUS)
+    for i, e in k["plus"].items():
+        assert w[int(i)] == e
+    A.mult(v, w, m.MINUS)
+    for i, e in k["minus"].items():
+        assert w[int(i)] == e
+
+
+def test_spmv_empty_and_rectangular(m):
+    O, A = pair(np.zeros(4, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    assert np.array_equal((A * dv(m, np.ones(3))).numpy(), np.zeros(3))
+    rp = np.array([0, 2, 3], np.int32)
+    ci = np.array([0, 4, 2], np.int32)
+    va = np.array([1.5, -2.0, 3.0])
+    O, A = pair(rp, ci, va, ncols=5)
+    x = np.arange(5.0)
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+
+
+def test_spmv_dot_fused(m):
+    import ctypes as C
+    rp, ci, va = family("diff3d_16")
+    O, A = pair(rp, ci, va)
+    p = np.random.default_rng(0).uniform(-1, 1, A.num_rows())
+    q = m.dense_vector(A.num_rows())
+    d = C.c_double()
+    m._lib.check(m._lib.lib().mtl4g_spmv_dot(A.h, dv(m, p).ptr, q.ptr, C.byref(d), m._stream()))
+    qr, dr = orc.fused_spmv_dot(O, p)
+    assert np.array_equal(q.numpy(), qr)
+    assert abs(d.value - dr) <= 1e-13 * abs(dr)
+
+
+# ---------------------------------------------------------------- reductions
+@pytest.mark.parametrize("n", [1, 5, 255, 256, 257, 4096, 100003, 3_000_001])
+def test_reductions(m, n):
+    rng = np.random.default_rng(n)
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    va, vb = dv(m, a), dv(m, b)
+    exact = math.fsum(a * b)
+    assert abs(m.dot(va, vb) - exact) <= 1e-13 * np.abs(a * b).sum()
+    assert abs(m.unary_dot(va) - math.fsum(a * a)) <= 1e-13 * (a * a).sum()
+    assert abs(m.two_norm(va) - math.sqrt(math.fsum(a * a))) <= 1e-13 * math.sqrt((a * a).sum())
+    # deterministic: same inputs, same bits
+    assert m.dot(va, vb) == m.dot(va, vb)
+
+
+def test_fuse_kats(m):
+    import ctypes as C
+    k = KATS["fuse"]
+    n = k["n"]
+    r = dv(m, np.full(n, 6.0))
+    q = dv(m, np.full(n, 2.0))
+    rho = C.c_double()
+    m._lib.check(m._lib.lib().mtl4g_axpy_unary_dot(n, k["alpha"], q.ptr, r.ptr, C.byref(rho), m._stream()))
+    assert abs(rho.value - k["axpy_unary_dot"]) < 1e-9
+    assert np.array_equal(r.numpy(), np.full(n, 6.0 - 7.8 * 2.0))
+    x7 = dv(m, np.full(n, 7.0))
+    assert m.unary_dot(x7) == k["unary_dot_7"]
+    assert abs(m.two_norm(x7) - k["two_norm_7"]) < 1e-4
+    assert abs(m.dot(r, dv(m, np.full(n, 2.0))) - k["dot_r_x2"]) < 1e-9
+
+
+def test_axpy_bitwise(m):
+    import ctypes as C
+    rng = np.random.default_rng(4)
+    n = 12345
+    v, w = rng.standard_normal(n), rng.standard_normal(n)
+    dw = dv(m, w)
+    m._lib.check(m._lib.lib().mtl4g_axpy(n, 0.37, dv(m, v).ptr, dw.ptr, m._stream()))
+    assert np.array_equal(dw.numpy(), w + 0.37 * v)
+
+
+# ---------------------------------------------------------------- preconditioners
+@pytest.mark.parametrize("fam", ["lap2d_64", "diff3d_16", "convdiff_40", "spd_2000", "lap3d_12"])
+@pytest.mark.parametrize("kind", ["diagonal", "ilu_0", "ic_0"])
+@pytest.mark.parametrize("vb", [8, 4])
+def test_pc_solve_bitwise(m, fam, kind, vb):
+    if kind == "ic_0" and fam == "convdiff_40":
+        pytest.skip("ic_0 needs a symmetric matrix")
+    rp, ci, va = family(fam, seed=5)
+    O, A = pair(rp, ci, va)
+    OP = orc.Preconditioner(kind, O, vb)
+    cls = getattr(m.pc, kind)
+    P = cls(A, m.F64 if vb == 8 else m.F32)
+    x = np.random.default_rng(1).uniform(-1, 1, A.num_rows())
+    got = P.solve(dv(m, x)).numpy()
+    ref = OP.solve(x)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    if kind != "diagonal":
+        for which in ("L", "U"):
+            R = OP.factor(which)
+            rpg, cig, vag = getattr(P, "get_" + which)()
+            assert np.array_equal(rpg, R.rowptr) and np.array_equal(cig, R.col)
+            assert np.array_equal(vag, R.val)
+    if kind == "ilu_0":
+        got = P.adjoint_solve(dv(m, x)).numpy()
+        assert np.array_equal(got, OP.adjoint_solve(x))
+
+
+def test_ic0_golden_vector(m):
+    k = KATS["ic_0_solve"]
+    A = m.laplacian_setup(*k["grid"])
+    P = m.pc.ic_0(A, m.F32)
+    y = P.solve(dv(m, k["x"])).numpy()
+    assert np.linalg.norm(y - np.array(k["y"])) <= k["tol"]
+
+
+def test_pc_levels(m):
+    A = m.laplacian_setup(30, 20)
+    P = m.pc.ic_0(A)
+    lo, up = P.levels()
+    assert lo == 30 + 20 - 1 and up == 30 + 20 - 1
+
+
+def test_missing_diagonal(m):
+    rp = np.array([0, 1, 2], np.int32)
+    ci = np.array([1, 0], np.int32)
+    A = m.compressed2D(2, 2, rp, ci, np.ones(2))
+    with pytest.raises(m.missing_diagonal):
+        m.pc.ic_0(A)
+    with pytest.raises(m.missing_diagonal):
+        m.pc.ilu_0(A)
+
+
+def test_pc_not_square(m):
+    A = m.compressed2D(2, 3, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.ones(2))
+    with pytest.raises(m.matrix_not_square):
+        m.pc.diagonal(A)
+
+
+# ---------------------------------------------------------------- CG
+def config1():
+    rp, ci, va = family("lap2d_64")
+    xt = np.random.default_rng(0).uniform(-1, 1, 4096)
+    return rp, ci, va, xt
+
+
+def test_cg_config1_parity(m):
+    rp, ci, va, xt = config1()
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, xt)
+    xo, st, ito, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=2000, rtol=1e-10)
+    assert st == 0
+    P = m.pc.diagonal(A)
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+    err = m.cg(A, x, dv(m, b), P, it)
+    assert err == 0
+    xg = x.numpy()
+    assert abs(it.iterations() - ito.i) <= 2, (it.iterations(), ito.i)
+    assert np.linalg.norm(xg - xo) <= 1e-12 * np.linalg.norm(xo)
+    assert it.resid() <= 1e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("fam", ["lap2d_64", "diff3d_16", "spd_2000", "lap3d_12"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ic_0", "ilu_0"])
+def test_cg_families(m, fam, kind):
+    rp, ci, va = family(fam, seed=2)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(9).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OP = None if kind == "identity" else orc.Preconditioner(kind, O)
+    xo, st, ito, ho = orc.cg(O, b, OP, max_iter=3000, rtol=1e-10, hist_cap=4000)
+    P = getattr(m.pc, kind)(A)
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+    err = m.cg(A, x, dv(m, b), P, it)
+    assert err == st == 0
+    assert abs(it.iterations() - ito.i) <= 2, (it.iterations(), ito.i)
+    assert np.linalg.norm(x.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_cg_iteration_semantics(m):
+    rp, ci, va = family("lap2d_64")
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.ones(A.num_rows()))
+    P = m.pc.diagonal(A)
+    # max_iter -> error 1
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 3, 1e-12)
+    assert m.cg(A, x, dv(m, b), P, it) == 1 and it.iterations() == 3
+    xo, st, ito, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=3, rtol=1e-12)
+    assert np.array_equal(x.numpy(), xo) or np.linalg.norm(x.numpy() - xo) <= 1e-13 * np.linalg.norm(xo)
+    # b = 0 -> 0 iterations
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(0.0, 10, 1e-10)
+    assert m.cg(A, x, m.dense_vector(A.num_rows()), P, it) == 0 and it.iterations() == 0
+    # absolute tolerance only (absolut_termination_criterion_test.cpp:27-39)
+    k = KATS["absolute_termination"]
+    A4 = m.laplacian_setup(*k["grid"])
+    b4 = A4 * m.dense_vector(A4.num_rows(), 1.0)
+    x4 = m.dense_vector(A4.num_rows())
+    it = m.cyclic_iteration(b4, k["max_iter"], 0.0, k["atol"], 5)
+    assert m.cg(A4, x4, b4, m.pc.diagonal(A4), it) == 0
+    r = b4.numpy() - (A4 * x4).numpy()
+    assert np.linalg.norm(r) <= k["atol"]
+
+
+def test_cg_ilu0_kat(m):
+    k = KATS["ilu_0_cg"]
+    A = m.laplacian_setup(*k["grid"])
+    b = A * m.dense_vector(A.num_rows(), 1.0)
+    x = m.dense_vector(A.num_rows())
+    it = m.cyclic_iteration(b, k["max_iter"], k["rtol"], 0.0, 1)
+    assert m.cg(A, x, b, m.pc.ilu_0(A), it) == 0
+    assert it.iterations() <= k["max_iterations_allowed"]
+
+
+def test_solver_handle_reuse(m):
+    rp, ci, va, xt = config1()
+    O, A = pair(rp, ci, va)
+    b = dv(m, orc.spmv(O, xt))
+    P = m.pc.diagonal(A)
+    S = m.Solver(A, P)
+    out = []
+    for _ in range(3):
+        x = m.dense_vector(A.num_rows())
+        it = m.basic_iteration(b, 2000, 1e-10)
+        assert S.solve(x, b, it) == 0
+        out.append((it.iterations(), x.numpy()))
+    assert all(o[0] == out[0][0] and np.array_equal(o[1], out[0][1]) for o in out)
+
+
+# ---------------------------------------------------------------- BiCGSTAB
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ilu_0"])
+def test_bicgstab_families(m, fam, kind):
+    rp, ci, va = family(fam, seed=4)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(3).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OP = None if kind == "identity" else orc.Preconditioner(kind, O)
+    xo, st, ito, _ = orc.bicgstab(O, b, OP, max_iter=2000, rtol=1e-10)
+    P = getattr(m.pc, kind)(A)
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+    err = m.bicgstab(A, x, dv(m, b), P, it)
+    assert err == st == 0
+    assert abs(it.iterations() - ito.i) <= 2, (it.iterations(), ito.i)
+    # both solutions carry the solve's own error (about cond(A)*rtol): they
+    # must agree to within that error, measured on the oracle against xt
+    own = max(np.linalg.norm(xo - xt), 1e-12 * np.linalg.norm(xt))
+    assert np.linalg.norm(x.numpy() - xo) <= 10 * own
+    res = np.linalg.norm(b - orc.spmv(O, x.numpy()))
+    assert res <= 1e-9 * np.linalg.norm(b)
+
+
+def test_bicgstab_breakdown(m):
+    A = m.compressed2D(2, 2, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.zeros(2))
+    x = m.dense_vector(2)
+    with pytest.raises(m.unexpected_orthogonality):
+        m.bicgstab(A, x, m.dense_vector(2, 1.0), None, m.basic_iteration(2.0 ** 0.5, 10, 1e-8))
+
+
+# ---------------------------------------------------------------- mixed precision
+@pytest.mark.parametrize("fmt", ["f32", "bf16"])
+def test_ir_cg_reaches_fp64_accuracy(m, fmt):
+    rp, ci, va = family("diff3d_16", seed=1)
+    O, A = pair(rp, ci, va)
+    xt = np.random.default_rng(1).uniform(-1, 1, A.num_rows())
+    b = orc.spmv(O, xt)
+    P32 = m.pc.diagonal(A, m.F32)
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 100, 1e-10)
+    err, st = m.ir_solve(A, x, dv(m, b), it, P32, value_format=m.F32 if fmt == "f32" else m.BF16,
+                         inner_rtol=1e-4 if fmt == "f32" else 1e-2)
+    assert err == 0
+    res = np.linalg.norm(b - orc.spmv(O, x.numpy())) / np.linalg.norm(b)
+    assert res <= 1.5e-10, res
+    xo, _, _, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=5000, rtol=1e-12)
+    assert np.linalg.norm(x.numpy() - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_ir_bicgstab_ilu(m):
+    rp, ci, va = family("convdiff_40", seed=1)
+    O, A = pair(rp, ci, va)
+    xt = np.random.default_rng(1).uniform(-1, 1, A.num_rows())
+    b = orc.spmv(O, xt)
+    P32 = m.pc.ilu_0(A, m.F32)
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 100, 1e-10)
+    err, st = m.ir_solve(A, x, dv(m, b), it, P32, method=m.BICGSTAB, inner_rtol=1e-4)
+    assert err == 0
+    res = np.linalg.norm(b - orc.spmv(O, x.numpy())) / np.linalg.norm(b)
+    assert res <= 1.5e-10
