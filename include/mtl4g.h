@@ -176,21 +176,33 @@ int mtl4g_solver_solve(mtl4g_solver S, double* d_x, const double* d_b, mtl4g_ite
                        double* h_hist, int32_t hist_cap, void* stream);
 
 /* ---- mixed-precision iterative refinement (no MTL4 counterpart) ------ */
-/* Outer loop in FP64 on the device: r = b - A*x, stop when
- * ||r|| <= rtol*norm_r0 (or atol); otherwise solve A*d = r with the inner
- * method in FP32 arithmetic on FP32 or BF16 stored values (inner stop:
- * inner_rtol relative to ||r||), then x += d in FP64. */
+/* The residual r = b - A*x, the solution update and the stopping norm are
+ * FP64 on the device (stop: ||r|| <= rtol*norm_r0 or atol, as in
+ * basic_iteration); the Krylov recurrence runs in FP32 arithmetic on FP32
+ * or BF16 stored values.
+ *  MTL4G_IR_RELIABLE: one FP32 CG/BiCGSTAB recurrence; its updates are
+ *    accumulated into x in FP64 and r is recomputed in FP64 ("reliable
+ *    updates") whenever the FP32 residual fell by inner_rtol since the last
+ *    refresh, and whenever the FP32 recurrence believes it has converged.
+ *    it->max_iter bounds the FP32 iterations; it->i returns them.
+ *  MTL4G_IR_RESTART: classic refinement: solve A*d = r in FP32 to
+ *    inner_rtol*||r|| (at most inner_max_iter iterations), x += d, repeat
+ *    (at most max_outer steps); it->i returns the refinement steps. */
+#define MTL4G_IR_RELIABLE 0
+#define MTL4G_IR_RESTART  1
+
 typedef struct {
     int32_t method;          /* MTL4G_CG or MTL4G_BICGSTAB */
-    int32_t value_format;    /* MTL4G_F32 or MTL4G_BF16 for the inner matrix */
-    double inner_rtol;       /* inner relative tolerance, e.g. 1e-5 */
-    int32_t inner_max_iter;  /* per inner solve */
-    int32_t max_outer;       /* refinement steps */
+    int32_t value_format;    /* MTL4G_F32 or MTL4G_BF16 for the FP32 recurrence */
+    int32_t mode;            /* MTL4G_IR_RELIABLE or MTL4G_IR_RESTART */
+    double inner_rtol;       /* reliable: refresh trigger (e.g. 0.1); restart: inner tolerance */
+    int32_t inner_max_iter;  /* restart: per inner solve */
+    int32_t max_outer;       /* restart: refinement steps */
 } mtl4g_ir_params;
 
 typedef struct {
-    int32_t outer;           /* refinement steps taken */
-    int32_t inner_total;     /* inner iterations summed over all steps */
+    int32_t outer;           /* FP64 residual refreshes (reliable) or refinement steps */
+    int32_t inner_total;     /* FP32 iterations */
     double relres;           /* final FP64 ||b - A x|| / norm_r0 */
 } mtl4g_ir_stats;
 

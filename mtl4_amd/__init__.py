@@ -437,12 +437,18 @@ class Solver:
         return it.error
 
 
-def ir_solve(A, x, b, it, P32=None, method=CG, value_format=F32, inner_rtol=1e-5,
-             inner_max_iter=10000, max_outer=50):
-    """Mixed-precision iterative refinement: FP64 residual/correction/norm on
-    the device, inner Krylov solve in FP32 arithmetic (FP32 or BF16 values).
+def ir_solve(A, x, b, it, P32=None, method=CG, value_format=F32, mode="reliable",
+             inner_rtol=None, inner_max_iter=10000, max_outer=50):
+    """Mixed-precision refinement: FP64 residual, correction and stopping norm
+    on the device; Krylov recurrence in FP32 arithmetic (FP32 or BF16 values).
+    mode "reliable": one FP32 recurrence with FP64 residual refreshes when the
+    FP32 residual fell by inner_rtol (default 0.1); mode "restart": classic
+    refinement with inner solves to inner_rtol (default 1e-5).
     Returns (error_code, stats dict)."""
-    prm = _lib.IRParams(method, value_format, inner_rtol, inner_max_iter, max_outer)
+    md = _lib.IR_RELIABLE if mode == "reliable" else _lib.IR_RESTART
+    if inner_rtol is None:
+        inner_rtol = 0.1 if md == _lib.IR_RELIABLE else 1e-5
+    prm = _lib.IRParams(method, value_format, md, inner_rtol, inner_max_iter, max_outer)
     stats = _lib.IRStats()
     st = it._state()
     status = _L().mtl4g_ir_solve(A.h, P32.h if P32 is not None else None, x.ptr, b.ptr,

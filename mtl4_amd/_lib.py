@@ -26,9 +26,12 @@ class Iteration(C.Structure):
                 ("resid", C.c_double)]
 
 
+IR_RELIABLE, IR_RESTART = 0, 1
+
+
 class IRParams(C.Structure):
-    _fields_ = [("method", C.c_int32), ("value_format", C.c_int32), ("inner_rtol", C.c_double),
-                ("inner_max_iter", C.c_int32), ("max_outer", C.c_int32)]
+    _fields_ = [("method", C.c_int32), ("value_format", C.c_int32), ("mode", C.c_int32),
+                ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32), ("max_outer", C.c_int32)]
 
 
 class IRStats(C.Structure):

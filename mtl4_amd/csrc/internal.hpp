@@ -84,6 +84,15 @@ __device__ __forceinline__ float load_val<float, uint16_t>(const uint16_t* p, in
     return bf16_to_f32(p[j]);
 }
 
+template <typename AT, typename MT>
+__device__ __forceinline__ AT load_val_nt(const MT* p, int64_t j) {
+    return static_cast<AT>(__builtin_nontemporal_load(p + j));
+}
+template <>
+__device__ __forceinline__ float load_val_nt<float, uint16_t>(const uint16_t* p, int64_t j) {
+    return bf16_to_f32(__builtin_nontemporal_load(p + j));
+}
+
 // Agent-scope loads/stores for values exchanged between workgroups
 // inside one kernel (partials, level flags); they bypass non-coherent
 // per-XCD cache lines.
@@ -119,31 +128,60 @@ __device__ __forceinline__ T block_sum(T v, T* sh /* BS/64 */) {
     return r;
 }
 
-// Last-block pattern: every block stores its partials, then the block that
-// arrives last (ticket nblocks-1) reduces them in index order and resets
-// the ticket for the next launch (no memset needed, graph-replay safe).
+// Publish a value for other workgroups of the same kernel: an agent-scope
+// atomic exchange is performed at the device-coherent level, so no L2
+// write-back fence (buffer_wbl2, which a __threadfence() costs every
+// workgroup on a multi-XCD part) is needed.
+__device__ __forceinline__ void publish(double* p, double v) {
+    (void)__hip_atomic_exchange(reinterpret_cast<unsigned long long*>(p),
+                                (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish(float* p, float v) {
+    (void)__hip_atomic_exchange(reinterpret_cast<unsigned*>(p), __float_as_uint(v),
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until this wave's memory operations (the publishes above) have
+// completed, before the atomic that announces them.
+__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Last-block pattern: lane 0 of every block publishes its partials, then the
+// block that arrives last (ticket nblocks-1) reduces them in index order with
+// agent-scope loads and resets the ticket for the next launch (no memset
+// needed, graph-replay safe).
 template <int BS>
 __device__ __forceinline__ bool arrive_last(unsigned* ticket, unsigned nblocks) {
     __shared__ int am_last;
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned prev = atomicAdd(ticket, 1u);
+        wait_memory();
+        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         am_last = (prev == nblocks - 1);
     }
     __syncthreads();
-    if (am_last) {
-        __threadfence();
-        if (threadIdx.x == 0) st_agent(ticket, 0u);
-    }
+    if (am_last && threadIdx.x == 0) st_agent(ticket, 0u);
     return am_last != 0;
 }
 
-// Sum of partials[0..n) in a fixed order, valid in all threads.
+// Sum of partials[0..n) in a fixed order, valid in all threads.  Each lane
+// issues U independent agent-scope loads before it adds them, so the last
+// workgroup pays one memory round trip per U*BS partials rather than one
+// per BS.
 template <int BS, typename T>
 __device__ __forceinline__ T reduce_partials(const T* partials, int n, int stride, T* sh) {
+    constexpr int U = 16;
     T s = 0;
-    for (int i = threadIdx.x; i < n; i += BS) s += ld_agent(partials + (size_t)i * stride);
+    for (int base = threadIdx.x; base < n; base += BS * U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = base + u * BS;
+            v[u] = i < n ? ld_agent(partials + (size_t)i * stride) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) s += v[u];
+    }
     return block_sum<BS>(s, sh);
 }
 
@@ -183,21 +221,22 @@ __device__ __forceinline__ AT csr_block_rows(const int32_t* __restrict__ rowptr,
         s = block_sum<BS>(s, sh);
         return s;   // valid for lane 0 == the row
     }
-    constexpr int U = 4;
-    for (int base = t; base < nz; base += BS * U) {
-        int c[U];
-        AT v[U];
+    // The whole tile is requested at once: U = TILE/BS column indices and
+    // values per lane (non-temporal: streamed once, they should not evict the
+    // x window from L2), then U independent gathers of x.
+    constexpr int U = TILE / BS;
+    int c[U];
+    AT v[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            int k = base + u * BS;
-            c[u] = k < nz ? col[j0 + k] : 0;
-            v[u] = k < nz ? load_val<AT>(val, (int64_t)j0 + k) : AT(0);
-        }
+    for (int u = 0; u < U; u++) {
+        const int k = t + u * BS;
+        c[u] = k < nz ? __builtin_nontemporal_load(col + j0 + k) : 0;
+        v[u] = k < nz ? load_val_nt<AT>(val, (int64_t)j0 + k) : AT(0);
+    }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            int k = base + u * BS;
-            if (k < nz) prod[k] = v[u] * static_cast<AT>(x[c[u]]);
-        }
+    for (int u = 0; u < U; u++) {
+        const int k = t + u * BS;
+        if (k < nz) prod[k] = v[u] * static_cast<AT>(x[c[u]]);
     }
     __syncthreads();
     AT s = 0;

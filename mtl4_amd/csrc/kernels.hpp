@@ -21,6 +21,45 @@
 namespace mtl4g {
 
 constexpr int kMaxTile = 4096;
+constexpr int kFanIn = 128;                                  // workgroups per ticket group
+constexpr int kMaxGroups = kMaxPartials / kFanIn;            // 512
+constexpr int kTicketStride = 32;                            // one ticket per 128-byte line
+constexpr size_t kTicketWords = (size_t)(1 + kMaxGroups) * kTicketStride;
+constexpr size_t kPartialWords = 2 * (size_t)(kMaxPartials + kMaxGroups);
+
+// Two-level combine of per-workgroup partial sums.  Workgroup b publishes its
+// NP partials, then takes the ticket of its group of kFanIn workgroups; the
+// last of a group sums the group's partials (index order) and takes the top
+// ticket; the last group sums the group sums (index order) and runs
+// epi.finish.  Same-address atomics are thus at most kFanIn deep instead of
+// one per workgroup, and the result does not depend on arrival order.
+// partials: NP*G block values then NP*ngroups group values; tickets: words
+// [(1+g)*kTicketStride] per group and [0] on top, all left at zero.
+template <int BS, typename AT, typename Epi>
+__device__ __forceinline__ void combine_partials(const AT* part, AT* partials, unsigned* tickets,
+                                                 int b, int G, const Epi& epi, AT* sh) {
+    constexpr int NP = Epi::NP;
+    const int ngroups = (G + kFanIn - 1) / kFanIn;
+    const int g = b / kFanIn;
+    const int gsize = min(kFanIn, G - g * kFanIn);
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        AT v = block_sum<BS>(part[k], sh);
+        if (threadIdx.x == 0) publish(partials + (size_t)k * G + b, v);
+    }
+    if (!arrive_last<BS>(tickets + (size_t)(1 + g) * kTicketStride, (unsigned)gsize)) return;
+    AT* gpart = partials + (size_t)NP * G;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        AT v = reduce_partials<BS>(partials + (size_t)k * G + (size_t)g * kFanIn, gsize, 1, sh);
+        if (threadIdx.x == 0) publish(gpart + (size_t)k * ngroups + g, v);
+    }
+    if (!arrive_last<BS>(tickets, (unsigned)ngroups)) return;
+    AT tot[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) tot[k] = reduce_partials<BS>(gpart + (size_t)k * ngroups, ngroups, 1, sh);
+    if (threadIdx.x == 0) epi.finish(tot);
+}
 
 template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
 __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
@@ -44,20 +83,7 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
         AT s = csr_block_rows<BS, TILE, AT>(rowptr, col, val, x, r0, r1, prod, sh, row);
         if (row < r1) epi(row, s, part);
     }
-    if constexpr (Epi::NP > 0) {
-        const unsigned G = gridDim.x;
-#pragma unroll
-        for (int k = 0; k < Epi::NP; k++) {
-            AT v = block_sum<BS>(part[k], sh);
-            if (threadIdx.x == 0) st_agent(partials + (size_t)k * G + b, v);
-        }
-        if (arrive_last<BS>(ticket, G)) {
-            AT tot[Epi::NP];
-#pragma unroll
-            for (int k = 0; k < Epi::NP; k++) tot[k] = reduce_partials<BS>(partials + (size_t)k * G, G, 1, sh);
-            if (threadIdx.x == 0) epi.finish(tot);
-        }
-    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
 template <int BS, typename AT, typename Epi>
@@ -77,17 +103,7 @@ __global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, un
     for (int k = 0; k < NP; k++) part[k] = AT(0);
     for (int64_t i = lo + threadIdx.x; i < hi; i += BS) epi(i, part);
     if constexpr (Epi::NP > 0) {
-#pragma unroll
-        for (int k = 0; k < Epi::NP; k++) {
-            AT v = block_sum<BS>(part[k], sh);
-            if (threadIdx.x == 0) st_agent(partials + (size_t)k * G + b, v);
-        }
-        if (arrive_last<BS>(ticket, G)) {
-            AT tot[Epi::NP];
-#pragma unroll
-            for (int k = 0; k < Epi::NP; k++) tot[k] = reduce_partials<BS>(partials + (size_t)k * G, G, 1, sh);
-            if (threadIdx.x == 0) epi.finish(tot);
-        }
+        combine_partials<BS>(part, partials, ticket, b, (int)G, epi, sh);
     }
 }
 

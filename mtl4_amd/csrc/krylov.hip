@@ -44,6 +44,49 @@ __device__ __forceinline__ void iter_finished(Ctl* c, double resid, double* hist
     c->first = c->i <= 1;
 }
 
+__device__ __forceinline__ bool conv_test(const Ctl* c, double resid) {
+    return c->norm_r0 == 0 ? (resid <= c->atol) : (resid <= c->rtol * c->norm_r0 || resid <= c->atol);
+}
+
+// Mixed precision, loop head on the FP32 recurrence residual: it only
+// estimates convergence, so an estimated stop (or the iteration limit)
+// pauses the recurrence and asks for an FP64 refresh, which decides.
+__device__ __forceinline__ void mixed_check(Ctl* c, double resid32) {
+    if (!isfinite(resid32) || conv_test(c, resid32)) { c->want_ru = 1; c->pause = 1; return; }
+    if (c->i >= c->max_iter) { c->want_ru = 1; c->pause = 1; c->limit = 1; return; }
+    c->i++;
+    c->first = c->i <= 1;
+    if (resid32 <= c->delta * c->rmax) c->want_ru = 1;
+}
+
+// FP64 refresh done: stop on the true residual, or resume the recurrence
+// from the replaced residual (rho and beta of CG recomputed from it).
+__device__ __forceinline__ void ru_finish(Ctl* c, double rr64, double rho_new, bool cg_fused) {
+    const double resid = sqrt(rr64);
+    c->resid = resid;
+    if (conv_test(c, resid)) { c->done = 1; return; }
+    if (!isfinite(resid)) { c->status = MTL4G_ERR_NONFINITE; c->done = 1; return; }
+    if (c->limit || (c->init && c->i >= c->max_iter)) { c->error = 1; c->done = 1; return; }
+    if (c->pause || c->init) {
+        c->i++;
+        c->first = c->i <= 1;
+        c->pause = 0;
+        c->init = 0;
+    }
+    if (c->restart) {   // BiCGSTAB half step that FP64 did not confirm: restart p
+        c->first = 1;
+        c->restart = 0;
+        c->half = 0;
+    }
+    c->rmax = resid;
+    c->want_ru = 0;
+    c->ru_count++;
+    if (cg_fused) {
+        c->rho = rho_new;
+        c->beta = c->first ? 0.0 : (double)((float)rho_new / (float)c->rho_1);
+    }
+}
+
 // ---------------------------------------------------------------- CG epilogues
 // r = b - A*x; r.r (and r.(d.*r) for the diagonal PC)  (cg.hpp:92-94)
 template <typename VT, typename DT, int PCM>
@@ -82,7 +125,7 @@ struct EpiCgRho {
     static constexpr int NP = 1;
     const VT *r, *z;
     Ctl* c;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
     __device__ void operator()(int64_t i, VT* part) const { part[0] += r[i] * z[i]; }
     __device__ void finish(const VT* tot) const {
@@ -102,7 +145,7 @@ struct EpiCgP {
     Ctl* c;
     VT beta;
     int first;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {
         beta = static_cast<VT>(c->beta);
         first = c->first;
@@ -124,7 +167,7 @@ struct EpiCgQ {
     const VT* p;
     VT* q;
     Ctl* c;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
     __device__ void operator()(int row, VT s, VT* part) const {
         q[row] = s;
@@ -150,7 +193,7 @@ struct EpiCgXR {
     Ctl* c;
     double* hist;
     VT alpha;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
     __device__ void operator()(int64_t i, VT* part) const {
         x[i] = x[i] + alpha * p[i];
@@ -164,8 +207,9 @@ struct EpiCgXR {
     }
     __device__ void finish(const VT* tot) const {
         c->rr = (double)tot[0];
-        iter_finished(c, (double)std::sqrt(std::fabs(tot[0])), hist);
-        if (!c->done && PCM != PCM_GENERAL) {
+        if (c->mixed) mixed_check(c, (double)std::sqrt(std::fabs(tot[0])));
+        else iter_finished(c, (double)std::sqrt(std::fabs(tot[0])), hist);
+        if (!c->done && !c->pause && PCM != PCM_GENERAL) {
             const VT rho = PCM == PCM_DIAG ? tot[1] : tot[0];
             c->beta = (double)(rho / static_cast<VT>(c->rho_1));
             c->rho = (double)rho;
@@ -201,7 +245,7 @@ struct EpiBiRho {
     static constexpr int NP = 1;
     const VT *rt, *r;
     Ctl* c;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
     __device__ void operator()(int64_t i, VT* part) const { part[0] += rt[i] * r[i]; }
     __device__ void finish(const VT* tot) const {
@@ -225,7 +269,7 @@ struct EpiBiP {
     Ctl* c;
     VT beta, omega;
     int first;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {
         beta = static_cast<VT>(c->beta);
         omega = static_cast<VT>(c->omega);
@@ -244,7 +288,7 @@ struct EpiBiV {
     const VT* rt;
     VT* v;
     Ctl* c;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
     __device__ void operator()(int row, VT s, VT* part) const {
         v[row] = s;
@@ -266,7 +310,7 @@ struct EpiBiS {
     Ctl* c;
     double* hist;
     VT alpha;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
     __device__ void operator()(int64_t i, VT* part) const {
         const VT t = r[i] - alpha * v[i];
@@ -274,6 +318,16 @@ struct EpiBiS {
         part[0] += t * t;
     }
     __device__ void finish(const VT* tot) const {
+        if (c->mixed) {   // estimated stop: x += alpha*phat, then let FP64 decide
+            const double r = (double)std::sqrt(tot[0]);
+            if (!isfinite(r) || conv_test(c, r)) {
+                c->half = 1;
+                c->want_ru = 1;
+                c->pause = 1;
+                c->restart = 1;
+            }
+            return;
+        }
         const int i_before = c->i;
         iter_finished(c, (double)std::sqrt(tot[0]), hist);
         if (c->done) c->half = 1;
@@ -289,7 +343,7 @@ struct EpiBiHalf {
     VT* x;
     Ctl* c;
     VT alpha;
-    __device__ bool skip() const { return !(c->done && c->half == 1); }
+    __device__ bool skip() const { return c->half != 1; }
     __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
     __device__ void operator()(int64_t i, VT*) const { x[i] = x[i] + alpha * phat[i]; }
     __device__ void finish(const VT*) const { c->half = 2; }
@@ -302,7 +356,7 @@ struct EpiBiT {
     const VT* s;
     VT* t;
     Ctl* c;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
     __device__ void operator()(int row, VT v, VT* part) const {
         t[row] = v;
@@ -322,7 +376,7 @@ struct EpiBiXR {
     Ctl* c;
     double* hist;
     VT alpha, omega;
-    __device__ bool skip() const { return c->done != 0; }
+    __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {
         alpha = static_cast<VT>(c->alpha);
         omega = static_cast<VT>(c->omega);
@@ -335,7 +389,49 @@ struct EpiBiXR {
     }
     __device__ void finish(const VT* tot) const {
         c->rho_2 = c->rho_1;
-        iter_finished(c, (double)std::sqrt(tot[0]), hist);
+        if (c->mixed) mixed_check(c, (double)std::sqrt(tot[0]));
+        else iter_finished(c, (double)std::sqrt(tot[0]), hist);
+    }
+};
+
+// ---------------------------------------------------------------- FP64 refresh
+// x += double(x32); x32 = 0
+struct EpiRu1 {
+    static constexpr int NP = 0;
+    double* x;
+    float* x32;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0 || c->want_ru == 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double*) const {
+        x[i] = x[i] + (double)x32[i];
+        x32[i] = 0.0f;
+    }
+    __device__ void finish(const double*) const {}
+};
+
+// r = b - A*x in FP64; r32 = float(r); ||r||^2 and, for CG with a fused
+// preconditioner, rho = r32.(d .* r32) of the replaced residual
+template <typename DT, int PCM>
+struct EpiRu2 {
+    static constexpr int NP = 2;
+    const double* b;
+    float* r32;
+    const DT* d;
+    Ctl* c;
+    int cg;
+    __device__ bool skip() const { return c->done != 0 || c->want_ru == 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int row, double s, double* part) const {
+        const double v = b[row] - s;
+        const float f = (float)v;
+        r32[row] = f;
+        part[0] += v * v;
+        if (PCM == PCM_DIAG) part[1] += (double)(f * ((float)d[row] * f));
+        else if (PCM == PCM_NONE) part[1] += (double)(f * f);
+    }
+    __device__ void finish(const double* tot) const {
+        ru_finish(c, tot[0], tot[1], cg && PCM != PCM_GENERAL);
     }
 };
 
@@ -350,7 +446,11 @@ static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_p
         M4_HIP(hipMemset(W.vecs.p, 0, W.vecs.n * sizeof(VT)));
         W.n = n;
     }
-    if (!W.partials.p) M4_TRY(W.partials.alloc((size_t)2 * kMaxPartials));
+    if (!W.partials.p) M4_TRY(W.partials.alloc(kPartialWords));
+    if (!W.tickets.p) {
+        M4_TRY(W.tickets.alloc(kTicketWords));
+        M4_HIP(hipMemset(W.tickets.p, 0, kTicketWords * sizeof(unsigned)));
+    }
     if (!W.ctl.p) {
         M4_TRY(W.ctl.alloc(1));
         M4_HIP(hipMemset(W.ctl.p, 0, sizeof(Ctl)));
@@ -378,7 +478,7 @@ static int cg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s
     VT* y0 = z + n;
     Ctl* c = W.ctl.p;
     VT* part = reinterpret_cast<VT*>(W.partials.p);
-    unsigned* tk = &c->ticket[0];
+    unsigned* tk = W.tickets.p;
     if (PCM == PCM_GENERAL) {
         M4_TRY(pc_apply<VT>(P, r, z, y0, &c->done, W.pc_counters.p, s));
         M4_TRY(launch_vec<VT>(n, EpiCgRho<VT>{r, z, c}, part, tk, s));
@@ -396,7 +496,7 @@ static int cg_init(const mtl4g_matrix_s* A, const MT* val, const DT* d, Workspac
     VT* r = W.vecs.p + 2 * n;
     Ctl* c = W.ctl.p;
     return launch_csr<VT>(A, val, x, EpiCgInit<VT, DT, PCM>{b, r, d, c, W.hist.p},
-                          reinterpret_cast<VT*>(W.partials.p), &c->ticket[0], s);
+                          reinterpret_cast<VT*>(W.partials.p), W.tickets.p, s);
 }
 
 template <typename VT, typename MT>
@@ -407,7 +507,7 @@ static int bicg_init(const mtl4g_matrix_s* A, const MT* val, Workspace<VT>& W, c
     VT* rt = W.vecs.p + 7 * n;
     Ctl* c = W.ctl.p;
     return launch_csr<VT>(A, val, x, EpiBiInit<VT>{b, r, rt, c, W.hist.p},
-                          reinterpret_cast<VT*>(W.partials.p), &c->ticket[0], s);
+                          reinterpret_cast<VT*>(W.partials.p), W.tickets.p, s);
 }
 
 template <typename VT, typename MT>
@@ -424,7 +524,7 @@ static int bicg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc
     }
     Ctl* c = W.ctl.p;
     VT* part = reinterpret_cast<VT*>(W.partials.p);
-    unsigned* tk = &c->ticket[0];
+    unsigned* tk = W.tickets.p;
     M4_TRY(launch_vec<VT>(n, EpiBiRho<VT>{rt, r, c}, part, tk, s));
     M4_TRY(launch_vec<VT>(n, EpiBiP<VT>{r, v, p, c, VT(0), VT(0), 0}, part, tk, s));
     if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s));
@@ -556,6 +656,110 @@ template int krylov_run<double>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, i
                                 double*, const double*, mtl4g_iteration*, double*, int32_t, hipStream_t);
 template int krylov_run<float>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<float>&,
                                float*, const float*, mtl4g_iteration*, double*, int32_t, hipStream_t);
+
+// ---------------------------------------------------------------- mixed precision
+template <typename MT>
+static int mixed_typed(mtl4g_matrix_s* A, const MT* val32, const mtl4g_pc_s* P, int method,
+                       Workspace<float>& W, float* x32, double* x, const double* b, int32_t budget,
+                       hipStream_t s) {
+    const int64_t n = A->n;
+    const int pcm = (!P || P->kind == MTL4G_PC_IDENTITY) ? PCM_NONE
+                    : P->kind == MTL4G_PC_DIAGONAL          ? PCM_DIAG
+                                                             : PCM_GENERAL;
+    if (pcm == PCM_DIAG && P->fmt != MTL4G_F32) {
+        set_error("mixed precision needs a preconditioner stored in FP32");
+        return MTL4G_ERR_INVALID;
+    }
+    const float* d = pcm == PCM_DIAG ? P->inv_diag32.p : nullptr;
+    Ctl* c = W.ctl.p;
+    double* part = W.partials.p;
+    unsigned* tk = W.tickets.p;
+    float* r32 = W.vecs.p + (method == MTL4G_CG ? 2 : 6) * n;
+    const int cg = method == MTL4G_CG;
+    auto refresh = [&]() -> int {
+        M4_TRY(launch_vec<double>(n, EpiRu1{x, x32, c}, part, tk, s));
+        if (pcm == PCM_DIAG)
+            return launch_csr<double>(A, A->val.p, x, EpiRu2<float, PCM_DIAG>{b, r32, d, c, cg}, part, tk, s);
+        if (pcm == PCM_NONE)
+            return launch_csr<double>(A, A->val.p, x, EpiRu2<float, PCM_NONE>{b, r32, d, c, cg}, part, tk, s);
+        return launch_csr<double>(A, A->val.p, x, EpiRu2<float, PCM_GENERAL>{b, r32, d, c, cg}, part, tk, s);
+    };
+    auto iteration = [&]() -> int {
+        if (method == MTL4G_BICGSTAB) return bicg_iteration<float, MT>(A, val32, P, W, x32, s);
+        if (pcm == PCM_NONE) return cg_iteration<float, MT, float, PCM_NONE>(A, val32, P, d, W, x32, s);
+        if (pcm == PCM_DIAG) return cg_iteration<float, MT, float, PCM_DIAG>(A, val32, P, d, W, x32, s);
+        return cg_iteration<float, MT, float, PCM_GENERAL>(A, val32, P, d, W, x32, s);
+    };
+    M4_TRY(refresh());   // initial FP64 residual (Ctl::init set)
+    if (method == MTL4G_BICGSTAB) {   // rtilde = r
+        float* rt = W.vecs.p + 7 * n;
+        M4_HIP(hipMemcpyAsync(rt, r32, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    int32_t launched = 0, chunk = 2;
+    for (;;) {
+        M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        M4_HIP(hipStreamSynchronize(s));
+        if (W.h_ctl->done) break;
+        if (launched > budget) {
+            set_error("mixed solver did not stop after %d launched iterations", launched);
+            return MTL4G_ERR_INVALID;
+        }
+        for (int k = 0; k < chunk; k++) M4_TRY(iteration());
+        M4_TRY(refresh());
+        launched += chunk;
+        chunk = std::min(chunk * 2, 16);
+    }
+    return MTL4G_OK;
+}
+
+int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, double delta,
+              int32_t max_refresh, Workspace<float>& W, float* x32, double* x, const double* b,
+              mtl4g_iteration* it, int32_t* refreshes, hipStream_t s) {
+    (void)max_refresh;
+    if (A->n != A->m) return MTL4G_ERR_NOT_SQUARE;
+    if (P && P->n != A->n) return MTL4G_ERR_SIZE;
+    if (A->n == 0) {
+        it->resid = 0.0;
+        it->error = 0;
+        return 0;
+    }
+    const int nvec = method == MTL4G_CG ? 5 : 9;
+    M4_TRY(ensure_workspace(W, A->n, nvec, P, 0));
+    M4_TRY(ensure_lowp_values(A, fmt, s));
+    M4_HIP(hipMemsetAsync(x32, 0, (size_t)A->n * sizeof(float), s));
+    Ctl init;
+    std::memset(&init, 0, sizeof init);
+    init.norm_r0 = it->norm_r0;
+    init.rtol = it->rtol;
+    init.atol = it->atol;
+    init.max_iter = it->max_iter;
+    init.i = it->i;
+    init.mixed = 1;
+    init.init = 1;
+    init.want_ru = 1;
+    init.delta = delta;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
+    M4_HIP(hipGetLastError());
+    const int32_t budget = std::max(0, it->max_iter - it->i) + 64;
+    int st;
+    if (fmt == MTL4G_BF16) st = mixed_typed<uint16_t>(A, A->valbf.p, P, method, W, x32, x, b, budget, s);
+    else st = mixed_typed<float>(A, A->val32.p, P, method, W, x32, x, b, budget, s);
+    M4_TRY(st);
+    if (P) M4_TRY(pc_check_error(P, s));
+    M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    M4_HIP(hipStreamSynchronize(s));
+    const Ctl& c = *W.h_ctl;
+    it->i = c.i;
+    it->error = c.error;
+    it->resid = c.resid;
+    if (refreshes) *refreshes = c.ru_count;
+    if (c.status != 0) {
+        set_error(c.status == MTL4G_ERR_BREAKDOWN ? "unexpected orthogonality (breakdown)"
+                                                   : "non-finite residual");
+        return c.status;
+    }
+    return c.error;
+}
 
 }  // namespace mtl4g
 

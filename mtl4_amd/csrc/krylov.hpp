@@ -10,9 +10,11 @@ namespace mtl4g {
 struct Ctl {
     double rho, rho_1, rr, alpha, alpha_1, beta, omega, gamma, rho_2;
     double norm_r0, rtol, atol, resid;
-    double inner_scale;
+    double rmax, delta;      // mixed precision: residual at the last FP64 refresh, trigger
     int32_t i, max_iter, error, done, first, half, status;
     int32_t hist_n, hist_cap;
+    // mixed precision (reliable updates): FP32 recurrence, FP64 refreshes
+    int32_t mixed, pause, want_ru, ru_count, limit, init, restart;
     int32_t pad;
     unsigned ticket[4];
 };
@@ -22,7 +24,8 @@ template <typename VT>
 struct Workspace {
     int32_t n = 0;
     DevBuf<VT> vecs;          // method-specific vectors
-    DevBuf<double> partials;  // 2 x kMaxPartials
+    DevBuf<double> partials;  // kPartialWords
+    DevBuf<unsigned> tickets; // kTicketWords, zero between launches
     DevBuf<Ctl> ctl;
     DevBuf<double> hist;
     DevBuf<unsigned> pc_counters;
@@ -38,6 +41,15 @@ template <typename VT>
 int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int value_format,
                Workspace<VT>& W, VT* x, const VT* b, mtl4g_iteration* it, double* h_hist,
                int32_t hist_cap, hipStream_t s);
+
+// Mixed precision with reliable updates: one FP32 recurrence (values in
+// value_format), solution accumulated in FP64, the true residual b - A*x
+// recomputed in FP64 whenever the FP32 residual fell by delta since the last
+// refresh, and the stop decision taken on the FP64 residual.
+int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P32, int method, int value_format,
+              double delta, int32_t max_refresh, Workspace<float>& W, float* x32,
+              double* x, const double* b, mtl4g_iteration* it, int32_t* refreshes,
+              hipStream_t s);
 
 }  // namespace mtl4g
 

@@ -205,17 +205,18 @@ struct EpiAxpyUnary {
     __device__ void finish(const double* tot) const { *out = tot[0]; }
 };
 
-// Stream-ordered scratch for one reduction: partials, ticket and result.
+// Stream-ordered scratch for one reduction: tickets, result and partials.
 struct Scratch {
+    static constexpr size_t kTicketBytes = kTicketWords * sizeof(unsigned);
     char* base = nullptr;
     hipStream_t s = nullptr;
-    double* partials() { return reinterpret_cast<double*>(base + 256); }
     unsigned* ticket() { return reinterpret_cast<unsigned*>(base); }
-    double* result() { return reinterpret_cast<double*>(base + 64); }
+    double* result() { return reinterpret_cast<double*>(base + kTicketBytes); }
+    double* partials() { return reinterpret_cast<double*>(base + kTicketBytes + 256); }
     int get(hipStream_t st) {
         s = st;
-        M4_HIP(hipMallocAsync((void**)&base, 256 + sizeof(double) * 4 * kMaxPartials, st));
-        M4_HIP(hipMemsetAsync(base, 0, 256, st));
+        M4_HIP(hipMallocAsync((void**)&base, kTicketBytes + 256 + sizeof(double) * kPartialWords, st));
+        M4_HIP(hipMemsetAsync(base, 0, kTicketBytes + 256, st));
         return MTL4G_OK;
     }
     ~Scratch() {

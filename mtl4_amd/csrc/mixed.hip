@@ -21,7 +21,6 @@ namespace mtl4g {
 
 struct IrCtl {
     double rr;
-    unsigned ticket[4];
 };
 
 // r = b - A*x in FP64, r32 = float(r), ||r||^2
@@ -55,6 +54,7 @@ struct EpiIrCorrect {
 struct IrWorkspace {
     DevBuf<float> r32, d32;
     DevBuf<double> partials;
+    DevBuf<unsigned> tickets;
     DevBuf<IrCtl> ctl;
     Workspace<float> inner;
 };
@@ -73,12 +73,31 @@ extern "C" int mtl4g_ir_solve(mtl4g_matrix A, mtl4g_pc P32, double* x, const dou
         return MTL4G_ERR_INVALID;
     }
     if (P32 && P32->n != A->n) return MTL4G_ERR_SIZE;
+    if (prm->method != MTL4G_CG && prm->method != MTL4G_BICGSTAB) return MTL4G_ERR_INVALID;
     hipStream_t s = S(stream);
     const int32_t n = A->n;
+    if (prm->mode == MTL4G_IR_RELIABLE) {
+        Workspace<float> W32;
+        DevBuf<float> x32;
+        M4_TRY(x32.alloc(n));
+        int32_t refreshes = 0;
+        it->error = 0;
+        int st = mixed_run(A, P32, prm->method, prm->value_format, prm->inner_rtol,
+                           prm->max_outer, W32, x32.p, x, b, it, &refreshes, s);
+        if (stats) {
+            stats->outer = refreshes;
+            stats->inner_total = it->i;
+            stats->relres = it->norm_r0 != 0 ? it->resid / it->norm_r0 : it->resid;
+        }
+        return st;
+    }
+    if (prm->mode != MTL4G_IR_RESTART) return MTL4G_ERR_INVALID;
     IrWorkspace W;
     M4_TRY(W.r32.alloc(n));
     M4_TRY(W.d32.alloc(n));
-    M4_TRY(W.partials.alloc(kMaxPartials));
+    M4_TRY(W.partials.alloc(kPartialWords));
+    M4_TRY(W.tickets.alloc(kTicketWords));
+    M4_HIP(hipMemsetAsync(W.tickets.p, 0, kTicketWords * sizeof(unsigned), s));
     M4_TRY(W.ctl.alloc(1));
     M4_HIP(hipMemsetAsync(W.ctl.p, 0, sizeof(IrCtl), s));
     M4_TRY(ensure_lowp_values(A, prm->value_format, s));
@@ -88,7 +107,7 @@ extern "C" int mtl4g_ir_solve(mtl4g_matrix A, mtl4g_pc P32, double* x, const dou
     it->error = 0;
     for (;;) {
         M4_TRY(launch_csr<double>(A, A->val.p, x, EpiIrResid{b, nullptr, W.r32.p, W.ctl.p},
-                                  W.partials.p, &W.ctl.p->ticket[0], s));
+                                  W.partials.p, W.tickets.p, s));
         IrCtl h;
         M4_HIP(hipMemcpyAsync(&h, W.ctl.p, sizeof(IrCtl), hipMemcpyDeviceToHost, s));
         M4_HIP(hipStreamSynchronize(s));

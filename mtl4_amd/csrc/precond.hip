@@ -38,7 +38,8 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
                                               const int* skip) {
     if (skip && *skip) return;
     __shared__ int s_b, s_ok, s_final;
-    if (threadIdx.x == 0) s_b = (int)atomicAdd(&counters[0], 1u);
+    if (threadIdx.x == 0)
+        s_b = (int)__hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int b = s_b;
     const int L = S.blk_level[b];
@@ -56,8 +57,10 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
         s_ok = ok;
     }
     __syncthreads();
+    // Values of earlier levels are read with agent-scope loads and results
+    // are published with agent-scope atomics, so no L2 write-back fences are
+    // needed between workgroups of this kernel.
     if (s_ok) {
-        __threadfence();
         const int k = S.lev_ptr[L] + (b - S.lev_blk[L]) * BS + (int)threadIdx.x;
         if (k < S.lev_ptr[L + 1]) {
             const int r = S.perm[k];
@@ -65,19 +68,19 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
             for (int j = S.rowptr[k]; j < S.rowptr[k + 1]; j++)
                 rr -= static_cast<VT>(S.val[j]) * ld_agent(w + S.col[j]);
             if (!S.unit) rr = rr * static_cast<VT>(S.dia[k]);
-            st_agent(w + r, rr);
+            publish(w + r, rr);
         }
     }
-    __threadfence();
+    wait_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned need = (unsigned)(S.lev_blk[L + 1] - S.lev_blk[L]);
-        const unsigned done = atomicAdd(&counters[2 + L], 1u) + 1u;
+        const unsigned done =
+            __hip_atomic_fetch_add(&counters[2 + L], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         s_final = (L == S.nlev - 1 && done == need);
     }
     __syncthreads();
     if (s_final) {   // every other workgroup has published: reset for the next launch
-        __threadfence();
         if (threadIdx.x == 0 && ld_agent(&counters[1]) != 0u) atomicOr(err, 1);
         for (int i = threadIdx.x; i < S.nlev + 2; i += BS) st_agent(&counters[i], 0u);
     }

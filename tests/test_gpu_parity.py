@@ -309,7 +309,10 @@ def test_bicgstab_families(m, fam, kind):
     it = m.basic_iteration(dv(m, b), 2000, 1e-10)
     err = m.bicgstab(A, x, dv(m, b), P, it)
     assert err == st == 0
-    assert abs(it.iterations() - ito.i) <= 2, (it.iterations(), ito.i)
+    # BiCGSTAB's residual is not monotone, so the tree-ordered dots (MTL4
+    # uses 4 unrolled partial sums) can move the step at which it first
+    # crosses the tolerance by a few iterations; bound: 5% + 2
+    assert abs(it.iterations() - ito.i) <= 2 + 0.05 * ito.i, (it.iterations(), ito.i)
     # both solutions carry the solve's own error (about cond(A)*rtol): they
     # must agree to within that error, measured on the oracle against xt
     own = max(np.linalg.norm(xo - xt), 1e-12 * np.linalg.norm(xt))
@@ -326,33 +329,84 @@ def test_bicgstab_breakdown(m):
 
 
 # ---------------------------------------------------------------- mixed precision
+# The bar: the FP64 residual ||b - A x|| / ||b|| computed by the oracle is
+# <= 1e-10 (the stop is taken on the device's FP64 residual), and the
+# solution agrees with the oracle's FP64 solve to within the accuracy that
+# residual allows (cond(A) * 1e-10).
+def _fp64_reference(O, b, kind):
+    xo, st, _, _ = orc.cg(O, b, orc.Preconditioner(kind, O), max_iter=20000, rtol=1e-13)
+    return xo
+
+
+@pytest.mark.parametrize("mode", ["reliable", "restart"])
 @pytest.mark.parametrize("fmt", ["f32", "bf16"])
-def test_ir_cg_reaches_fp64_accuracy(m, fmt):
-    rp, ci, va = family("diff3d_16", seed=1)
+@pytest.mark.parametrize("kind", ["diagonal", "ic_0", "identity"])
+def test_mixed_cg_reaches_fp64_accuracy(m, mode, fmt, kind):
+    if fmt == "bf16" and kind == "ic_0":
+        pytest.skip("ic_0 factors are stored in FP32; the BF16 variant covers the matrix values")
+    # BF16 values perturb A by ~2^-9 relative; refinement contracts only while
+    # cond(A) * 2^-9 < 1, so the BF16 variant is checked on the well-conditioned
+    # family (the diffusion matrices with kappa contrast 1e4 do not qualify)
+    fam = "diff3d_16" if fmt == "f32" else "spd_2000"
+    rp, ci, va = family(fam, seed=1)
     O, A = pair(rp, ci, va)
     xt = np.random.default_rng(1).uniform(-1, 1, A.num_rows())
     b = orc.spmv(O, xt)
-    P32 = m.pc.diagonal(A, m.F32)
+    P32 = getattr(m.pc, kind)(A, m.F32)
     x = m.dense_vector(A.num_rows())
-    it = m.basic_iteration(dv(m, b), 100, 1e-10)
+    maxit = 20000 if mode == "reliable" else 200
+    it = m.basic_iteration(dv(m, b), maxit, 1e-10)
+    inner = None if mode == "reliable" else (1e-4 if fmt == "f32" else 1e-2)
     err, st = m.ir_solve(A, x, dv(m, b), it, P32, value_format=m.F32 if fmt == "f32" else m.BF16,
-                         inner_rtol=1e-4 if fmt == "f32" else 1e-2)
+                         mode=mode, inner_rtol=inner)
     assert err == 0
     res = np.linalg.norm(b - orc.spmv(O, x.numpy())) / np.linalg.norm(b)
-    assert res <= 1.5e-10, res
-    xo, _, _, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=5000, rtol=1e-12)
+    assert res <= 1.0e-10 * 1.01, res
+    xo = _fp64_reference(O, b, "diagonal")
     assert np.linalg.norm(x.numpy() - xo) <= 1e-6 * np.linalg.norm(xo)
+    if mode == "reliable" and fmt == "f32":
+        # one FP32 recurrence: close to the FP64 iteration count
+        xf, _, itf, _ = orc.cg(O, b, None if kind == "identity" else orc.Preconditioner(kind, O),
+                               max_iter=20000, rtol=1e-10)
+        assert st["inner_total"] <= 1.5 * itf.i + 20, (st, itf.i)
 
 
-def test_ir_bicgstab_ilu(m):
+@pytest.mark.parametrize("mode", ["reliable", "restart"])
+def test_mixed_bicgstab_ilu(m, mode):
     rp, ci, va = family("convdiff_40", seed=1)
     O, A = pair(rp, ci, va)
     xt = np.random.default_rng(1).uniform(-1, 1, A.num_rows())
     b = orc.spmv(O, xt)
     P32 = m.pc.ilu_0(A, m.F32)
     x = m.dense_vector(A.num_rows())
-    it = m.basic_iteration(dv(m, b), 100, 1e-10)
-    err, st = m.ir_solve(A, x, dv(m, b), it, P32, method=m.BICGSTAB, inner_rtol=1e-4)
+    it = m.basic_iteration(dv(m, b), 5000 if mode == "reliable" else 100, 1e-10)
+    err, st = m.ir_solve(A, x, dv(m, b), it, P32, method=m.BICGSTAB, mode=mode,
+                         inner_rtol=None if mode == "reliable" else 1e-4)
     assert err == 0
     res = np.linalg.norm(b - orc.spmv(O, x.numpy())) / np.linalg.norm(b)
-    assert res <= 1.5e-10
+    assert res <= 1.0e-10 * 1.01, res
+
+
+def test_mixed_bf16_reports_stagnation(m):
+    # ill-conditioned case: the BF16 operator cannot drive the FP64 residual
+    # to 1e-10; the solve must stop with an error, not loop
+    rp, ci, va = family("diff3d_16", seed=1)
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.random.default_rng(1).uniform(-1, 1, A.num_rows()))
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+    try:
+        err, st = m.ir_solve(A, x, dv(m, b), it, m.pc.diagonal(A, m.F32), value_format=m.BF16)
+    except m.nonfinite_error:
+        err = -6
+    assert err != 0 or it.resid() <= 1e-10 * np.linalg.norm(b)
+
+
+def test_mixed_iteration_limit(m):
+    rp, ci, va = family("diff3d_16", seed=1)
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.ones(A.num_rows()))
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 5, 1e-10)
+    err, st = m.ir_solve(A, x, dv(m, b), it, m.pc.diagonal(A, m.F32))
+    assert err == 1 and it.iterations() == 5
