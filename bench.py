@@ -102,6 +102,7 @@ def main():
     rp, ci, va = gen.diffusion3d(g, seed=0)
     n, nnz = len(rp) - 1, len(ci)
     A = m.compressed2D(n, n, rp, ci, va)
+    A.autotune()                          # CSR kernel and shape for this matrix
     xt = m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
     b = A * xt
     bnorm = m.two_norm(b)
@@ -111,14 +112,17 @@ def main():
     t_spmv = timed_spmv(m, torch, A, xt, y, args.spmv_reps)
     by = spmv_bytes(n, nnz)
     spmv_gbs = by / t_spmv / 1e9
+    kern = A.get_kernel()
     rr = gen.random_sparse(2_000_000, seed=0)
     R = m.compressed2D(2_000_000, 2_000_000, *rr)
+    R.autotune()
     xr = m.dense_vector(np.random.default_rng(2).uniform(-1.0, 1.0, 2_000_000))
     yr = m.dense_vector(2_000_000)
     t_rnd = timed_spmv(m, torch, R, xr, yr, args.spmv_reps)
     rnd_gbs = spmv_bytes(2_000_000, len(rr[1])) / t_rnd / 1e9
     traffic = load_traffic()
-    roofline = {"bound": "hbm", "kernel": "k_csr<EpiAssign> (y = A*x, FP64, diffusion3d 160^3)",
+    kname = ("k_csrv<%d lanes/row>" % kern[1]) if kern[0] == m._lib.SPMV_VECTOR else "k_csr (LDS tiles)"
+    roofline = {"bound": "hbm", "kernel": "%s<EpiAssign> (y = A*x, FP64, diffusion3d %d^3)" % (kname, g),
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": by, "avg_launch_us": round(t_spmv * 1e6, 2)}

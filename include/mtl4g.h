@@ -24,6 +24,7 @@
 #ifndef MTL4G_H
 #define MTL4G_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -94,6 +95,15 @@ int mtl4g_version(void);
 /* Sets the device used by later calls on this thread. */
 int mtl4g_set_device(int device);
 
+/* Device storage for dense_vector (vector/dense_vector.hpp:50-269 keeps its
+ * elements in a contiguous_memory_block; here the block lives in HBM).
+ * kind: 0 host->device, 1 device->host, 2 device->device. */
+int mtl4g_malloc(void** d_ptr, size_t bytes);
+int mtl4g_free(void* d_ptr);
+int mtl4g_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int mtl4g_memset(void* d_ptr, int value, size_t bytes, void* stream);
+int mtl4g_stream_synchronize(void* stream);
+
 /* ---- compressed2D --------------------------------------------------- */
 /* mtl::compressed2D<double>(m, n, nnz, Major, Minor, Entries): copies the
  * arrays (matrix/compressed2D.hpp:355-364).  int32 row pointers and column
@@ -108,9 +118,18 @@ int mtl4g_csr_info(mtl4g_matrix A, int32_t* nrows, int32_t* ncols, int64_t* nnz)
 /* Overwrite the values (same pattern), e.g. after an inserter with
  * update_store on existing entries (matrix/inserter.hpp:30-48). */
 int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
-/* Row-block partition used by the SpMV kernels: at most rows_per_block
- * rows and nnz_per_block non-zeros per workgroup (0 = library default). */
+/* CSR kernel choice.  MTL4G_SPMV_VECTOR: `group` lanes (4..64) per row,
+ * `rows_per_block` rows per workgroup; MTL4G_SPMV_TILE: row blocks of at
+ * most rows_per_block rows and nnz_per_block non-zeros staged through LDS.
+ * Both sum every row in column order.  0 = library default. */
+#define MTL4G_SPMV_TILE   0
+#define MTL4G_SPMV_VECTOR 1
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
+int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
+int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);
+/* Autotuner: times y = A*x for the candidate kernels and shapes on this
+ * matrix and keeps the fastest; h_best_us (may be NULL) gets its time. */
+int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream);
 
 /* w (=,+=,-=) A*v: smat_cvec_mult, operation/mat_vec_mult.hpp:613-683,
  * dispatched from vector/crtp_base_vector.hpp:93-104,191-200,268-277.
@@ -210,6 +229,59 @@ typedef struct {
 int mtl4g_ir_solve(mtl4g_matrix A, mtl4g_pc P32, double* d_x, const double* d_b,
                    mtl4g_iteration* it, const mtl4g_ir_params* prm, mtl4g_ir_stats* st,
                    void* stream);
+
+/* ---- batched solves of many independent systems (no MTL4 counterpart) -- */
+/* itl::cg with pc::identity or pc::diagonal (cg.hpp:38-111) run for many
+ * systems by one persistent launch: each work item is solved entirely by
+ * one workgroup, so results do not depend on the batch they ran in.
+ * Precision MTL4G_F64: FP64 as MTL4; MTL4G_F32 / MTL4G_BF16: FP32 recurrence
+ * with FP64 reliable updates (FP64 stop, see mtl4g_ir_solve). */
+typedef struct mtl4g_batch_s* mtl4g_batch;
+
+/* One work item: system sys, right-hand side at d_b + b_off, start vector and
+ * result at d_x + x_off, workspace at d_work + ws_off (doubles,
+ * mtl4g_batch_workspace_elems of the system size). */
+typedef struct {
+    int32_t sys;
+    int32_t max_iter;
+    int64_t b_off, x_off, ws_off;
+    double rtol, atol;
+} mtl4g_batch_item;
+
+/* basic_iteration's state after an item: iterations, error (0/1), status
+ * (< 0: non-finite), FP64 refreshes (mixed precision), resid, norm_r0 = ||b||. */
+typedef struct {
+    int32_t iterations, error, status, refreshes;
+    double resid, norm_r0;
+} mtl4g_batch_result;
+
+/* Systems in one block-diagonal CSR on the host: system k owns rows
+ * [sys_rowptr[k], sys_rowptr[k+1]) and only columns in that range (absolute
+ * column indices, sorted rows).  The arrays are copied to the device. */
+int mtl4g_batch_create(mtl4g_batch* B, int32_t nsys, const int32_t* sys_rowptr,
+                       const int32_t* rowptr, const int32_t* col, const double* val, int where);
+int mtl4g_batch_destroy(mtl4g_batch B);
+int mtl4g_batch_info(mtl4g_batch B, int32_t* nsys, int32_t* nrows, int64_t* nnz);
+int64_t mtl4g_batch_workspace_elems(int32_t n, int precision);
+/* Launch shape (threads per workgroup 256/512/1024, resident workgroups per CU). */
+int mtl4g_batch_set_shape(mtl4g_batch B, int precision, int32_t threads, int32_t per_cu);
+int mtl4g_batch_get_shape(mtl4g_batch B, int precision, int32_t* threads, int32_t* per_cu);
+/* Every system once: x and b laid out like the rows (d_x + sys_rowptr[k]);
+ * systems run largest first; h_results (may be NULL) indexed by system. */
+int mtl4g_batch_solve(mtl4g_batch B, int pc_kind, int precision, double* d_x, const double* d_b,
+                      double rtol, double atol, int32_t max_iter, mtl4g_batch_result* h_results,
+                      void* stream);
+/* Autotuner: times the whole batch for every launch shape (x reset from
+ * d_x0 each time) and keeps the fastest for this precision. */
+int mtl4g_batch_tune(mtl4g_batch B, int pc_kind, int precision, double* d_x, const double* d_x0,
+                     const double* d_b, double rtol, int32_t max_iter, double* h_best_ms,
+                     void* stream);
+/* A list of work items already on the device (d_queue: one zero-able word);
+ * asynchronous on the stream.  Used by continuous batching. */
+int mtl4g_batch_solve_items(mtl4g_batch B, int pc_kind, int precision,
+                            const mtl4g_batch_item* d_items, int32_t nitems, const double* d_b,
+                            double* d_x, double* d_work, mtl4g_batch_result* d_results,
+                            unsigned* d_queue, void* stream);
 
 #ifdef __cplusplus
 }

@@ -137,7 +137,22 @@ class compressed2D:
         check(_L().mtl4g_csr_set_values(self.h, self.val.ctypes.data_as(C.c_void_p), 0))
 
     def tune(self, rows_per_block=0, nnz_per_block=0):
+        """LDS-tile CSR kernel with this row-block partition."""
         check(_L().mtl4g_csr_tune(self.h, rows_per_block, nnz_per_block), "tune")
+
+    def set_kernel(self, kernel, group=0, rows_per_block=0):
+        check(_L().mtl4g_csr_set_kernel(self.h, kernel, group, rows_per_block), "set_kernel")
+
+    def get_kernel(self):
+        k, g, r = C.c_int(), C.c_int32(), C.c_int32()
+        check(_L().mtl4g_csr_get_kernel(self.h, C.byref(k), C.byref(g), C.byref(r)))
+        return k.value, g.value, r.value
+
+    def autotune(self):
+        """Time the CSR kernel candidates on this matrix, keep the fastest (us)."""
+        us = C.c_double()
+        check(_L().mtl4g_csr_autotune(self.h, C.byref(us), _stream()), "autotune")
+        return us.value
 
     def mult(self, v, w, assign=ASSIGN):
         """w (=,+=,-=) A*v  (operation/mat_vec_mult.hpp:613-683)."""
@@ -457,3 +472,64 @@ def ir_solve(A, x, b, it, P32=None, method=CG, value_format=F32, mode="reliable"
     check(status, "ir_solve")
     return it.error, {"outer": stats.outer, "inner_total": stats.inner_total,
                       "relres": stats.relres}
+
+
+class Batch:
+    """Many independent systems, solved by one persistent launch.
+
+    Batch(systems) with systems a list of (rowptr, col, val) CSR triples (local
+    column indices); kept in HBM as one block-diagonal matrix."""
+
+    def __init__(self, systems):
+        sys_row = [0]
+        rps, cols, vals = [], [], []
+        nnz = 0
+        for (rp, ci, va) in systems:
+            rp = np.asarray(rp, np.int64)
+            off = sys_row[-1]
+            rps.append(rp[:-1] + nnz)
+            cols.append(np.asarray(ci, np.int64) + off)
+            vals.append(np.asarray(va, np.float64))
+            nnz += int(rp[-1])
+            sys_row.append(off + len(rp) - 1)
+        self.sys_row = np.asarray(sys_row, np.int32)
+        self.rowptr = np.concatenate(rps + [np.array([nnz])]).astype(np.int32)
+        self.col = np.concatenate(cols).astype(np.int32)
+        self.val = np.concatenate(vals)
+        self.nsys = len(systems)
+        self.nrows = int(self.sys_row[-1])
+        h = C.c_void_p()
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        check(_L().mtl4g_batch_create(C.byref(h), self.nsys, p(self.sys_row), p(self.rowptr),
+                                      p(self.col), p(self.val), 0), "batch_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.mtl4g_batch_destroy(h)
+            self.h = None
+
+    def solve(self, x, b, pc_kind=_lib.PC_DIAGONAL, precision=F64, rtol=1e-10, atol=0.0,
+              max_iter=100000):
+        """Every system once; x, b device vectors laid out like the rows.
+        Returns a list of result dicts, one per system."""
+        res = (_lib.BatchResult * self.nsys)()
+        check(_L().mtl4g_batch_solve(self.h, pc_kind, precision, x.ptr, b.ptr, rtol, atol,
+                                     max_iter, res, _stream()), "batch_solve")
+        return [{"iterations": r.iterations, "error": r.error, "status": r.status,
+                 "refreshes": r.refreshes, "resid": r.resid, "norm_r0": r.norm_r0} for r in res]
+
+    def tune(self, x, x0, b, pc_kind=_lib.PC_DIAGONAL, precision=F64, rtol=1e-10, max_iter=100000):
+        ms = C.c_double()
+        check(_L().mtl4g_batch_tune(self.h, pc_kind, precision, x.ptr, x0.ptr, b.ptr, rtol,
+                                    max_iter, C.byref(ms), _stream()), "batch_tune")
+        return ms.value, self.shape(precision)
+
+    def shape(self, precision=F64):
+        t, pcu = C.c_int32(), C.c_int32()
+        check(_L().mtl4g_batch_get_shape(self.h, precision, C.byref(t), C.byref(pcu)))
+        return t.value, pcu.value
+
+    def set_shape(self, precision, threads, per_cu):
+        check(_L().mtl4g_batch_set_shape(self.h, precision, threads, per_cu), "set_shape")

@@ -38,6 +38,20 @@ class IRStats(C.Structure):
     _fields_ = [("outer", C.c_int32), ("inner_total", C.c_int32), ("relres", C.c_double)]
 
 
+SPMV_TILE, SPMV_VECTOR = 0, 1
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("sys", C.c_int32), ("max_iter", C.c_int32), ("b_off", C.c_int64),
+                ("x_off", C.c_int64), ("ws_off", C.c_int64), ("rtol", C.c_double),
+                ("atol", C.c_double)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("error", C.c_int32), ("status", C.c_int32),
+                ("refreshes", C.c_int32), ("resid", C.c_double), ("norm_r0", C.c_double)]
+
+
 def declared_functions():
     """Names of every function include/mtl4g.h declares."""
     text = open(HEADER).read()
@@ -78,6 +92,23 @@ _SIGS = {
     "mtl4g_solver_solve": (_INT, [_P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_ir_solve": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), C.POINTER(IRParams),
                               C.POINTER(IRStats), _P]),
+    "mtl4g_malloc": (_INT, [_PP, C.c_size_t]),
+    "mtl4g_free": (_INT, [_P]),
+    "mtl4g_memcpy": (_INT, [_P, _P, C.c_size_t, _INT, _P]),
+    "mtl4g_memset": (_INT, [_P, _INT, C.c_size_t, _P]),
+    "mtl4g_stream_synchronize": (_INT, [_P]),
+    "mtl4g_csr_set_kernel": (_INT, [_P, _INT, _I32, _I32]),
+    "mtl4g_csr_get_kernel": (_INT, [_P, C.POINTER(_INT), C.POINTER(_I32), C.POINTER(_I32)]),
+    "mtl4g_csr_autotune": (_INT, [_P, C.POINTER(_D), _P]),
+    "mtl4g_batch_create": (_INT, [_PP, _I32, _P, _P, _P, _P, _INT]),
+    "mtl4g_batch_destroy": (_INT, [_P]),
+    "mtl4g_batch_info": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64)]),
+    "mtl4g_batch_workspace_elems": (_I64, [_I32, _INT]),
+    "mtl4g_batch_set_shape": (_INT, [_P, _INT, _I32, _I32]),
+    "mtl4g_batch_get_shape": (_INT, [_P, _INT, C.POINTER(_I32), C.POINTER(_I32)]),
+    "mtl4g_batch_solve": (_INT, [_P, _INT, _INT, _P, _P, _D, _D, _I32, C.POINTER(BatchResult), _P]),
+    "mtl4g_batch_tune": (_INT, [_P, _INT, _INT, _P, _P, _P, _D, _I32, C.POINTER(_D), _P]),
+    "mtl4g_batch_solve_items": (_INT, [_P, _INT, _INT, _P, _I32, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -1,0 +1,57 @@
+// batched.hpp -- many independent systems solved by one persistent launch.
+#pragma once
+
+#include "internal.hpp"
+
+namespace mtl4g {
+
+struct BatchDev {
+    const int32_t* rowptr;    // block-diagonal CSR over all systems
+    const int32_t* col;       // absolute column indices
+    const double* val;
+    const float* val32;
+    const uint16_t* valbf;
+    const double* dinv;       // 1 / A[i][i]
+    const float* dinv32;
+    const int32_t* sys_row;   // first row of each system, nsys+1
+};
+
+// Launch shape picked by the autotuner.
+struct BatchShape {
+    int bs = 256;            // threads per workgroup
+    int per_cu = 2;          // resident workgroups per CU
+};
+
+}  // namespace mtl4g
+
+struct mtl4g_batch_s {
+    int device = 0;
+    int32_t nsys = 0, nrows = 0;
+    int64_t nnz = 0;
+    int32_t max_n = 0;
+    std::vector<int32_t> h_sys_row, h_rowptr;
+    std::vector<double> h_val, h_diag;
+    mtl4g::DevBuf<int32_t> sys_row, rowptr, col;
+    mtl4g::DevBuf<double> val, dinv;
+    mtl4g::DevBuf<float> val32, dinv32;
+    mtl4g::DevBuf<uint16_t> valbf;
+    // solve-all scratch (items in cost order, workspace, queue)
+    mtl4g::DevBuf<mtl4g_batch_item> items;
+    mtl4g::DevBuf<mtl4g_batch_result> results;
+    mtl4g::DevBuf<double> work;
+    mtl4g::DevBuf<unsigned> queue;
+    std::vector<int32_t> order;   // systems by decreasing estimated cost
+    mtl4g::BatchShape shape[3];   // per precision (F64, F32, BF16)
+    bool tuned[3] = {false, false, false};
+    int cus = 0;
+};
+
+namespace mtl4g {
+// Workspace elements (doubles) one item of system size n needs.
+int64_t batch_ws_elems(int32_t n, int precision);
+// Launch the persistent kernel over `nitems` items (queue is reset inside).
+int batch_launch(mtl4g_batch_s* B, int pc_kind, int precision, const BatchShape& sh,
+                 const mtl4g_batch_item* d_items, int nitems, const double* d_b, double* d_x,
+                 double* d_work, mtl4g_batch_result* d_out, unsigned* d_queue, hipStream_t s);
+int batch_ensure_values(mtl4g_batch_s* B, int precision);
+}  // namespace mtl4g

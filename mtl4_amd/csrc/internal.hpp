@@ -193,26 +193,35 @@ __device__ __forceinline__ int xcd_logical_block(int g, int per_xcd) {
 }
 
 // ----------------------------------------------------------------- CSR core
+// Row block descriptor: rows [r0, r1), non-zeros [j0, j1).
+struct RowBlock {
+    int32_t r0, r1, j0, j1;
+};
+
 // One workgroup, rows [r0, r1) of a row block whose non-zeros fit the LDS
 // tile: all lanes stream val/col coalesced and form the products into LDS,
 // then lane t sums row r0+t in column order (the order of
 // mat_vec_mult.hpp:656-682, so FP64 results match MTL4 bit for bit).
 // Long single rows (nnz > TILE) are summed by the whole workgroup.
-template <int BS, int TILE, typename AT, typename MT, typename XT>
+// The row bounds and the epilogue's per-row operands (epi.preload) are
+// requested together with the value/index stream, so a workgroup waits on
+// three dependent memory round trips: descriptor, stream, gathers.
+template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
 __device__ __forceinline__ AT csr_block_rows(const int32_t* __restrict__ rowptr,
                                              const int32_t* __restrict__ col,
                                              const MT* __restrict__ val,
-                                             const XT* __restrict__ x, int r0, int r1,
-                                             AT* prod, AT* sh, int& row) {
+                                             const XT* __restrict__ x, const RowBlock& d,
+                                             AT* prod, AT* sh, int& row, const Epi& epi,
+                                             typename Epi::Pre& pre) {
     const int t = threadIdx.x;
-    row = r0 + t;
-    const int j0 = rowptr[r0];
-    const int j1 = rowptr[r1];
-    const int nz = j1 - j0;
+    row = d.r0 + t;
+    const int j0 = d.j0;
+    const int nz = d.j1 - d.j0;
     int a = 0, b = 0;
-    if (row < r1) {
+    if (row < d.r1) {
         a = rowptr[row] - j0;
         b = rowptr[row + 1] - j0;
+        pre = epi.preload(row);
     }
     if (nz > TILE) {  // one long row: strided partial sums, fixed-order combine
         AT s = 0;
@@ -256,8 +265,11 @@ struct mtl4g_matrix_s {
     mtl4g::DevBuf<float> val32;       // FP32 copy (created on demand)
     mtl4g::DevBuf<uint16_t> valbf;    // BF16 copy (created on demand)
     mtl4g::DevBuf<double> diag;       // A[i][i] (0 where absent)
-    mtl4g::DevBuf<int32_t> blk;       // row-block partition
-    int nb = 0;                       // number of row blocks
+    mtl4g::DevBuf<mtl4g::RowBlock> desc;   // row-block partition (LDS tile kernel)
+    int nb = 0;                            // number of row blocks
+    int kernel = MTL4G_SPMV_VECTOR;        // CSR kernel: vector groups or LDS tiles
+    int gs = 8;                            // vector kernel: lanes per row
+    int vrows = 256;                       // vector kernel: rows per workgroup
     int rows_per_block = 0, nnz_per_block = 0;
     int32_t max_row = 0;
     std::vector<int32_t> h_rowptr, h_col;   // host pattern (factorizations)

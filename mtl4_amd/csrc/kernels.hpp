@@ -10,7 +10,10 @@
 //   static constexpr int NP;                       // partial sums produced
 //   bool skip() const;                             // uniform early exit
 //   void prepare();                                // load scalars once per thread
-//   void operator()(int row_or_i, [AT s,] AT* part) const;
+//   k_csr:  struct Pre; Pre preload(int row) const; // per-row operands, fetched
+//           void operator()(int row, AT s, const Pre&, AT* part) const;
+//                                                   // with the matrix stream
+//   k_vec:  void operator()(int64_t i, AT* part) const;
 //   void finish(const AT* totals) const;           // last workgroup, lane 0
 // Partial sums are combined in block-index order by the last workgroup to
 // arrive, so every reduction is deterministic and needs no memset.
@@ -66,22 +69,78 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
                                             const int32_t* __restrict__ col,
                                             const MT* __restrict__ val,
                                             const XT* __restrict__ x,
-                                            const int32_t* __restrict__ blk, int nb, int per_xcd,
+                                            const RowBlock* __restrict__ desc, int nb, int per_xcd,
                                             Epi epi, AT* partials, unsigned* ticket) {
+    const int b = xcd_logical_block(blockIdx.x, per_xcd);
+    const RowBlock d = desc[b < nb ? b : nb - 1];   // requested with the stop flag
     if (epi.skip()) return;
     epi.prepare();
     __shared__ AT prod[TILE];
     __shared__ AT sh[BS / kWave];
     constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
-    const int b = xcd_logical_block(blockIdx.x, per_xcd);
     AT part[NP];
 #pragma unroll
     for (int k = 0; k < NP; k++) part[k] = AT(0);
     if (b < nb) {
-        const int r0 = blk[b], r1 = blk[b + 1];
         int row;
-        AT s = csr_block_rows<BS, TILE, AT>(rowptr, col, val, x, r0, r1, prod, sh, row);
-        if (row < r1) epi(row, s, part);
+        typename Epi::Pre pre{};
+        AT s = csr_block_rows<BS, TILE, AT>(rowptr, col, val, x, d, prod, sh, row, epi, pre);
+        if (row < d.r1) epi(row, s, pre, part);
+    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
+}
+
+// Vector CSR: a group of GS lanes per row, 64/GS rows per wavefront pass.
+// The lanes of a group read consecutive non-zeros of their row, so a
+// wavefront's value/index loads cover consecutive rows contiguously; no LDS,
+// no workgroup barrier.  The row sum keeps MTL4's order: the group's
+// products are added one after the other in column order, each taken from
+// its lane with a shuffle (mat_vec_mult.hpp:656-682, bit for bit).
+template <int GS, int BS, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_csrv(const int32_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ col,
+                                             const MT* __restrict__ val,
+                                             const XT* __restrict__ x, int nrows, int rows_per_block,
+                                             int nb, int per_xcd, Epi epi, AT* partials,
+                                             unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    __shared__ AT sh[BS / kWave];
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    constexpr int GPW = kWave / GS;       // row groups per wavefront
+    constexpr int WPB = BS / kWave;       // wavefronts per workgroup
+    const int b = xcd_logical_block(blockIdx.x, per_xcd);
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int g = lane / GS, l = lane % GS;
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+    if (b < nb) {
+        const int row_begin = b * rows_per_block;
+        const int row_end = min(nrows, row_begin + rows_per_block);
+        for (int rb = row_begin + wave * GPW; rb < row_end; rb += WPB * GPW) {
+            const int r = rb + g;
+            const bool valid = r < row_end;
+            int a = 0, e = 0;
+            typename Epi::Pre pre{};
+            if (valid) {
+                a = rowptr[r];
+                e = rowptr[r + 1];
+                if (l == 0) pre = epi.preload(r);
+            }
+            AT s = AT(0);
+            for (int base = a; __any(base < e); base += GS) {
+                const int j = base + l;
+                const AT pj = j < e ? load_val_nt<AT>(val, j) * static_cast<AT>(x[__builtin_nontemporal_load(col + j)])
+                                    : AT(0);
+#pragma unroll
+                for (int k = 0; k < GS; k++) {
+                    const AT v = __shfl(pj, g * GS + k, kWave);
+                    if (base + k < e) s += v;
+                }
+            }
+            if (valid && l == 0) epi(r, s, pre, part);
+        }
     }
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
@@ -111,6 +170,30 @@ __global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, un
 template <typename AT, typename MT, typename XT, typename Epi>
 int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& epi,
                AT* partials, unsigned* ticket, hipStream_t s) {
+    if (A->n == 0) return MTL4G_OK;
+    if (A->kernel == MTL4G_SPMV_VECTOR) {
+        const int rpb = A->vrows;
+        const int nbv = (A->n + rpb - 1) / rpb;
+        const int per = (nbv + kXcd - 1) / kXcd;
+        const int grid = per * kXcd;
+        if (Epi::NP > 0 && grid > kMaxPartials) {
+            set_error("matrix too large for one reduction pass (%d blocks)", grid);
+            return MTL4G_ERR_UNSUPPORTED;
+        }
+#define M4_LAUNCHV(G)                                                                          \
+    hipLaunchKernelGGL((k_csrv<G, 256, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,         \
+                       A->rowptr.p, A->col.p, val, x, A->n, rpb, nbv, per, epi, partials, ticket)
+        switch (A->gs) {
+        case 4: M4_LAUNCHV(4); break;
+        case 8: M4_LAUNCHV(8); break;
+        case 16: M4_LAUNCHV(16); break;
+        case 32: M4_LAUNCHV(32); break;
+        default: M4_LAUNCHV(64); break;
+        }
+#undef M4_LAUNCHV
+        M4_HIP(hipGetLastError());
+        return MTL4G_OK;
+    }
     if (A->nb == 0) return MTL4G_OK;
     const int per = (A->nb + kXcd - 1) / kXcd;
     const int grid = per * kXcd;
@@ -121,7 +204,7 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
     const int tile = A->nnz_per_block;
 #define M4_LAUNCH(T)                                                                          \
     hipLaunchKernelGGL((k_csr<256, T, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,         \
-                       A->rowptr.p, A->col.p, val, x, A->blk.p, A->nb, per, epi, partials, ticket)
+                       A->rowptr.p, A->col.p, val, x, A->desc.p, A->nb, per, epi, partials, ticket)
     if (tile <= 1024) M4_LAUNCH(1024);
     else if (tile <= 2048) M4_LAUNCH(2048);
     else M4_LAUNCH(4096);

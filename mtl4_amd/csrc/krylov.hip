@@ -92,6 +92,9 @@ __device__ __forceinline__ void ru_finish(Ctl* c, double rr64, double rho_new, b
 template <typename VT, typename DT, int PCM>
 struct EpiCgInit {
     static constexpr int NP = PCM == PCM_DIAG ? 2 : 1;
+    struct Pre {
+        VT b, d;
+    };
     const VT* b;
     VT* r;
     const DT* d;
@@ -99,12 +102,15 @@ struct EpiCgInit {
     double* hist;
     __device__ bool skip() const { return false; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, VT s, VT* part) const {
-        const VT v = b[row] - s;
+    __device__ Pre preload(int row) const {
+        return Pre{b[row], PCM == PCM_DIAG ? static_cast<VT>(d[row]) : VT(0)};
+    }
+    __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
+        const VT v = pre.b - s;
         r[row] = v;
         part[0] += v * v;
         if (PCM == PCM_DIAG) {
-            const VT z = static_cast<VT>(d[row]) * v;
+            const VT z = pre.d * v;
             part[1] += v * z;
         }
     }
@@ -164,14 +170,18 @@ struct EpiCgP {
 template <typename VT>
 struct EpiCgQ {
     static constexpr int NP = 1;
+    struct Pre {
+        VT p;
+    };
     const VT* p;
     VT* q;
     Ctl* c;
     __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, VT s, VT* part) const {
+    __device__ Pre preload(int row) const { return Pre{p[row]}; }
+    __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
         q[row] = s;
-        part[0] += p[row] * s;
+        part[0] += pre.p * s;
     }
     __device__ void finish(const VT* tot) const {
         c->alpha_1 = (double)tot[0];
@@ -222,14 +232,18 @@ struct EpiCgXR {
 template <typename VT>
 struct EpiBiInit {
     static constexpr int NP = 1;
+    struct Pre {
+        VT b;
+    };
     const VT* b;
     VT *r, *rt;
     Ctl* c;
     double* hist;
     __device__ bool skip() const { return false; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, VT s, VT* part) const {
-        const VT v = b[row] - s;
+    __device__ Pre preload(int row) const { return Pre{b[row]}; }
+    __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
+        const VT v = pre.b - s;
         r[row] = v;
         rt[row] = v;
         part[0] += v * v;
@@ -285,14 +299,18 @@ struct EpiBiP {
 template <typename VT>
 struct EpiBiV {
     static constexpr int NP = 1;
+    struct Pre {
+        VT rt;
+    };
     const VT* rt;
     VT* v;
     Ctl* c;
     __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, VT s, VT* part) const {
+    __device__ Pre preload(int row) const { return Pre{rt[row]}; }
+    __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
         v[row] = s;
-        part[0] += rt[row] * s;
+        part[0] += pre.rt * s;
     }
     __device__ void finish(const VT* tot) const {
         c->gamma = (double)tot[0];
@@ -353,14 +371,18 @@ struct EpiBiHalf {
 template <typename VT>
 struct EpiBiT {
     static constexpr int NP = 2;
+    struct Pre {
+        VT s;
+    };
     const VT* s;
     VT* t;
     Ctl* c;
     __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, VT v, VT* part) const {
+    __device__ Pre preload(int row) const { return Pre{s[row]}; }
+    __device__ void operator()(int row, VT v, const Pre& pre, VT* part) const {
         t[row] = v;
-        part[0] += v * s[row];
+        part[0] += v * pre.s;
         part[1] += v * v;
     }
     __device__ void finish(const VT* tot) const { c->omega = (double)(tot[0] / tot[1]); }
@@ -415,6 +437,10 @@ struct EpiRu1 {
 template <typename DT, int PCM>
 struct EpiRu2 {
     static constexpr int NP = 2;
+    struct Pre {
+        double b;
+        float d;
+    };
     const double* b;
     float* r32;
     const DT* d;
@@ -422,12 +448,15 @@ struct EpiRu2 {
     int cg;
     __device__ bool skip() const { return c->done != 0 || c->want_ru == 0; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, double s, double* part) const {
-        const double v = b[row] - s;
+    __device__ Pre preload(int row) const {
+        return Pre{b[row], PCM == PCM_DIAG ? (float)d[row] : 0.0f};
+    }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
+        const double v = pre.b - s;
         const float f = (float)v;
         r32[row] = f;
         part[0] += v * v;
-        if (PCM == PCM_DIAG) part[1] += (double)(f * ((float)d[row] * f));
+        if (PCM == PCM_DIAG) part[1] += (double)(f * (pre.d * f));
         else if (PCM == PCM_NONE) part[1] += (double)(f * f);
     }
     __device__ void finish(const double* tot) const {

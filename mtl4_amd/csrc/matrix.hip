@@ -69,8 +69,11 @@ int build_row_blocks(mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block) {
     A->nb = (int)blk.size() - 1;
     A->rows_per_block = rows_per_block;
     A->nnz_per_block = nnz_per_block;
-    M4_TRY(A->blk.alloc(blk.size()));
-    M4_HIP(hipMemcpy(A->blk.p, blk.data(), blk.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    std::vector<RowBlock> desc(std::max(A->nb, 1));
+    for (int b = 0; b < A->nb; b++)
+        desc[b] = RowBlock{blk[b], blk[b + 1], rp[blk[b]], rp[blk[b + 1]]};
+    M4_TRY(A->desc.alloc(desc.size()));
+    M4_HIP(hipMemcpy(A->desc.p, desc.data(), desc.size() * sizeof(RowBlock), hipMemcpyHostToDevice));
     return MTL4G_OK;
 }
 
@@ -102,13 +105,17 @@ int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
 template <typename AT, int MODE>
 struct EpiAssign {
     static constexpr int NP = 0;
+    struct Pre {
+        AT y;
+    };
     AT* y;
     __device__ bool skip() const { return false; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, AT s, AT*) const {
+    __device__ Pre preload(int row) const { return Pre{MODE == MTL4G_ASSIGN ? AT(0) : y[row]}; }
+    __device__ void operator()(int row, AT s, const Pre& pre, AT*) const {
         if (MODE == MTL4G_ASSIGN) y[row] = s;
-        else if (MODE == MTL4G_PLUS) y[row] = y[row] + s;
-        else y[row] = y[row] - s;
+        else if (MODE == MTL4G_PLUS) y[row] = pre.y + s;
+        else y[row] = pre.y - s;
     }
     __device__ void finish(const AT*) const {}
 };
@@ -116,14 +123,18 @@ struct EpiAssign {
 // (lazy(q)= A*p) || (lazy(alpha)= lazy_dot(p,q)), result to *out
 struct EpiSpmvDot {
     static constexpr int NP = 1;
+    struct Pre {
+        double p;
+    };
     const double* p;
     double* q;
     double* out;
     __device__ bool skip() const { return false; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, double s, double* part) const {
+    __device__ Pre preload(int row) const { return Pre{p[row]}; }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
         q[row] = s;
-        part[0] += p[row] * s;
+        part[0] += pre.p * s;
     }
     __device__ void finish(const double* tot) const { *out = tot[0]; }
 };
@@ -269,6 +280,40 @@ int mtl4g_set_device(int device) {
     return MTL4G_OK;
 }
 
+int mtl4g_malloc(void** p, size_t bytes) {
+    if (!p) return MTL4G_ERR_INVALID;
+    M4_HIP(hipMalloc(p, bytes ? bytes : 1));
+    return MTL4G_OK;
+}
+
+int mtl4g_free(void* p) {
+    if (p) M4_HIP(hipFree(p));
+    return MTL4G_OK;
+}
+
+int mtl4g_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream) {
+    if (bytes == 0) return MTL4G_OK;
+    if (!dst || !src || kind < 0 || kind > 2) return MTL4G_ERR_INVALID;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                            : kind == 1 ? hipMemcpyDeviceToHost
+                                        : hipMemcpyDeviceToDevice;
+    M4_HIP(hipMemcpyAsync(dst, src, bytes, k, S(stream)));
+    if (kind != 2) M4_HIP(hipStreamSynchronize(S(stream)));
+    return MTL4G_OK;
+}
+
+int mtl4g_memset(void* p, int value, size_t bytes, void* stream) {
+    if (bytes == 0) return MTL4G_OK;
+    if (!p) return MTL4G_ERR_INVALID;
+    M4_HIP(hipMemsetAsync(p, value, bytes, S(stream)));
+    return MTL4G_OK;
+}
+
+int mtl4g_stream_synchronize(void* stream) {
+    M4_HIP(hipStreamSynchronize(S(stream)));
+    return MTL4G_OK;
+}
+
 int mtl4g_csr_create(mtl4g_matrix* out, int32_t nrows, int32_t ncols, int64_t nnz,
                      const int32_t* rowptr, const int32_t* col, const double* val, int where) {
     if (!out || nrows < 0 || ncols < 0 || nnz < 0 || (nrows > 0 && !rowptr) ||
@@ -337,6 +382,11 @@ int mtl4g_csr_create(mtl4g_matrix* out, int32_t nrows, int32_t ncols, int64_t nn
     if (e == hipSuccess && nrows) e = hipMemcpy(A->diag.p, diag.data(), (size_t)nrows * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(hip_status(e, "hipMemcpy", __FILE__, __LINE__));
     if ((st = build_row_blocks(A, 0, 0))) return fail(st);
+    // row-length binning: lanes per row from the average row length
+    const double avg = nrows ? (double)nnz / nrows : 0.0;
+    A->gs = avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
+    A->vrows = std::max(4, 2048 / A->gs);
+    A->kernel = MTL4G_SPMV_VECTOR;
     *out = A;
     return MTL4G_OK;
 }
@@ -374,7 +424,79 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where) {
 
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block) {
     if (!A) return MTL4G_ERR_INVALID;
-    return build_row_blocks(A, rows_per_block, nnz_per_block);
+    M4_TRY(build_row_blocks(A, rows_per_block, nnz_per_block));
+    A->kernel = MTL4G_SPMV_TILE;
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block) {
+    if (!A) return MTL4G_ERR_INVALID;
+    if (kernel == MTL4G_SPMV_TILE) return mtl4g_csr_tune(A, rows_per_block, 0);
+    if (kernel != MTL4G_SPMV_VECTOR) return MTL4G_ERR_INVALID;
+    if (group == 0) group = A->gs;
+    if (group != 4 && group != 8 && group != 16 && group != 32 && group != 64) {
+        set_error("vector CSR group must be 4, 8, 16, 32 or 64 lanes");
+        return MTL4G_ERR_INVALID;
+    }
+    if (rows_per_block <= 0) rows_per_block = std::max(4, 2048 / group);
+    A->kernel = MTL4G_SPMV_VECTOR;
+    A->gs = group;
+    A->vrows = rows_per_block;
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block) {
+    if (!A) return MTL4G_ERR_INVALID;
+    if (kernel) *kernel = A->kernel;
+    if (group) *group = A->kernel == MTL4G_SPMV_VECTOR ? A->gs : 0;
+    if (rows_per_block) *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR ? A->vrows : A->rows_per_block;
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
+    if (!A) return MTL4G_ERR_INVALID;
+    hipStream_t s = S(stream);
+    if (A->n == 0) return MTL4G_OK;
+    struct Cand {
+        int kernel, a, b;
+    };
+    std::vector<Cand> cands = {{MTL4G_SPMV_TILE, 256, 2048}, {MTL4G_SPMV_TILE, 128, 1024}};
+    for (int g : {4, 8, 16, 32, 64})
+        for (int f : {1, 2, 4}) cands.push_back({MTL4G_SPMV_VECTOR, g, std::max(4, f * 1024 / g)});
+    double *x = nullptr, *y = nullptr;
+    M4_HIP(hipMallocAsync((void**)&x, (size_t)A->m * 8 + 8, s));
+    M4_HIP(hipMallocAsync((void**)&y, (size_t)A->n * 8 + 8, s));
+    M4_HIP(hipMemsetAsync(x, 0, (size_t)A->m * 8, s));
+    hipEvent_t e0, e1;
+    M4_HIP(hipEventCreate(&e0));
+    M4_HIP(hipEventCreate(&e1));
+    double best = 1e30;
+    Cand win = cands[0];
+    for (const Cand& c : cands) {
+        int st = c.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, c.a, c.b) : mtl4g_csr_set_kernel(A, c.kernel, c.a, c.b);
+        if (st) continue;
+        if (launch_spmv(A, x, y, MTL4G_ASSIGN, s)) continue;   // warm-up
+        const int reps = 5;
+        M4_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) M4_TRY(launch_spmv(A, x, y, MTL4G_ASSIGN, s));
+        M4_HIP(hipEventRecord(e1, s));
+        M4_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        M4_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1000.0 * ms / reps;
+        if (us < best) {
+            best = us;
+            win = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFreeAsync(x, s);
+    (void)hipFreeAsync(y, s);
+    M4_TRY(win.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, win.a, win.b)
+                                         : mtl4g_csr_set_kernel(A, win.kernel, win.a, win.b));
+    if (h_best_us) *h_best_us = best;
+    return MTL4G_OK;
 }
 
 int mtl4g_spmv(mtl4g_matrix A, const double* v, double* w, int assign, void* stream) {

@@ -26,14 +26,18 @@ struct IrCtl {
 // r = b - A*x in FP64, r32 = float(r), ||r||^2
 struct EpiIrResid {
     static constexpr int NP = 1;
+    struct Pre {
+        double b;
+    };
     const double* b;
     double* r;
     float* r32;
     IrCtl* c;
     __device__ bool skip() const { return false; }
     __device__ void prepare() {}
-    __device__ void operator()(int row, double s, double* part) const {
-        const double v = b[row] - s;
+    __device__ Pre preload(int row) const { return Pre{b[row]}; }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
+        const double v = pre.b - s;
         r32[row] = (float)v;
         part[0] += v * v;
     }

@@ -58,6 +58,32 @@ def test_spmv_partitions(m, tile):
     assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
 
 
+@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("fam", ["lap2d_64", "random_3000", "long_rows", "diff3d_16"])
+def test_spmv_vector_groups_bitwise(m, fam, group):
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    A.set_kernel(m._lib.SPMV_VECTOR, group, 0)
+    assert A.get_kernel()[:2] == (m._lib.SPMV_VECTOR, group)
+    x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    y0 = np.random.default_rng(9).uniform(-1, 1, A.num_rows())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+    w = dv(m, y0)
+    A.mult(dv(m, x), w, m.MINUS)
+    yr = y0.copy()
+    orc.spmv(O, x, yr, orc.MINUS)
+    assert np.array_equal(w.numpy(), yr)
+
+
+def test_spmv_autotune_keeps_results(m):
+    rp, ci, va = family("random_3000", seed=2)
+    O, A = pair(rp, ci, va)
+    us = A.autotune()
+    assert us > 0
+    x = np.random.default_rng(3).uniform(-1, 1, A.num_cols())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+
+
 def test_spmv_kat(m):
     k = KATS["matrix_vector_product"]
     A = m.laplacian_setup(*k["grid"])
