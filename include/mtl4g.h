@@ -95,6 +95,28 @@ int mtl4g_version(void);
 /* Sets the device used by later calls on this thread. */
 int mtl4g_set_device(int device);
 
+/* Summation order of the FP64 reductions (dot, unary_dot, two_norm, the
+ * fused reductions and those inside cg / bicgstab / solver_solve).
+ * MTL4G_ORDER_TREE (default): every workgroup sums its share, the partial
+ *   sums are combined in workgroup order -- deterministic, full bandwidth,
+ *   but rounded differently from MTL4.
+ * MTL4G_ORDER_MTL4: MTL4's own sequential order -- four accumulators taking
+ *   the indices i = k (mod 4), the remainder into the first, finished as
+ *   dot<4> ((t0+t1)+(t2+t3)) (operation/dot.hpp:136-147), reduction<4>
+ *   t0+(t1+(t2+t3)) (vector/reduction.hpp:28-82), the fused SpMV dot
+ *   ((t0+t1)+t2)+t3 (operation/fused_expr.hpp:57-77, dot_index_evaluator)
+ *   and the fused update norm (t0+t1)+(t2+t3)
+ *   (vector/reduction_index_evaluator.hpp:22-41).  Products and updates are
+ *   computed per element as before, so every FP64 solve then reproduces
+ *   MTL4's CPU result bit for bit; the sums run on four lanes, so this mode
+ *   is for validation on small systems (about n/4 dependent additions per
+ *   reduction).  No MTL4 counterpart; process-wide, read at each launch.
+ * Mixed-precision and batched solves always use the tree order. */
+#define MTL4G_ORDER_TREE 0
+#define MTL4G_ORDER_MTL4 1
+int mtl4g_set_reduction_order(int order);
+int mtl4g_get_reduction_order(void);
+
 /* Device storage for dense_vector (vector/dense_vector.hpp:50-269 keeps its
  * elements in a contiguous_memory_block; here the block lives in HBM).
  * kind: 0 host->device, 1 device->host, 2 device->device. */
@@ -282,6 +304,51 @@ int mtl4g_batch_solve_items(mtl4g_batch B, int pc_kind, int precision,
                             const mtl4g_batch_item* d_items, int32_t nitems, const double* d_b,
                             double* d_x, double* d_work, mtl4g_batch_result* d_results,
                             unsigned* d_queue, void* stream);
+
+/* ---- serving loop: continuous batching (no MTL4 counterpart) --------- */
+/* Matrices stay resident in HBM (one mtl4g_batch).  Requests (matrix, number
+ * of right-hand sides) are admitted in arrival order while their device
+ * footprint (b, x and workspace of every right-hand side) fits the cap; the
+ * admitted requests are packed into persistent batched launches on `streams`
+ * concurrent streams as soon as a stream is free, so launches overlap on the
+ * device.  Systems with more than big_rows rows are solved by the
+ * whole-GPU engine on a stream of their own.  Every solve is itl::cg with
+ * the identity or diagonal preconditioner at rtol; precision as in
+ * mtl4g_batch_solve. */
+typedef struct mtl4g_server_s* mtl4g_server;
+
+typedef struct {
+    int32_t streams;        /* concurrent batched launches (e.g. 4) */
+    int32_t max_items;      /* right-hand sides per launch (e.g. 4096) */
+    int32_t precision;      /* MTL4G_F64, MTL4G_F32 or MTL4G_BF16 */
+    int32_t pc_kind;        /* MTL4G_PC_IDENTITY or MTL4G_PC_DIAGONAL */
+    int32_t max_iter;
+    int32_t big_rows;       /* larger systems use the whole-GPU engine */
+    double rtol;
+    double cap_bytes;       /* admission cap on the in-flight working set */
+} mtl4g_server_params;
+
+typedef struct {
+    double wall_s;          /* first arrival to last completion */
+    int64_t solves;         /* right-hand sides solved */
+    int32_t launches;       /* batched launches */
+    int32_t max_inflight;   /* most launches on the device at once */
+    double peak_bytes;      /* largest admitted working set */
+    int32_t errors;         /* solves that did not converge */
+    int32_t big_solves;     /* solves taken by the whole-GPU engine */
+} mtl4g_server_stats;
+
+int mtl4g_server_create(mtl4g_server* S, mtl4g_batch B, const mtl4g_server_params* prm);
+int mtl4g_server_destroy(mtl4g_server S);
+/* Replays a request trace in real time: request r arrives arrival_s[r]
+ * seconds after the start for matrix mat[r] with nrhs[r] right-hand sides
+ * generated on the device from seed[r] (values uniform in [-1,1]).
+ * latency_s[r] = completion - arrival.  Optional outputs: iters (one per
+ * right-hand side, requests in order), x_out + x_off[r] (the solutions of
+ * request r, nrhs[r] vectors of the matrix size one after the other). */
+int mtl4g_server_run(mtl4g_server S, int32_t nreq, const int32_t* mat, const int32_t* nrhs,
+                     const double* arrival_s, const uint64_t* seed, double* latency_s,
+                     int32_t* iters, double* x_out, const int64_t* x_off, mtl4g_server_stats* st);
 
 #ifdef __cplusplus
 }

@@ -18,9 +18,9 @@ import sys
 import numpy as np
 
 from . import _lib
-from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, PLUS, MtlError, check,
-                   incompatible_size, logic_error, matrix_not_square, missing_diagonal,
-                   nonfinite_error, unexpected_orthogonality)
+from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, ORDER_MTL4, ORDER_TREE, PLUS,
+                   MtlError, check, incompatible_size, logic_error, matrix_not_square,
+                   missing_diagonal, nonfinite_error, unexpected_orthogonality)
 
 __all__ = ["compressed2D", "dense_vector", "pc", "solve", "adjoint_solve", "basic_iteration",
            "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "dot", "unary_dot",
@@ -40,6 +40,33 @@ def _stream():
 
 def _L():
     return _lib.lib()
+
+
+def set_reduction_order(order):
+    """ORDER_TREE (default: parallel, deterministic) or ORDER_MTL4 (MTL4's
+    sequential unrolled order: FP64 reductions and solves then reproduce
+    MTL4's CPU bits; meant for validation on small systems)."""
+    check(_L().mtl4g_set_reduction_order(order), "set_reduction_order")
+
+
+def get_reduction_order():
+    return _L().mtl4g_get_reduction_order()
+
+
+class reduction_order:
+    """with reduction_order(ORDER_MTL4): ... -- sets the order, restores it."""
+
+    def __init__(self, order):
+        self.order = order
+
+    def __enter__(self):
+        self.saved = get_reduction_order()
+        set_reduction_order(self.order)
+        return self
+
+    def __exit__(self, *exc):
+        set_reduction_order(self.saved)
+        return False
 
 
 # ---------------------------------------------------------------- containers
@@ -119,7 +146,7 @@ class compressed2D:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and getattr(_lib, "_lib", None) is not None:
             _lib._lib.mtl4g_csr_destroy(h)
             self.h = None
 
@@ -218,7 +245,7 @@ class _PC:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and getattr(_lib, "_lib", None) is not None:
             _lib._lib.mtl4g_pc_destroy(h)
             self.h = None
 
@@ -440,7 +467,7 @@ class Solver:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and getattr(_lib, "_lib", None) is not None:
             _lib._lib.mtl4g_solver_destroy(h)
             self.h = None
 
@@ -506,7 +533,7 @@ class Batch:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and getattr(_lib, "_lib", None) is not None:
             _lib._lib.mtl4g_batch_destroy(h)
             self.h = None
 
@@ -533,3 +560,45 @@ class Batch:
 
     def set_shape(self, precision, threads, per_cu):
         check(_L().mtl4g_batch_set_shape(self.h, precision, threads, per_cu), "set_shape")
+
+
+class Server:
+    """Serving loop over the matrices of a Batch (continuous batching)."""
+
+    def __init__(self, batch, precision=F32, pc_kind=_lib.PC_DIAGONAL, rtol=1e-10,
+                 max_iter=20000, streams=4, max_items=4096, big_rows=65536, cap_bytes=200e9):
+        self.batch = batch
+        self.prm = _lib.ServerParams(streams, max_items, precision, pc_kind, max_iter, big_rows,
+                                     rtol, cap_bytes)
+        h = C.c_void_p()
+        check(_L().mtl4g_server_create(C.byref(h), batch.h, C.byref(self.prm)), "server_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and getattr(_lib, "_lib", None) is not None:
+            _lib._lib.mtl4g_server_destroy(h)
+            self.h = None
+
+    def run(self, mat, nrhs, arrival_s, seed, want_x=False):
+        """Replay a trace; returns (latency_s, iterations per rhs, x list or None, stats)."""
+        mat = np.ascontiguousarray(mat, np.int32)
+        nrhs = np.ascontiguousarray(nrhs, np.int32)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        sd = np.ascontiguousarray(seed, np.uint64)
+        n = len(mat)
+        lat = np.zeros(n)
+        its = np.zeros(int(nrhs.sum()), np.int32)
+        sizes = np.diff(self.batch.sys_row)[mat].astype(np.int64)
+        xoff = np.zeros(n + 1, np.int64)
+        np.cumsum(sizes * nrhs, out=xoff[1:])
+        xo = np.zeros(int(xoff[-1])) if want_x else None
+        st = _lib.ServerStats()
+        p = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+        check(_L().mtl4g_server_run(self.h, n, p(mat), p(nrhs), p(arr), p(sd), p(lat), p(its),
+                                    p(xo), p(xoff) if want_x else None, C.byref(st)), "server_run")
+        xs = None
+        if want_x:
+            xs = [xo[xoff[r]:xoff[r + 1]].reshape(nrhs[r], sizes[r]) for r in range(n)]
+        stats = {f: getattr(st, f) for f, _ in _lib.ServerStats._fields_}
+        return lat, its, xs, stats

@@ -39,6 +39,7 @@ class IRStats(C.Structure):
 
 
 SPMV_TILE, SPMV_VECTOR = 0, 1
+ORDER_TREE, ORDER_MTL4 = 0, 1
 
 
 class BatchItem(C.Structure):
@@ -50,6 +51,18 @@ class BatchItem(C.Structure):
 class BatchResult(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("error", C.c_int32), ("status", C.c_int32),
                 ("refreshes", C.c_int32), ("resid", C.c_double), ("norm_r0", C.c_double)]
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("streams", C.c_int32), ("max_items", C.c_int32), ("precision", C.c_int32),
+                ("pc_kind", C.c_int32), ("max_iter", C.c_int32), ("big_rows", C.c_int32),
+                ("rtol", C.c_double), ("cap_bytes", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("solves", C.c_int64), ("launches", C.c_int32),
+                ("max_inflight", C.c_int32), ("peak_bytes", C.c_double), ("errors", C.c_int32),
+                ("big_solves", C.c_int32)]
 
 
 def declared_functions():
@@ -66,6 +79,8 @@ _SIGS = {
     "mtl4g_last_error": (C.c_char_p, []),
     "mtl4g_version": (_INT, []),
     "mtl4g_set_device": (_INT, [_INT]),
+    "mtl4g_set_reduction_order": (_INT, [_INT]),
+    "mtl4g_get_reduction_order": (_INT, []),
     "mtl4g_csr_create": (_INT, [_PP, _I32, _I32, _I64, _P, _P, _P, _INT]),
     "mtl4g_csr_destroy": (_INT, [_P]),
     "mtl4g_csr_info": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64)]),
@@ -109,6 +124,9 @@ _SIGS = {
     "mtl4g_batch_solve": (_INT, [_P, _INT, _INT, _P, _P, _D, _D, _I32, C.POINTER(BatchResult), _P]),
     "mtl4g_batch_tune": (_INT, [_P, _INT, _INT, _P, _P, _P, _D, _I32, C.POINTER(_D), _P]),
     "mtl4g_batch_solve_items": (_INT, [_P, _INT, _INT, _P, _I32, _P, _P, _P, _P, _P, _P]),
+    "mtl4g_server_create": (_INT, [_PP, _P, C.POINTER(ServerParams)]),
+    "mtl4g_server_destroy": (_INT, [_P]),
+    "mtl4g_server_run": (_INT, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(ServerStats)]),
 }
 
 _lib = None

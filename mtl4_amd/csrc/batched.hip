@@ -405,6 +405,7 @@ int mtl4g_batch_create(mtl4g_batch* out, int32_t nsys, const int32_t* sys_rowptr
     B->nnz = nnz;
     B->h_sys_row.assign(sys_rowptr, sys_rowptr + nsys + 1);
     B->h_rowptr.assign(rowptr, rowptr + nrows + 1);
+    B->h_col.assign(col, col + nnz);
     B->h_val.assign(val, val + nnz);
     B->h_diag.assign(nrows, 0.0);
     std::vector<double> cost(nsys);

@@ -29,7 +29,7 @@ struct mtl4g_batch_s {
     int32_t nsys = 0, nrows = 0;
     int64_t nnz = 0;
     int32_t max_n = 0;
-    std::vector<int32_t> h_sys_row, h_rowptr;
+    std::vector<int32_t> h_sys_row, h_rowptr, h_col;
     std::vector<double> h_val, h_diag;
     mtl4g::DevBuf<int32_t> sys_row, rowptr, col;
     mtl4g::DevBuf<double> val, dinv;

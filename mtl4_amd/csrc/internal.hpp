@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mtl4g.h"
@@ -26,6 +27,8 @@ namespace mtl4g {
 // ----------------------------------------------------------------- errors
 void set_error(const char* fmt, ...);
 int hip_status(hipError_t e, const char* what, const char* file, int line);
+// MTL4G_ORDER_TREE or MTL4G_ORDER_MTL4 (mtl4g_set_reduction_order)
+int reduction_order();
 
 #define M4_HIP(expr)                                                              \
     do {                                                                          \

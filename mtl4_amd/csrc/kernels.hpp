@@ -64,13 +64,16 @@ __device__ __forceinline__ void combine_partials(const AT* part, AT* partials, u
     if (threadIdx.x == 0) epi.finish(tot);
 }
 
-template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
+// ORD: instead of combining partial sums, every row stores its terms in
+// terms[k * tstride + row]; k_ordered then adds them in MTL4's order.
+template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi, bool ORD = false>
 __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
                                             const int32_t* __restrict__ col,
                                             const MT* __restrict__ val,
                                             const XT* __restrict__ x,
                                             const RowBlock* __restrict__ desc, int nb, int per_xcd,
-                                            Epi epi, AT* partials, unsigned* ticket) {
+                                            Epi epi, AT* partials, unsigned* ticket,
+                                            AT* terms, int64_t tstride) {
     const int b = xcd_logical_block(blockIdx.x, per_xcd);
     const RowBlock d = desc[b < nb ? b : nb - 1];   // requested with the stop flag
     if (epi.skip()) return;
@@ -85,9 +88,17 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
         int row;
         typename Epi::Pre pre{};
         AT s = csr_block_rows<BS, TILE, AT>(rowptr, col, val, x, d, prod, sh, row, epi, pre);
-        if (row < d.r1) epi(row, s, pre, part);
+        if (row < d.r1) {
+            if constexpr (ORD) {
+                epi(row, s, pre, part);
+#pragma unroll
+                for (int k = 0; k < NP; k++) terms[(size_t)k * tstride + row] = part[k];
+            } else {
+                epi(row, s, pre, part);
+            }
+        }
     }
-    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
+    if constexpr (Epi::NP > 0 && !ORD) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
 // Vector CSR: a group of GS lanes per row, 64/GS rows per wavefront pass.
@@ -145,8 +156,9 @@ __global__ __launch_bounds__(BS) void k_csrv(const int32_t* __restrict__ rowptr,
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
-template <int BS, typename AT, typename Epi>
-__global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, unsigned* ticket) {
+template <int BS, typename AT, typename Epi, bool ORD = false>
+__global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, unsigned* ticket,
+                                            AT* terms) {
     if (epi.skip()) return;
     epi.prepare();
     __shared__ AT sh[BS / kWave];
@@ -160,17 +172,133 @@ __global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, un
     AT part[NP];
 #pragma unroll
     for (int k = 0; k < NP; k++) part[k] = AT(0);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += BS) epi(i, part);
-    if constexpr (Epi::NP > 0) {
-        combine_partials<BS>(part, partials, ticket, b, (int)G, epi, sh);
+    if constexpr (ORD) {
+        for (int64_t i = lo + threadIdx.x; i < hi; i += BS) {
+#pragma unroll
+            for (int k = 0; k < NP; k++) part[k] = AT(0);
+            epi(i, part);
+#pragma unroll
+            for (int k = 0; k < NP; k++) terms[(size_t)k * n + i] = part[k];
+        }
+    } else {
+        for (int64_t i = lo + threadIdx.x; i < hi; i += BS) epi(i, part);
+        if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)G, epi, sh);
     }
 }
 
+// ---- MTL4-order reductions
+// MTL4 sums with U accumulators: a[k] holds the terms i = k (mod U) below
+// n - n % U in index order, a[0] then also the remainder; the forms below
+// combine them.
+enum RedOrder {
+    RO_DOT4 = 0,        // dot<4>: ((t0+t1)+(t2+t3)) + ((t4+t5)+(t6+t7)), t4..t7 = 0 (operation/dot.hpp:136-147)
+    RO_NORM4 = 1,       // reduction<4>: t0+(t1+(t2+t3)) (vector/reduction.hpp:28-82)
+    RO_FUSED_DOT = 2,   // dot_index_evaluator: ((t0+t1)+t2)+t3 (fused_expr.hpp:57-77)
+    RO_FUSED_NORM = 3,  // reduction_index_evaluator: (t0+t1)+(t2+t3) (reduction_index_evaluator.hpp:22-41)
+    RO_NORM8 = 4        // reduction<8> (unary_dot's default, unary_dot.hpp:56-60): t0+(t1+(...+(t6+t7)))
+};
+
+__host__ __device__ constexpr int ro_unroll(int order) { return order == RO_NORM8 ? 8 : 4; }
+
+template <typename E, typename = void>
+struct has_order : std::false_type {};
+template <typename E>
+struct has_order<E, std::void_t<decltype(E::order(0))>> : std::true_type {};
+
+template <typename AT>
+__device__ __forceinline__ AT mtl4_finish(int order, const AT* a) {
+    switch (order) {
+    case RO_DOT4: return ((a[0] + a[1]) + (a[2] + a[3])) + ((AT(0) + AT(0)) + (AT(0) + AT(0)));
+    case RO_NORM4: return a[0] + (a[1] + (a[2] + a[3]));
+    case RO_FUSED_DOT: return ((a[0] + a[1]) + a[2]) + a[3];
+    case RO_FUSED_NORM: return (a[0] + a[1]) + (a[2] + a[3]);
+    default: return a[0] + (a[1] + (a[2] + (a[3] + (a[4] + (a[5] + (a[6] + a[7]))))));
+    }
+}
+
+// One wavefront: lane k < U adds terms[k], terms[k+U], ... below n - n % U
+// one after the other (loads issued eight ahead), lane 0 then the remainder.
+template <typename AT, typename Epi>
+__global__ __launch_bounds__(kWave) void k_ordered(int64_t n, const AT* __restrict__ terms, Epi epi) {
+    if (epi.skip()) return;
+    epi.prepare();
+    constexpr int NP = Epi::NP;
+    __shared__ AT acc[NP][8];
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        const int U = ro_unroll(Epi::order(k));
+        const int64_t sb = n - n % U;
+        if (lane < U) {
+            const AT* t = terms + (size_t)k * n;
+            AT a = AT(0);
+            int64_t i = lane;
+            for (; i + 7 * U < sb; i += 8 * U) {
+                AT v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = t[i + U * u];
+#pragma unroll
+                for (int u = 0; u < 8; u++) a += v[u];
+            }
+            for (; i < sb; i += U) a += t[i];
+            if (lane == 0)
+                for (i = sb; i < n; i++) a += t[i];
+            acc[k][lane] = a;
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        AT tot[NP];
+#pragma unroll
+        for (int k = 0; k < NP; k++) tot[k] = mtl4_finish(Epi::order(k), acc[k]);
+        epi.finish(tot);
+    }
+}
+
+// True when this launch must run in MTL4's order; an error when it should
+// but the caller brought no terms buffer.
+template <typename AT, typename Epi>
+int want_ordered(const AT* terms, bool& ord) {
+    ord = false;
+    if constexpr (Epi::NP > 0 && has_order<Epi>::value && std::is_same<AT, double>::value) {
+        if (reduction_order() == MTL4G_ORDER_MTL4) {
+            if (!terms) {
+                set_error("MTL4-order reduction without a terms buffer");
+                return MTL4G_ERR_INVALID;
+            }
+            ord = true;
+        }
+    }
+    return MTL4G_OK;
+}
+
 // ---- host launchers
+// terms: Epi::NP * A->n elements, used when reductions run in MTL4's order.
 template <typename AT, typename MT, typename XT, typename Epi>
 int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& epi,
-               AT* partials, unsigned* ticket, hipStream_t s) {
+               AT* partials, unsigned* ticket, hipStream_t s, AT* terms = nullptr) {
     if (A->n == 0) return MTL4G_OK;
+    bool ord;
+    M4_TRY((want_ordered<AT, Epi>(terms, ord)));
+    if constexpr (Epi::NP > 0 && has_order<Epi>::value && std::is_same<AT, double>::value) {
+        if (ord) {   // the row-block kernel (rows are summed alike in every kernel)
+            const int per = (A->nb + kXcd - 1) / kXcd;
+            const int tile = A->nnz_per_block;
+#define M4_LAUNCHO(T)                                                                          \
+    hipLaunchKernelGGL((k_csr<256, T, AT, MT, XT, Epi, true>), dim3(per * kXcd), dim3(256), 0, s, \
+                       A->rowptr.p, A->col.p, val, x, A->desc.p, A->nb, per, epi, partials, ticket, \
+                       terms, (int64_t)A->n)
+            if (tile <= 1024) M4_LAUNCHO(1024);
+            else if (tile <= 2048) M4_LAUNCHO(2048);
+            else M4_LAUNCHO(4096);
+#undef M4_LAUNCHO
+            M4_HIP(hipGetLastError());
+            hipLaunchKernelGGL((k_ordered<AT, Epi>), dim3(1), dim3(kWave), 0, s, (int64_t)A->n,
+                               (const AT*)terms, epi);
+            M4_HIP(hipGetLastError());
+            return MTL4G_OK;
+        }
+    }
     if (A->kernel == MTL4G_SPMV_VECTOR) {
         const int rpb = A->vrows;
         const int nbv = (A->n + rpb - 1) / rpb;
@@ -204,7 +332,8 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
     const int tile = A->nnz_per_block;
 #define M4_LAUNCH(T)                                                                          \
     hipLaunchKernelGGL((k_csr<256, T, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,         \
-                       A->rowptr.p, A->col.p, val, x, A->desc.p, A->nb, per, epi, partials, ticket)
+                       A->rowptr.p, A->col.p, val, x, A->desc.p, A->nb, per, epi, partials, ticket, \
+                       (AT*)nullptr, (int64_t)0)
     if (tile <= 1024) M4_LAUNCH(1024);
     else if (tile <= 2048) M4_LAUNCH(2048);
     else M4_LAUNCH(4096);
@@ -214,10 +343,23 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
 }
 
 template <typename AT, typename Epi>
-int launch_vec(int64_t n, const Epi& epi, AT* partials, unsigned* ticket, hipStream_t s) {
+int launch_vec(int64_t n, const Epi& epi, AT* partials, unsigned* ticket, hipStream_t s,
+               AT* terms = nullptr) {
     if (n <= 0) return MTL4G_OK;
+    bool ord;
+    M4_TRY((want_ordered<AT, Epi>(terms, ord)));
+    if constexpr (Epi::NP > 0 && has_order<Epi>::value && std::is_same<AT, double>::value) {
+        if (ord) {
+            hipLaunchKernelGGL((k_vec<256, AT, Epi, true>), dim3(blocks_for(n)), dim3(256), 0, s, n,
+                               epi, partials, ticket, terms);
+            M4_HIP(hipGetLastError());
+            hipLaunchKernelGGL((k_ordered<AT, Epi>), dim3(1), dim3(kWave), 0, s, n, (const AT*)terms, epi);
+            M4_HIP(hipGetLastError());
+            return MTL4G_OK;
+        }
+    }
     hipLaunchKernelGGL((k_vec<256, AT, Epi>), dim3(blocks_for(n)), dim3(256), 0, s, n, epi,
-                       partials, ticket);
+                       partials, ticket, (AT*)nullptr);
     M4_HIP(hipGetLastError());
     return MTL4G_OK;
 }

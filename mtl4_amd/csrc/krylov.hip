@@ -92,6 +92,7 @@ __device__ __forceinline__ void ru_finish(Ctl* c, double rr64, double rho_new, b
 template <typename VT, typename DT, int PCM>
 struct EpiCgInit {
     static constexpr int NP = PCM == PCM_DIAG ? 2 : 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
     struct Pre {
         VT b, d;
     };
@@ -129,6 +130,7 @@ struct EpiCgInit {
 template <typename VT>
 struct EpiCgRho {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
     const VT *r, *z;
     Ctl* c;
     __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
@@ -170,6 +172,7 @@ struct EpiCgP {
 template <typename VT>
 struct EpiCgQ {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_FUSED_DOT; }
     struct Pre {
         VT p;
     };
@@ -196,6 +199,7 @@ struct EpiCgQ {
 template <typename VT, typename DT, int PCM>
 struct EpiCgXR {
     static constexpr int NP = PCM == PCM_DIAG ? 2 : 1;
+    __host__ __device__ static constexpr int order(int k) { return k == 0 ? RO_FUSED_NORM : RO_DOT4; }
     VT* x;
     VT* r;
     const VT *p, *q;
@@ -232,6 +236,7 @@ struct EpiCgXR {
 template <typename VT>
 struct EpiBiInit {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_NORM4; }
     struct Pre {
         VT b;
     };
@@ -257,6 +262,7 @@ struct EpiBiInit {
 template <typename VT>
 struct EpiBiRho {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
     const VT *rt, *r;
     Ctl* c;
     __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
@@ -299,6 +305,7 @@ struct EpiBiP {
 template <typename VT>
 struct EpiBiV {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
     struct Pre {
         VT rt;
     };
@@ -323,6 +330,7 @@ struct EpiBiV {
 template <typename VT>
 struct EpiBiS {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_NORM4; }
     const VT *r, *v;
     VT* s;
     Ctl* c;
@@ -371,6 +379,7 @@ struct EpiBiHalf {
 template <typename VT>
 struct EpiBiT {
     static constexpr int NP = 2;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
     struct Pre {
         VT s;
     };
@@ -393,6 +402,7 @@ struct EpiBiT {
 template <typename VT>
 struct EpiBiXR {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_NORM4; }
     VT *x, *r;
     const VT *shat, *phat, *s, *t;
     Ctl* c;
@@ -487,6 +497,8 @@ static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_p
     if (!W.h_ctl) M4_HIP(hipHostMalloc((void**)&W.h_ctl, sizeof(Ctl), hipHostMallocDefault));
     if (hist_cap > 0 && W.hist.n < (size_t)hist_cap) M4_TRY(W.hist.alloc(hist_cap));
     if (!W.hist.p) M4_TRY(W.hist.alloc(1));
+    if (sizeof(VT) == 8 && reduction_order() == MTL4G_ORDER_MTL4 && W.terms.n < 2 * (size_t)n)
+        M4_TRY(W.terms.alloc(2 * (size_t)n));
     size_t words = P && (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) ? pc_counter_words(P) : 1;
     if (W.pc_counters.n < words) {
         M4_TRY(W.pc_counters.alloc(words));
@@ -507,14 +519,15 @@ static int cg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s
     VT* y0 = z + n;
     Ctl* c = W.ctl.p;
     VT* part = reinterpret_cast<VT*>(W.partials.p);
+    VT* tm = reinterpret_cast<VT*>(W.terms.p);
     unsigned* tk = W.tickets.p;
     if (PCM == PCM_GENERAL) {
         M4_TRY(pc_apply<VT>(P, r, z, y0, &c->done, W.pc_counters.p, s));
-        M4_TRY(launch_vec<VT>(n, EpiCgRho<VT>{r, z, c}, part, tk, s));
+        M4_TRY(launch_vec<VT>(n, EpiCgRho<VT>{r, z, c}, part, tk, s, tm));
     }
     M4_TRY(launch_vec<VT>(n, EpiCgP<VT, DT, PCM>{r, d, z, p, c, VT(0), 0}, part, tk, s));
-    M4_TRY(launch_csr<VT>(A, val, p, EpiCgQ<VT>{p, q, c}, part, tk, s));
-    M4_TRY(launch_vec<VT>(n, EpiCgXR<VT, DT, PCM>{x, r, p, q, d, c, W.hist.p, VT(0)}, part, tk, s));
+    M4_TRY(launch_csr<VT>(A, val, p, EpiCgQ<VT>{p, q, c}, part, tk, s, tm));
+    M4_TRY(launch_vec<VT>(n, EpiCgXR<VT, DT, PCM>{x, r, p, q, d, c, W.hist.p, VT(0)}, part, tk, s, tm));
     return MTL4G_OK;
 }
 
@@ -525,7 +538,8 @@ static int cg_init(const mtl4g_matrix_s* A, const MT* val, const DT* d, Workspac
     VT* r = W.vecs.p + 2 * n;
     Ctl* c = W.ctl.p;
     return launch_csr<VT>(A, val, x, EpiCgInit<VT, DT, PCM>{b, r, d, c, W.hist.p},
-                          reinterpret_cast<VT*>(W.partials.p), W.tickets.p, s);
+                          reinterpret_cast<VT*>(W.partials.p), W.tickets.p, s,
+                          reinterpret_cast<VT*>(W.terms.p));
 }
 
 template <typename VT, typename MT>
@@ -536,7 +550,8 @@ static int bicg_init(const mtl4g_matrix_s* A, const MT* val, Workspace<VT>& W, c
     VT* rt = W.vecs.p + 7 * n;
     Ctl* c = W.ctl.p;
     return launch_csr<VT>(A, val, x, EpiBiInit<VT>{b, r, rt, c, W.hist.p},
-                          reinterpret_cast<VT*>(W.partials.p), W.tickets.p, s);
+                          reinterpret_cast<VT*>(W.partials.p), W.tickets.p, s,
+                          reinterpret_cast<VT*>(W.terms.p));
 }
 
 template <typename VT, typename MT>
@@ -553,16 +568,17 @@ static int bicg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc
     }
     Ctl* c = W.ctl.p;
     VT* part = reinterpret_cast<VT*>(W.partials.p);
+    VT* tm = reinterpret_cast<VT*>(W.terms.p);
     unsigned* tk = W.tickets.p;
-    M4_TRY(launch_vec<VT>(n, EpiBiRho<VT>{rt, r, c}, part, tk, s));
+    M4_TRY(launch_vec<VT>(n, EpiBiRho<VT>{rt, r, c}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiBiP<VT>{r, v, p, c, VT(0), VT(0), 0}, part, tk, s));
     if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s));
-    M4_TRY(launch_csr<VT>(A, val, phat, EpiBiV<VT>{rt, v, c}, part, tk, s));
-    M4_TRY(launch_vec<VT>(n, EpiBiS<VT>{r, v, sv, c, W.hist.p, VT(0)}, part, tk, s));
+    M4_TRY(launch_csr<VT>(A, val, phat, EpiBiV<VT>{rt, v, c}, part, tk, s, tm));
+    M4_TRY(launch_vec<VT>(n, EpiBiS<VT>{r, v, sv, c, W.hist.p, VT(0)}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiBiHalf<VT>{phat, x, c, VT(0)}, part, tk, s));
     if (!ident) M4_TRY(pc_apply<VT>(P, sv, shat, y0, &c->done, W.pc_counters.p, s));
-    M4_TRY(launch_csr<VT>(A, val, shat, EpiBiT<VT>{sv, t, c}, part, tk, s));
-    M4_TRY(launch_vec<VT>(n, EpiBiXR<VT>{x, r, shat, phat, sv, t, c, W.hist.p, VT(0), VT(0)}, part, tk, s));
+    M4_TRY(launch_csr<VT>(A, val, shat, EpiBiT<VT>{sv, t, c}, part, tk, s, tm));
+    M4_TRY(launch_vec<VT>(n, EpiBiXR<VT>{x, r, shat, phat, sv, t, c, W.hist.p, VT(0), VT(0)}, part, tk, s, tm));
     return MTL4G_OK;
 }
 

@@ -29,6 +29,7 @@ struct Workspace {
     DevBuf<Ctl> ctl;
     DevBuf<double> hist;
     DevBuf<unsigned> pc_counters;
+    DevBuf<double> terms;     // 2n, reductions in MTL4's order (FP64 only)
     Ctl* h_ctl = nullptr;     // pinned mirror for polling
     ~Workspace() {
         if (h_ctl) (void)hipHostFree(h_ctl);

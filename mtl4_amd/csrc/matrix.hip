@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -20,6 +21,9 @@
 namespace mtl4g {
 
 static thread_local std::string g_last_error;
+static std::atomic<int> g_reduction_order{MTL4G_ORDER_TREE};
+
+int reduction_order() { return g_reduction_order.load(std::memory_order_relaxed); }
 
 void set_error(const char* fmt, ...) {
     char buf[1024];
@@ -123,6 +127,7 @@ struct EpiAssign {
 // (lazy(q)= A*p) || (lazy(alpha)= lazy_dot(p,q)), result to *out
 struct EpiSpmvDot {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_FUSED_DOT; }
     struct Pre {
         double p;
     };
@@ -170,6 +175,7 @@ int launch_spmv_f32(const mtl4g_matrix_s* A, int fmt, const float* x, float* y, 
 // ---- vector kernels: block b owns a contiguous chunk; fixed partial layout
 struct EpiDot {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_DOT4; }
     const double *a, *b;
     double* out;
     __device__ bool skip() const { return false; }
@@ -178,15 +184,17 @@ struct EpiDot {
     __device__ void finish(const double* tot) const { *out = tot[0]; }
 };
 
+// unary_dot(v) (reduction<8>) or, with SQRT, two_norm(v) (reduction<4>)
+template <int SQRT>
 struct EpiUnaryDot {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return SQRT ? RO_NORM4 : RO_NORM8; }
     const double* a;
     double* out;
-    int sqrt_result;
     __device__ bool skip() const { return false; }
     __device__ void prepare() {}
     __device__ void operator()(int64_t i, double* part) const { part[0] += a[i] * a[i]; }
-    __device__ void finish(const double* tot) const { *out = sqrt_result ? sqrt(tot[0]) : tot[0]; }
+    __device__ void finish(const double* tot) const { *out = SQRT ? sqrt(tot[0]) : tot[0]; }
 };
 
 struct EpiAxpy {
@@ -202,6 +210,7 @@ struct EpiAxpy {
 
 struct EpiAxpyUnary {
     static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_FUSED_NORM; }
     double alpha;
     const double* q;
     double* r;
@@ -216,17 +225,22 @@ struct EpiAxpyUnary {
     __device__ void finish(const double* tot) const { *out = tot[0]; }
 };
 
-// Stream-ordered scratch for one reduction: tickets, result and partials.
+// Stream-ordered scratch for one reduction: tickets, result, partials and,
+// when reductions run in MTL4's order, n terms.
 struct Scratch {
     static constexpr size_t kTicketBytes = kTicketWords * sizeof(unsigned);
     char* base = nullptr;
     hipStream_t s = nullptr;
+    bool ordered = false;
     unsigned* ticket() { return reinterpret_cast<unsigned*>(base); }
     double* result() { return reinterpret_cast<double*>(base + kTicketBytes); }
     double* partials() { return reinterpret_cast<double*>(base + kTicketBytes + 256); }
-    int get(hipStream_t st) {
+    double* terms() { return ordered ? partials() + kPartialWords : nullptr; }
+    int get(hipStream_t st, int64_t n) {
         s = st;
-        M4_HIP(hipMallocAsync((void**)&base, kTicketBytes + 256 + sizeof(double) * kPartialWords, st));
+        ordered = reduction_order() == MTL4G_ORDER_MTL4;
+        const size_t words = kPartialWords + (ordered ? (size_t)n : 0);
+        M4_HIP(hipMallocAsync((void**)&base, kTicketBytes + 256 + sizeof(double) * words, st));
         M4_HIP(hipMemsetAsync(base, 0, kTicketBytes + 256, st));
         return MTL4G_OK;
     }
@@ -237,7 +251,8 @@ struct Scratch {
 
 template <typename Epi>
 static int run_vec(int64_t n, const Epi& epi, Scratch* sc, hipStream_t s) {
-    return launch_vec<double>(n, epi, sc ? sc->partials() : nullptr, sc ? sc->ticket() : nullptr, s);
+    return launch_vec<double>(n, epi, sc ? sc->partials() : nullptr, sc ? sc->ticket() : nullptr, s,
+                              sc ? sc->terms() : nullptr);
 }
 
 static int fetch(Scratch& sc, double* h, hipStream_t s) {
@@ -274,6 +289,17 @@ const char* mtl4g_status_string(int st) {
 const char* mtl4g_last_error(void) { return g_last_error.c_str(); }
 
 int mtl4g_version(void) { return 100; }
+
+int mtl4g_set_reduction_order(int order) {
+    if (order != MTL4G_ORDER_TREE && order != MTL4G_ORDER_MTL4) {
+        set_error("reduction order %d", order);
+        return MTL4G_ERR_INVALID;
+    }
+    mtl4g::g_reduction_order.store(order, std::memory_order_relaxed);
+    return MTL4G_OK;
+}
+
+int mtl4g_get_reduction_order(void) { return mtl4g::reduction_order(); }
 
 int mtl4g_set_device(int device) {
     M4_HIP(hipSetDevice(device));
@@ -516,8 +542,9 @@ int mtl4g_spmv_dot(mtl4g_matrix A, const double* p, double* q, double* h_dot, vo
     hipStream_t s = S(stream);
     if (A->n == 0) { *h_dot = 0; return MTL4G_OK; }
     Scratch sc;
-    M4_TRY(sc.get(s));
-    M4_TRY(launch_csr<double>(A, A->val.p, p, EpiSpmvDot{p, q, sc.result()}, sc.partials(), sc.ticket(), s));
+    M4_TRY(sc.get(s, A->n));
+    M4_TRY(launch_csr<double>(A, A->val.p, p, EpiSpmvDot{p, q, sc.result()}, sc.partials(), sc.ticket(), s,
+                              sc.terms()));
     return fetch(sc, h_dot, s);
 }
 
@@ -526,7 +553,7 @@ int mtl4g_dot(int64_t n, const double* v, const double* w, double* h, void* stre
     if (n == 0) { *h = 0; return MTL4G_OK; }
     hipStream_t s = S(stream);
     Scratch sc;
-    M4_TRY(sc.get(s));
+    M4_TRY(sc.get(s, n));
     M4_TRY(run_vec(n, EpiDot{v, w, sc.result()}, &sc, s));
     return fetch(sc, h, s);
 }
@@ -536,8 +563,8 @@ int mtl4g_unary_dot(int64_t n, const double* v, double* h, void* stream) {
     if (n == 0) { *h = 0; return MTL4G_OK; }
     hipStream_t s = S(stream);
     Scratch sc;
-    M4_TRY(sc.get(s));
-    M4_TRY(run_vec(n, EpiUnaryDot{v, sc.result(), 0}, &sc, s));
+    M4_TRY(sc.get(s, n));
+    M4_TRY(run_vec(n, EpiUnaryDot<0>{v, sc.result()}, &sc, s));
     return fetch(sc, h, s);
 }
 
@@ -546,8 +573,8 @@ int mtl4g_two_norm(int64_t n, const double* v, double* h, void* stream) {
     if (n == 0) { *h = 0; return MTL4G_OK; }
     hipStream_t s = S(stream);
     Scratch sc;
-    M4_TRY(sc.get(s));
-    M4_TRY(run_vec(n, EpiUnaryDot{v, sc.result(), 1}, &sc, s));
+    M4_TRY(sc.get(s, n));
+    M4_TRY(run_vec(n, EpiUnaryDot<1>{v, sc.result()}, &sc, s));
     return fetch(sc, h, s);
 }
 
@@ -562,7 +589,7 @@ int mtl4g_axpy_unary_dot(int64_t n, double alpha, const double* q, double* r, do
     if (n == 0) { *h = 0; return MTL4G_OK; }
     hipStream_t s = S(stream);
     Scratch sc;
-    M4_TRY(sc.get(s));
+    M4_TRY(sc.get(s, n));
     M4_TRY(run_vec(n, EpiAxpyUnary{alpha, q, r, sc.result()}, &sc, s));
     return fetch(sc, h, s);
 }

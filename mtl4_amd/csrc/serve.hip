@@ -1,0 +1,482 @@
+// serve.hip -- serving loop with continuous batching (no MTL4 counterpart).
+//
+// One host event loop replays the request trace in real time and drives the
+// device:
+//   arrivals  -> FIFO of pending requests
+//   admission -> a request enters only when b, x and workspace for all its
+//                right-hand sides fit the arena (device working-set cap); no
+//                overtaking, so a large request is not starved
+//   launch    -> whenever one of `streams` launch slots is free, every
+//                admitted-and-waiting request (up to max_items right-hand
+//                sides) is packed into one persistent batched launch on that
+//                slot's stream; launches on different slots overlap
+//   complete  -> an event per launch; its requests complete, memory returns
+//                to the arena, waiting requests can be admitted
+// Systems above big_rows rows go to a worker thread that runs the whole-GPU
+// Krylov engine on its own stream, overlapping with the batched launches.
+// Right-hand sides are generated on the device from a per-request seed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "batched.hpp"
+#include "kernels.hpp"
+#include "krylov.hpp"
+#include "precond.hpp"
+
+namespace mtl4g {
+
+// splitmix64 of (seed, i) mapped to [-1, 1)
+__device__ __forceinline__ double hash_uniform(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+struct GenItem {
+    int64_t b_off, x_off;
+    int32_t n, pad;
+    uint64_t seed;
+};
+
+// b = uniform(seed), x = 0 for every item (one workgroup per item)
+__global__ void k_gen_items(const GenItem* __restrict__ items, int nitems, double* base) {
+    for (int k = blockIdx.x; k < nitems; k += gridDim.x) {
+        const GenItem g = items[k];
+        for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
+            base[g.b_off + i] = hash_uniform(g.seed, (uint64_t)i);
+            base[g.x_off + i] = 0.0;
+        }
+    }
+}
+
+// first-fit allocator over the device arena (units: doubles, 64-aligned)
+struct Arena {
+    int64_t cap = 0, used = 0, peak = 0;
+    std::map<int64_t, int64_t> free_;
+    void init(int64_t c) {
+        cap = c;
+        free_.clear();
+        free_[0] = c;
+        used = peak = 0;
+    }
+    static int64_t round(int64_t n) { return (n + 63) / 64 * 64; }
+    bool alloc(int64_t n, int64_t& off) {
+        n = round(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                off = it->first;
+                const int64_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                used += n;
+                peak = std::max(peak, used);
+                return true;
+            }
+        return false;
+    }
+    void release(int64_t off, int64_t n) {
+        n = round(n);
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Req {
+    int32_t id, mat, nrhs, n;
+    double arrival;
+    uint64_t seed;
+    int64_t off = 0, elems = 0;
+    bool big = false;
+};
+
+struct Slot {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    std::vector<int32_t> reqs;
+    int nitems = 0;
+    mtl4g_batch_item* h_items = nullptr;
+    GenItem* h_gen = nullptr;
+    mtl4g_batch_result* h_res = nullptr;
+    DevBuf<mtl4g_batch_item> d_items;
+    DevBuf<GenItem> d_gen;
+    DevBuf<mtl4g_batch_result> d_res;
+    DevBuf<unsigned> d_queue;
+};
+
+}  // namespace mtl4g
+
+struct mtl4g_server_s {
+    mtl4g_batch_s* B = nullptr;
+    mtl4g_server_params prm{};
+    mtl4g::DevBuf<double> arena;
+    std::vector<std::unique_ptr<mtl4g::Slot>> slots;
+    // whole-GPU engine for large systems
+    std::vector<mtl4g_matrix_s*> big_A;
+    std::vector<mtl4g_pc_s*> big_P;
+    hipStream_t big_s = nullptr;
+    ~mtl4g_server_s() {
+        for (auto& slp : slots) {
+            mtl4g::Slot& sl = *slp;
+            if (sl.ev) (void)hipEventDestroy(sl.ev);
+            if (sl.s) (void)hipStreamDestroy(sl.s);
+            if (sl.h_items) (void)hipHostFree(sl.h_items);
+            if (sl.h_gen) (void)hipHostFree(sl.h_gen);
+            if (sl.h_res) (void)hipHostFree(sl.h_res);
+        }
+        for (auto* p : big_P) delete p;
+        for (auto* a : big_A) delete a;
+        if (big_s) (void)hipStreamDestroy(big_s);
+    }
+};
+
+namespace mtl4g {
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Big-system worker: whole-GPU engine, one right-hand side after the other.
+struct BigJob {
+    Req r;
+};
+
+}  // namespace mtl4g
+
+using namespace mtl4g;
+
+extern "C" {
+
+int mtl4g_server_create(mtl4g_server* out, mtl4g_batch B, const mtl4g_server_params* prm) {
+    if (!out || !B || !prm || prm->streams < 1 || prm->streams > 16 || prm->max_items < 1 || prm->cap_bytes <= 0 ||
+        (prm->pc_kind != MTL4G_PC_IDENTITY && prm->pc_kind != MTL4G_PC_DIAGONAL))
+        return MTL4G_ERR_INVALID;
+    *out = nullptr;
+    mtl4g_server_s* S_ = new (std::nothrow) mtl4g_server_s;
+    if (!S_) return MTL4G_ERR_OOM;
+    auto fail = [&](int st) { delete S_; return st; };
+    S_->B = B;
+    S_->prm = *prm;
+    int st = batch_ensure_values(B, prm->precision);
+    if (st) return fail(st);
+    const int64_t cap_elems = (int64_t)(prm->cap_bytes / 8.0);
+    if ((st = S_->arena.alloc((size_t)cap_elems))) return fail(st);
+    for (int k = 0; k < prm->streams; k++) S_->slots.emplace_back(new Slot);
+    for (auto& slp : S_->slots) {
+        Slot& sl = *slp;
+        if (hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking) || hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) ||
+            hipHostMalloc((void**)&sl.h_items, sizeof(mtl4g_batch_item) * prm->max_items, hipHostMallocDefault) ||
+            hipHostMalloc((void**)&sl.h_gen, sizeof(GenItem) * prm->max_items, hipHostMallocDefault) ||
+            hipHostMalloc((void**)&sl.h_res, sizeof(mtl4g_batch_result) * prm->max_items, hipHostMallocDefault))
+            return fail(hip_status(hipErrorOutOfMemory, "server slot", __FILE__, __LINE__));
+        if ((st = sl.d_items.alloc(prm->max_items)) || (st = sl.d_gen.alloc(prm->max_items)) ||
+            (st = sl.d_res.alloc(prm->max_items)) || (st = sl.d_queue.alloc(1)))
+            return fail(st);
+    }
+    // large systems: their own CSR handle and preconditioner for the whole-GPU engine
+    S_->big_A.assign(B->nsys, nullptr);
+    S_->big_P.assign(B->nsys, nullptr);
+    for (int32_t k = 0; k < B->nsys; k++) {
+        const int32_t a = B->h_sys_row[k], e = B->h_sys_row[k + 1], n = e - a;
+        if (n <= prm->big_rows) continue;
+        const int32_t j0 = B->h_rowptr[a], j1 = B->h_rowptr[e];
+        std::vector<int32_t> rp(n + 1), ci(j1 - j0);
+        for (int32_t r = 0; r <= n; r++) rp[r] = B->h_rowptr[a + r] - j0;
+        for (int32_t j = j0; j < j1; j++) ci[j - j0] = B->h_col[j] - a;
+        mtl4g_matrix A = nullptr;
+        if ((st = mtl4g_csr_create(&A, n, n, j1 - j0, rp.data(), ci.data(), B->h_val.data() + j0, 0))) return fail(st);
+        S_->big_A[k] = A;
+        mtl4g_pc P = nullptr;
+        const int fmt = prm->precision == MTL4G_F64 ? MTL4G_F64 : MTL4G_F32;
+        if ((st = mtl4g_pc_create(&P, A, prm->pc_kind, fmt))) return fail(st);
+        S_->big_P[k] = P;
+    }
+    if (hipStreamCreateWithFlags(&S_->big_s, hipStreamNonBlocking)) return fail(MTL4G_ERR_HIP);
+    *out = S_;
+    return MTL4G_OK;
+}
+
+int mtl4g_server_destroy(mtl4g_server S_) {
+    delete S_;
+    return MTL4G_OK;
+}
+
+int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const int32_t* nrhs,
+                     const double* arrival_s, const uint64_t* seed, double* latency_s, int32_t* iters,
+                     double* x_out, const int64_t* x_off, mtl4g_server_stats* stats) {
+    if (!S_ || nreq < 0 || (nreq && (!mat || !nrhs || !arrival_s || !seed || !latency_s))) return MTL4G_ERR_INVALID;
+    mtl4g_batch_s* B = S_->B;
+    const mtl4g_server_params& prm = S_->prm;
+    Arena arena;
+    arena.init((int64_t)S_->arena.n);
+    double* base = S_->arena.p;
+    std::vector<Req> reqs(nreq);
+    std::vector<int64_t> iter_off(nreq + 1, 0);
+    for (int32_t r = 0; r < nreq; r++) {
+        if (mat[r] < 0 || mat[r] >= B->nsys || nrhs[r] < 1 || nrhs[r] > prm.max_items) {
+            set_error("request %d: matrix %d / nrhs %d out of range", r, mat[r], nrhs[r]);
+            return MTL4G_ERR_INVALID;
+        }
+        Req& q = reqs[r];
+        q.id = r;
+        q.mat = mat[r];
+        q.nrhs = nrhs[r];
+        q.n = B->h_sys_row[q.mat + 1] - B->h_sys_row[q.mat];
+        q.arrival = arrival_s[r];
+        q.seed = seed[r];
+        q.big = S_->big_A[q.mat] != nullptr;
+        const int64_t per = Arena::round(q.n) * 2 + (q.big ? 0 : Arena::round(batch_ws_elems(q.n, MTL4G_F64)));
+        q.elems = per * q.nrhs;
+        if (q.elems > arena.cap) {
+            set_error("request %d needs %lld bytes, above the cap", r, (long long)(q.elems * 8));
+            return MTL4G_ERR_INVALID;
+        }
+        iter_off[r + 1] = iter_off[r] + q.nrhs;
+    }
+    std::vector<int32_t> order(nreq);
+    for (int32_t r = 0; r < nreq; r++) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return reqs[a].arrival < reqs[b].arrival; });
+
+    // big-system worker
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int32_t> big_todo;
+    std::deque<std::pair<int32_t, double>> big_done;
+    std::atomic<bool> stop{false};
+    std::atomic<int> worker_err{0};
+    std::atomic<int32_t> big_errors{0}, big_solves{0};
+    Workspace<double> W64;
+    Workspace<float> W32;
+    DevBuf<float> x32;
+    std::thread worker([&]() {
+        (void)hipSetDevice(B->device);
+        for (;;) {
+            int32_t id;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop.load() || !big_todo.empty(); });
+                if (big_todo.empty()) return;
+                id = big_todo.front();
+                big_todo.pop_front();
+            }
+            const Req& q = reqs[id];
+            const int64_t pn = Arena::round(q.n);
+            for (int32_t k = 0; k < q.nrhs; k++) {
+                GenItem g{q.off + k * 2 * pn, q.off + k * 2 * pn + pn, q.n, 0, q.seed * 131 + (uint64_t)k};
+                GenItem* dg = nullptr;
+                if (hipMallocAsync((void**)&dg, sizeof(GenItem), S_->big_s) ||
+                    hipMemcpyAsync(dg, &g, sizeof g, hipMemcpyHostToDevice, S_->big_s)) {
+                    worker_err = MTL4G_ERR_HIP;
+                    break;
+                }
+                hipLaunchKernelGGL(k_gen_items, dim3(1), dim3(256), 0, S_->big_s, dg, 1, base);
+                (void)hipFreeAsync(dg, S_->big_s);
+                double* bb = base + g.b_off;
+                double* xx = base + g.x_off;
+                mtl4g_iteration it{};
+                it.max_iter = prm.max_iter;
+                it.rtol = prm.rtol;
+                // norm_r0 = ||b|| as basic_iteration(b, ...)
+                double nb = 0;
+                if (mtl4g_two_norm(q.n, bb, &nb, S_->big_s)) { worker_err = MTL4G_ERR_HIP; break; }
+                it.norm_r0 = nb;
+                int st;
+                if (prm.precision == MTL4G_F64) {
+                    st = krylov_run<double>(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, MTL4G_F64, W64, xx, bb, &it,
+                                            nullptr, 0, S_->big_s);
+                } else {
+                    if (x32.n < (size_t)q.n && x32.alloc(q.n)) { worker_err = MTL4G_ERR_OOM; break; }
+                    int32_t rf = 0;
+                    st = mixed_run(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, prm.precision, 0.1, 0, W32, x32.p, xx,
+                                   bb, &it, &rf, S_->big_s);
+                }
+                if (st != 0) big_errors++;
+                big_solves++;
+                if (iters) iters[iter_off[id] + k] = it.i;
+            }
+            (void)hipStreamSynchronize(S_->big_s);
+            std::lock_guard<std::mutex> lk(mu);
+            big_done.emplace_back(id, now_s());
+        }
+    });
+
+    const double t0 = now_s();
+    size_t next = 0;
+    std::deque<int32_t> pending;   // arrived, not yet admitted (FIFO)
+    std::deque<int32_t> admitted;  // admitted small requests waiting for a launch slot
+    int32_t completed = 0, launches = 0, max_inflight = 0, errors = 0;
+    int64_t solves = 0;
+    double t_last = t0;
+    int status = MTL4G_OK;
+    auto complete = [&](int32_t id, double t) {
+        latency_s[id] = t - (t0 + reqs[id].arrival);
+        t_last = std::max(t_last, t);
+        if (x_out && x_off) {
+            const Req& q = reqs[id];
+            const int64_t pn = Arena::round(q.n);
+            const int64_t stride = q.big ? 2 * pn : 2 * pn + Arena::round(batch_ws_elems(q.n, MTL4G_F64));
+            for (int32_t k = 0; k < q.nrhs; k++)
+                (void)hipMemcpy(x_out + x_off[id] + (int64_t)k * q.n, base + q.off + k * stride + pn,
+                                (size_t)q.n * 8, hipMemcpyDeviceToHost);
+        }
+        arena.release(reqs[id].off, reqs[id].elems);
+        completed++;
+    };
+    while (completed < nreq && status == MTL4G_OK) {
+        bool progress = false;
+        const double now = now_s();
+        // arrivals
+        while (next < order.size() && t0 + reqs[order[next]].arrival <= now) {
+            pending.push_back(order[next++]);
+            progress = true;
+        }
+        // completions of batched launches
+        int inflight = 0;
+        for (auto& slp : S_->slots) {
+            Slot& sl = *slp;
+            if (!sl.busy) continue;
+            hipError_t q = hipEventQuery(sl.ev);
+            if (q == hipErrorNotReady) {
+                inflight++;
+                continue;
+            }
+            if (q != hipSuccess) {
+                status = hip_status(q, "batched launch", __FILE__, __LINE__);
+                break;
+            }
+            const double t = now_s();
+            int item = 0;
+            for (int32_t id : sl.reqs) {
+                for (int32_t k = 0; k < reqs[id].nrhs; k++, item++) {
+                    const mtl4g_batch_result& res = sl.h_res[item];
+                    if (res.error || res.status) errors++;
+                    if (iters) iters[iter_off[id] + k] = res.iterations;
+                    solves++;
+                }
+                complete(id, t);
+            }
+            sl.busy = false;
+            sl.reqs.clear();
+            progress = true;
+        }
+        // completions of the big-system worker
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            while (!big_done.empty()) {
+                auto d = big_done.front();
+                big_done.pop_front();
+                solves += reqs[d.first].nrhs;
+                complete(d.first, d.second);
+                progress = true;
+            }
+        }
+        if (worker_err) status = worker_err;
+        // admission: FIFO while the working set fits
+        while (!pending.empty()) {
+            Req& q = reqs[pending.front()];
+            int64_t off;
+            if (!arena.alloc(q.elems, off)) break;
+            q.off = off;
+            pending.pop_front();
+            if (q.big) {
+                std::lock_guard<std::mutex> lk(mu);
+                big_todo.push_back(q.id);
+                cv.notify_one();
+            } else {
+                admitted.push_back(q.id);
+            }
+            progress = true;
+        }
+        // launches: every free slot takes what is admitted
+        for (auto& slp : S_->slots) {
+            Slot& sl = *slp;
+            if (sl.busy || admitted.empty()) continue;
+            int nit = 0;
+            while (!admitted.empty() && nit + reqs[admitted.front()].nrhs <= prm.max_items) {
+                const Req& q = reqs[admitted.front()];
+                const int64_t pn = Arena::round(q.n);
+                const int64_t stride = 2 * pn + Arena::round(batch_ws_elems(q.n, MTL4G_F64));
+                for (int32_t k = 0; k < q.nrhs; k++, nit++) {
+                    const int64_t o = q.off + k * stride;
+                    sl.h_gen[nit] = GenItem{o, o + pn, q.n, 0, q.seed * 131 + (uint64_t)k};
+                    sl.h_items[nit] = mtl4g_batch_item{q.mat, prm.max_iter, o, o + pn, o + 2 * pn, prm.rtol, 0.0};
+                }
+                sl.reqs.push_back(q.id);
+                admitted.pop_front();
+            }
+            sl.nitems = nit;
+            hipStream_t s = sl.s;
+            int st = 0;
+            if (hipMemcpyAsync(sl.d_gen.p, sl.h_gen, nit * sizeof(GenItem), hipMemcpyHostToDevice, s) ||
+                hipMemcpyAsync(sl.d_items.p, sl.h_items, nit * sizeof(mtl4g_batch_item), hipMemcpyHostToDevice, s))
+                st = MTL4G_ERR_HIP;
+            if (!st) {
+                hipLaunchKernelGGL(k_gen_items, dim3(std::min(nit, 4096)), dim3(256), 0, s, sl.d_gen.p, nit, base);
+                st = batch_launch(B, prm.pc_kind, prm.precision, B->shape[prm.precision == MTL4G_F64 ? 0 : prm.precision == MTL4G_F32 ? 1 : 2],
+                                  sl.d_items.p, nit, base, base, base, sl.d_res.p, sl.d_queue.p, s);
+            }
+            if (!st && (hipMemcpyAsync(sl.h_res, sl.d_res.p, nit * sizeof(mtl4g_batch_result), hipMemcpyDeviceToHost, s) ||
+                        hipEventRecord(sl.ev, s)))
+                st = MTL4G_ERR_HIP;
+            if (st) {
+                status = st;
+                break;
+            }
+            sl.busy = true;
+            launches++;
+            progress = true;
+        }
+        int busy = 0;
+        for (auto& slp : S_->slots) busy += slp->busy;
+        max_inflight = std::max(max_inflight, busy);
+        if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+        cv.notify_all();
+    }
+    worker.join();
+    for (auto& slp : S_->slots) {
+        if (slp->busy) (void)hipStreamSynchronize(slp->s);
+        slp->busy = false;
+        slp->reqs.clear();
+    }
+    if (stats) {
+        stats->wall_s = t_last - t0;
+        stats->solves = solves;
+        stats->launches = launches;
+        stats->max_inflight = max_inflight;
+        stats->peak_bytes = (double)arena.peak * 8.0;
+        stats->errors = errors + big_errors.load();
+        stats->big_solves = big_solves.load();
+    }
+    return status;
+}
+
+}  // extern "C"

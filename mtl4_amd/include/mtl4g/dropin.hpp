@@ -156,7 +156,10 @@ class dense_vector {
     template <std::size_t N>
     explicit dense_vector(const Value (&a)[N]) : h_(a, a + N) {}
     dense_vector(std::initializer_list<Value> l) : h_(l) {}
-    dense_vector(const dense_vector& o) : h_(o.begin_(), o.begin_() + o.size()) {}
+    dense_vector(const dense_vector& o) {
+        const Value* p = o.host_read_();   // the live copy may be on the device
+        h_.assign(p, p + o.size());
+    }
     dense_vector(dense_vector&& o) noexcept { swap_(o); }
     template <typename Expr, typename = decltype(std::declval<const Expr&>().assign_to(std::declval<dense_vector&>()))>
     dense_vector(const Expr& e) {

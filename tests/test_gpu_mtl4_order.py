@@ -1,0 +1,160 @@
+"""MTL4-order reductions: with mtl4g_set_reduction_order(MTL4G_ORDER_MTL4)
+every FP64 reduction sums in MTL4's unrolled sequential order, so the
+reductions and whole cg / bicgstab solves must reproduce the CPU oracle
+(MTL4's algorithm at -ffp-contract=off) bit for bit: same iteration count,
+same final residual, same solution bits.
+
+The data are chosen order-sensitive (magnitudes spread over 16 decades): a
+sum in any other order -- the default tree order, or the oracle with another
+unroll factor -- gives different bits, which each test checks first."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import orc
+from helpers import family, pair
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 4, 5, 7, 8, 9, 15, 16, 17, 33, 1000, 4099, 100003]
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mtl4_amd
+    return mtl4_amd
+
+
+@pytest.fixture
+def mtl4_order(m):
+    with m.reduction_order(m.ORDER_MTL4):
+        yield
+    assert m.get_reduction_order() == m.ORDER_TREE
+
+
+def dv(m, a):
+    return m.dense_vector(np.asarray(a, dtype=np.float64))
+
+
+def wide(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) * 10.0 ** rng.uniform(-8, 8, n)
+
+
+def test_order_switch(m):
+    assert m.get_reduction_order() == m.ORDER_TREE
+    with pytest.raises(m.logic_error):
+        m.set_reduction_order(7)
+    with m.reduction_order(m.ORDER_MTL4):
+        assert m.get_reduction_order() == m.ORDER_MTL4
+    assert m.get_reduction_order() == m.ORDER_TREE
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_reductions_bitwise(m, mtl4_order, n):
+    a, b = wide(n, n), wide(n, n + 1)
+    va, vb = dv(m, a), dv(m, b)
+    assert m.dot(va, vb) == orc.dot(a, b, 4)
+    assert m.unary_dot(va) == orc.unary_dot(a, 8)      # unary_dot<8> is MTL4's default
+    assert m.two_norm(va) == orc.two_norm(a)           # two_norm<4>
+    if n >= 100:   # the data must tell the orders apart
+        assert orc.dot(a, b, 4) != orc.dot(a, b, 8) or orc.dot(a, b, 4) != orc.dot(a, b, 1)
+        assert orc.unary_dot(a, 8) != orc.unary_dot(a, 4)
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_fused_axpy_unary_dot_bitwise(m, mtl4_order, n):
+    q, r = wide(n, 3 * n), wide(n, 3 * n + 1)
+    alpha = 0.7310585786300049
+    rg = dv(m, r)
+    rho = C.c_double()
+    m._lib.check(m._lib.lib().mtl4g_axpy_unary_dot(n, alpha, dv(m, q).ptr, rg.ptr, C.byref(rho),
+                                                   m._stream()))
+    ro = r.copy()
+    rho_o = orc.fused_axpy_unary_dot(alpha, q, ro)
+    assert np.array_equal(rg.numpy(), ro)
+    assert rho.value == rho_o
+
+
+@pytest.mark.parametrize("fam", ["lap2d_64", "random_3000", "long_rows", "diff3d_16"])
+def test_fused_spmv_dot_bitwise(m, mtl4_order, fam):
+    rp, ci, va = family(fam, seed=5)
+    O, A = pair(rp, ci, va)
+    p = wide(A.num_rows(), 17)
+    q = m.dense_vector(A.num_rows())
+    d = C.c_double()
+    m._lib.check(m._lib.lib().mtl4g_spmv_dot(A.h, dv(m, p).ptr, q.ptr, C.byref(d), m._stream()))
+    qo, do = orc.fused_spmv_dot(O, p)
+    assert np.array_equal(q.numpy(), qo)
+    assert d.value == do
+
+
+def _solve_both(m, method, fam, kind, seed, max_iter=3000):
+    rp, ci, va = family(fam, seed=seed)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(seed + 100).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OP = None if kind == "identity" else orc.Preconditioner(kind, O)
+    xo, st, ito, _ = getattr(orc, method)(O, b, OP, max_iter=max_iter, rtol=1e-10)
+    P = getattr(m.pc, kind)(A)
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), max_iter, 1e-10)
+    err = getattr(m, method)(A, x, dv(m, b), P, it)
+    return (err, it, x.numpy()), (st, ito, xo)
+
+
+@pytest.mark.parametrize("fam", ["lap2d_64", "diff3d_16", "spd_2000", "lap3d_12"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ic_0", "ilu_0"])
+def test_cg_bitwise(m, mtl4_order, fam, kind):
+    (err, it, x), (st, ito, xo) = _solve_both(m, "cg", fam, kind, seed=2)
+    assert err == st == 0
+    assert it.norm_r0() == ito.norm_r0
+    assert it.iterations() == ito.i
+    assert it.resid() == ito.resid
+    assert np.array_equal(x, xo)
+
+
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ilu_0"])
+def test_bicgstab_bitwise(m, mtl4_order, fam, kind):
+    (err, it, x), (st, ito, xo) = _solve_both(m, "bicgstab", fam, kind, seed=4, max_iter=2000)
+    assert err == st == 0
+    assert it.iterations() == ito.i
+    assert it.resid() == ito.resid
+    assert np.array_equal(x, xo)
+
+
+def test_cg_config1_bitwise(m, mtl4_order):
+    """CG + diagonal on the 64x64 Laplacian, the reference's CPU test case."""
+    rp, ci, va = family("lap2d_64")
+    O, A = pair(rp, ci, va)
+    xt = np.random.default_rng(0).uniform(-1, 1, 4096)
+    b = orc.spmv(O, xt)
+    xo, st, ito, ho = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=2000, rtol=1e-10,
+                             hist_cap=4000)
+    x = m.dense_vector(4096)
+    it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+    assert m.cg(A, x, dv(m, b), m.pc.diagonal(A), it) == st == 0
+    assert it.iterations() == ito.i and it.resid() == ito.resid
+    assert np.array_equal(x.numpy(), xo)
+
+
+def test_solver_handle_follows_order(m):
+    """A solver made in tree order and run in MTL4 order gets its terms
+    buffer on first use and then matches the oracle bit for bit."""
+    rp, ci, va = family("lap2d_64")
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.ones(4096))
+    S = m.Solver(A, m.pc.diagonal(A), method=m.CG)
+    x = m.dense_vector(4096)
+    it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+    S.solve(x, dv(m, b), it)
+    with m.reduction_order(m.ORDER_MTL4):
+        x2 = m.dense_vector(4096)
+        it2 = m.basic_iteration(dv(m, b), 2000, 1e-10)
+        S.solve(x2, dv(m, b), it2)
+    xo, st, ito, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=2000, rtol=1e-10)
+    assert it2.iterations() == ito.i and np.array_equal(x2.numpy(), xo)
+    assert abs(it.iterations() - ito.i) <= 2
