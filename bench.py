@@ -121,7 +121,12 @@ def main():
     t_rnd = timed_spmv(m, torch, R, xr, yr, args.spmv_reps)
     rnd_gbs = spmv_bytes(2_000_000, len(rr[1])) / t_rnd / 1e9
     traffic = load_traffic()
-    kname = ("k_csrv<%d lanes/row>" % kern[1]) if kern[0] == m._lib.SPMV_VECTOR else "k_csr (LDS tiles)"
+    if kern[0] == m._lib.SPMV_VECTOR:
+        kname = "k_csrv<%d lanes/row>" % kern[1]
+    elif kern[0] == m._lib.SPMV_STREAM:
+        kname = "k_csrp (persistent LDS tiles, %d workgroups/CU, %d rows/block)" % (kern[1], kern[2])
+    else:
+        kname = "k_csr (LDS tiles, %d rows/block)" % kern[2]
     roofline = {"bound": "hbm", "kernel": "%s<EpiAssign> (y = A*x, FP64, diffusion3d %d^3)" % (kname, g),
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,

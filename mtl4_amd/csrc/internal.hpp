@@ -13,6 +13,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -274,6 +275,8 @@ struct mtl4g_matrix_s {
     int gs = 8;                            // vector kernel: lanes per row
     int vrows = 256;                       // vector kernel: rows per workgroup
     int rows_per_block = 0, nnz_per_block = 0;
+    int cus = 256;                         // compute units of the device
+    int per_cu = 4;                        // stream kernel: workgroups per CU
     int32_t max_row = 0;
     std::vector<int32_t> h_rowptr, h_col;   // host pattern (factorizations)
     std::vector<double> h_val;

@@ -101,6 +101,105 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
     if constexpr (Epi::NP > 0 && !ORD) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
+// Persistent row-block kernel with the next tile's stream in flight.
+// Workgroup g runs on XCD g % 8 and owns every (gridDim/8)-th row block of
+// that XCD's contiguous eighth of the matrix, so the XCD's workgroups sweep
+// one window of rows together and share its x in L2.  Per tile, a lane
+// issues the gathers of x for the current tile, then the descriptor, row
+// bounds, epilogue operands and value/index stream of its next tile, and
+// only then waits for the gathers: the next tile's HBM reads overlap the
+// current tile's gathers, products, row sums and epilogue.  Products go to
+// one of two LDS buffers in turn, so one barrier per tile suffices.  Rows
+// are summed in column order, as in k_csr; a longer-than-TILE single row is
+// summed by the whole workgroup (not overlapped).
+template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ col,
+                                             const MT* __restrict__ val,
+                                             const XT* __restrict__ x,
+                                             const RowBlock* __restrict__ desc, int nb, Epi epi,
+                                             AT* partials, unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    constexpr int U = TILE / BS;
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    __shared__ AT prod[2][TILE];
+    __shared__ AT sh[BS / kWave];
+    const int t = threadIdx.x;
+    const int slots = gridDim.x / kXcd;
+    const int per = (nb + kXcd - 1) / kXcd;
+    const int xcd = blockIdx.x % kXcd;
+    const int hi = min(nb, (xcd + 1) * per);
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+
+    RowBlock d{};
+    int c[U];
+    AT v[U];
+    int ra = 0, re = 0;
+    typename Epi::Pre pre{};
+    auto fetch = [&](int bb) {
+        d = desc[bb];
+        const int row = d.r0 + t;
+        const int nz = d.j1 - d.j0;
+        if (row < d.r1) {
+            ra = rowptr[row] - d.j0;
+            re = rowptr[row + 1] - d.j0;
+            pre = epi.preload(row);
+        }
+        if (nz <= TILE) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = t + u * BS;
+                c[u] = k < nz ? __builtin_nontemporal_load(col + d.j0 + k) : 0;
+                v[u] = k < nz ? load_val_nt<AT>(val, (int64_t)d.j0 + k) : AT(0);
+            }
+        }
+    };
+
+    int b = xcd * per + blockIdx.x / kXcd;
+    if (b < hi) fetch(b);
+    int buf = 0;
+    while (b < hi) {
+        const RowBlock dc = d;
+        const int nz = dc.j1 - dc.j0;
+        const int row = dc.r0 + t;
+        const int a = ra, e = re;
+        const typename Epi::Pre pc = pre;
+        const int bn = b + slots;
+        if (nz > TILE) {
+            AT s = AT(0);
+            for (int k = t; k < nz; k += BS)
+                s += load_val<AT>(val, (int64_t)dc.j0 + k) * static_cast<AT>(x[col[dc.j0 + k]]);
+            s = block_sum<BS>(s, sh);
+            if (bn < hi) fetch(bn);
+            if (t == 0) epi(row, s, pc, part);
+        } else {
+            AT g[U], vc[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = t + u * BS;
+                g[u] = k < nz ? static_cast<AT>(x[c[u]]) : AT(0);
+                vc[u] = v[u];
+            }
+            if (bn < hi) fetch(bn);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = t + u * BS;
+                if (k < nz) prod[buf][k] = vc[u] * g[u];
+            }
+            __syncthreads();
+            AT s = AT(0);
+            for (int k = a; k < e; k++) s += prod[buf][k];
+            if (row < dc.r1) epi(row, s, pc, part);
+            buf ^= 1;
+        }
+        b = bn;
+    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
+}
+
 // Vector CSR: a group of GS lanes per row, 64/GS rows per wavefront pass.
 // The lanes of a group read consecutive non-zeros of their row, so a
 // wavefront's value/index loads cover consecutive rows contiguously; no LDS,
@@ -323,6 +422,19 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
         return MTL4G_OK;
     }
     if (A->nb == 0) return MTL4G_OK;
+    if (A->kernel == MTL4G_SPMV_STREAM) {
+        const int want = A->cus * A->per_cu;
+        const int grid = std::max(kXcd, std::min(want, (A->nb + kXcd - 1) / kXcd * kXcd) / kXcd * kXcd);
+        const int tile = A->nnz_per_block;
+#define M4_LAUNCHP(T)                                                                          \
+    hipLaunchKernelGGL((k_csrp<256, T, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,         \
+                       A->rowptr.p, A->col.p, val, x, A->desc.p, A->nb, epi, partials, ticket)
+        if (tile <= 1024) M4_LAUNCHP(1024);
+        else M4_LAUNCHP(2048);
+#undef M4_LAUNCHP
+        M4_HIP(hipGetLastError());
+        return MTL4G_OK;
+    }
     const int per = (A->nb + kXcd - 1) / kXcd;
     const int grid = per * kXcd;
     if (Epi::NP > 0 && grid > kMaxPartials) {

@@ -356,6 +356,8 @@ int mtl4g_csr_create(mtl4g_matrix* out, int32_t nrows, int32_t ncols, int64_t nn
     if (!A) return MTL4G_ERR_OOM;
     auto fail = [&](int st) { delete A; return st; };
     (void)hipGetDevice(&A->device);
+    (void)hipDeviceGetAttribute(&A->cus, hipDeviceAttributeMultiprocessorCount, A->device);
+    if (A->cus <= 0) A->cus = 256;
     A->n = nrows;
     A->m = ncols;
     A->nnz = nnz;
@@ -458,6 +460,17 @@ int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block) {
     if (!A) return MTL4G_ERR_INVALID;
     if (kernel == MTL4G_SPMV_TILE) return mtl4g_csr_tune(A, rows_per_block, 0);
+    if (kernel == MTL4G_SPMV_STREAM) {
+        if (group == 0) group = A->per_cu;
+        if (group < 1 || group > 8) {
+            set_error("stream kernel: 1 to 8 workgroups per compute unit");
+            return MTL4G_ERR_INVALID;
+        }
+        if (rows_per_block > 0 || !A->desc.p) M4_TRY(build_row_blocks(A, rows_per_block, 0));
+        A->kernel = MTL4G_SPMV_STREAM;
+        A->per_cu = group;
+        return MTL4G_OK;
+    }
     if (kernel != MTL4G_SPMV_VECTOR) return MTL4G_ERR_INVALID;
     if (group == 0) group = A->gs;
     if (group != 4 && group != 8 && group != 16 && group != 32 && group != 64) {
@@ -474,7 +487,7 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block) {
     if (!A) return MTL4G_ERR_INVALID;
     if (kernel) *kernel = A->kernel;
-    if (group) *group = A->kernel == MTL4G_SPMV_VECTOR ? A->gs : 0;
+    if (group) *group = A->kernel == MTL4G_SPMV_VECTOR ? A->gs : A->kernel == MTL4G_SPMV_STREAM ? A->per_cu : 0;
     if (rows_per_block) *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR ? A->vrows : A->rows_per_block;
     return MTL4G_OK;
 }
@@ -484,11 +497,15 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     hipStream_t s = S(stream);
     if (A->n == 0) return MTL4G_OK;
     struct Cand {
-        int kernel, a, b;
+        int kernel, a, b, rows, nnz;   // stream kernel: a = workgroups per CU on (rows, nnz) blocks
     };
-    std::vector<Cand> cands = {{MTL4G_SPMV_TILE, 256, 2048}, {MTL4G_SPMV_TILE, 128, 1024}};
+    std::vector<Cand> cands = {{MTL4G_SPMV_TILE, 256, 2048, 0, 0}, {MTL4G_SPMV_TILE, 128, 1024, 0, 0}};
+    // persistent grids stay within what one CU holds at once (LDS: two
+    // product buffers of TILE doubles per workgroup)
+    for (int f : {2, 3, 4}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 256, 2048});
+    for (int f : {4, 6, 8}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 128, 1024});
     for (int g : {4, 8, 16, 32, 64})
-        for (int f : {1, 2, 4}) cands.push_back({MTL4G_SPMV_VECTOR, g, std::max(4, f * 1024 / g)});
+        for (int f : {1, 2, 4}) cands.push_back({MTL4G_SPMV_VECTOR, g, std::max(4, f * 1024 / g), 0, 0});
     double *x = nullptr, *y = nullptr;
     M4_HIP(hipMallocAsync((void**)&x, (size_t)A->m * 8 + 8, s));
     M4_HIP(hipMallocAsync((void**)&y, (size_t)A->n * 8 + 8, s));
@@ -499,7 +516,10 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     double best = 1e30;
     Cand win = cands[0];
     for (const Cand& c : cands) {
-        int st = c.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, c.a, c.b) : mtl4g_csr_set_kernel(A, c.kernel, c.a, c.b);
+        int st = c.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, c.a, c.b)
+                 : c.kernel == MTL4G_SPMV_STREAM ? (mtl4g_csr_tune(A, c.rows, c.nnz) ||
+                                                    mtl4g_csr_set_kernel(A, c.kernel, c.a, 0))
+                                                 : mtl4g_csr_set_kernel(A, c.kernel, c.a, c.b);
         if (st) continue;
         if (launch_spmv(A, x, y, MTL4G_ASSIGN, s)) continue;   // warm-up
         const int reps = 5;
@@ -519,6 +539,7 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     (void)hipEventDestroy(e1);
     (void)hipFreeAsync(x, s);
     (void)hipFreeAsync(y, s);
+    if (win.kernel == MTL4G_SPMV_STREAM) M4_TRY(mtl4g_csr_tune(A, win.rows, win.nnz));
     M4_TRY(win.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, win.a, win.b)
                                          : mtl4g_csr_set_kernel(A, win.kernel, win.a, win.b));
     if (h_best_us) *h_best_us = best;

@@ -75,6 +75,41 @@ def test_spmv_vector_groups_bitwise(m, fam, group):
     assert np.array_equal(w.numpy(), yr)
 
 
+@pytest.mark.parametrize("shape", [(256, 2048, 2), (256, 2048, 4), (128, 1024, 8), (64, 1024, 1)])
+@pytest.mark.parametrize("fam", ["lap2d_5", "lap2d_64", "random_3000", "long_rows", "diff3d_16"])
+def test_spmv_stream_bitwise(m, fam, shape):
+    """Persistent stream kernel: same bits as MTL4's row sums, for =, -=,
+    and fused into CG's SpMV+dot (whole solve against the tile kernel)."""
+    rows, nnz, per_cu = shape
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    A.tune(rows, nnz)
+    A.set_kernel(m._lib.SPMV_STREAM, per_cu, 0)
+    assert A.get_kernel()[:2] == (m._lib.SPMV_STREAM, per_cu)
+    x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    y0 = np.random.default_rng(9).uniform(-1, 1, A.num_rows())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+    w = dv(m, y0)
+    A.mult(dv(m, x), w, m.MINUS)
+    yr = y0.copy()
+    orc.spmv(O, x, yr, orc.MINUS)
+    assert np.array_equal(w.numpy(), yr)
+    if fam in ("lap2d_64", "diff3d_16"):
+        b = orc.spmv(O, x)
+        sols = []
+        for kernel in (m._lib.SPMV_STREAM, m._lib.SPMV_TILE):
+            if kernel == m._lib.SPMV_TILE:
+                A.tune(rows, nnz)
+            xs = m.dense_vector(A.num_rows())
+            it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+            assert m.cg(A, xs, dv(m, b), m.pc.diagonal(A), it) == 0
+            sols.append((it.iterations(), xs.numpy()))
+        # same partition, same partial-sum order only when the grids agree;
+        # the bar is the CG one: +-2 iterations, solutions within 1e-9
+        assert abs(sols[0][0] - sols[1][0]) <= 2
+        assert np.linalg.norm(sols[0][1] - sols[1][1]) <= 1e-9 * np.linalg.norm(sols[1][1])
+
+
 def test_spmv_autotune_keeps_results(m):
     rp, ci, va = family("random_3000", seed=2)
     O, A = pair(rp, ci, va)
