@@ -293,10 +293,16 @@ int64_t mtl4g_batch_workspace_elems(int32_t n, int precision);
 int mtl4g_batch_set_shape(mtl4g_batch B, int precision, int32_t threads, int32_t per_cu);
 int mtl4g_batch_get_shape(mtl4g_batch B, int precision, int32_t* threads, int32_t* per_cu);
 /* Every system once: x and b laid out like the rows (d_x + sys_rowptr[k]);
- * systems run largest first; h_results (may be NULL) indexed by system. */
+ * systems run largest first; h_results (may be NULL) indexed by system.
+ * The launch (queue reset + persistent kernel) is a captured hipGraph,
+ * recorded once per preconditioner, precision, shape and buffer set and
+ * replayed after that; it runs on a stream of the batch's own, ordered
+ * after the work already on `stream` and before the work enqueued next. */
 int mtl4g_batch_solve(mtl4g_batch B, int pc_kind, int precision, double* d_x, const double* d_b,
                       double rtol, double atol, int32_t max_iter, mtl4g_batch_result* h_results,
                       void* stream);
+/* Number of graph captures and replays made by mtl4g_batch_solve so far. */
+int mtl4g_batch_graph_info(mtl4g_batch B, int64_t* captures, int64_t* replays);
 /* Autotuner: times the whole batch for every launch shape (x reset from
  * d_x0 each time) and keeps the fastest for this precision. */
 int mtl4g_batch_tune(mtl4g_batch B, int pc_kind, int precision, double* d_x, const double* d_x0,

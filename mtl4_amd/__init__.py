@@ -558,6 +558,12 @@ class Batch:
         check(_L().mtl4g_batch_get_shape(self.h, precision, C.byref(t), C.byref(pcu)))
         return t.value, pcu.value
 
+    def graph_info(self):
+        """(captures, replays) of the solve-all graph."""
+        c, r = C.c_int64(), C.c_int64()
+        check(_L().mtl4g_batch_graph_info(self.h, C.byref(c), C.byref(r)), "graph_info")
+        return c.value, r.value
+
     def set_shape(self, precision, threads, per_cu):
         check(_L().mtl4g_batch_set_shape(self.h, precision, threads, per_cu), "set_shape")
 

@@ -118,6 +118,7 @@ _SIGS = {
     "mtl4g_batch_create": (_INT, [_PP, _I32, _P, _P, _P, _P, _INT]),
     "mtl4g_batch_destroy": (_INT, [_P]),
     "mtl4g_batch_info": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64)]),
+    "mtl4g_batch_graph_info": (_INT, [_P, C.POINTER(_I64), C.POINTER(_I64)]),
     "mtl4g_batch_workspace_elems": (_I64, [_I32, _INT]),
     "mtl4g_batch_set_shape": (_INT, [_P, _INT, _I32, _I32]),
     "mtl4g_batch_get_shape": (_INT, [_P, _INT, C.POINTER(_I32), C.POINTER(_I32)]),

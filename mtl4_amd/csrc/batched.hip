@@ -378,6 +378,61 @@ static int prepare_all(mtl4g_batch_s* Bt, int precision, int32_t max_iter, doubl
     return MTL4G_OK;
 }
 
+// Solve-all through a captured graph: recorded once per (preconditioner,
+// precision, shape, buffers) and replayed; the queue is reset by a kernel
+// inside the graph (no memset node).
+static int batch_graph_launch(mtl4g_batch_s* B, int pc_kind, int precision, const double* b, double* x,
+                              hipStream_t s) {
+    const BatchShape sh = B->shape[prec_index(precision)];
+    BatchGraphKey key;
+    key.pc_kind = pc_kind;
+    key.precision = precision;
+    key.bs = sh.bs;
+    key.per_cu = sh.per_cu;
+    key.b = b;
+    key.x = x;
+    key.items = B->items.p;
+    key.work = B->work.p;
+    key.results = B->results.p;
+    M4_TRY(batch_ensure_values(B, precision));   // allocations stay outside the capture
+    if (!B->gstream) {
+        M4_HIP(hipStreamCreateWithFlags(&B->gstream, hipStreamNonBlocking));
+        M4_HIP(hipEventCreateWithFlags(&B->gev[0], hipEventDisableTiming));
+        M4_HIP(hipEventCreateWithFlags(&B->gev[1], hipEventDisableTiming));
+    }
+    if (!B->gexec || !(key == B->gkey)) {
+        if (B->gexec) {
+            (void)hipGraphExecDestroy(B->gexec);
+            B->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        M4_HIP(hipStreamBeginCapture(B->gstream, hipStreamCaptureModeThreadLocal));
+        const int st = batch_launch(B, pc_kind, precision, sh, B->items.p, B->nsys, b, x, B->work.p,
+                                    B->results.p, B->queue.p, B->gstream);
+        const hipError_t e = hipStreamEndCapture(B->gstream, &g);
+        if (st != MTL4G_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return hip_status(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        const hipError_t ei = hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            B->gexec = nullptr;
+            return hip_status(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        B->gkey = key;
+        B->graph_captures++;
+    }
+    M4_HIP(hipEventRecord(B->gev[0], s));
+    M4_HIP(hipStreamWaitEvent(B->gstream, B->gev[0], 0));
+    M4_HIP(hipGraphLaunch(B->gexec, B->gstream));
+    M4_HIP(hipEventRecord(B->gev[1], B->gstream));
+    M4_HIP(hipStreamWaitEvent(s, B->gev[1], 0));
+    B->graph_launches++;
+    return MTL4G_OK;
+}
+
 }  // namespace mtl4g
 
 using namespace mtl4g;
@@ -464,6 +519,13 @@ int mtl4g_batch_solve_items(mtl4g_batch B, int pc_kind, int precision, const mtl
                         results, queue, S(stream));
 }
 
+int mtl4g_batch_graph_info(mtl4g_batch B, int64_t* captures, int64_t* replays) {
+    if (!B) return MTL4G_ERR_INVALID;
+    if (captures) *captures = B->graph_captures;
+    if (replays) *replays = B->graph_launches;
+    return MTL4G_OK;
+}
+
 int mtl4g_batch_info(mtl4g_batch B, int32_t* nsys, int32_t* nrows, int64_t* nnz) {
     if (!B) return MTL4G_ERR_INVALID;
     if (nsys) *nsys = B->nsys;
@@ -494,8 +556,7 @@ int mtl4g_batch_solve(mtl4g_batch B, int pc_kind, int precision, double* d_x, co
         return MTL4G_ERR_INVALID;
     hipStream_t s = S(stream);
     M4_TRY(prepare_all(B, precision, max_iter, rtol, atol, s));
-    M4_TRY(batch_launch(B, pc_kind, precision, B->shape[prec_index(precision)], B->items.p, B->nsys, d_b, d_x,
-                        B->work.p, B->results.p, B->queue.p, s));
+    M4_TRY(batch_graph_launch(B, pc_kind, precision, d_b, d_x, s));
     if (h_results) {
         // results come back in item order; report them per system
         std::vector<mtl4g_batch_result> res(B->nsys);

@@ -22,6 +22,16 @@ struct BatchShape {
     int per_cu = 2;          // resident workgroups per CU
 };
 
+// What a captured solve-all graph was recorded with (kernel arguments).
+struct BatchGraphKey {
+    int pc_kind = -1, precision = -1, bs = 0, per_cu = 0;
+    const void *b = nullptr, *x = nullptr, *items = nullptr, *work = nullptr, *results = nullptr;
+    bool operator==(const BatchGraphKey& o) const {
+        return pc_kind == o.pc_kind && precision == o.precision && bs == o.bs && per_cu == o.per_cu &&
+               b == o.b && x == o.x && items == o.items && work == o.work && results == o.results;
+    }
+};
+
 }  // namespace mtl4g
 
 struct mtl4g_batch_s {
@@ -44,6 +54,19 @@ struct mtl4g_batch_s {
     mtl4g::BatchShape shape[3];   // per precision (F64, F32, BF16)
     bool tuned[3] = {false, false, false};
     int cus = 0;
+    // solve-all as a captured graph (queue reset + persistent kernel),
+    // replayed on a stream of its own ordered after and before the caller's
+    hipGraphExec_t gexec = nullptr;
+    mtl4g::BatchGraphKey gkey;
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev[2] = {nullptr, nullptr};
+    int64_t graph_launches = 0, graph_captures = 0;
+    ~mtl4g_batch_s() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (hipEvent_t e : gev)
+            if (e) (void)hipEventDestroy(e);
+        if (gstream) (void)hipStreamDestroy(gstream);
+    }
 };
 
 namespace mtl4g {

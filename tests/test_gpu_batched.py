@@ -131,3 +131,46 @@ def test_batch_rejects_coupled_systems(m):
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     st = m._lib.lib().mtl4g_batch_create(C.byref(h), 2, p(sys_row), p(rp), p(ci), p(va), 0)
     assert st == m._lib.ERR_INVALID
+
+
+def test_batch_graph_replays_match_direct_launch(m):
+    """mtl4g_batch_solve records the launch once as a graph and replays it:
+    three replays with eager work and synchronizations in between must each
+    give the bits of a direct (uncaptured) launch of the same items."""
+    import ctypes as C
+    import torch
+    systems = make_systems(24, seed=13)
+    bs = rhs(systems, seed=8)
+    B = m.Batch(systems)
+    b = m.dense_vector(np.concatenate(bs))
+    x = m.dense_vector(B.nrows)
+    outs = []
+    for rep in range(3):
+        x.t.zero_()
+        res = B.solve(x, b, precision=m.F64, rtol=1e-10)
+        outs.append((x.numpy(), [r["iterations"] for r in res]))
+        junk = torch.randn(1 << 20, device="cuda", dtype=torch.float64)
+        torch.cuda.synchronize()
+        assert float(junk.abs().sum()) > 0          # eager work between replays
+    assert B.graph_info() == (1, 3)
+    # direct launch of the same items (device item list from the last solve)
+    L = m._lib.lib()
+    items = (m._lib.BatchItem * B.nsys)()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    order = sorted(range(B.nsys), key=lambda k: -(len(systems[k][1]) * np.sqrt(len(systems[k][0]) - 1)))
+    ws = np.concatenate([[0], np.cumsum([L.mtl4g_batch_workspace_elems(len(s[0]) - 1, m.F64) for s in systems])])
+    for j, k in enumerate(order):
+        items[j] = m._lib.BatchItem(k, 100000, int(B.sys_row[k]), int(B.sys_row[k]), int(ws[k]), 1e-10, 0.0)
+    ditems = torch.empty(C.sizeof(items), dtype=torch.uint8, device="cuda")
+    ditems.copy_(torch.frombuffer(bytearray(items), dtype=torch.uint8))
+    work = torch.zeros(int(ws[-1]), dtype=torch.float64, device="cuda")
+    dres = torch.zeros(B.nsys * C.sizeof(m._lib.BatchResult), dtype=torch.uint8, device="cuda")
+    queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+    xd = m.dense_vector(B.nrows)
+    m._lib.check(L.mtl4g_batch_solve_items(B.h, m._lib.PC_DIAGONAL, m.F64, ditems.data_ptr(), B.nsys,
+                                           b.ptr, xd.ptr, work.data_ptr(), dres.data_ptr(),
+                                           queue.data_ptr(), m._stream()))
+    torch.cuda.synchronize()
+    for xs, its in outs:
+        assert np.array_equal(xs, xd.numpy())
+        assert its == outs[0][1]
