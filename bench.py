@@ -77,6 +77,168 @@ def cpu_baseline(rp, ci, va, b, iters_gpu, cpu_iters):
                       "scaled to the GPU's %d iterations" % (cpu_iters, dt, per_iter * 1e3, iters_gpu)}
 
 
+def guarded(fn):
+    """Run one extra measurement; a failure is recorded, not fatal."""
+    try:
+        return fn()
+    except Exception as e:   # the headline line must still be printed
+        return {"error": "%s: %s" % (type(e).__name__, e)}
+
+
+def relres_of(m, A, x, b, bnorm):
+    r = m.dense_vector(A.num_rows())
+    r.assign(b)
+    A.mult(x, r, m.MINUS)
+    return m.two_norm(r) / bnorm
+
+
+def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
+    """One FP64 solve and one mixed-precision (FP32 values, reliable
+    updates) solve of A x = b with the given preconditioner; relative
+    residuals in FP64 and the mixed solution's error against the FP64 one."""
+    n = A.num_rows()
+    P64 = getattr(m.pc, pc)(A)
+    P32 = getattr(m.pc, pc)(A, m.F32)
+    S = m.Solver(A, P64, method=method)
+    x = m.dense_vector(n)
+    it = m.basic_iteration(bnorm, 100000, 1e-10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    err = S.solve(x, b, it)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    xm = m.dense_vector(n)
+    itm = m.basic_iteration(bnorm, 100000, 1e-10)
+    t0 = time.perf_counter()
+    errm, st = m.ir_solve(A, xm, b, itm, P32, method=method, value_format=m.F32, mode="reliable")
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    ref = x.t if x_ref is None else x_ref
+    rel_err = float(torch.linalg.vector_norm(xm.t - ref) / torch.linalg.vector_norm(ref))
+    return {"fp64": {"solves_per_s": round(1 / t64, 4), "iterations": it.iterations(), "error_code": err,
+                     "relres": relres_of(m, A, x, b, bnorm)},
+            "mixed_f32": {"solves_per_s": round(1 / tm, 4), "fp32_iterations": st["inner_total"],
+                          "fp64_refreshes": st["outer"], "error_code": errm,
+                          "relres": relres_of(m, A, xm, b, bnorm), "rel_err_vs_fp64": rel_err},
+            "speedup": round(t64 / tm, 3), "steps": 1}
+
+
+def c3_section(m, torch, gen):
+    """Mixed-precision cases: cg + ic_0 on 128^3 diffusion; bicgstab + ilu_0
+    on the 1024^2 upwind convection-diffusion problem (Peclet 10)."""
+    out = {}
+    rp, ci, va = gen.diffusion3d(128, seed=0)
+    n = len(rp) - 1
+    A = m.compressed2D(n, n, rp, ci, va)
+    A.autotune()
+    b = A * m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
+    out["cg_ic0_diffusion3d_128"] = pair_section(m, torch, A, b, m.two_norm(b), "ic_0", m.CG)
+    del A, b
+    rp, ci, va = gen.convection_diffusion2d(1024, peclet=10.0, seed=0)
+    n = len(rp) - 1
+    A = m.compressed2D(n, n, rp, ci, va)
+    A.autotune()
+    b = A * m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
+    out["bicgstab_ilu0_convdiff_1024"] = pair_section(m, torch, A, b, m.two_norm(b), "ilu_0", m.BICGSTAB)
+    return out
+
+
+def batch_systems(gen, count, seed=0):
+    """Uneven SPD systems: N log-uniform in [256, 65536], families 2D 5-point,
+    3D 7-point and random diagonally dominant (about 12 non-zeros per row)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(256), np.log(65536)))))
+        fam = int(rng.integers(0, 3))
+        if fam == 0:
+            a = max(2, int(round(np.sqrt(n))))
+            out.append(gen.laplacian_setup(a, max(2, int(round(n / a)))))
+        elif fam == 1:
+            a = max(2, int(round(n ** (1.0 / 3.0))))
+            out.append(gen.laplacian3d(a, a, max(2, int(round(n / (a * a))))))
+        else:
+            out.append(gen.random_spd(n, seed=seed * 100000 + k))
+    return out
+
+
+def batched_section(m, torch, gen, count):
+    """All systems in one persistent graph-replayed launch, cg + diagonal to
+    relative residual 1e-10: FP64 against mixed precision (FP32 values,
+    FP64 reliable updates)."""
+    t0 = time.perf_counter()
+    systems = batch_systems(gen, count)
+    B = m.Batch(systems)
+    rows = int(B.nrows)
+    b = m.dense_vector(np.random.default_rng(7).uniform(-1.0, 1.0, rows))
+    x = m.dense_vector(rows)
+    x0 = m.dense_vector(rows)
+    setup_s = time.perf_counter() - t0
+    out = {"systems": count, "rows": rows, "nnz": int(len(B.col)), "setup_s": round(setup_s, 2)}
+    for name, prec in (("fp64", m.F64), ("mixed_f32", m.F32)):
+        tune_ms, shape = B.tune(x, x0, b, precision=prec, rtol=1e-10)
+        times = []
+        for rep in range(3):
+            x.t.zero_()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            res = B.solve(x, b, precision=prec, rtol=1e-10)
+            times.append(time.perf_counter() - t1)
+        its = np.array([r["iterations"] for r in res])
+        errs = sum(1 for r in res if r["error"] or r["status"])
+        rel = np.array([r["resid"] / r["norm_r0"] for r in res if r["norm_r0"] > 0])
+        tbest = min(times)
+        out[name] = {"systems_per_s": round(count / tbest, 2), "batch_ms": round(1000 * tbest, 3),
+                     "shape": list(shape), "tune_best_ms": round(tune_ms, 3), "errors": errs,
+                     "iterations_mean": round(float(its.mean()), 1), "iterations_max": int(its.max()),
+                     "max_relres_reported": float(rel.max()) if rel.size else None}
+    out["mixed_speedup"] = round(out["mixed_f32"]["systems_per_s"] / out["fp64"]["systems_per_s"], 3)
+    out["graph"] = dict(zip(("captures", "replays"), B.graph_info()))
+    return out
+
+
+def serving_section(m, torch, gen, nreq):
+    """Continuous batching over 512 resident matrices (the batched families
+    plus 3D stencils up to 96^3), Poisson arrivals, 1-8 right-hand sides per
+    request, mixed precision to 1e-10, against plain FP64 per-request
+    launches on one stream."""
+    systems = batch_systems(gen, 480, seed=1)
+    rng = np.random.default_rng(3)
+    for k in range(32):
+        a = int(round(np.exp(rng.uniform(np.log(32), np.log(96)))))
+        systems.append(gen.laplacian3d(a, a, a))
+    B = m.Batch(systems)
+    out = {"matrices": len(systems), "requests_per_rate": nreq, "cap_bytes": 200e9}
+    rates = (2000.0, 8000.0, 20000.0)
+    traces = {}
+    for rate in rates:
+        rg = np.random.default_rng(int(rate))
+        traces[rate] = (rg.integers(0, len(systems), nreq), rg.integers(1, 9, nreq),
+                        np.cumsum(rg.exponential(1.0 / rate, nreq)),
+                        rg.integers(1, 2 ** 62, nreq, dtype=np.uint64))
+        out["%dk_req_s" % int(rate / 1000)] = {}
+    for name, kw in (("mixed", dict(precision=m.F32, streams=4, max_items=1024)),
+                     ("fp64_per_request", dict(precision=m.F64, streams=1, max_items=8,
+                                                per_request=True))):
+        Sv = m.Server(B, rtol=1e-10, big_rows=65536, cap_bytes=200e9, **kw)
+        for rate in rates:
+            mat, nrhs, arr, seeds = traces[rate]
+            lat, its, _, st = Sv.run(mat, nrhs, arr, seeds)
+            out["%dk_req_s" % int(rate / 1000)][name] = {
+                "p50_ms": round(1000 * float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(1000 * float(np.percentile(lat, 99)), 3),
+                "solves_per_s": round(st["solves"] / st["wall_s"], 1),
+                "wall_s": round(st["wall_s"], 3), "launches": st["launches"],
+                "max_inflight": st["max_inflight"], "errors": st["errors"],
+                "big_solves": st["big_solves"], "peak_bytes": st["peak_bytes"]}
+        del Sv
+    for rate in rates:
+        row = out["%dk_req_s" % int(rate / 1000)]
+        row["speedup_solves_per_s"] = round(row["mixed"]["solves_per_s"] /
+                                            row["fp64_per_request"]["solves_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +249,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--only-spmv", action="store_true", help="SpMV timing only (profiling runs)")
+    ap.add_argument("--sections", default="all",
+                    help="extra measurements: all, none, or a list of ic0,c3,batched,serving")
+    ap.add_argument("--batch-systems", type=int, default=4096)
+    ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -211,6 +377,18 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(rp, ci, va, b.numpy(), iters, args.cpu_iters)
 
+    sections = set(args.sections.split(",")) if args.sections != "all" else {"ic0", "c3", "batched", "serving"}
+    extra = {}
+    if "ic0" in sections:
+        extra["cg_ic0_160"] = guarded(lambda: pair_section(m, torch, A, b, bnorm, "ic_0", m.CG, x64))
+    del S, P, P32
+    if "c3" in sections:
+        extra["c3"] = guarded(lambda: c3_section(m, torch, gen))
+    if "batched" in sections:
+        extra["batched"] = guarded(lambda: batched_section(m, torch, gen, args.batch_systems))
+    if "serving" in sections:
+        extra["serving"] = guarded(lambda: serving_section(m, torch, gen, args.serve_requests))
+
     v = args.steps / t_fp64
     mixed["speedup_vs_fp64"] = round(mixed["solves_per_s"] / v, 3)
     mixed_bf16["speedup_vs_fp64"] = round(mixed_bf16["solves_per_s"] / v, 3)
@@ -244,6 +422,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

@@ -336,6 +336,8 @@ typedef struct {
     int32_t big_rows;       /* larger systems use the whole-GPU engine */
     double rtol;
     double cap_bytes;       /* admission cap on the in-flight working set */
+    int32_t per_request;    /* 1: one request per launch (the plain per-request
+                               baseline; with streams = 1, no overlap) */
 } mtl4g_server_params;
 
 typedef struct {

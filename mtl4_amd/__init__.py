@@ -572,10 +572,11 @@ class Server:
     """Serving loop over the matrices of a Batch (continuous batching)."""
 
     def __init__(self, batch, precision=F32, pc_kind=_lib.PC_DIAGONAL, rtol=1e-10,
-                 max_iter=20000, streams=4, max_items=4096, big_rows=65536, cap_bytes=200e9):
+                 max_iter=20000, streams=4, max_items=4096, big_rows=65536, cap_bytes=200e9,
+                 per_request=False):
         self.batch = batch
         self.prm = _lib.ServerParams(streams, max_items, precision, pc_kind, max_iter, big_rows,
-                                     rtol, cap_bytes)
+                                     rtol, cap_bytes, int(per_request))
         h = C.c_void_p()
         check(_L().mtl4g_server_create(C.byref(h), batch.h, C.byref(self.prm)), "server_create")
         self.h = h

@@ -56,7 +56,7 @@ class BatchResult(C.Structure):
 class ServerParams(C.Structure):
     _fields_ = [("streams", C.c_int32), ("max_items", C.c_int32), ("precision", C.c_int32),
                 ("pc_kind", C.c_int32), ("max_iter", C.c_int32), ("big_rows", C.c_int32),
-                ("rtol", C.c_double), ("cap_bytes", C.c_double)]
+                ("rtol", C.c_double), ("cap_bytes", C.c_double), ("per_request", C.c_int32)]
 
 
 class ServerStats(C.Structure):

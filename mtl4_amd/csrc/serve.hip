@@ -417,7 +417,8 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
             Slot& sl = *slp;
             if (sl.busy || admitted.empty()) continue;
             int nit = 0;
-            while (!admitted.empty() && nit + reqs[admitted.front()].nrhs <= prm.max_items) {
+            while (!admitted.empty() && nit + reqs[admitted.front()].nrhs <= prm.max_items &&
+                   !(prm.per_request && nit > 0)) {
                 const Req& q = reqs[admitted.front()];
                 const int64_t pn = Arena::round(q.n);
                 const int64_t stride = 2 * pn + Arena::round(batch_ws_elems(q.n, MTL4G_F64));
