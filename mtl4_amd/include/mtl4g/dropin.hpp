@@ -9,6 +9,7 @@
 //   itl::pc::{identity,diagonal,ilu_0,ic_0}<Matrix[, Value]> P(A);  y = solve(P, x);
 //   itl::basic_iteration / cyclic_iteration / noisy_iteration<double> iter(b, max, rtol[, atol]);
 //   int err = itl::cg(A, x, b, P, iter);  itl::bicgstab(A, x, b, P, iter);
+//   mtl::io::matrix_market_istream(name) >> A;  A = mtl::io::matrix_market(name);
 // Host containers keep MTL4's storage (row-major CSR with std::size_t
 // indices by default, contiguous vectors); operations on the path run on the
 // device, with the matrix mirrored once into HBM and vectors mirrored on
@@ -18,9 +19,11 @@
 #include <mtl4g.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <complex>
 #include <cstddef>
+#include <fstream>
 #include <initializer_list>
 #include <iostream>
 #include <map>
@@ -56,6 +59,12 @@ struct missing_diagonal : runtime_error {
 };
 struct device_error : runtime_error {
     explicit device_error(const char* s) : runtime_error(s) {}
+};
+struct io_error : std::runtime_error {
+    explicit io_error(const char* s = "I/O error") : std::runtime_error(s) {}
+};
+struct file_not_found : io_error {
+    explicit file_not_found(const char* s = "File not found") : io_error(s) {}
 };
 
 #ifndef MTL_THROW_IF
@@ -454,6 +463,18 @@ class compressed2D {
           data_(Entries, Entries + nnz) {}
     compressed2D(const compressed2D& o)
         : nr_(o.nr_), nc_(o.nc_), starts_(o.starts_), indices_(o.indices_), data_(o.data_) {}
+    // From a file description, e.g. compressed2D<double> A(mtl::io::matrix_market(name))
+    template <typename File,
+              typename = decltype(std::declval<const File&>().read_to_(std::declval<compressed2D&>()))>
+    explicit compressed2D(const File& f) : starts_(1, 0) {
+        f.read_to_(*this);
+    }
+    template <typename File,
+              typename = decltype(std::declval<const File&>().read_to_(std::declval<compressed2D&>()))>
+    compressed2D& operator=(const File& f) {
+        f.read_to_(*this);
+        return *this;
+    }
     compressed2D& operator=(const compressed2D& o) {
         if (this != &o) {
             nr_ = o.nr_;
@@ -699,6 +720,172 @@ using mat::num_cols;
 using mat::num_rows;
 using mat::set_to_zero;
 using mat::trans;
+
+// ---------------------------------------------------------------- Matrix Market IO
+// io/matrix_market.hpp: header "%%MatrixMarket matrix <coordinate|array>
+// <real|integer|pattern|complex> <general|symmetric|skew-symmetric|hermitian>",
+// '%' comment lines, then "rows cols [entries]".  Entries go through an
+// inserter (later duplicates overwrite); off-diagonal entries of symmetric,
+// skew-symmetric and Hermitian files are mirrored (negated for skew);
+// pattern entries are stored as 0; complex values cannot go into a real
+// matrix; array files are read column by column.
+namespace io {
+
+class matrix_market_istream {
+  public:
+    explicit matrix_market_istream(const char* p) : own_(new std::ifstream(p)), in_(*own_) { check_(p); }
+    explicit matrix_market_istream(const std::string& s) : matrix_market_istream(s.c_str()) {}
+    explicit matrix_market_istream(std::istream& s = std::cin) : in_(s) { check_(nullptr); }
+
+    template <typename V, typename P>
+    matrix_market_istream& operator>>(mat::compressed2D<V, P>& A) {
+        read_(A);
+        return *this;
+    }
+    void close() {
+        if (own_) own_->close();
+    }
+
+  private:
+    enum symmetry_t { general, symmetric, skew, hermitian };
+
+    void check_(const char* name) {
+        if (in_.good()) return;
+        std::string msg("matrix_market_istream: Error in input stream!\n");
+        if (name) msg += std::string("Probably file ") + name + " not found.\n";
+        throw file_not_found(msg.c_str());
+    }
+    static std::string lower_(std::string s) {
+        for (char& ch : s) ch = (char)std::tolower((unsigned char)ch);
+        return s;
+    }
+
+    template <typename V, typename P>
+    void read_(mat::compressed2D<V, P>& A) {
+        typedef typename mat::compressed2D<V, P>::size_type size_type;
+        std::string marker, type, layout, field, sym;
+        in_ >> marker >> type >> layout >> field >> sym;
+        MTL_THROW_IF(marker != "%%MatrixMarket", runtime_error("File not in Matrix Market format"));
+        MTL_THROW_IF(type != "matrix", runtime_error("Try to read matrix from non-matrix file"));
+        layout = lower_(layout);
+        sym = lower_(sym);
+        symmetry_t symm;
+        if (sym == "general") symm = general;
+        else if (sym == "symmetric") symm = symmetric;
+        else if (sym == "skew-symmetric") symm = skew;
+        else if (sym == "hermitian") symm = hermitian;
+        else throw runtime_error("Unknown symmetry in Matrix Market file");
+        MTL_THROW_IF(layout != "coordinate" && layout != "array",
+                     runtime_error("Unknown sparsity in Matrix Market file"));
+        for (;;) {   // comment lines
+            char first = 0;
+            in_ >> first;
+            if (first != '%') {
+                in_.putback(first);
+                break;
+            }
+            std::string rest;
+            std::getline(in_, rest);
+        }
+        std::size_t nr = 0, nc = 0, nnz = 0;
+        in_ >> nr >> nc;
+        if (layout == "coordinate") in_ >> nnz;
+        int kind;   // 0 real, 1 integer, 2 pattern
+        if (field == "real") kind = 0;
+        else if (field == "integer") kind = 1;
+        else if (field == "pattern") kind = 2;
+        else if (field == "complex") throw runtime_error("Cannot convert complex value in real\n");
+        else throw runtime_error("Unknown tag for matrix value type in file");
+        A.change_dim((size_type)nr, (size_type)nc);
+        mat::inserter<mat::compressed2D<V, P>> ins(A);
+        auto value = [&]() -> V {
+            if (kind == 0) {
+                double v = 0;
+                in_ >> v;
+                return V(v);
+            }
+            if (kind == 1) {
+                long v = 0;
+                in_ >> v;
+                return V(v);
+            }
+            return V(0);
+        };
+        auto put = [&](std::size_t r, std::size_t c, V v) {
+            ins[(size_type)r][(size_type)c] << v;
+            if (r == c) return;
+            if (symm == symmetric || symm == hermitian) ins[(size_type)c][(size_type)r] << v;
+            else if (symm == skew) ins[(size_type)c][(size_type)r] << -v;
+        };
+        if (layout == "coordinate") {
+            for (std::size_t k = 0; k < nnz; k++) {
+                std::size_t r = 0, c = 0;
+                in_ >> r >> c;
+                if (!in_) break;
+                put(r - 1, c - 1, value());
+            }
+        } else {
+            for (std::size_t c = 0; c < nc; c++)
+                for (std::size_t r = 0; r < nr; r++) put(r, c, value());
+        }
+    }
+
+    std::unique_ptr<std::ifstream> own_;
+    std::istream& in_;
+};
+
+// Coordinate real general, 17 significant digits (values read back exactly).
+class matrix_market_ostream {
+  public:
+    explicit matrix_market_ostream(const char* p) : own_(new std::ofstream(p)), out_(*own_) {}
+    explicit matrix_market_ostream(const std::string& s) : matrix_market_ostream(s.c_str()) {}
+    explicit matrix_market_ostream(std::ostream& s = std::cout) : out_(s) {}
+
+    template <typename V, typename P>
+    matrix_market_ostream& operator<<(const mat::compressed2D<V, P>& A) {
+        out_ << "%%MatrixMarket matrix coordinate real general\n";
+        out_ << A.num_rows() << " " << A.num_cols() << " " << A.nnz() << "\n";
+        const std::streamsize prec = out_.precision(17);
+        for (std::size_t r = 0; r < (std::size_t)A.num_rows(); r++)
+            for (std::size_t j = A.ref_major()[r]; j < (std::size_t)A.ref_major()[r + 1]; j++)
+                out_ << r + 1 << " " << A.ref_minor()[j] + 1 << " " << A.address_data()[j] << "\n";
+        out_.precision(prec);
+        return *this;
+    }
+    void close() {
+        if (own_) own_->close();
+    }
+
+  private:
+    std::unique_ptr<std::ofstream> own_;
+    std::ostream& out_;
+};
+
+// mtl::io::matrix_market(name): read with `compressed2D<double> A(matrix_market(name))`
+// or `A = matrix_market(name)`, write with `matrix_market(name) = A`
+// (io/matrix_file.hpp:21-40).
+class matrix_market {
+  public:
+    explicit matrix_market(const std::string& fname) : fname_(fname) {}
+    explicit matrix_market(const char* fname) : fname_(fname) {}
+    template <typename V, typename P>
+    matrix_market& operator=(const mat::compressed2D<V, P>& A) {
+        matrix_market_ostream out(fname_);
+        out << A;
+        return *this;
+    }
+    template <typename V, typename P>
+    void read_to_(mat::compressed2D<V, P>& A) const {
+        matrix_market_istream in(fname_);
+        in >> A;
+    }
+    const std::string& file_name() const { return fname_; }
+
+  private:
+    std::string fname_;
+};
+
+}  // namespace io
 
 namespace vec {
 template <typename M, typename V>
