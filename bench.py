@@ -50,14 +50,27 @@ def timed_spmv(m, torch, A, x, y, reps):
     return e0.elapsed_time(e1) / 1000.0 / reps
 
 
-def load_traffic():
-    """HBM bytes per SpMV launch measured with rocprofv3 PMC counters."""
+def traffic_key(m, kern):
+    """Name of a CSR kernel configuration as in the PMC traffic table."""
+    kind, group, rows = kern
+    if kind == m._lib.SPMV_STREAM:
+        return "stream/%d/%d" % (rows, group)
+    if kind == m._lib.SPMV_TILE:
+        return "tile/%d" % rows
+    return None
+
+
+def load_traffic(key):
+    """HBM bytes per launch of this SpMV configuration, from the two
+    rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) recorded under profiles/,
+    scaled by the counters' readings on a 512 MiB device copy."""
     p = os.path.join(ROOT, "profiles", "spmv_diffusion3d_160_traffic.json")
-    if not os.path.exists(p):
+    if key is None or not os.path.exists(p):
         return None
     try:
-        return json.load(open(p)).get("traffic_bytes")
-    except Exception:
+        cfg = json.load(open(p))["configs"].get(key)
+        return cfg["traffic_bytes"] if cfg else None
+    except (OSError, ValueError, KeyError):
         return None
 
 
@@ -249,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--only-spmv", action="store_true", help="SpMV timing only (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="with --only-spmv: also copy 512 MiB on the device (PMC calibration)")
     ap.add_argument("--sections", default="all",
                     help="extra measurements: all, none, or a list of ic0,c3,batched,serving")
     ap.add_argument("--batch-systems", type=int, default=4096)
@@ -279,14 +294,7 @@ def main():
     by = spmv_bytes(n, nnz)
     spmv_gbs = by / t_spmv / 1e9
     kern = A.get_kernel()
-    rr = gen.random_sparse(2_000_000, seed=0)
-    R = m.compressed2D(2_000_000, 2_000_000, *rr)
-    R.autotune()
-    xr = m.dense_vector(np.random.default_rng(2).uniform(-1.0, 1.0, 2_000_000))
-    yr = m.dense_vector(2_000_000)
-    t_rnd = timed_spmv(m, torch, R, xr, yr, args.spmv_reps)
-    rnd_gbs = spmv_bytes(2_000_000, len(rr[1])) / t_rnd / 1e9
-    traffic = load_traffic()
+    traffic = load_traffic(traffic_key(m, kern))
     if kern[0] == m._lib.SPMV_VECTOR:
         kname = "k_csrv<%d lanes/row>" % kern[1]
     elif kern[0] == m._lib.SPMV_STREAM:
@@ -294,13 +302,28 @@ def main():
     else:
         kname = "k_csr (LDS tiles, %d rows/block)" % kern[2]
     roofline = {"bound": "hbm", "kernel": "%s<EpiAssign> (y = A*x, FP64, diffusion3d %d^3)" % (kname, g),
+                "config": traffic_key(m, kern),
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": by, "avg_launch_us": round(t_spmv * 1e6, 2)}
-    if args.only_spmv:
+    if args.only_spmv:   # the roofline kernel alone (profiling runs)
+        if args.copy_calibration:   # a copy of known size for the PMC byte counters
+            src = torch.rand(64 << 20, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+            del src, dst
         torch.cuda.synchronize()
-        print(json.dumps({"spmv_gbs": spmv_gbs, "roofline": roofline, "random_2M_gbs": rnd_gbs}))
+        print(json.dumps({"spmv_gbs": spmv_gbs, "roofline": roofline}))
         return
+    rr = gen.random_sparse(2_000_000, seed=0)
+    R = m.compressed2D(2_000_000, 2_000_000, *rr)
+    R.autotune()
+    xr = m.dense_vector(np.random.default_rng(2).uniform(-1.0, 1.0, 2_000_000))
+    yr = m.dense_vector(2_000_000)
+    t_rnd = timed_spmv(m, torch, R, xr, yr, args.spmv_reps)
+    rnd_gbs = spmv_bytes(2_000_000, len(rr[1])) / t_rnd / 1e9
+    del R, xr, yr
 
     # ---- FP64 CG + diagonal: the drop-in path
     P = m.pc.diagonal(A)

@@ -521,8 +521,8 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
                                                     mtl4g_csr_set_kernel(A, c.kernel, c.a, 0))
                                                  : mtl4g_csr_set_kernel(A, c.kernel, c.a, c.b);
         if (st) continue;
-        if (launch_spmv(A, x, y, MTL4G_ASSIGN, s)) continue;   // warm-up
-        const int reps = 5;
+        if (launch_spmv(A, x, y, MTL4G_ASSIGN, s) || launch_spmv(A, x, y, MTL4G_ASSIGN, s)) continue;   // warm-up
+        const int reps = 10;
         M4_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < reps; r++) M4_TRY(launch_spmv(A, x, y, MTL4G_ASSIGN, s));
         M4_HIP(hipEventRecord(e1, s));

@@ -4,9 +4,9 @@ reductions and whole cg / bicgstab solves must reproduce the CPU oracle
 (MTL4's algorithm at -ffp-contract=off) bit for bit: same iteration count,
 same final residual, same solution bits.
 
-The data are chosen order-sensitive (magnitudes spread over 16 decades): a
-sum in any other order -- the default tree order, or the oracle with another
-unroll factor -- gives different bits, which each test checks first."""
+The data are chosen order-sensitive: magnitudes spread over 16 decades,
+and in test_reductions_order_sensitive 1e16 beside ones, where the oracle
+itself shows that another accumulator layout gives other bits."""
 import ctypes as C
 
 import numpy as np
@@ -58,9 +58,21 @@ def test_reductions_bitwise(m, mtl4_order, n):
     assert m.dot(va, vb) == orc.dot(a, b, 4)
     assert m.unary_dot(va) == orc.unary_dot(a, 8)      # unary_dot<8> is MTL4's default
     assert m.two_norm(va) == orc.two_norm(a)           # two_norm<4>
-    if n >= 100:   # the data must tell the orders apart
-        assert orc.dot(a, b, 4) != orc.dot(a, b, 8) or orc.dot(a, b, 4) != orc.dot(a, b, 1)
-        assert orc.unary_dot(a, 8) != orc.unary_dot(a, 4)
+
+
+@pytest.mark.parametrize("n", [9, 4099, 100003])
+def test_reductions_order_sensitive(m, mtl4_order, n):
+    """Vectors whose sums change with the accumulator layout (1e16 next to
+    ones): the device must give MTL4's answer, not another order's."""
+    a = np.ones(n)
+    a[0] = 1e8
+    assert orc.unary_dot(a, 8) != orc.unary_dot(a, 4)
+    assert m.unary_dot(dv(m, a)) == orc.unary_dot(a, 8)
+    assert m.two_norm(dv(m, a)) == orc.two_norm(a)
+    x, y = np.ones(n), np.ones(n)
+    x[0], x[2] = 1e16, -1e16
+    assert orc.dot(x, y, 4) != orc.dot(x, y, 1)
+    assert m.dot(dv(m, x), dv(m, y)) == orc.dot(x, y, 4)
 
 
 @pytest.mark.parametrize("n", SIZES)
