@@ -90,6 +90,14 @@ def cpu_baseline(rp, ci, va, b, iters_gpu, cpu_iters):
                       "scaled to the GPU's %d iterations" % (cpu_iters, dt, per_iter * 1e3, iters_gpu)}
 
 
+T_START = time.perf_counter()
+
+
+def log(msg):
+    """Progress on stderr (the JSON result is the only line on stdout)."""
+    print("[bench %7.1fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
+
+
 def guarded(fn):
     """Run one extra measurement; a failure is recorded, not fatal."""
     try:
@@ -110,6 +118,7 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     updates) solve of A x = b with the given preconditioner; relative
     residuals in FP64 and the mixed solution's error against the FP64 one."""
     n = A.num_rows()
+    log("%s: preconditioners (%s, n=%d)" % ("cg" if method == m.CG else "bicgstab", pc, n))
     P64 = getattr(m.pc, pc)(A)
     P32 = getattr(m.pc, pc)(A, m.F32)
     S = m.Solver(A, P64, method=method)
@@ -120,6 +129,7 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     err = S.solve(x, b, it)
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t0
+    log("  FP64 solve: %d iterations, %.3f s" % (it.iterations(), t64))
     xm = m.dense_vector(n)
     itm = m.basic_iteration(bnorm, 100000, 1e-10)
     t0 = time.perf_counter()
@@ -140,6 +150,7 @@ def c3_section(m, torch, gen):
     """Mixed-precision cases: cg + ic_0 on 128^3 diffusion; bicgstab + ilu_0
     on the 1024^2 upwind convection-diffusion problem (Peclet 10)."""
     out = {}
+    log("c3: diffusion3d 128^3")
     rp, ci, va = gen.diffusion3d(128, seed=0)
     n = len(rp) - 1
     A = m.compressed2D(n, n, rp, ci, va)
@@ -147,6 +158,7 @@ def c3_section(m, torch, gen):
     b = A * m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
     out["cg_ic0_diffusion3d_128"] = pair_section(m, torch, A, b, m.two_norm(b), "ic_0", m.CG)
     del A, b
+    log("c3: convection-diffusion 1024^2")
     rp, ci, va = gen.convection_diffusion2d(1024, peclet=10.0, seed=0)
     n = len(rp) - 1
     A = m.compressed2D(n, n, rp, ci, va)
@@ -180,7 +192,9 @@ def batched_section(m, torch, gen, count):
     relative residual 1e-10: FP64 against mixed precision (FP32 values,
     FP64 reliable updates)."""
     t0 = time.perf_counter()
+    log("batched: generating %d systems" % count)
     systems = batch_systems(gen, count)
+    log("batched: uploading")
     B = m.Batch(systems)
     rows = int(B.nrows)
     b = m.dense_vector(np.random.default_rng(7).uniform(-1.0, 1.0, rows))
@@ -189,7 +203,9 @@ def batched_section(m, torch, gen, count):
     setup_s = time.perf_counter() - t0
     out = {"systems": count, "rows": rows, "nnz": int(len(B.col)), "setup_s": round(setup_s, 2)}
     for name, prec in (("fp64", m.F64), ("mixed_f32", m.F32)):
+        log("batched: %s tune" % name)
         tune_ms, shape = B.tune(x, x0, b, precision=prec, rtol=1e-10)
+        log("batched: %s solves (tuned %.1f ms)" % (name, tune_ms))
         times = []
         for rep in range(3):
             x.t.zero_()
@@ -215,6 +231,7 @@ def serving_section(m, torch, gen, nreq):
     plus 3D stencils up to 96^3), Poisson arrivals, 1-8 right-hand sides per
     request, mixed precision to 1e-10, against plain FP64 per-request
     launches on one stream."""
+    log("serving: generating 512 matrices")
     systems = batch_systems(gen, 480, seed=1)
     rng = np.random.default_rng(3)
     for k in range(32):
@@ -235,6 +252,7 @@ def serving_section(m, torch, gen, nreq):
                                                 per_request=True))):
         Sv = m.Server(B, rtol=1e-10, big_rows=65536, cap_bytes=200e9, **kw)
         for rate in rates:
+            log("serving: %s at %d req/s" % (name, rate))
             mat, nrhs, arr, seeds = traces[rate]
             lat, its, _, st = Sv.run(mat, nrhs, arr, seeds)
             out["%dk_req_s" % int(rate / 1000)][name] = {
@@ -280,6 +298,7 @@ def main():
     torch.cuda.set_device(0)
 
     g = args.grid
+    log("diffusion3d %d^3: generating" % g)
     rp, ci, va = gen.diffusion3d(g, seed=0)
     n, nnz = len(rp) - 1, len(ci)
     A = m.compressed2D(n, n, rp, ci, va)
@@ -325,6 +344,7 @@ def main():
     rnd_gbs = spmv_bytes(2_000_000, len(rr[1])) / t_rnd / 1e9
     del R, xr, yr
 
+    log("SpMV %.1f GB/s; FP64 cg + diagonal" % spmv_gbs)
     # ---- FP64 CG + diagonal: the drop-in path
     P = m.pc.diagonal(A)
     S = m.Solver(A, P)
@@ -391,12 +411,14 @@ def main():
                 "p50_ms": round(1000 * float(np.percentile(latm, 50)), 3),
                 "p99_ms": round(1000 * float(np.percentile(latm, 99)), 3)}
 
+    log("mixed precision")
     mixed = run_mixed(m.F32, args.steps, args.warmup, 100000)
     # BF16-stored values: one solve, capped at 3x the FP64 iteration count
     # (on this conditioning it stagnates; the record says so)
     mixed_bf16 = run_mixed(m.BF16, 1, 0, 3 * iters)
 
     cpu = None
+    log("CPU baseline")
     if not args.no_cpu:
         cpu = cpu_baseline(rp, ci, va, b.numpy(), iters, args.cpu_iters)
 

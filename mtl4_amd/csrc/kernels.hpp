@@ -423,7 +423,10 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
     }
     if (A->nb == 0) return MTL4G_OK;
     if (A->kernel == MTL4G_SPMV_STREAM) {
-        const int want = A->cus * A->per_cu;
+        int pcu = A->per_cu;
+        if (sizeof(MT) == 4 && A->per_cu_lowp[0] > 0) pcu = A->per_cu_lowp[0];
+        if (sizeof(MT) == 2 && A->per_cu_lowp[1] > 0) pcu = A->per_cu_lowp[1];
+        const int want = A->cus * pcu;
         const int grid = std::max(kXcd, std::min(want, (A->nb + kXcd - 1) / kXcd * kXcd) / kXcd * kXcd);
         const int tile = A->nnz_per_block;
 #define M4_LAUNCHP(T)                                                                          \

@@ -89,8 +89,9 @@ static uint16_t f32_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
+static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s);
+
 int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
-    (void)s;
     if (fmt == MTL4G_F32 && !A->val32.p) {
         std::vector<float> v(A->h_val.begin(), A->h_val.end());
         M4_TRY(A->val32.alloc(v.size()));
@@ -102,6 +103,7 @@ int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
         M4_TRY(A->valbf.alloc(v.size()));
         M4_HIP(hipMemcpy(A->valbf.p, v.data(), v.size() * 2, hipMemcpyHostToDevice));
     }
+    if ((fmt == MTL4G_F32 || fmt == MTL4G_BF16) && !A->lowp_tuned[fmt - 1]) M4_TRY(tune_lowp(A, fmt, s));
     return MTL4G_OK;
 }
 
@@ -159,6 +161,47 @@ static int spmv32_mode(const mtl4g_matrix_s* A, int fmt, const float* x, float* 
     if (fmt == MTL4G_F32)
         return launch_csr<float>(A, A->val32.p, x, EpiAssign<float, MODE>{y}, (float*)nullptr, nullptr, s);
     return launch_csr<float>(A, A->valbf.p, x, EpiAssign<float, MODE>{y}, (float*)nullptr, nullptr, s);
+}
+
+// Workgroups per CU of the stream kernel on the FP32 / BF16 values: fewer
+// bytes per non-zero and fewer registers than FP64, so the best persistent
+// grid can differ.  Timed on the first low-precision use of the matrix.
+static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
+    A->lowp_tuned[fmt - 1] = true;
+    A->per_cu_lowp[fmt - 1] = 0;
+    if (A->kernel != MTL4G_SPMV_STREAM || A->n == 0 || A->m == 0) return MTL4G_OK;
+    float *x = nullptr, *y = nullptr;
+    M4_HIP(hipMallocAsync((void**)&x, (size_t)A->m * 4, s));
+    M4_HIP(hipMallocAsync((void**)&y, (size_t)A->n * 4, s));
+    M4_HIP(hipMemsetAsync(x, 0, (size_t)A->m * 4, s));
+    hipEvent_t e0, e1;
+    M4_HIP(hipEventCreate(&e0));
+    M4_HIP(hipEventCreate(&e1));
+    double best = 1e30;
+    int win = 0;
+    for (int f = 2; f <= 6; f++) {
+        A->per_cu_lowp[fmt - 1] = f;
+        int st = launch_spmv_f32(A, fmt, x, y, MTL4G_ASSIGN, s);
+        if (!st) st = launch_spmv_f32(A, fmt, x, y, MTL4G_ASSIGN, s);
+        if (st) continue;
+        const int reps = 10;
+        M4_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) M4_TRY(launch_spmv_f32(A, fmt, x, y, MTL4G_ASSIGN, s));
+        M4_HIP(hipEventRecord(e1, s));
+        M4_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        M4_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            win = f;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFreeAsync(x, s);
+    (void)hipFreeAsync(y, s);
+    A->per_cu_lowp[fmt - 1] = win;
+    return MTL4G_OK;
 }
 
 int launch_spmv_f32(const mtl4g_matrix_s* A, int fmt, const float* x, float* y, int assign,
@@ -454,6 +497,8 @@ int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block
     if (!A) return MTL4G_ERR_INVALID;
     M4_TRY(build_row_blocks(A, rows_per_block, nnz_per_block));
     A->kernel = MTL4G_SPMV_TILE;
+    A->lowp_tuned[0] = A->lowp_tuned[1] = false;
+    A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
     return MTL4G_OK;
 }
 
@@ -469,6 +514,8 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
         if (rows_per_block > 0 || !A->desc.p) M4_TRY(build_row_blocks(A, rows_per_block, 0));
         A->kernel = MTL4G_SPMV_STREAM;
         A->per_cu = group;
+        A->lowp_tuned[0] = A->lowp_tuned[1] = false;
+        A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
         return MTL4G_OK;
     }
     if (kernel != MTL4G_SPMV_VECTOR) return MTL4G_ERR_INVALID;
@@ -504,8 +551,12 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     // product buffers of TILE doubles per workgroup)
     for (int f : {2, 3, 4}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 256, 2048});
     for (int f : {4, 6, 8}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 128, 1024});
-    for (int g : {4, 8, 16, 32, 64})
+    // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
+    const double avg = A->n ? (double)A->nnz / A->n : 0.0;
+    for (int g : {4, 8, 16, 32, 64}) {
+        if (g >= 16 && avg < 4.0 * g) continue;
         for (int f : {1, 2, 4}) cands.push_back({MTL4G_SPMV_VECTOR, g, std::max(4, f * 1024 / g), 0, 0});
+    }
     double *x = nullptr, *y = nullptr;
     M4_HIP(hipMallocAsync((void**)&x, (size_t)A->m * 8 + 8, s));
     M4_HIP(hipMallocAsync((void**)&y, (size_t)A->n * 8 + 8, s));
