@@ -57,6 +57,8 @@ def traffic_key(m, kern):
         return "stream/%d/%d" % (rows, group)
     if kind == m._lib.SPMV_TILE:
         return "tile/%d" % rows
+    if kind == m._lib.SPMV_SCALAR:
+        return "scalar/%d" % group
     return None
 
 
@@ -131,11 +133,16 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     t64 = time.perf_counter() - t0
     log("  FP64 solve: %d iterations, %.3f s" % (it.iterations(), t64))
     xm = m.dense_vector(n)
-    itm = m.basic_iteration(bnorm, 100000, 1e-10)
+    # the FP32 recurrence may stagnate: at most 3x the FP64 iteration count
+    itm = m.basic_iteration(bnorm, 3 * max(it.iterations(), 10), 1e-10)
     t0 = time.perf_counter()
-    errm, st = m.ir_solve(A, xm, b, itm, P32, method=method, value_format=m.F32, mode="reliable")
+    try:
+        errm, st = m.ir_solve(A, xm, b, itm, P32, method=method, value_format=m.F32, mode="reliable")
+    except m.nonfinite_error:
+        errm, st = -6, {"inner_total": itm.iterations(), "outer": -1}
     torch.cuda.synchronize()
     tm = time.perf_counter() - t0
+    log("  mixed solve: %d FP32 iterations, error %d, %.3f s" % (st["inner_total"], errm, tm))
     ref = x.t if x_ref is None else x_ref
     rel_err = float(torch.linalg.vector_norm(xm.t - ref) / torch.linalg.vector_norm(ref))
     return {"fp64": {"solves_per_s": round(1 / t64, 4), "iterations": it.iterations(), "error_code": err,
@@ -318,6 +325,8 @@ def main():
         kname = "k_csrv<%d lanes/row>" % kern[1]
     elif kern[0] == m._lib.SPMV_STREAM:
         kname = "k_csrp (persistent LDS tiles, %d workgroups/CU, %d rows/block)" % (kern[1], kern[2])
+    elif kern[0] == m._lib.SPMV_SCALAR:
+        kname = "k_csrs (one lane per row, %d values per step)" % kern[1]
     else:
         kname = "k_csr (LDS tiles, %d rows/block)" % kern[2]
     roofline = {"bound": "hbm", "kernel": "%s<EpiAssign> (y = A*x, FP64, diffusion3d %d^3)" % (kname, g),

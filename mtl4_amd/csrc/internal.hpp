@@ -277,6 +277,7 @@ struct mtl4g_matrix_s {
     int rows_per_block = 0, nnz_per_block = 0;
     int cus = 256;                         // compute units of the device
     int per_cu = 4;                        // stream kernel: workgroups per CU
+    int uv = 8;                            // scalar kernel: values requested per lane-step
     int per_cu_lowp[2] = {0, 0};           // the same for FP32 / BF16 values (0: as FP64)
     bool lowp_tuned[2] = {false, false};
     int32_t max_row = 0;

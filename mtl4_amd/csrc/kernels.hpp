@@ -1,10 +1,16 @@
-// kernels.hpp -- the two kernel shapes every operation of the library is
-// built from, each parameterized by an epilogue functor:
+// kernels.hpp -- the kernel shapes every operation of the library is built
+// from, each parameterized by an epilogue functor:
 //
-//   k_csr: one workgroup per CSR row block (SpMV and everything fused into
-//          it: residuals, dots of the product with another vector);
+//   CSR kernels (SpMV and everything fused into it: residuals, dots of the
+//   product with another vector), all summing each row in column order:
+//     k_csr:  one workgroup per row block, products staged in LDS;
+//     k_csrp: the same row blocks walked by a persistent grid that keeps the
+//             next block's stream in flight;
+//     k_csrs: one lane per row;
+//     k_csrv: a group of lanes per row, products added through shuffles;
 //   k_vec: one workgroup per contiguous chunk of a vector (axpy-type
-//          updates fused with their reductions).
+//          updates fused with their reductions);
+//   k_ordered: the final sums of a reduction in MTL4's own order.
 //
 // Epilogue concept:
 //   static constexpr int NP;                       // partial sums produced
@@ -198,6 +204,52 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
         b = bn;
     }
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
+}
+
+// Scalar CSR: one lane per row, RPT consecutive rows per lane-pass.  The
+// lane requests the first UV values and indices of its row at once (more in
+// a loop for longer rows), then the gathers, then sums in column order --
+// no LDS, no barrier.  For stencil rows the k-th gathers of neighbouring
+// lanes are neighbouring x entries (coalesced), and the strided value and
+// index loads of a wavefront cover its rows' contiguous span, reused from
+// the vector cache across k.  Rows map to workgroups XCD-aware.
+template <int BS, int UV, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_csrs(const int32_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ col,
+                                             const MT* __restrict__ val,
+                                             const XT* __restrict__ x, int nrows, int nb, int per_xcd,
+                                             Epi epi, AT* partials, unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    __shared__ AT sh[BS / kWave];
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+    const int b = xcd_logical_block(blockIdx.x, per_xcd);
+    const int row = b * BS + (int)threadIdx.x;
+    if (b < nb && row < nrows) {
+        const int a = rowptr[row], e = rowptr[row + 1];
+        const typename Epi::Pre pre = epi.preload(row);
+        AT s = AT(0);
+        for (int j0 = a; j0 < e; j0 += UV) {
+            int c[UV];
+            AT v[UV];
+#pragma unroll
+            for (int u = 0; u < UV; u++) {
+                c[u] = j0 + u < e ? col[j0 + u] : 0;   // cached: neighbouring lanes reuse the lines
+                v[u] = j0 + u < e ? load_val<AT>(val, (int64_t)j0 + u) : AT(0);
+            }
+            AT g[UV];
+#pragma unroll
+            for (int u = 0; u < UV; u++) g[u] = j0 + u < e ? static_cast<AT>(x[c[u]]) : AT(0);
+#pragma unroll
+            for (int u = 0; u < UV; u++)
+                if (j0 + u < e) s += v[u] * g[u];
+        }
+        epi(row, s, pre, part);
+    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
 // Vector CSR: a group of GS lanes per row, 64/GS rows per wavefront pass.
@@ -397,6 +449,23 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
             M4_HIP(hipGetLastError());
             return MTL4G_OK;
         }
+    }
+    if (A->kernel == MTL4G_SPMV_SCALAR) {
+        const int nbs = (A->n + 255) / 256;
+        const int per = (nbs + kXcd - 1) / kXcd;
+        const int grid = per * kXcd;
+        if (Epi::NP > 0 && grid > kMaxPartials) {
+            set_error("matrix too large for one reduction pass (%d blocks)", grid);
+            return MTL4G_ERR_UNSUPPORTED;
+        }
+        if (A->uv <= 4)
+            hipLaunchKernelGGL((k_csrs<256, 4, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s, A->rowptr.p,
+                               A->col.p, val, x, A->n, nbs, per, epi, partials, ticket);
+        else
+            hipLaunchKernelGGL((k_csrs<256, 8, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s, A->rowptr.p,
+                               A->col.p, val, x, A->n, nbs, per, epi, partials, ticket);
+        M4_HIP(hipGetLastError());
+        return MTL4G_OK;
     }
     if (A->kernel == MTL4G_SPMV_VECTOR) {
         const int rpb = A->vrows;

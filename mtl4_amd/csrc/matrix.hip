@@ -518,6 +518,17 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
         A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
         return MTL4G_OK;
     }
+    if (kernel == MTL4G_SPMV_SCALAR) {
+        if (group == 0) group = A->uv;
+        if (group != 4 && group != 8) {
+            set_error("scalar CSR: 4 or 8 values per step");
+            return MTL4G_ERR_INVALID;
+        }
+        A->kernel = MTL4G_SPMV_SCALAR;
+        A->uv = group;
+        A->lowp_tuned[0] = A->lowp_tuned[1] = false;
+        return MTL4G_OK;
+    }
     if (kernel != MTL4G_SPMV_VECTOR) return MTL4G_ERR_INVALID;
     if (group == 0) group = A->gs;
     if (group != 4 && group != 8 && group != 16 && group != 32 && group != 64) {
@@ -534,7 +545,11 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block) {
     if (!A) return MTL4G_ERR_INVALID;
     if (kernel) *kernel = A->kernel;
-    if (group) *group = A->kernel == MTL4G_SPMV_VECTOR ? A->gs : A->kernel == MTL4G_SPMV_STREAM ? A->per_cu : 0;
+    if (group)
+        *group = A->kernel == MTL4G_SPMV_VECTOR   ? A->gs
+                 : A->kernel == MTL4G_SPMV_STREAM ? A->per_cu
+                 : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
+                                                  : 0;
     if (rows_per_block) *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR ? A->vrows : A->rows_per_block;
     return MTL4G_OK;
 }
@@ -551,6 +566,7 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     // product buffers of TILE doubles per workgroup)
     for (int f : {2, 3, 4}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 256, 2048});
     for (int f : {4, 6, 8}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 128, 1024});
+    for (int u : {4, 8}) cands.push_back({MTL4G_SPMV_SCALAR, u, 0, 0, 0});
     // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
     const double avg = A->n ? (double)A->nnz / A->n : 0.0;
     for (int g : {4, 8, 16, 32, 64}) {

@@ -65,10 +65,27 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
         if (k < S.lev_ptr[L + 1]) {
             const int r = S.perm[k];
             VT rr = v[r];
-            for (int j = S.rowptr[k]; j < S.rowptr[k + 1]; j++)
-                rr -= static_cast<VT>(S.val[j]) * ld_agent(w + S.col[j]);
+            // eight entries at a time: indices and values first, then the
+            // eight solution values they name (one round trip each), then
+            // the subtractions in the row's column order
+            const int a = S.rowptr[k], e = S.rowptr[k + 1];
+            for (int j0 = a; j0 < e; j0 += 8) {
+                int c[8];
+                VT m[8], wv[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const bool in = j0 + u < e;
+                    c[u] = in ? S.col[j0 + u] : 0;
+                    m[u] = in ? static_cast<VT>(S.val[j0 + u]) : VT(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) wv[u] = j0 + u < e ? ld_agent(w + c[u]) : VT(0);
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (j0 + u < e) rr -= m[u] * wv[u];
+            }
             if (!S.unit) rr = rr * static_cast<VT>(S.dia[k]);
-            publish(w + r, rr);
+            st_agent(w + r, rr);   // the L1 writes through; wait_memory() below waits for it
         }
     }
     wait_memory();

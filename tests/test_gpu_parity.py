@@ -110,6 +110,32 @@ def test_spmv_stream_bitwise(m, fam, shape):
         assert np.linalg.norm(sols[0][1] - sols[1][1]) <= 1e-9 * np.linalg.norm(sols[1][1])
 
 
+@pytest.mark.parametrize("uv", [4, 8])
+@pytest.mark.parametrize("fam", ["lap2d_5", "lap2d_64", "random_3000", "long_rows", "diff3d_16"])
+def test_spmv_scalar_bitwise(m, fam, uv):
+    """One lane per row: same bits as MTL4's row sums, =, -=, and inside CG."""
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    A.set_kernel(m._lib.SPMV_SCALAR, uv, 0)
+    assert A.get_kernel()[:2] == (m._lib.SPMV_SCALAR, uv)
+    x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    y0 = np.random.default_rng(9).uniform(-1, 1, A.num_rows())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+    w = dv(m, y0)
+    A.mult(dv(m, x), w, m.MINUS)
+    yr = y0.copy()
+    orc.spmv(O, x, yr, orc.MINUS)
+    assert np.array_equal(w.numpy(), yr)
+    if fam == "lap2d_64":
+        b = orc.spmv(O, x)
+        xo, st, ito, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=3000, rtol=1e-10)
+        xs = m.dense_vector(A.num_rows())
+        it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+        assert m.cg(A, xs, dv(m, b), m.pc.diagonal(A), it) == 0
+        assert abs(it.iterations() - ito.i) <= 2
+        assert np.linalg.norm(xs.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
 def test_spmv_autotune_keeps_results(m):
     rp, ci, va = family("random_3000", seed=2)
     O, A = pair(rp, ci, va)
