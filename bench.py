@@ -156,22 +156,19 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
 def c3_section(m, torch, gen):
     """Mixed-precision cases: cg + ic_0 on 128^3 diffusion; bicgstab + ilu_0
     on the 1024^2 upwind convection-diffusion problem (Peclet 10)."""
-    out = {}
+    def case(make, pc, method):
+        rp, ci, va = make()
+        n = len(rp) - 1
+        A = m.compressed2D(n, n, rp, ci, va)
+        A.autotune()
+        b = A * m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
+        return pair_section(m, torch, A, b, m.two_norm(b), pc, method)
+
     log("c3: diffusion3d 128^3")
-    rp, ci, va = gen.diffusion3d(128, seed=0)
-    n = len(rp) - 1
-    A = m.compressed2D(n, n, rp, ci, va)
-    A.autotune()
-    b = A * m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
-    out["cg_ic0_diffusion3d_128"] = pair_section(m, torch, A, b, m.two_norm(b), "ic_0", m.CG)
-    del A, b
+    out = {"cg_ic0_diffusion3d_128": guarded(lambda: case(lambda: gen.diffusion3d(128, seed=0), "ic_0", m.CG))}
     log("c3: convection-diffusion 1024^2")
-    rp, ci, va = gen.convection_diffusion2d(1024, peclet=10.0, seed=0)
-    n = len(rp) - 1
-    A = m.compressed2D(n, n, rp, ci, va)
-    A.autotune()
-    b = A * m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
-    out["bicgstab_ilu0_convdiff_1024"] = pair_section(m, torch, A, b, m.two_norm(b), "ilu_0", m.BICGSTAB)
+    out["bicgstab_ilu0_convdiff_1024"] = guarded(lambda: case(
+        lambda: gen.convection_diffusion2d(1024, peclet=10.0, seed=0), "ilu_0", m.BICGSTAB))
     return out
 
 
@@ -254,7 +251,7 @@ def serving_section(m, torch, gen, nreq):
                         np.cumsum(rg.exponential(1.0 / rate, nreq)),
                         rg.integers(1, 2 ** 62, nreq, dtype=np.uint64))
         out["%dk_req_s" % int(rate / 1000)] = {}
-    for name, kw in (("mixed", dict(precision=m.F32, streams=4, max_items=1024)),
+    for name, kw in (("mixed", dict(precision=m.F32, streams=4, max_items=128)),
                      ("fp64_per_request", dict(precision=m.F64, streams=1, max_items=8,
                                                 per_request=True))):
         Sv = m.Server(B, rtol=1e-10, big_rows=65536, cap_bytes=200e9, **kw)

@@ -785,7 +785,10 @@ int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, doubl
     init.delta = delta;
     hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
     M4_HIP(hipGetLastError());
-    const int32_t budget = std::max(0, it->max_iter - it->i) + 64;
+    // a safety net against a device loop that never stops: every counted
+    // iteration can leave at most one chunk (16) of launches skipped behind a
+    // pause for an FP64 refresh
+    const int32_t budget = (std::max(0, it->max_iter - it->i) + 8) * 16 + 64;
     int st;
     if (fmt == MTL4G_BF16) st = mixed_typed<uint16_t>(A, A->valbf.p, P, method, W, x32, x, b, budget, s);
     else st = mixed_typed<float>(A, A->val32.p, P, method, W, x32, x, b, budget, s);

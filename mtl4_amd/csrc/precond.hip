@@ -39,7 +39,7 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
     if (skip && *skip) return;
     __shared__ int s_b, s_ok, s_final;
     if (threadIdx.x == 0)
-        s_b = (int)__hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_b = (int)__hip_atomic_fetch_add(sweep_counter(counters, 0), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int b = s_b;
     const int L = S.blk_level[b];
@@ -48,9 +48,10 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
         if (L > 0) {
             const unsigned need = (unsigned)(S.lev_blk[L] - S.lev_blk[L - 1]);
             unsigned it = 0;
-            while (ld_agent(&counters[2 + L - 1]) < need) {
-                if (ld_agent(&counters[1]) != 0u) { ok = 0; break; }
-                if (++it > kSpinLimit) { st_agent(&counters[1], 1u); ok = 0; break; }
+            const unsigned* lev = sweep_counter(counters, 2 + L - 1);
+            while (ld_agent(lev) < need) {
+                if ((++it & 63u) == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) { ok = 0; break; }
+                if (it > kSpinLimit) { st_agent(sweep_counter(counters, 1), 1u); ok = 0; break; }
                 __builtin_amdgcn_s_sleep(2);
             }
         }
@@ -93,13 +94,13 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
     if (threadIdx.x == 0) {
         const unsigned need = (unsigned)(S.lev_blk[L + 1] - S.lev_blk[L]);
         const unsigned done =
-            __hip_atomic_fetch_add(&counters[2 + L], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            __hip_atomic_fetch_add(sweep_counter(counters, 2 + L), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         s_final = (L == S.nlev - 1 && done == need);
     }
     __syncthreads();
     if (s_final) {   // every other workgroup has published: reset for the next launch
-        if (threadIdx.x == 0 && ld_agent(&counters[1]) != 0u) atomicOr(err, 1);
-        for (int i = threadIdx.x; i < S.nlev + 2; i += BS) st_agent(&counters[i], 0u);
+        if (threadIdx.x == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) atomicOr(err, 1);
+        for (int i = threadIdx.x; i < S.nlev + 2; i += BS) st_agent(sweep_counter(counters, i), 0u);
     }
 }
 
@@ -202,8 +203,8 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
         M4_TRY(up(S->val32, v32));
         M4_TRY(up(S->dia32, d32));
     }
-    M4_TRY(S->counters.alloc((size_t)nlev + 2));
-    M4_HIP(hipMemset(S->counters.p, 0, ((size_t)nlev + 2) * sizeof(unsigned)));
+    M4_TRY(S->counters.alloc(((size_t)nlev + 2) * kCounterStride));
+    M4_HIP(hipMemset(S->counters.p, 0, ((size_t)nlev + 2) * kCounterStride * sizeof(unsigned)));
     return MTL4G_OK;
 }
 
@@ -236,7 +237,7 @@ static int grid_for(int64_t n) {
 }
 
 size_t pc_counter_words(const mtl4g_pc_s* P) {
-    return (size_t)P->lower.nlev + 2 + (size_t)P->upper.nlev + 2;
+    return ((size_t)P->lower.nlev + 2 + (size_t)P->upper.nlev + 2) * kCounterStride;
 }
 
 template <typename VT>
@@ -259,7 +260,7 @@ int pc_apply(const mtl4g_pc_s* P, const VT* r, VT* z, VT* y0, const int* skip,
     default:
         M4_TRY(launch_sweep<VT>(P->lower, P->fmt, r, y0, skip, P->err.p, counters, s));
         M4_TRY(launch_sweep<VT>(P->upper, P->fmt, y0, z, skip, P->err.p,
-                                counters ? counters + P->lower.nlev + 2 : nullptr, s));
+                                counters ? counters + ((size_t)P->lower.nlev + 2) * kCounterStride : nullptr, s));
         return MTL4G_OK;
     }
     M4_HIP(hipGetLastError());

@@ -5,6 +5,14 @@
 
 namespace mtl4g {
 
+// Sweep counters live one per 128-byte line: waiting workgroups poll the
+// counter of one level, and neighbouring levels' counters must not share
+// the line that is being incremented.
+constexpr int kCounterStride = 32;
+__host__ __device__ inline unsigned* sweep_counter(unsigned* base, int i) {
+    return base + (size_t)i * kCounterStride;
+}
+
 // One triangular sweep  w[r] = (v[r] - sum_j T[r][c_j] * w[c_j]) * dia[r]
 // stored in level order: position k of the sweep handles row perm[k]; the
 // off-diagonal entries of that row follow in the order MTL4 subtracts them.
@@ -14,7 +22,7 @@ struct Sweep {
     DevBuf<int32_t> perm, rowptr, col, lev_ptr, lev_blk, blk_level;
     DevBuf<double> val64, dia64;
     DevBuf<float> val32, dia32;
-    DevBuf<unsigned> counters;   // default counters: [0] ticket, [1] abort, [2..2+nlev) level done counts
+    DevBuf<unsigned> counters;   // default counters: 0 ticket, 1 abort, 2..2+nlev level done counts
     std::vector<int32_t> h_lev_ptr;
 };
 
@@ -30,8 +38,8 @@ struct SweepRows {
 
 int upload_sweep(const SweepRows& rows, int value_format, Sweep* S);
 
-// counters: nlev+2 zeroed words owned by the caller (NULL = the sweep's own);
-// each launch leaves them zeroed again.
+// counters: (nlev+2) * kCounterStride zeroed words owned by the caller (NULL =
+// the sweep's own); each launch leaves them zeroed again.
 template <typename VT>
 int launch_sweep(const Sweep& S, int value_format, const VT* v, VT* w, const int* skip, int* err,
                  unsigned* counters, hipStream_t s);
