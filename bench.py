@@ -55,6 +55,8 @@ def traffic_key(m, kern):
     kind, group, rows = kern
     if kind == m._lib.SPMV_STREAM:
         return "stream/%d/%d" % (rows, group)
+    if kind == m._lib.SPMV_STREAM2:
+        return "stream2/%d/%d" % (rows, group)
     if kind == m._lib.SPMV_TILE:
         return "tile/%d" % rows
     if kind == m._lib.SPMV_SCALAR:
@@ -322,6 +324,8 @@ def main():
         kname = "k_csrv<%d lanes/row>" % kern[1]
     elif kern[0] == m._lib.SPMV_STREAM:
         kname = "k_csrp (persistent LDS tiles, %d workgroups/CU, %d rows/block)" % (kern[1], kern[2])
+    elif kern[0] == m._lib.SPMV_STREAM2:
+        kname = "k_csrp (persistent LDS tiles, two blocks ahead, %d workgroups/CU, %d rows/block)" % (kern[1], kern[2])
     elif kern[0] == m._lib.SPMV_SCALAR:
         kname = "k_csrs (one lane per row, %d values per step)" % kern[1]
     else:

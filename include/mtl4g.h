@@ -146,6 +146,8 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
  * MTL4G_SPMV_STREAM: the same row blocks walked by a persistent grid of
  * `group` workgroups per compute unit (1..8), each keeping its next
  * block's stream in flight while it sums the current one;
+ * MTL4G_SPMV_STREAM2: the stream kernel with two row blocks requested
+ * ahead instead of one;
  * MTL4G_SPMV_SCALAR: one lane per row, `group` (4 or 8) values and
  * indices requested per step, no LDS.
  * All sum every row in column order.  0 = library default. */
@@ -153,6 +155,7 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
 #define MTL4G_SPMV_VECTOR 1
 #define MTL4G_SPMV_STREAM 2
 #define MTL4G_SPMV_SCALAR 3
+#define MTL4G_SPMV_STREAM2 4
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);

@@ -107,18 +107,18 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
     if constexpr (Epi::NP > 0 && !ORD) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
-// Persistent row-block kernel with the next tile's stream in flight.
+// Persistent row-block kernel with the next tiles' streams in flight.
 // Workgroup g runs on XCD g % 8 and owns every (gridDim/8)-th row block of
 // that XCD's contiguous eighth of the matrix, so the XCD's workgroups sweep
-// one window of rows together and share its x in L2.  Per tile, a lane
-// issues the gathers of x for the current tile, then the descriptor, row
-// bounds, epilogue operands and value/index stream of its next tile, and
-// only then waits for the gathers: the next tile's HBM reads overlap the
-// current tile's gathers, products, row sums and epilogue.  Products go to
-// one of two LDS buffers in turn, so one barrier per tile suffices.  Rows
-// are summed in column order, as in k_csr; a longer-than-TILE single row is
-// summed by the whole workgroup (not overlapped).
-template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
+// one window of rows together and share its x in L2.  A workgroup keeps
+// DEPTH row blocks requested ahead: per tile, a lane issues the gathers of x
+// for the current tile, then the descriptor, row bounds, epilogue operands
+// and value/index stream of the tile DEPTH steps ahead, and only then waits
+// for the gathers (the wait counter counts in issue order, so the newer
+// requests stay in flight).  Products go to one of two LDS buffers in turn,
+// so one barrier per tile suffices.  Rows are summed in column order, as in
+// k_csr; a longer-than-TILE single row is summed by the whole workgroup.
+template <int BS, int TILE, int DEPTH, typename AT, typename MT, typename XT, typename Epi>
 __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
                                              const int32_t* __restrict__ col,
                                              const MT* __restrict__ val,
@@ -140,68 +140,73 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
 #pragma unroll
     for (int k = 0; k < NP; k++) part[k] = AT(0);
 
-    RowBlock d{};
-    int c[U];
-    AT v[U];
-    int ra = 0, re = 0;
-    typename Epi::Pre pre{};
-    auto fetch = [&](int bb) {
-        d = desc[bb];
-        const int row = d.r0 + t;
-        const int nz = d.j1 - d.j0;
-        if (row < d.r1) {
-            ra = rowptr[row] - d.j0;
-            re = rowptr[row + 1] - d.j0;
-            pre = epi.preload(row);
+    struct Stage {
+        RowBlock d;
+        int c[U];
+        AT v[U];
+        int ra, re;
+        typename Epi::Pre pre;
+    };
+    Stage st[DEPTH];   // requested tiles, st[0] the next to sum
+    auto fetch = [&](int bb, Stage& q) {
+        q.d = desc[bb];
+        const int row = q.d.r0 + t;
+        const int nz = q.d.j1 - q.d.j0;
+        q.ra = q.re = 0;
+        if (row < q.d.r1) {
+            q.ra = rowptr[row] - q.d.j0;
+            q.re = rowptr[row + 1] - q.d.j0;
+            q.pre = epi.preload(row);
         }
         if (nz <= TILE) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int k = t + u * BS;
-                c[u] = k < nz ? __builtin_nontemporal_load(col + d.j0 + k) : 0;
-                v[u] = k < nz ? load_val_nt<AT>(val, (int64_t)d.j0 + k) : AT(0);
+                q.c[u] = k < nz ? __builtin_nontemporal_load(col + q.d.j0 + k) : 0;
+                q.v[u] = k < nz ? load_val_nt<AT>(val, (int64_t)q.d.j0 + k) : AT(0);
             }
         }
     };
 
     int b = xcd * per + blockIdx.x / kXcd;
-    if (b < hi) fetch(b);
+#pragma unroll
+    for (int i = 0; i < DEPTH; i++)
+        if (b + i * slots < hi) fetch(b + i * slots, st[i]);
     int buf = 0;
     while (b < hi) {
-        const RowBlock dc = d;
-        const int nz = dc.j1 - dc.j0;
-        const int row = dc.r0 + t;
-        const int a = ra, e = re;
-        const typename Epi::Pre pc = pre;
-        const int bn = b + slots;
+        const Stage cur = st[0];
+#pragma unroll
+        for (int i = 0; i + 1 < DEPTH; i++) st[i] = st[i + 1];
+        const int nz = cur.d.j1 - cur.d.j0;
+        const int row = cur.d.r0 + t;
+        const int bn = b + DEPTH * slots;   // the tile requested in this step
         if (nz > TILE) {
             AT s = AT(0);
             for (int k = t; k < nz; k += BS)
-                s += load_val<AT>(val, (int64_t)dc.j0 + k) * static_cast<AT>(x[col[dc.j0 + k]]);
+                s += load_val<AT>(val, (int64_t)cur.d.j0 + k) * static_cast<AT>(x[col[cur.d.j0 + k]]);
             s = block_sum<BS>(s, sh);
-            if (bn < hi) fetch(bn);
-            if (t == 0) epi(row, s, pc, part);
+            if (bn < hi) fetch(bn, st[DEPTH - 1]);
+            if (t == 0) epi(row, s, cur.pre, part);
         } else {
-            AT g[U], vc[U];
+            AT g[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int k = t + u * BS;
-                g[u] = k < nz ? static_cast<AT>(x[c[u]]) : AT(0);
-                vc[u] = v[u];
+                g[u] = k < nz ? static_cast<AT>(x[cur.c[u]]) : AT(0);
             }
-            if (bn < hi) fetch(bn);
+            if (bn < hi) fetch(bn, st[DEPTH - 1]);
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int k = t + u * BS;
-                if (k < nz) prod[buf][k] = vc[u] * g[u];
+                if (k < nz) prod[buf][k] = cur.v[u] * g[u];
             }
             __syncthreads();
             AT s = AT(0);
-            for (int k = a; k < e; k++) s += prod[buf][k];
-            if (row < dc.r1) epi(row, s, pc, part);
+            for (int k = cur.ra; k < cur.re; k++) s += prod[buf][k];
+            if (row < cur.d.r1) epi(row, s, cur.pre, part);
             buf ^= 1;
         }
-        b = bn;
+        b += slots;
     }
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
 }
@@ -491,18 +496,23 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
         return MTL4G_OK;
     }
     if (A->nb == 0) return MTL4G_OK;
-    if (A->kernel == MTL4G_SPMV_STREAM) {
+    if (A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2) {
         int pcu = A->per_cu;
         if (sizeof(MT) == 4 && A->per_cu_lowp[0] > 0) pcu = A->per_cu_lowp[0];
         if (sizeof(MT) == 2 && A->per_cu_lowp[1] > 0) pcu = A->per_cu_lowp[1];
         const int want = A->cus * pcu;
         const int grid = std::max(kXcd, std::min(want, (A->nb + kXcd - 1) / kXcd * kXcd) / kXcd * kXcd);
         const int tile = A->nnz_per_block;
-#define M4_LAUNCHP(T)                                                                          \
-    hipLaunchKernelGGL((k_csrp<256, T, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,         \
+#define M4_LAUNCHP(T, D)                                                                       \
+    hipLaunchKernelGGL((k_csrp<256, T, D, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,      \
                        A->rowptr.p, A->col.p, val, x, A->desc.p, A->nb, epi, partials, ticket)
-        if (tile <= 1024) M4_LAUNCHP(1024);
-        else M4_LAUNCHP(2048);
+        if (A->kernel == MTL4G_SPMV_STREAM2) {
+            if (tile <= 1024) M4_LAUNCHP(1024, 2);
+            else M4_LAUNCHP(2048, 2);
+        } else {
+            if (tile <= 1024) M4_LAUNCHP(1024, 1);
+            else M4_LAUNCHP(2048, 1);
+        }
 #undef M4_LAUNCHP
         M4_HIP(hipGetLastError());
         return MTL4G_OK;

@@ -59,6 +59,8 @@ __device__ __forceinline__ void mixed_check(Ctl* c, double resid32) {
     if (resid32 <= c->delta * c->rmax) c->want_ru = 1;
 }
 
+constexpr int32_t kMaxStall = 8;
+
 // FP64 refresh done: stop on the true residual, or resume the recurrence
 // from the replaced residual (rho and beta of CG recomputed from it).
 __device__ __forceinline__ void ru_finish(Ctl* c, double rr64, double rho_new, bool cg_fused) {
@@ -67,6 +69,14 @@ __device__ __forceinline__ void ru_finish(Ctl* c, double rr64, double rho_new, b
     if (conv_test(c, resid)) { c->done = 1; return; }
     if (!isfinite(resid)) { c->status = MTL4G_ERR_NONFINITE; c->done = 1; return; }
     if (c->limit || (c->init && c->i >= c->max_iter)) { c->error = 1; c->done = 1; return; }
+    // stagnation: the FP32 recurrence keeps asking for refreshes (its own
+    // residual estimate has fallen) while the true residual does not fall
+    // any more -- its attainable accuracy is reached; stop as MTL4 stops at
+    // the iteration limit instead of cycling until max_iter
+    if (!c->init) {
+        c->stall = resid > 0.5 * c->rmax ? c->stall + 1 : 0;
+        if (c->stall >= kMaxStall) { c->error = 1; c->done = 1; return; }
+    }
     if (c->pause || c->init) {
         c->i++;
         c->first = c->i <= 1;

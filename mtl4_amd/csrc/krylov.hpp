@@ -15,7 +15,7 @@ struct Ctl {
     int32_t hist_n, hist_cap;
     // mixed precision (reliable updates): FP32 recurrence, FP64 refreshes
     int32_t mixed, pause, want_ru, ru_count, limit, init, restart;
-    int32_t pad;
+    int32_t stall;           // consecutive FP64 refreshes that did not halve the residual
     unsigned ticket[4];
 };
 

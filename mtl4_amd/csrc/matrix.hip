@@ -169,7 +169,8 @@ static int spmv32_mode(const mtl4g_matrix_s* A, int fmt, const float* x, float* 
 static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     A->lowp_tuned[fmt - 1] = true;
     A->per_cu_lowp[fmt - 1] = 0;
-    if (A->kernel != MTL4G_SPMV_STREAM || A->n == 0 || A->m == 0) return MTL4G_OK;
+    if ((A->kernel != MTL4G_SPMV_STREAM && A->kernel != MTL4G_SPMV_STREAM2) || A->n == 0 || A->m == 0)
+        return MTL4G_OK;
     float *x = nullptr, *y = nullptr;
     M4_HIP(hipMallocAsync((void**)&x, (size_t)A->m * 4, s));
     M4_HIP(hipMallocAsync((void**)&y, (size_t)A->n * 4, s));
@@ -505,14 +506,14 @@ int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block) {
     if (!A) return MTL4G_ERR_INVALID;
     if (kernel == MTL4G_SPMV_TILE) return mtl4g_csr_tune(A, rows_per_block, 0);
-    if (kernel == MTL4G_SPMV_STREAM) {
+    if (kernel == MTL4G_SPMV_STREAM || kernel == MTL4G_SPMV_STREAM2) {
         if (group == 0) group = A->per_cu;
         if (group < 1 || group > 8) {
             set_error("stream kernel: 1 to 8 workgroups per compute unit");
             return MTL4G_ERR_INVALID;
         }
         if (rows_per_block > 0 || !A->desc.p) M4_TRY(build_row_blocks(A, rows_per_block, 0));
-        A->kernel = MTL4G_SPMV_STREAM;
+        A->kernel = kernel;
         A->per_cu = group;
         A->lowp_tuned[0] = A->lowp_tuned[1] = false;
         A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
@@ -547,7 +548,7 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
     if (kernel) *kernel = A->kernel;
     if (group)
         *group = A->kernel == MTL4G_SPMV_VECTOR   ? A->gs
-                 : A->kernel == MTL4G_SPMV_STREAM ? A->per_cu
+                 : A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2 ? A->per_cu
                  : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
                                                   : 0;
     if (rows_per_block) *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR ? A->vrows : A->rows_per_block;
@@ -566,6 +567,8 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     // product buffers of TILE doubles per workgroup)
     for (int f : {2, 3, 4}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 256, 2048});
     for (int f : {4, 6, 8}) cands.push_back({MTL4G_SPMV_STREAM, f, 0, 128, 1024});
+    for (int f : {2, 3, 4}) cands.push_back({MTL4G_SPMV_STREAM2, f, 0, 256, 2048});
+    for (int f : {4, 6}) cands.push_back({MTL4G_SPMV_STREAM2, f, 0, 128, 1024});
     for (int u : {4, 8}) cands.push_back({MTL4G_SPMV_SCALAR, u, 0, 0, 0});
     // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
     const double avg = A->n ? (double)A->nnz / A->n : 0.0;
@@ -584,7 +587,8 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     Cand win = cands[0];
     for (const Cand& c : cands) {
         int st = c.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, c.a, c.b)
-                 : c.kernel == MTL4G_SPMV_STREAM ? (mtl4g_csr_tune(A, c.rows, c.nnz) ||
+                 : c.kernel == MTL4G_SPMV_STREAM || c.kernel == MTL4G_SPMV_STREAM2
+                     ? (mtl4g_csr_tune(A, c.rows, c.nnz) ||
                                                     mtl4g_csr_set_kernel(A, c.kernel, c.a, 0))
                                                  : mtl4g_csr_set_kernel(A, c.kernel, c.a, c.b);
         if (st) continue;
@@ -606,7 +610,8 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     (void)hipEventDestroy(e1);
     (void)hipFreeAsync(x, s);
     (void)hipFreeAsync(y, s);
-    if (win.kernel == MTL4G_SPMV_STREAM) M4_TRY(mtl4g_csr_tune(A, win.rows, win.nnz));
+    if (win.kernel == MTL4G_SPMV_STREAM || win.kernel == MTL4G_SPMV_STREAM2)
+        M4_TRY(mtl4g_csr_tune(A, win.rows, win.nnz));
     M4_TRY(win.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, win.a, win.b)
                                          : mtl4g_csr_set_kernel(A, win.kernel, win.a, win.b));
     if (h_best_us) *h_best_us = best;

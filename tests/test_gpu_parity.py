@@ -75,17 +75,20 @@ def test_spmv_vector_groups_bitwise(m, fam, group):
     assert np.array_equal(w.numpy(), yr)
 
 
+@pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("shape", [(256, 2048, 2), (256, 2048, 4), (128, 1024, 8), (64, 1024, 1)])
 @pytest.mark.parametrize("fam", ["lap2d_5", "lap2d_64", "random_3000", "long_rows", "diff3d_16"])
-def test_spmv_stream_bitwise(m, fam, shape):
-    """Persistent stream kernel: same bits as MTL4's row sums, for =, -=,
-    and fused into CG's SpMV+dot (whole solve against the tile kernel)."""
+def test_spmv_stream_bitwise(m, fam, shape, depth):
+    """Persistent stream kernel (one or two row blocks requested ahead): same
+    bits as MTL4's row sums, for =, -=, and fused into CG's SpMV+dot (whole
+    solve against the tile kernel)."""
     rows, nnz, per_cu = shape
+    kernel = m._lib.SPMV_STREAM if depth == 1 else m._lib.SPMV_STREAM2
     rp, ci, va = family(fam, seed=6)
     O, A = pair(rp, ci, va)
     A.tune(rows, nnz)
-    A.set_kernel(m._lib.SPMV_STREAM, per_cu, 0)
-    assert A.get_kernel()[:2] == (m._lib.SPMV_STREAM, per_cu)
+    A.set_kernel(kernel, per_cu, 0)
+    assert A.get_kernel()[:2] == (kernel, per_cu)
     x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
     y0 = np.random.default_rng(9).uniform(-1, 1, A.num_rows())
     assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
@@ -97,8 +100,8 @@ def test_spmv_stream_bitwise(m, fam, shape):
     if fam in ("lap2d_64", "diff3d_16"):
         b = orc.spmv(O, x)
         sols = []
-        for kernel in (m._lib.SPMV_STREAM, m._lib.SPMV_TILE):
-            if kernel == m._lib.SPMV_TILE:
+        for k in (kernel, m._lib.SPMV_TILE):
+            if k == m._lib.SPMV_TILE:
                 A.tune(rows, nnz)
             xs = m.dense_vector(A.num_rows())
             it = m.basic_iteration(dv(m, b), 3000, 1e-10)
