@@ -50,6 +50,29 @@ def timed_spmv(m, torch, A, x, y, reps):
     return e0.elapsed_time(e1) / 1000.0 / reps
 
 
+def hbm_context(torch):
+    """What plain library kernels reach on this GPU (context for the
+    roofline, not a ceiling of the SpMV): a 2 GiB device copy (torch copy_)
+    and a 2 GiB read (torch.sum), 5 repetitions each after a warm-up."""
+    src = torch.ones(256 << 20, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    nbytes = src.numel() * 8
+    out = {}
+    for name, fn, moved in (("copy_GBs", lambda: dst.copy_(src), 2 * nbytes),
+                            ("sum_read_GBs", lambda: torch.sum(src), nbytes)):
+        fn()
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(5):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        out[name] = round(moved / (e0.elapsed_time(e1) / 5 / 1000.0) / 1e9, 1)
+    del src, dst
+    return out
+
+
 def traffic_key(m, kern):
     """Name of a CSR kernel configuration as in the PMC traffic table."""
     kind, group, rows = kern
@@ -345,6 +368,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"spmv_gbs": spmv_gbs, "roofline": roofline}))
         return
+    roofline["hbm_context"] = hbm_context(torch)
     rr = gen.random_sparse(2_000_000, seed=0)
     R = m.compressed2D(2_000_000, 2_000_000, *rr)
     R.autotune()

@@ -107,7 +107,106 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
     if constexpr (Epi::NP > 0 && !ORD) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
-// Persistent row-block kernel with the next tiles' streams in flight.
+// Persistent row-block kernel with the next tile's stream in flight.
+// Workgroup g runs on XCD g % 8 and owns every (gridDim/8)-th row block of
+// that XCD's contiguous eighth of the matrix, so the XCD's workgroups sweep
+// one window of rows together and share its x in L2.  Per tile, a lane
+// issues the gathers of x for the current tile, then the descriptor, row
+// bounds, epilogue operands and value/index stream of its next tile, and
+// only then waits for the gathers: the next tile's HBM reads overlap the
+// current tile's gathers, products, row sums and epilogue.  Products go to
+// one of two LDS buffers in turn, so one barrier per tile suffices.  Rows
+// are summed in column order, as in k_csr; a longer-than-TILE single row is
+// summed by the whole workgroup (not overlapped).
+template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ col,
+                                             const MT* __restrict__ val,
+                                             const XT* __restrict__ x,
+                                             const RowBlock* __restrict__ desc, int nb, Epi epi,
+                                             AT* partials, unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    constexpr int U = TILE / BS;
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    __shared__ AT prod[2][TILE];
+    __shared__ AT sh[BS / kWave];
+    const int t = threadIdx.x;
+    const int slots = gridDim.x / kXcd;
+    const int per = (nb + kXcd - 1) / kXcd;
+    const int xcd = blockIdx.x % kXcd;
+    const int hi = min(nb, (xcd + 1) * per);
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+
+    RowBlock d{};
+    int c[U];
+    AT v[U];
+    int ra = 0, re = 0;
+    typename Epi::Pre pre{};
+    auto fetch = [&](int bb) {
+        d = desc[bb];
+        const int row = d.r0 + t;
+        const int nz = d.j1 - d.j0;
+        if (row < d.r1) {
+            ra = rowptr[row] - d.j0;
+            re = rowptr[row + 1] - d.j0;
+            pre = epi.preload(row);
+        }
+        if (nz <= TILE) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = t + u * BS;
+                c[u] = k < nz ? __builtin_nontemporal_load(col + d.j0 + k) : 0;
+                v[u] = k < nz ? load_val_nt<AT>(val, (int64_t)d.j0 + k) : AT(0);
+            }
+        }
+    };
+
+    int b = xcd * per + blockIdx.x / kXcd;
+    if (b < hi) fetch(b);
+    int buf = 0;
+    while (b < hi) {
+        const RowBlock dc = d;
+        const int nz = dc.j1 - dc.j0;
+        const int row = dc.r0 + t;
+        const int a = ra, e = re;
+        const typename Epi::Pre pc = pre;
+        const int bn = b + slots;
+        if (nz > TILE) {
+            AT s = AT(0);
+            for (int k = t; k < nz; k += BS)
+                s += load_val<AT>(val, (int64_t)dc.j0 + k) * static_cast<AT>(x[col[dc.j0 + k]]);
+            s = block_sum<BS>(s, sh);
+            if (bn < hi) fetch(bn);
+            if (t == 0) epi(row, s, pc, part);
+        } else {
+            AT g[U], vc[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = t + u * BS;
+                g[u] = k < nz ? static_cast<AT>(x[c[u]]) : AT(0);
+                vc[u] = v[u];
+            }
+            if (bn < hi) fetch(bn);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = t + u * BS;
+                if (k < nz) prod[buf][k] = vc[u] * g[u];
+            }
+            __syncthreads();
+            AT s = AT(0);
+            for (int k = a; k < e; k++) s += prod[buf][k];
+            if (row < dc.r1) epi(row, s, pc, part);
+            buf ^= 1;
+        }
+        b = bn;
+    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
+}
+
+// k_csrp with DEPTH row blocks requested ahead instead of one.
 // Workgroup g runs on XCD g % 8 and owns every (gridDim/8)-th row block of
 // that XCD's contiguous eighth of the matrix, so the XCD's workgroups sweep
 // one window of rows together and share its x in L2.  A workgroup keeps
@@ -119,7 +218,7 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
 // so one barrier per tile suffices.  Rows are summed in column order, as in
 // k_csr; a longer-than-TILE single row is summed by the whole workgroup.
 template <int BS, int TILE, int DEPTH, typename AT, typename MT, typename XT, typename Epi>
-__global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
+__global__ __launch_bounds__(BS) void k_csrp2(const int32_t* __restrict__ rowptr,
                                              const int32_t* __restrict__ col,
                                              const MT* __restrict__ val,
                                              const XT* __restrict__ x,
@@ -312,6 +411,15 @@ __global__ __launch_bounds__(BS) void k_csrv(const int32_t* __restrict__ rowptr,
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
+// Element epilogues may split operator()(i, part) into In load(i) (all
+// reads of element i) and store(i, const In&, part) (the arithmetic, writes
+// and partial sums); k_vec then batches the loads of several elements.
+template <typename E, typename = void>
+struct has_split : std::false_type {};
+template <typename E>
+struct has_split<E, std::void_t<typename E::In>> : std::true_type {};
+constexpr int kVecBatch = 4;
+
 template <int BS, typename AT, typename Epi, bool ORD = false>
 __global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, unsigned* ticket,
                                             AT* terms) {
@@ -337,7 +445,22 @@ __global__ __launch_bounds__(BS) void k_vec(int64_t n, Epi epi, AT* partials, un
             for (int k = 0; k < NP; k++) terms[(size_t)k * n + i] = part[k];
         }
     } else {
-        for (int64_t i = lo + threadIdx.x; i < hi; i += BS) epi(i, part);
+        if constexpr (has_split<Epi>::value) {
+            // loads of kVecBatch elements first, then their updates in the same
+            // element order as the plain loop (partial sums unchanged): the
+            // loads of later elements need not wait for earlier stores
+            int64_t i0 = lo + threadIdx.x;
+            for (; i0 + (kVecBatch - 1) * BS < hi; i0 += kVecBatch * BS) {
+                typename Epi::In in[kVecBatch];
+#pragma unroll
+                for (int u = 0; u < kVecBatch; u++) in[u] = epi.load(i0 + u * BS);
+#pragma unroll
+                for (int u = 0; u < kVecBatch; u++) epi.store(i0 + u * BS, in[u], part);
+            }
+            for (; i0 < hi; i0 += BS) epi.store(i0, epi.load(i0), part);
+        } else {
+            for (int64_t i = lo + threadIdx.x; i < hi; i += BS) epi(i, part);
+        }
         if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)G, epi, sh);
     }
 }
@@ -429,6 +552,21 @@ int want_ordered(const AT* terms, bool& ord) {
 }
 
 // ---- host launchers
+// Grid of a persistent kernel: cus * per_cu workgroups (a multiple of the
+// XCD count), no more than the row blocks, and no more per CU than can be
+// resident together (queried once per kernel instantiation).
+template <auto KERNEL>
+int persistent_grid(int cus, int per_cu, int nb) {
+    static const int resident = []() {
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, KERNEL, 256, 0) != hipSuccess || nblk < 1)
+            nblk = 1;
+        return nblk;
+    }();
+    const int want = cus * std::min(per_cu, resident);
+    return std::max(kXcd, std::min(want, (nb + kXcd - 1) / kXcd * kXcd) / kXcd * kXcd);
+}
+
 // terms: Epi::NP * A->n elements, used when reductions run in MTL4's order.
 template <typename AT, typename MT, typename XT, typename Epi>
 int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& epi,
@@ -500,18 +638,21 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
         int pcu = A->per_cu;
         if (sizeof(MT) == 4 && A->per_cu_lowp[0] > 0) pcu = A->per_cu_lowp[0];
         if (sizeof(MT) == 2 && A->per_cu_lowp[1] > 0) pcu = A->per_cu_lowp[1];
-        const int want = A->cus * pcu;
-        const int grid = std::max(kXcd, std::min(want, (A->nb + kXcd - 1) / kXcd * kXcd) / kXcd * kXcd);
         const int tile = A->nnz_per_block;
-#define M4_LAUNCHP(T, D)                                                                       \
-    hipLaunchKernelGGL((k_csrp<256, T, D, AT, MT, XT, Epi>), dim3(grid), dim3(256), 0, s,      \
-                       A->rowptr.p, A->col.p, val, x, A->desc.p, A->nb, epi, partials, ticket)
+        // a persistent grid must be resident at once: never more workgroups
+        // per CU than this instantiation's registers and LDS allow
+#define M4_LAUNCHP(KERNEL)                                                                     \
+    do {                                                                                       \
+        const int g = persistent_grid<KERNEL>(A->cus, pcu, A->nb);                            \
+        hipLaunchKernelGGL((KERNEL), dim3(g), dim3(256), 0, s, A->rowptr.p, A->col.p, val, x,   \
+                           A->desc.p, A->nb, epi, partials, ticket);                          \
+    } while (0)
         if (A->kernel == MTL4G_SPMV_STREAM2) {
-            if (tile <= 1024) M4_LAUNCHP(1024, 2);
-            else M4_LAUNCHP(2048, 2);
+            if (tile <= 1024) M4_LAUNCHP((k_csrp2<256, 1024, 2, AT, MT, XT, Epi>));
+            else M4_LAUNCHP((k_csrp2<256, 2048, 2, AT, MT, XT, Epi>));
         } else {
-            if (tile <= 1024) M4_LAUNCHP(1024, 1);
-            else M4_LAUNCHP(2048, 1);
+            if (tile <= 1024) M4_LAUNCHP((k_csrp<256, 1024, AT, MT, XT, Epi>));
+            else M4_LAUNCHP((k_csrp<256, 2048, AT, MT, XT, Epi>));
         }
 #undef M4_LAUNCHP
         M4_HIP(hipGetLastError());

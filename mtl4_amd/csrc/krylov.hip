@@ -168,13 +168,21 @@ struct EpiCgP {
         beta = static_cast<VT>(c->beta);
         first = c->first;
     }
-    __device__ void operator()(int64_t i, VT*) const {
-        VT zi;
-        if (PCM == PCM_NONE) zi = r[i];
-        else if (PCM == PCM_DIAG) zi = static_cast<VT>(d[i]) * r[i];
-        else zi = z[i];
-        p[i] = first ? zi : zi + beta * p[i];
+    struct In {
+        VT zi, p;
+    };
+    __device__ In load(int64_t i) const {
+        In in;
+        if (PCM == PCM_NONE) in.zi = r[i];
+        else if (PCM == PCM_DIAG) in.zi = static_cast<VT>(d[i]) * r[i];
+        else in.zi = z[i];
+        in.p = first ? VT(0) : p[i];
+        return in;
     }
+    __device__ void store(int64_t i, const In& in, VT*) const {
+        p[i] = first ? in.zi : in.zi + beta * in.p;
+    }
+    __device__ void operator()(int64_t i, VT* part) const { store(i, load(i), part); }
     __device__ void finish(const VT*) const {}
 };
 
@@ -219,16 +227,23 @@ struct EpiCgXR {
     VT alpha;
     __device__ bool skip() const { return c->done != 0 || c->pause != 0; }
     __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
-    __device__ void operator()(int64_t i, VT* part) const {
-        x[i] = x[i] + alpha * p[i];
-        const VT v = r[i] - alpha * q[i];
+    struct In {
+        VT x, p, r, q, d;
+    };
+    __device__ In load(int64_t i) const {
+        return In{x[i], p[i], r[i], q[i], PCM == PCM_DIAG ? static_cast<VT>(d[i]) : VT(0)};
+    }
+    __device__ void store(int64_t i, const In& in, VT* part) const {
+        x[i] = in.x + alpha * in.p;
+        const VT v = in.r - alpha * in.q;
         r[i] = v;
         part[0] += v * v;
         if (PCM == PCM_DIAG) {
-            const VT z = static_cast<VT>(d[i]) * v;
+            const VT z = in.d * v;
             part[1] += v * z;
         }
     }
+    __device__ void operator()(int64_t i, VT* part) const { store(i, load(i), part); }
     __device__ void finish(const VT* tot) const {
         c->rr = (double)tot[0];
         if (c->mixed) mixed_check(c, (double)std::sqrt(std::fabs(tot[0])));
