@@ -73,11 +73,11 @@ def hbm_context(torch):
     return out
 
 
-def traffic_key(m, kern, index_bytes=4):
+def traffic_key(m, kern):
     """Name of a CSR kernel configuration as in the PMC traffic table."""
     kind, group, rows = kern
     if kind == m._lib.SPMV_STREAM:
-        return "stream%s/%d/%d" % ("16" if index_bytes == 2 else "", rows, group)
+        return "stream/%d/%d" % (rows, group)
     if kind == m._lib.SPMV_STREAM2:
         return "stream2/%d/%d" % (rows, group)
     if kind == m._lib.SPMV_TILE:
@@ -342,8 +342,7 @@ def main():
     by = spmv_bytes(n, nnz)
     spmv_gbs = by / t_spmv / 1e9
     kern = A.get_kernel()
-    ib = A.index_bytes()
-    traffic = load_traffic(traffic_key(m, kern, ib))
+    traffic = load_traffic(traffic_key(m, kern))
     if kern[0] == m._lib.SPMV_VECTOR:
         kname = "k_csrv<%d lanes/row>" % kern[1]
     elif kern[0] == m._lib.SPMV_STREAM:
@@ -355,7 +354,7 @@ def main():
     else:
         kname = "k_csr (LDS tiles, %d rows/block)" % kern[2]
     roofline = {"bound": "hbm", "kernel": "%s<EpiAssign> (y = A*x, FP64, diffusion3d %d^3)" % (kname, g),
-                "config": traffic_key(m, kern, ib), "index_bytes_read": ib,
+                "config": traffic_key(m, kern),
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": by, "avg_launch_us": round(t_spmv * 1e6, 2)}

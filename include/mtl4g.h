@@ -159,10 +159,6 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);
-/* Bytes per column index the stream kernel reads: 2 when every entry's
- * column is within +-32767 of its row block's first row (then stored as
- * that offset), else 4.  Results do not depend on it. */
-int mtl4g_csr_index_bytes(mtl4g_matrix A, int* bytes);
 /* Autotuner: times y = A*x for the candidate kernels and shapes on this
  * matrix and keeps the fastest; h_best_us (may be NULL) gets its time. */
 int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream);

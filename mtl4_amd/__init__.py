@@ -175,12 +175,6 @@ class compressed2D:
         check(_L().mtl4g_csr_get_kernel(self.h, C.byref(k), C.byref(g), C.byref(r)))
         return k.value, g.value, r.value
 
-    def index_bytes(self):
-        """Bytes per column index the current SpMV kernel reads (2 or 4)."""
-        b = C.c_int()
-        check(_L().mtl4g_csr_index_bytes(self.h, C.byref(b)), "index_bytes")
-        return b.value
-
     def autotune(self):
         """Time the CSR kernel candidates on this matrix, keep the fastest (us)."""
         us = C.c_double()

@@ -270,8 +270,6 @@ struct mtl4g_matrix_s {
     mtl4g::DevBuf<uint16_t> valbf;    // BF16 copy (created on demand)
     mtl4g::DevBuf<double> diag;       // A[i][i] (0 where absent)
     mtl4g::DevBuf<mtl4g::RowBlock> desc;   // row-block partition (LDS tile kernel)
-    mtl4g::DevBuf<int16_t> col16;          // col - first row of the entry's row block
-    bool idx16 = false;                    // col16 holds every entry (stream kernel reads it)
     int nb = 0;                            // number of row blocks
     int kernel = MTL4G_SPMV_VECTOR;        // CSR kernel: vector groups or LDS tiles
     int gs = 8;                            // vector kernel: lanes per row

@@ -117,12 +117,10 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
 // current tile's gathers, products, row sums and epilogue.  Products go to
 // one of two LDS buffers in turn, so one barrier per tile suffices.  Rows
 // are summed in column order, as in k_csr; a longer-than-TILE single row is
-// summed by the whole workgroup (not overlapped).  IT = int16_t: column
-// indices stored as offsets from the row block's first row (when every
-// entry of the matrix fits), 2 bytes per non-zero instead of 4.
-template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi, typename IT = int32_t>
+// summed by the whole workgroup (not overlapped).
+template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
 __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
-                                             const IT* __restrict__ col,
+                                             const int32_t* __restrict__ col,
                                              const MT* __restrict__ val,
                                              const XT* __restrict__ x,
                                              const RowBlock* __restrict__ desc, int nb, Epi epi,
@@ -157,11 +155,10 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
             pre = epi.preload(row);
         }
         if (nz <= TILE) {
-            const int base = sizeof(IT) == 2 ? d.r0 : 0;
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int k = t + u * BS;
-                c[u] = k < nz ? base + (int)__builtin_nontemporal_load(col + d.j0 + k) : 0;
+                c[u] = k < nz ? __builtin_nontemporal_load(col + d.j0 + k) : 0;
                 v[u] = k < nz ? load_val_nt<AT>(val, (int64_t)d.j0 + k) : AT(0);
             }
         }
@@ -178,10 +175,9 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
         const typename Epi::Pre pc = pre;
         const int bn = b + slots;
         if (nz > TILE) {
-            const int base = sizeof(IT) == 2 ? dc.r0 : 0;
             AT s = AT(0);
             for (int k = t; k < nz; k += BS)
-                s += load_val<AT>(val, (int64_t)dc.j0 + k) * static_cast<AT>(x[base + (int)col[dc.j0 + k]]);
+                s += load_val<AT>(val, (int64_t)dc.j0 + k) * static_cast<AT>(x[col[dc.j0 + k]]);
             s = block_sum<BS>(s, sh);
             if (bn < hi) fetch(bn);
             if (t == 0) epi(row, s, pc, part);
@@ -645,21 +641,18 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
         const int tile = A->nnz_per_block;
         // a persistent grid must be resident at once: never more workgroups
         // per CU than this instantiation's registers and LDS allow
-#define M4_LAUNCHP(KERNEL, COL)                                                                \
+#define M4_LAUNCHP(KERNEL)                                                                     \
     do {                                                                                       \
         const int g = persistent_grid<KERNEL>(A->cus, pcu, A->nb);                            \
-        hipLaunchKernelGGL((KERNEL), dim3(g), dim3(256), 0, s, A->rowptr.p, (COL), val, x,      \
+        hipLaunchKernelGGL((KERNEL), dim3(g), dim3(256), 0, s, A->rowptr.p, A->col.p, val, x,   \
                            A->desc.p, A->nb, epi, partials, ticket);                          \
     } while (0)
         if (A->kernel == MTL4G_SPMV_STREAM2) {
-            if (tile <= 1024) M4_LAUNCHP((k_csrp2<256, 1024, 2, AT, MT, XT, Epi>), A->col.p);
-            else M4_LAUNCHP((k_csrp2<256, 2048, 2, AT, MT, XT, Epi>), A->col.p);
-        } else if (A->idx16) {
-            if (tile <= 1024) M4_LAUNCHP((k_csrp<256, 1024, AT, MT, XT, Epi, int16_t>), A->col16.p);
-            else M4_LAUNCHP((k_csrp<256, 2048, AT, MT, XT, Epi, int16_t>), A->col16.p);
+            if (tile <= 1024) M4_LAUNCHP((k_csrp2<256, 1024, 2, AT, MT, XT, Epi>));
+            else M4_LAUNCHP((k_csrp2<256, 2048, 2, AT, MT, XT, Epi>));
         } else {
-            if (tile <= 1024) M4_LAUNCHP((k_csrp<256, 1024, AT, MT, XT, Epi>), A->col.p);
-            else M4_LAUNCHP((k_csrp<256, 2048, AT, MT, XT, Epi>), A->col.p);
+            if (tile <= 1024) M4_LAUNCHP((k_csrp<256, 1024, AT, MT, XT, Epi>));
+            else M4_LAUNCHP((k_csrp<256, 2048, AT, MT, XT, Epi>));
         }
 #undef M4_LAUNCHP
         M4_HIP(hipGetLastError());

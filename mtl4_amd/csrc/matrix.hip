@@ -78,28 +78,6 @@ int build_row_blocks(mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block) {
         desc[b] = RowBlock{blk[b], blk[b + 1], rp[blk[b]], rp[blk[b + 1]]};
     M4_TRY(A->desc.alloc(desc.size()));
     M4_HIP(hipMemcpy(A->desc.p, desc.data(), desc.size() * sizeof(RowBlock), hipMemcpyHostToDevice));
-    // 16-bit column offsets from each block's first row, when all fit
-    A->idx16 = false;
-    if (A->nnz > 0) {
-        std::vector<int16_t> c16(A->nnz);
-        bool fits = true;
-        for (int b = 0; b < A->nb && fits; b++) {
-            const int32_t r0 = blk[b];
-            for (int32_t j = rp[blk[b]]; j < rp[blk[b + 1]]; j++) {
-                const int32_t dlt = A->h_col[j] - r0;
-                if (dlt < -32768 || dlt > 32767) {
-                    fits = false;
-                    break;
-                }
-                c16[j] = (int16_t)dlt;
-            }
-        }
-        if (fits) {
-            M4_TRY(A->col16.alloc(c16.size()));
-            M4_HIP(hipMemcpy(A->col16.p, c16.data(), c16.size() * sizeof(int16_t), hipMemcpyHostToDevice));
-            A->idx16 = true;
-        }
-    }
     return MTL4G_OK;
 }
 
@@ -574,12 +552,6 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
                  : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
                                                   : 0;
     if (rows_per_block) *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR ? A->vrows : A->rows_per_block;
-    return MTL4G_OK;
-}
-
-int mtl4g_csr_index_bytes(mtl4g_matrix A, int* bytes) {
-    if (!A || !bytes) return MTL4G_ERR_INVALID;
-    *bytes = A->idx16 && A->kernel == MTL4G_SPMV_STREAM ? 2 : 4;
     return MTL4G_OK;
 }
 

@@ -113,21 +113,6 @@ def test_spmv_stream_bitwise(m, fam, shape, depth):
         assert np.linalg.norm(sols[0][1] - sols[1][1]) <= 1e-9 * np.linalg.norm(sols[1][1])
 
 
-def test_spmv_stream_index_width(m):
-    """16-bit column offsets only when every entry fits; both widths give
-    MTL4's bits."""
-    from mtl4_amd import generators as gen
-    for (rp, ci, va), width in ((gen.laplacian3d(20, 20, 20), 2), (gen.random_sparse(40000, seed=3), 4)):
-        O, A = pair(rp, ci, va)
-        A.tune(256, 2048)
-        A.set_kernel(m._lib.SPMV_STREAM, 4, 0)
-        assert A.index_bytes() == width
-        x = np.random.default_rng(5).uniform(-1, 1, A.num_cols())
-        assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
-        A.tune(256, 2048)                      # the tile kernel always reads 4-byte indices
-        assert A.index_bytes() == 4
-
-
 @pytest.mark.parametrize("uv", [4, 8])
 @pytest.mark.parametrize("fam", ["lap2d_5", "lap2d_64", "random_3000", "long_rows", "diff3d_16"])
 def test_spmv_scalar_bitwise(m, fam, uv):
