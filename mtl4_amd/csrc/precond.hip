@@ -2,13 +2,13 @@
 //
 // Factorizations run on the host in FP64 in MTL4's loop order (so the
 // stored factors are the ones MTL4 computes); the triangular sweeps run on
-// the GPU, level scheduled.  A sweep is one kernel: workgroups take tickets
-// in level order, each owns up to kSweepBS rows of one dependency level;
-// a row waits only for the rows it reads (their values replace a sentinel;
-// bounded wait, abort flag), solves with MTL4's per-row subtraction order
-// (FP64 results equal MTL4's bit for bit) and publishes its value.  The
-// last workgroup to finish resets the counters, so a sweep needs no memset
-// between launches.
+// the GPU, level scheduled.  A sweep is one persistent kernel: workgroups
+// take tickets in order, each owns up to kSweepBS rows of one dependency
+// level, waits until every workgroup of the previous level has published
+// (bounded wait, abort flag), solves its rows with MTL4's per-row
+// subtraction order (FP64 results equal MTL4's bit for bit) and publishes.
+// The last workgroup of the last level resets the counters, so a sweep
+// needs no memset between launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,98 +32,75 @@ struct SweepView {
     int unit;
 };
 
-// A row's value doubles as its ready flag: the output vector is filled with
-// a sentinel (a signalling-NaN bit pattern that no arithmetic produces)
-// before the sweep, and a row waits until each row it reads holds something
-// else.  Workgroups take tickets in level order, so every row a workgroup
-// waits for belongs to a workgroup that is already running: waits end.
-template <typename VT>
-struct Sentinel;
-template <>
-struct Sentinel<double> {
-    __device__ static bool is(double v) { return __double_as_longlong(v) == (long long)0xFFF73E1A55C0FFEEull; }
-    __device__ static double value() { return __longlong_as_double((long long)0xFFF73E1A55C0FFEEull); }
-};
-template <>
-struct Sentinel<float> {
-    __device__ static bool is(float v) { return __float_as_uint(v) == 0xFFB3E1A5u; }
-    __device__ static float value() { return __uint_as_float(0xFFB3E1A5u); }
-};
-
-template <typename VT>
-__global__ void k_fill_sentinel(int64_t n, VT* w, const int* skip) {
-    if (skip && *skip) return;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        w[i] = Sentinel<VT>::value();
-}
-
 template <int BS, typename MT, typename VT>
 __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __restrict__ v,
                                               VT* w, unsigned* counters, int* err,
                                               const int* skip) {
     if (skip && *skip) return;
-    __shared__ int s_b, s_final;
-    unsigned* abort_flag = sweep_counter(counters, 1);
+    __shared__ int s_b, s_ok, s_final;
     if (threadIdx.x == 0)
         s_b = (int)__hip_atomic_fetch_add(sweep_counter(counters, 0), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int b = s_b;
     const int L = S.blk_level[b];
-    const int k = S.lev_ptr[L] + (b - S.lev_blk[L]) * BS + (int)threadIdx.x;
-    if (k < S.lev_ptr[L + 1]) {
-        const int r = S.perm[k];
-        VT rr = v[r];
-        bool ok = true;
-        // eight entries at a time: indices and values, then the eight
-        // solution values they name (re-read until none is the sentinel),
-        // then the subtractions in the row's column order
-        const int a = S.rowptr[k], e = S.rowptr[k + 1];
-        for (int j0 = a; j0 < e; j0 += 8) {
-            int c[8];
-            VT m[8], wv[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const bool in = j0 + u < e;
-                c[u] = in ? S.col[j0 + u] : 0;
-                m[u] = in ? static_cast<VT>(S.val[j0 + u]) : VT(0);
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        if (L > 0) {
+            const unsigned need = (unsigned)(S.lev_blk[L] - S.lev_blk[L - 1]);
+            unsigned it = 0;
+            const unsigned* lev = sweep_counter(counters, 2 + L - 1);
+            while (ld_agent(lev) < need) {
+                if ((++it & 63u) == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) { ok = 0; break; }
+                if (it > kSpinLimit) { st_agent(sweep_counter(counters, 1), 1u); ok = 0; break; }
+                __builtin_amdgcn_s_sleep(2);
             }
-            for (unsigned it = 0;;) {
-                bool pending = false;
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    // Values of earlier levels are read with agent-scope loads and results
+    // are published with agent-scope atomics, so no L2 write-back fences are
+    // needed between workgroups of this kernel.
+    if (s_ok) {
+        const int k = S.lev_ptr[L] + (b - S.lev_blk[L]) * BS + (int)threadIdx.x;
+        if (k < S.lev_ptr[L + 1]) {
+            const int r = S.perm[k];
+            VT rr = v[r];
+            // eight entries at a time: indices and values first, then the
+            // eight solution values they name (one round trip each), then
+            // the subtractions in the row's column order
+            const int a = S.rowptr[k], e = S.rowptr[k + 1];
+            for (int j0 = a; j0 < e; j0 += 8) {
+                int c[8];
+                VT m[8], wv[8];
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
-                    wv[u] = j0 + u < e ? ld_agent(w + c[u]) : VT(0);
-                    pending |= Sentinel<VT>::is(wv[u]);
+                    const bool in = j0 + u < e;
+                    c[u] = in ? S.col[j0 + u] : 0;
+                    m[u] = in ? static_cast<VT>(S.val[j0 + u]) : VT(0);
                 }
-                if (!pending || !ok) break;
-                ++it;
-                if ((it & 63u) == 0 && ld_agent(abort_flag) != 0u) ok = false;
-                if (it > kSpinLimit) {
-                    st_agent(abort_flag, 1u);
-                    ok = false;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (j0 + u < e) rr -= m[u] * wv[u];
+                for (int u = 0; u < 8; u++) wv[u] = j0 + u < e ? ld_agent(w + c[u]) : VT(0);
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (j0 + u < e) rr -= m[u] * wv[u];
+            }
+            if (!S.unit) rr = rr * static_cast<VT>(S.dia[k]);
+            st_agent(w + r, rr);   // the L1 writes through; wait_memory() below waits for it
         }
-        if (!S.unit) rr = rr * static_cast<VT>(S.dia[k]);
-        if (Sentinel<VT>::is(rr)) rr = VT(NAN);   // never publish the flag pattern itself
-        st_agent(w + r, rr);   // after an abort too: rows waiting on this one stop waiting
     }
     wait_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
+        const unsigned need = (unsigned)(S.lev_blk[L + 1] - S.lev_blk[L]);
         const unsigned done =
-            __hip_atomic_fetch_add(sweep_counter(counters, 2), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        s_final = done == gridDim.x;
+            __hip_atomic_fetch_add(sweep_counter(counters, 2 + L), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        s_final = (L == S.nlev - 1 && done == need);
     }
     __syncthreads();
-    if (s_final && threadIdx.x == 0) {   // every workgroup has finished: reset for the next launch
-        if (ld_agent(abort_flag) != 0u) atomicOr(err, 1);
-        st_agent(sweep_counter(counters, 0), 0u);
-        st_agent(sweep_counter(counters, 2), 0u);
-        st_agent(abort_flag, 0u);
+    if (s_final) {   // every other workgroup has published: reset for the next launch
+        if (threadIdx.x == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) atomicOr(err, 1);
+        for (int i = threadIdx.x; i < S.nlev + 2; i += BS) st_agent(sweep_counter(counters, i), 0u);
     }
 }
 
@@ -236,11 +213,6 @@ int launch_sweep(const Sweep& S, int fmt, const VT* v, VT* w, const int* skip, i
                  unsigned* counters, hipStream_t s) {
     if (S.n == 0) return MTL4G_OK;
     if (!counters) counters = S.counters.p;
-    {
-        const int64_t g = std::min<int64_t>((S.n + 1023) / 1024, 2048);
-        hipLaunchKernelGGL((k_fill_sentinel<VT>), dim3((unsigned)std::max<int64_t>(g, 1)), dim3(256), 0, s,
-                           (int64_t)S.n, w, skip);
-    }
     if (fmt == MTL4G_F64) {
         SweepView<double, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
                                 S.blk_level.p, S.val64.p, S.dia64.p, S.nlev, S.unit ? 1 : 0};
