@@ -280,6 +280,9 @@ struct mtl4g_matrix_s {
     int uv = 8;                            // scalar kernel: values requested per lane-step
     int per_cu_lowp[2] = {0, 0};           // the same for FP32 / BF16 values (0: as FP64)
     bool lowp_tuned[2] = {false, false};
+    bool lowp_wide[2] = {false, false};    // FP32 / BF16 stream kernel on the wide partition
+    mtl4g::DevBuf<mtl4g::RowBlock> desc_wide;   // blocks of <= 512 rows / 4096 non-zeros
+    int nb_wide = 0;
     int32_t max_row = 0;
     std::vector<int32_t> h_rowptr, h_col;   // host pattern (factorizations)
     std::vector<double> h_val;

@@ -117,8 +117,30 @@ __global__ __launch_bounds__(BS) void k_csr(const int32_t* __restrict__ rowptr,
 // current tile's gathers, products, row sums and epilogue.  Products go to
 // one of two LDS buffers in turn, so one barrier per tile suffices.  Rows
 // are summed in column order, as in k_csr; a longer-than-TILE single row is
-// summed by the whole workgroup (not overlapped).
-template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi>
+// summed by the whole workgroup (not overlapped).  RPL rows per lane: lane t
+// sums rows t, t + BS, ... of the block (blocks of up to RPL * BS rows).
+// Two-element loads of the value/index streams (FP64 and FP32 values): lane
+// t requests elements (base + 2p, base + 2p + 1) of pairs p = t + u*BS, with
+// base = j0 rounded down to even, so every pair load is aligned; elements
+// outside [j0, j1) are loaded (the arrays carry two elements of padding) and
+// ignored.  A block then holds at most TILE - 2 non-zeros.
+template <typename T>
+struct pair_of;
+template <>
+struct pair_of<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct pair_of<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct pair_of<uint16_t> {
+    typedef uint16_t type __attribute__((ext_vector_type(2)));
+};
+typedef int int_pair __attribute__((ext_vector_type(2)));
+
+template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi, int RPL = 1>
 __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
                                              const int32_t* __restrict__ col,
                                              const MT* __restrict__ val,
@@ -128,6 +150,8 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
     if (epi.skip()) return;
     epi.prepare();
     constexpr int U = TILE / BS;
+    constexpr bool PAIR = sizeof(MT) >= 4 && U % 2 == 0;
+    constexpr int LONG = PAIR ? TILE - 2 : TILE;   // blocks above this are single long rows
     constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
     __shared__ AT prod[2][TILE];
     __shared__ AT sh[BS / kWave];
@@ -143,18 +167,38 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
     RowBlock d{};
     int c[U];
     AT v[U];
-    int ra = 0, re = 0;
-    typename Epi::Pre pre{};
+    int ra[RPL] = {}, re[RPL] = {};
+    typename Epi::Pre pre[RPL]{};
     auto fetch = [&](int bb) {
         d = desc[bb];
-        const int row = d.r0 + t;
         const int nz = d.j1 - d.j0;
-        if (row < d.r1) {
-            ra = rowptr[row] - d.j0;
-            re = rowptr[row + 1] - d.j0;
-            pre = epi.preload(row);
+#pragma unroll
+        for (int q = 0; q < RPL; q++) {
+            const int row = d.r0 + t + q * BS;
+            if (row < d.r1) {
+                ra[q] = rowptr[row] - d.j0;
+                re[q] = rowptr[row + 1] - d.j0;
+                pre[q] = epi.preload(row);
+            } else {
+                ra[q] = re[q] = 0;
+            }
         }
-        if (nz <= TILE) {
+        if constexpr (PAIR) {
+            if (nz <= LONG) {
+                const int base = d.j0 & ~1;
+#pragma unroll
+                for (int u = 0; u < U / 2; u++) {
+                    const int e0 = min(base + 2 * (t + u * BS), d.j1 & ~1);   // in bounds; unused beyond j1
+                    const int_pair cc = __builtin_nontemporal_load(reinterpret_cast<const int_pair*>(col + e0));
+                    const typename pair_of<MT>::type vv =
+                        __builtin_nontemporal_load(reinterpret_cast<const typename pair_of<MT>::type*>(val + e0));
+                    c[2 * u] = cc.x;
+                    c[2 * u + 1] = cc.y;
+                    v[2 * u] = static_cast<AT>(vv.x);
+                    v[2 * u + 1] = static_cast<AT>(vv.y);
+                }
+            }
+        } else if (nz <= TILE) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int k = t + u * BS;
@@ -171,34 +215,49 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
         const RowBlock dc = d;
         const int nz = dc.j1 - dc.j0;
         const int row = dc.r0 + t;
-        const int a = ra, e = re;
-        const typename Epi::Pre pc = pre;
+        int a[RPL], e[RPL];
+        typename Epi::Pre pc[RPL];
+#pragma unroll
+        for (int q = 0; q < RPL; q++) {
+            a[q] = ra[q];
+            e[q] = re[q];
+            pc[q] = pre[q];
+        }
         const int bn = b + slots;
-        if (nz > TILE) {
+        if (nz > LONG) {
             AT s = AT(0);
             for (int k = t; k < nz; k += BS)
                 s += load_val<AT>(val, (int64_t)dc.j0 + k) * static_cast<AT>(x[col[dc.j0 + k]]);
             s = block_sum<BS>(s, sh);
             if (bn < hi) fetch(bn);
-            if (t == 0) epi(row, s, pc, part);
+            if (t == 0) epi(row, s, pc[0], part);
         } else {
+            // tile-relative index of the element lane t holds in slot u
+            const int pbase = (dc.j0 & ~1) - dc.j0;
+            auto slot_k = [&](int u) {
+                if constexpr (PAIR) return pbase + 2 * (t + (u / 2) * BS) + (u & 1);
+                else return t + u * BS;
+            };
             AT g[U], vc[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int k = t + u * BS;
-                g[u] = k < nz ? static_cast<AT>(x[c[u]]) : AT(0);
+                const int k = slot_k(u);
+                g[u] = (k >= 0 && k < nz) ? static_cast<AT>(x[c[u]]) : AT(0);
                 vc[u] = v[u];
             }
             if (bn < hi) fetch(bn);
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int k = t + u * BS;
-                if (k < nz) prod[buf][k] = vc[u] * g[u];
+                const int k = slot_k(u);
+                if (k >= 0 && k < nz) prod[buf][k] = vc[u] * g[u];
             }
             __syncthreads();
-            AT s = AT(0);
-            for (int k = a; k < e; k++) s += prod[buf][k];
-            if (row < dc.r1) epi(row, s, pc, part);
+#pragma unroll
+            for (int q = 0; q < RPL; q++) {
+                AT s = AT(0);
+                for (int k = a[q]; k < e[q]; k++) s += prod[buf][k];
+                if (row + q * BS < dc.r1) epi(row + q * BS, s, pc[q], part);
+            }
             buf ^= 1;
         }
         b = bn;
@@ -636,8 +695,19 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
     if (A->nb == 0) return MTL4G_OK;
     if (A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2) {
         int pcu = A->per_cu;
-        if (sizeof(MT) == 4 && A->per_cu_lowp[0] > 0) pcu = A->per_cu_lowp[0];
-        if (sizeof(MT) == 2 && A->per_cu_lowp[1] > 0) pcu = A->per_cu_lowp[1];
+        const int li = sizeof(MT) == 4 ? 0 : 1;   // FP32 / BF16 values
+        if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) pcu = A->per_cu_lowp[li];
+        if constexpr (sizeof(MT) < 8) {
+            // low-precision values: larger blocks, two rows per lane
+            if (A->kernel == MTL4G_SPMV_STREAM && A->lowp_wide[li] && A->nb_wide > 0) {
+                const int g = persistent_grid<k_csrp<256, 4096, AT, MT, XT, Epi, 2>>(A->cus, pcu, A->nb_wide);
+                hipLaunchKernelGGL((k_csrp<256, 4096, AT, MT, XT, Epi, 2>), dim3(g), dim3(256), 0, s,
+                                   A->rowptr.p, A->col.p, val, x, A->desc_wide.p, A->nb_wide, epi, partials,
+                                   ticket);
+                M4_HIP(hipGetLastError());
+                return MTL4G_OK;
+            }
+        }
         const int tile = A->nnz_per_block;
         // a persistent grid must be resident at once: never more workgroups
         // per CU than this instantiation's registers and LDS allow

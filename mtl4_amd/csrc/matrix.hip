@@ -48,6 +48,26 @@ int blocks_for(int64_t n) {
 }
 
 // ------------------------------------------------------------------ partition
+// Row blocks of at most rows_per_block rows and nnz_per_block non-zeros (a
+// longer row gets a block of its own), in row order.
+static std::vector<RowBlock> partition_rows(const mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block) {
+    const std::vector<int32_t>& rp = A->h_rowptr;
+    std::vector<RowBlock> out;
+    if (nnz_per_block >= 64) nnz_per_block -= 2;   // headroom for aligned pair loads
+    int32_t r = 0;
+    while (r < A->n) {
+        const int32_t r0 = r;
+        int64_t nz = 0;
+        while (r < A->n && r - r0 < rows_per_block && nz + (rp[r + 1] - rp[r]) <= nnz_per_block) {
+            nz += rp[r + 1] - rp[r];
+            r++;
+        }
+        if (r == r0) r++;
+        out.push_back(RowBlock{r0, r, rp[r0], rp[r]});
+    }
+    return out;
+}
+
 int build_row_blocks(mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block) {
     if (rows_per_block <= 0) rows_per_block = 256;
     if (nnz_per_block <= 0) nnz_per_block = 2048;
@@ -58,12 +78,14 @@ int build_row_blocks(mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block) {
     std::vector<int32_t> blk;
     blk.reserve(A->n / 64 + 2);
     const std::vector<int32_t>& rp = A->h_rowptr;
+    // two non-zeros of headroom for the stream kernel's aligned pair loads
+    const int cap = nnz_per_block >= 64 ? nnz_per_block - 2 : nnz_per_block;
     int32_t r = 0;
     blk.push_back(0);
     while (r < A->n) {
         int32_t r0 = r;
         int64_t nz = 0;
-        while (r < A->n && r - r0 < rows_per_block && nz + (rp[r + 1] - rp[r]) <= nnz_per_block) {
+        while (r < A->n && r - r0 < rows_per_block && nz + (rp[r + 1] - rp[r]) <= cap) {
             nz += rp[r + 1] - rp[r];
             r++;
         }
@@ -94,13 +116,13 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s);
 int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     if (fmt == MTL4G_F32 && !A->val32.p) {
         std::vector<float> v(A->h_val.begin(), A->h_val.end());
-        M4_TRY(A->val32.alloc(v.size()));
+        M4_TRY(A->val32.alloc(v.size() + 2));   // padding for aligned pair loads
         M4_HIP(hipMemcpy(A->val32.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
     }
     if (fmt == MTL4G_BF16 && !A->valbf.p) {
         std::vector<uint16_t> v(A->h_val.size());
         for (size_t i = 0; i < v.size(); i++) v[i] = f32_to_bf16((float)A->h_val[i]);
-        M4_TRY(A->valbf.alloc(v.size()));
+        M4_TRY(A->valbf.alloc(v.size() + 2));
         M4_HIP(hipMemcpy(A->valbf.p, v.data(), v.size() * 2, hipMemcpyHostToDevice));
     }
     if ((fmt == MTL4G_F32 || fmt == MTL4G_BF16) && !A->lowp_tuned[fmt - 1]) M4_TRY(tune_lowp(A, fmt, s));
@@ -169,8 +191,16 @@ static int spmv32_mode(const mtl4g_matrix_s* A, int fmt, const float* x, float* 
 static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     A->lowp_tuned[fmt - 1] = true;
     A->per_cu_lowp[fmt - 1] = 0;
+    A->lowp_wide[fmt - 1] = false;
     if ((A->kernel != MTL4G_SPMV_STREAM && A->kernel != MTL4G_SPMV_STREAM2) || A->n == 0 || A->m == 0)
         return MTL4G_OK;
+    if (A->kernel == MTL4G_SPMV_STREAM && !A->desc_wide.p) {   // wide partition for the candidates below
+        std::vector<RowBlock> w = partition_rows(A, 512, 4096);
+        M4_TRY(A->desc_wide.alloc(std::max<size_t>(w.size(), 1)));
+        if (!w.empty())
+            M4_HIP(hipMemcpy(A->desc_wide.p, w.data(), w.size() * sizeof(RowBlock), hipMemcpyHostToDevice));
+        A->nb_wide = (int)w.size();
+    }
     float *x = nullptr, *y = nullptr;
     M4_HIP(hipMallocAsync((void**)&x, (size_t)A->m * 4, s));
     M4_HIP(hipMallocAsync((void**)&y, (size_t)A->n * 4, s));
@@ -180,8 +210,19 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     M4_HIP(hipEventCreate(&e1));
     double best = 1e30;
     int win = 0;
-    for (int f = 2; f <= 6; f++) {
+    bool win_wide = false;
+    struct Cand {
+        int f;
+        bool wide;
+    };
+    std::vector<Cand> cands;
+    for (int f = 2; f <= 6; f++) cands.push_back({f, false});
+    if (A->kernel == MTL4G_SPMV_STREAM && A->nb_wide > 0)
+        for (int f = 2; f <= 4; f++) cands.push_back({f, true});
+    for (const Cand& cd : cands) {
+        const int f = cd.f;
         A->per_cu_lowp[fmt - 1] = f;
+        A->lowp_wide[fmt - 1] = cd.wide;
         int st = launch_spmv_f32(A, fmt, x, y, MTL4G_ASSIGN, s);
         if (!st) st = launch_spmv_f32(A, fmt, x, y, MTL4G_ASSIGN, s);
         if (st) continue;
@@ -195,6 +236,7 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
         if (ms < best) {
             best = ms;
             win = f;
+            win_wide = cd.wide;
         }
     }
     (void)hipEventDestroy(e0);
@@ -202,6 +244,7 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     (void)hipFreeAsync(x, s);
     (void)hipFreeAsync(y, s);
     A->per_cu_lowp[fmt - 1] = win;
+    A->lowp_wide[fmt - 1] = win_wide;
     return MTL4G_OK;
 }
 
@@ -445,8 +488,9 @@ int mtl4g_csr_create(mtl4g_matrix* out, int32_t nrows, int32_t ncols, int64_t nn
         }
     }
     int st;
-    if ((st = A->rowptr.alloc((size_t)nrows + 1)) || (st = A->col.alloc((size_t)nnz)) ||
-        (st = A->val.alloc((size_t)nnz)) || (st = A->diag.alloc((size_t)nrows)))
+    // col and val carry two elements of padding for the aligned pair loads
+    if ((st = A->rowptr.alloc((size_t)nrows + 1)) || (st = A->col.alloc((size_t)nnz + 2)) ||
+        (st = A->val.alloc((size_t)nnz + 2)) || (st = A->diag.alloc((size_t)nrows)))
         return fail(st);
     hipError_t e = hipMemcpy(A->rowptr.p, A->h_rowptr.data(), ((size_t)nrows + 1) * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && nnz) e = hipMemcpy(A->col.p, A->h_col.data(), (size_t)nnz * 4, hipMemcpyHostToDevice);
@@ -500,6 +544,7 @@ int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block
     A->kernel = MTL4G_SPMV_TILE;
     A->lowp_tuned[0] = A->lowp_tuned[1] = false;
     A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
+    A->lowp_wide[0] = A->lowp_wide[1] = false;
     return MTL4G_OK;
 }
 
@@ -517,6 +562,7 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
         A->per_cu = group;
         A->lowp_tuned[0] = A->lowp_tuned[1] = false;
         A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
+        A->lowp_wide[0] = A->lowp_wide[1] = false;
         return MTL4G_OK;
     }
     if (kernel == MTL4G_SPMV_SCALAR) {
