@@ -149,13 +149,19 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
  * MTL4G_SPMV_STREAM2: the stream kernel with two row blocks requested
  * ahead instead of one;
  * MTL4G_SPMV_SCALAR: one lane per row, `group` (4 or 8) values and
- * indices requested per step, no LDS.
+ * indices requested per step, no LDS;
+ * MTL4G_SPMV_SLICED: a sliced-ELL copy of the matrix (slices of 64 rows,
+ * each padded to its longest row, stored column-major) built on the
+ * device, one lane per row, a persistent grid of `group` workgroups per
+ * compute unit (1..8); refused when the padding would exceed half the
+ * non-zeros.
  * All sum every row in column order.  0 = library default. */
 #define MTL4G_SPMV_TILE   0
 #define MTL4G_SPMV_VECTOR 1
 #define MTL4G_SPMV_STREAM 2
 #define MTL4G_SPMV_SCALAR 3
 #define MTL4G_SPMV_STREAM2 4
+#define MTL4G_SPMV_SLICED 5
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);

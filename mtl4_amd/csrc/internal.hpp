@@ -283,6 +283,14 @@ struct mtl4g_matrix_s {
     bool lowp_wide[2] = {false, false};    // FP32 / BF16 stream kernel on the wide partition
     mtl4g::DevBuf<mtl4g::RowBlock> desc_wide;   // blocks of <= 512 rows / 4096 non-zeros
     int nb_wide = 0;
+    // sliced ELL (MTL4G_SPMV_SLICED): slices of 64 rows, column-major, padding column -1
+    std::vector<int64_t> h_sl_off;          // nsl + 1 entry offsets
+    mtl4g::DevBuf<int64_t> sl_off;
+    mtl4g::DevBuf<int32_t> sl_col;
+    mtl4g::DevBuf<double> sl_val;
+    mtl4g::DevBuf<float> sl_val32;
+    mtl4g::DevBuf<uint16_t> sl_valbf;
+    int nsl = 0;
     int32_t max_row = 0;
     std::vector<int32_t> h_rowptr, h_col;   // host pattern (factorizations)
     std::vector<double> h_val;

@@ -470,6 +470,118 @@ __global__ __launch_bounds__(BS) void k_csrv(const int32_t* __restrict__ rowptr,
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, b, (int)gridDim.x, epi, sh);
 }
 
+// Sliced ELL: rows in slices of 64 (one wavefront); slice s stores w_s
+// entries per row (its longest row), column-major: entry k of row 64s + l
+// at soff[s] + 64k + l, padding marked by column -1 and skipped.  A lane
+// sums its own row in column order, as every CSR kernel does, so the result
+// is the same bit for bit; no LDS, no barrier, no row pointers.  Each
+// wavefront walks its slices alone, in chunks of UNR entries, and requests
+// the next chunk's values and indices before it waits for the current
+// chunk's gathers of x.  XCD g % 8 owns a contiguous eighth of the slices
+// and its wavefronts take them in turn, so they share one window of x in L2.
+// Slice bookkeeping is wavefront-uniform (scalar registers and loads).  The
+// vector loads are unconditional -- entries beyond a slice's width reload
+// its last column, a chunk past the end of the array reads the padding the
+// arrays carry (UNR * 64 entries) -- so the wait for the gathers never
+// waits for the next chunk; two register sets alternate between the
+// current and the next chunk.
+constexpr int kSellUnroll = 8;                     // entries per lane per chunk
+constexpr int kSellPad = kSellUnroll * kWave;      // padding entries after the arrays
+
+template <int BS, int UNR, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_sell(const int64_t* __restrict__ soff,
+                                             const int32_t* __restrict__ col,
+                                             const MT* __restrict__ val,
+                                             const XT* __restrict__ x, int nrows, int nsl,
+                                             Epi epi, AT* partials, unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    constexpr int WPB = BS / kWave;
+    __shared__ AT sh[WPB];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int xcd = blockIdx.x % kXcd;
+    const int wstep = (int)(gridDim.x / kXcd) * WPB;   // wavefronts per XCD
+    const int per = (nsl + kXcd - 1) / kXcd;
+    const int hi = min(nsl, (xcd + 1) * per);
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+
+    struct Chunk {
+        int c[UNR];
+        AT v[UNR];
+        typename Epi::Pre pre;
+    };
+    // state of the chunk last requested: slice s, entries k0.. of width w at o
+    int s = xcd * per + (int)(blockIdx.x / kXcd) * WPB + wv;
+    int64_t o = 0;
+    int w = 0, k0 = 0;
+    auto fetch = [&](Chunk& ch) {
+        ch.pre = epi.preload(min(s * kWave + lane, nrows - 1));
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int slot = max(min(k0 + u, w - 1), 0);
+            const int64_t e = o + (int64_t)slot * kWave + lane;
+            ch.c[u] = __builtin_nontemporal_load(col + e);
+            ch.v[u] = load_val_nt<AT>(val, e);
+        }
+    };
+    auto slice = [&]() {
+        const int64_t a = soff[s], b = soff[s + 1];
+        o = ((int64_t)__builtin_amdgcn_readfirstlane((int)(a >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)a);
+        w = __builtin_amdgcn_readfirstlane((int)((b - a) / kWave));
+        k0 = 0;
+    };
+    AT acc = AT(0);
+    // one chunk: gathers of cur, then (unless this was the last) the request
+    // of the next chunk into nxt, then the products; false at the end
+    auto step = [&](const Chunk& cur, Chunk& nxt) -> bool {
+        AT g[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) g[u] = static_cast<AT>(x[max(cur.c[u], 0)]);
+        const int kc = k0, wc = w, row = s * kWave + lane;
+        // wavefront-uniform by construction (scalar branches)
+        const bool last = __builtin_amdgcn_readfirstlane(kc + UNR >= wc) != 0;
+        const int sn = __builtin_amdgcn_readfirstlane(last ? s + wstep : s);
+        auto sum = [&]() {
+#pragma unroll
+            for (int u = 0; u < UNR; u++)
+                if (kc + u < wc && cur.c[u] >= 0) acc += cur.v[u] * g[u];
+            if (last) {
+                if (row < nrows) epi(row, acc, cur.pre, part);
+                acc = AT(0);
+            }
+        };
+        if (__builtin_amdgcn_readfirstlane(sn >= hi)) {
+            sum();
+            return false;
+        }
+        s = sn;
+        if (last) slice();
+        else k0 = kc + UNR;
+        fetch(nxt);
+        // the products stay behind the requests of the next chunk
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) asm volatile("" : "+v"(g[u])::"memory");
+        sum();
+        return true;
+    };
+    if (s < hi) {
+        Chunk A, B;
+        slice();
+        fetch(A);
+        for (;;) {
+            if (!step(A, B)) break;
+            if (!step(B, A)) break;
+        }
+    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
+}
+
 // Element epilogues may split operator()(i, part) into In load(i) (all
 // reads of element i) and store(i, const In&, part) (the arithmetic, writes
 // and partial sums); k_vec then batches the loads of several elements.
@@ -689,6 +801,26 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
         default: M4_LAUNCHV(64); break;
         }
 #undef M4_LAUNCHV
+        M4_HIP(hipGetLastError());
+        return MTL4G_OK;
+    }
+    if (A->kernel == MTL4G_SPMV_SLICED) {
+        // the sliced copy of the value array the caller passed
+        const MT* sv = nullptr;
+        if constexpr (std::is_same<MT, double>::value) sv = val == A->val.p ? A->sl_val.p : nullptr;
+        else if constexpr (std::is_same<MT, float>::value) sv = val == A->val32.p ? A->sl_val32.p : nullptr;
+        else sv = val == A->valbf.p ? A->sl_valbf.p : nullptr;
+        if (!sv || !A->sl_off.p) {
+            set_error("sliced SpMV: no sliced copy of these values");
+            return MTL4G_ERR_INVALID;
+        }
+        int pcu = A->per_cu;
+        const int li = sizeof(MT) == 4 ? 0 : 1;
+        if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) pcu = A->per_cu_lowp[li];
+        const int units = (A->nsl + 3) / 4;   // four slices per workgroup pass
+        const int g = persistent_grid<k_sell<256, kSellUnroll, AT, MT, XT, Epi>>(A->cus, pcu, units);
+        hipLaunchKernelGGL((k_sell<256, kSellUnroll, AT, MT, XT, Epi>), dim3(g), dim3(256), 0, s, A->sl_off.p, A->sl_col.p,
+                           sv, x, A->n, A->nsl, epi, partials, ticket);
         M4_HIP(hipGetLastError());
         return MTL4G_OK;
     }

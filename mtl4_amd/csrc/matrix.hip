@@ -113,7 +113,97 @@ static uint16_t f32_to_bf16(float f) {
 
 static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s);
 
+// ---- sliced ELL copy of the matrix
+// Entries of the sliced layout: slices of 64 rows, each padded to its
+// longest row.
+static int64_t sliced_entries(const mtl4g_matrix_s* A) {
+    int64_t tot = 0;
+    for (int32_t s0 = 0; s0 < A->n; s0 += kWave) {
+        int32_t w = 0;
+        for (int32_t r = s0; r < std::min(A->n, s0 + kWave); r++) w = std::max(w, A->h_rowptr[r + 1] - A->h_rowptr[r]);
+        tot += (int64_t)w * kWave;
+    }
+    return tot;
+}
+
+// out[position of CSR entry j in the sliced layout] = src[j], pad elsewhere
+template <typename T, typename S, typename F>
+static void sliced_fill(const mtl4g_matrix_s* A, const S* src, T pad, F conv, std::vector<T>& out) {
+    out.assign((size_t)A->h_sl_off.back() + kSellPad, pad);   // the kernel may read the padding
+    for (int32_t s = 0; s < A->nsl; s++)
+        for (int l = 0; l < kWave; l++) {
+            const int32_t r = s * kWave + l;
+            if (r >= A->n) break;
+            for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++)
+                out[(size_t)(A->h_sl_off[s] + (int64_t)(j - A->h_rowptr[r]) * kWave + l)] = conv(src[j]);
+        }
+}
+
+template <typename T>
+static int upload(mtl4g::DevBuf<T>& d, const std::vector<T>& h) {
+    M4_TRY(d.alloc(h.size()));
+    if (!h.empty()) M4_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MTL4G_OK;
+}
+
+static void free_sliced(mtl4g_matrix_s* A) {
+    A->h_sl_off.clear();
+    A->sl_off.free();
+    A->sl_col.free();
+    A->sl_val.free();
+    A->sl_val32.free();
+    A->sl_valbf.free();
+    A->nsl = 0;
+}
+
+static int sliced_lowp_values(mtl4g_matrix_s* A, int fmt) {
+    if (A->h_sl_off.empty()) return MTL4G_OK;
+    if (fmt == MTL4G_F32 && !A->sl_val32.p) {
+        std::vector<float> v;
+        sliced_fill(A, A->h_val.data(), 0.0f, [](double d) { return (float)d; }, v);
+        M4_TRY(upload(A->sl_val32, v));
+    }
+    if (fmt == MTL4G_BF16 && !A->sl_valbf.p) {
+        std::vector<uint16_t> v;
+        sliced_fill(A, A->h_val.data(), (uint16_t)0, [](double d) { return f32_to_bf16((float)d); }, v);
+        M4_TRY(upload(A->sl_valbf, v));
+    }
+    return MTL4G_OK;
+}
+
+// Build the sliced copy (pattern and FP64 values; low-precision values for
+// the formats already in use).  Refused when padding exceeds half the nnz.
+static int build_sliced(mtl4g_matrix_s* A) {
+    if (!A->h_sl_off.empty()) return MTL4G_OK;
+    const int64_t ent = sliced_entries(A);
+    if (ent > A->nnz + A->nnz / 2 + kWave) {
+        set_error("sliced ELL: %lld padded entries for %lld non-zeros", (long long)ent, (long long)A->nnz);
+        return MTL4G_ERR_UNSUPPORTED;
+    }
+    A->nsl = (A->n + kWave - 1) / kWave;
+    A->h_sl_off.assign((size_t)A->nsl + 1, 0);
+    for (int32_t s = 0; s < A->nsl; s++) {
+        int32_t w = 0;
+        for (int32_t r = s * kWave; r < std::min(A->n, (s + 1) * kWave); r++)
+            w = std::max(w, A->h_rowptr[r + 1] - A->h_rowptr[r]);
+        A->h_sl_off[s + 1] = A->h_sl_off[s] + (int64_t)w * kWave;
+    }
+    int st;
+    std::vector<int32_t> c;
+    sliced_fill(A, A->h_col.data(), (int32_t)-1, [](int32_t j) { return j; }, c);
+    std::vector<double> v;
+    sliced_fill(A, A->h_val.data(), 0.0, [](double d) { return d; }, v);
+    if ((st = upload(A->sl_off, A->h_sl_off)) || (st = upload(A->sl_col, c)) || (st = upload(A->sl_val, v)) ||
+        (A->val32.p && (st = sliced_lowp_values(A, MTL4G_F32))) ||
+        (A->valbf.p && (st = sliced_lowp_values(A, MTL4G_BF16)))) {
+        free_sliced(A);
+        return st;
+    }
+    return MTL4G_OK;
+}
+
 int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
+    if (fmt == MTL4G_F32 || fmt == MTL4G_BF16) M4_TRY(sliced_lowp_values(A, fmt));
     if (fmt == MTL4G_F32 && !A->val32.p) {
         std::vector<float> v(A->h_val.begin(), A->h_val.end());
         M4_TRY(A->val32.alloc(v.size() + 2));   // padding for aligned pair loads
@@ -192,7 +282,8 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     A->lowp_tuned[fmt - 1] = true;
     A->per_cu_lowp[fmt - 1] = 0;
     A->lowp_wide[fmt - 1] = false;
-    if ((A->kernel != MTL4G_SPMV_STREAM && A->kernel != MTL4G_SPMV_STREAM2) || A->n == 0 || A->m == 0)
+    if ((A->kernel != MTL4G_SPMV_STREAM && A->kernel != MTL4G_SPMV_STREAM2 && A->kernel != MTL4G_SPMV_SLICED) ||
+        A->n == 0 || A->m == 0)
         return MTL4G_OK;
     if (A->kernel == MTL4G_SPMV_STREAM && !A->desc_wide.p) {   // wide partition for the candidates below
         std::vector<RowBlock> w = partition_rows(A, 512, 4096);
@@ -216,7 +307,10 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
         bool wide;
     };
     std::vector<Cand> cands;
-    for (int f = 2; f <= 6; f++) cands.push_back({f, false});
+    if (A->kernel == MTL4G_SPMV_SLICED)
+        for (int f : {2, 4, 6, 8}) cands.push_back({f, false});
+    else
+        for (int f = 2; f <= 6; f++) cands.push_back({f, false});
     if (A->kernel == MTL4G_SPMV_STREAM && A->nb_wide > 0)
         for (int f = 2; f <= 4; f++) cands.push_back({f, true});
     for (const Cand& cd : cands) {
@@ -535,6 +629,13 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where) {
     if (A->n) M4_HIP(hipMemcpy(A->diag.p, diag.data(), (size_t)A->n * 8, hipMemcpyHostToDevice));
     A->val32.free();
     A->valbf.free();
+    A->sl_val32.free();
+    A->sl_valbf.free();
+    if (!A->h_sl_off.empty()) {
+        std::vector<double> v;
+        sliced_fill(A, A->h_val.data(), 0.0, [](double d) { return d; }, v);
+        M4_TRY(upload(A->sl_val, v));
+    }
     return MTL4G_OK;
 }
 
@@ -558,6 +659,20 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
             return MTL4G_ERR_INVALID;
         }
         if (rows_per_block > 0 || !A->desc.p) M4_TRY(build_row_blocks(A, rows_per_block, 0));
+        A->kernel = kernel;
+        A->per_cu = group;
+        A->lowp_tuned[0] = A->lowp_tuned[1] = false;
+        A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
+        A->lowp_wide[0] = A->lowp_wide[1] = false;
+        return MTL4G_OK;
+    }
+    if (kernel == MTL4G_SPMV_SLICED) {
+        if (group == 0) group = A->per_cu;
+        if (group < 1 || group > 8) {
+            set_error("sliced kernel: 1 to 8 workgroups per compute unit");
+            return MTL4G_ERR_INVALID;
+        }
+        M4_TRY(build_sliced(A));
         A->kernel = kernel;
         A->per_cu = group;
         A->lowp_tuned[0] = A->lowp_tuned[1] = false;
@@ -594,7 +709,9 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
     if (kernel) *kernel = A->kernel;
     if (group)
         *group = A->kernel == MTL4G_SPMV_VECTOR   ? A->gs
-                 : A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2 ? A->per_cu
+                 : A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2 ||
+                           A->kernel == MTL4G_SPMV_SLICED
+                     ? A->per_cu
                  : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
                                                   : 0;
     if (rows_per_block) *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR ? A->vrows : A->rows_per_block;
@@ -616,6 +733,11 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     for (int f : {2, 3, 4}) cands.push_back({MTL4G_SPMV_STREAM2, f, 0, 256, 2048});
     for (int f : {4, 6}) cands.push_back({MTL4G_SPMV_STREAM2, f, 0, 128, 1024});
     for (int u : {4, 8}) cands.push_back({MTL4G_SPMV_SCALAR, u, 0, 0, 0});
+    // sliced ELL where slices of 64 rows pad little (a copy of the matrix)
+    const bool had_sliced = !A->h_sl_off.empty();
+    const bool try_sliced = had_sliced || sliced_entries(A) <= A->nnz + A->nnz / 8 + kWave;
+    if (try_sliced)
+        for (int f : {4, 6, 8}) cands.push_back({MTL4G_SPMV_SLICED, f, 0, 0, 0});
     // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
     const double avg = A->n ? (double)A->nnz / A->n : 0.0;
     for (int g : {4, 8, 16, 32, 64}) {
@@ -656,6 +778,7 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     (void)hipEventDestroy(e1);
     (void)hipFreeAsync(x, s);
     (void)hipFreeAsync(y, s);
+    if (win.kernel != MTL4G_SPMV_SLICED && !had_sliced) free_sliced(A);
     if (win.kernel == MTL4G_SPMV_STREAM || win.kernel == MTL4G_SPMV_STREAM2)
         M4_TRY(mtl4g_csr_tune(A, win.rows, win.nnz));
     M4_TRY(win.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, win.a, win.b)

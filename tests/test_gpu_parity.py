@@ -139,6 +139,79 @@ def test_spmv_scalar_bitwise(m, fam, uv):
         assert np.linalg.norm(xs.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
 
 
+def _sliced_entries(rp, n):
+    lens = np.diff(np.asarray(rp, np.int64))
+    return sum(int(lens[s:s + 64].max()) * 64 for s in range(0, n, 64) if lens[s:s + 64].size)
+
+
+def _spmv32(m, A, fmt, x):
+    import ctypes as C
+    torch = m._torch()
+    xs = torch.tensor(np.asarray(x, np.float32), device="cuda")
+    ys = torch.zeros(A.num_rows(), dtype=torch.float32, device="cuda")
+    m._lib.check(m._lib.lib().mtl4g_spmv_f32(A.h, fmt, C.c_void_p(xs.data_ptr()), C.c_void_p(ys.data_ptr()),
+                                             m.ASSIGN, m._stream()))
+    torch.cuda.synchronize()
+    return ys.cpu().numpy()
+
+
+@pytest.mark.parametrize("per_cu", [1, 4, 8])
+@pytest.mark.parametrize("fam", FAMILIES)
+def test_spmv_sliced_bitwise(m, fam, per_cu):
+    """Sliced ELL (slices of 64 rows, one lane per row): same bits as MTL4's
+    row sums for =, +=, -=; FP32 and BF16 values give the same bits as the
+    stream kernel on those values; refused where padding exceeds half the
+    non-zeros; CG on it within the CG bar of the tile kernel."""
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    ent = _sliced_entries(rp, n)
+    # FP32 / BF16 reference bits from the stream kernel, before the switch
+    x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    ref32 = {f: _spmv32(m, A, f, x) for f in (m.F32, m.BF16)}
+    if ent > len(ci) * 1.5 + 64:
+        with pytest.raises(m.MtlError):
+            A.set_kernel(m._lib.SPMV_SLICED, per_cu, 0)
+        return
+    A.set_kernel(m._lib.SPMV_SLICED, per_cu, 0)
+    assert A.get_kernel()[:2] == (m._lib.SPMV_SLICED, per_cu)
+    y0 = np.random.default_rng(9).uniform(-1, 1, A.num_rows())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+    for mode in (m.PLUS, m.MINUS):
+        w = dv(m, y0)
+        A.mult(dv(m, x), w, mode)
+        yr = y0.copy()
+        orc.spmv(O, x, yr, mode)
+        assert np.array_equal(w.numpy(), yr)
+    for f in (m.F32, m.BF16):
+        assert np.array_equal(_spmv32(m, A, f, x), ref32[f])
+    if fam in ("lap2d_64", "diff3d_16"):
+        b = orc.spmv(O, x)
+        xo, st, ito, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=3000, rtol=1e-10)
+        xs = m.dense_vector(n)
+        it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+        assert m.cg(A, xs, dv(m, b), m.pc.diagonal(A), it) == 0
+        assert abs(it.iterations() - ito.i) <= 2
+        assert np.linalg.norm(xs.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+        # mixed precision on the sliced FP32 copy
+        x32 = m.dense_vector(n)
+        it = m.basic_iteration(dv(m, b), 20000, 1e-10)
+        err, _ = m.ir_solve(A, x32, dv(m, b), it, m.pc.diagonal(A, m.F32), value_format=m.F32)
+        assert err == 0
+        assert np.linalg.norm(b - orc.spmv(O, x32.numpy())) <= 1.01e-10 * np.linalg.norm(b)
+
+
+def test_spmv_sliced_values_follow_set_values(m):
+    rp, ci, va = family("diff3d_16", seed=2)
+    O, A = pair(rp, ci, va)
+    A.set_kernel(m._lib.SPMV_SLICED, 4, 0)
+    x = np.random.default_rng(3).uniform(-1, 1, A.num_cols())
+    va2 = np.asarray(va) * 1.5 - 0.25
+    A.set_values(va2)
+    O2, _ = pair(rp, ci, va2)
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O2, x))
+
+
 def test_spmv_autotune_keeps_results(m):
     rp, ci, va = family("random_3000", seed=2)
     O, A = pair(rp, ci, va)
