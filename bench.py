@@ -36,6 +36,19 @@ def spmv_bytes(n, nnz, vb=8, ib=4):
     return nnz * (vb + ib) + (n + 1) * ib + n * vb + n * vb
 
 
+def sliced_bytes(rowptr, vb=8, ib=4):
+    """Algorithmic bytes of y = A*x on the sliced ELL layout: the padded
+    values+indices of slices of 64 rows, one 8-byte offset per slice, x
+    once, y once (no row pointers)."""
+    lens = np.diff(np.asarray(rowptr, dtype=np.int64))
+    n = len(lens)
+    nsl = (n + 63) // 64
+    padded = np.zeros(nsl * 64, dtype=np.int64)
+    padded[:n] = lens
+    entries = int(padded.reshape(nsl, 64).max(axis=1).sum()) * 64
+    return entries * (vb + ib) + (nsl + 1) * 8 + n * vb + n * vb
+
+
 def timed_spmv(m, torch, A, x, y, reps):
     """Average duration of the SpMV kernel, HIP events on the launch stream."""
     for _ in range(3):
@@ -84,6 +97,8 @@ def traffic_key(m, kern):
         return "tile/%d" % rows
     if kind == m._lib.SPMV_SCALAR:
         return "scalar/%d" % group
+    if kind == m._lib.SPMV_SLICED:
+        return "sliced/%d/%d" % (rows, group)
     return None
 
 
@@ -276,7 +291,7 @@ def serving_section(m, torch, gen, nreq):
                         np.cumsum(rg.exponential(1.0 / rate, nreq)),
                         rg.integers(1, 2 ** 62, nreq, dtype=np.uint64))
         out["%dk_req_s" % int(rate / 1000)] = {}
-    for name, kw in (("mixed", dict(precision=m.F32, streams=4, max_items=128)),
+    for name, kw in (("mixed", dict(precision=m.F32, streams=4, max_items=1024)),
                      ("fp64_per_request", dict(precision=m.F64, streams=1, max_items=8,
                                                 per_request=True))):
         Sv = m.Server(B, rtol=1e-10, big_rows=65536, cap_bytes=200e9, **kw)
@@ -339,11 +354,16 @@ def main():
     # ---- SpMV (the dominant kernel of the CG iteration)
     y = m.dense_vector(n)
     t_spmv = timed_spmv(m, torch, A, xt, y, args.spmv_reps)
-    by = spmv_bytes(n, nnz)
-    spmv_gbs = by / t_spmv / 1e9
     kern = A.get_kernel()
+    by_csr = spmv_bytes(n, nnz)
+    # bytes the chosen layout moves once (the sliced copy has no row pointers
+    # but pads each slice of 64 rows to its longest row)
+    by = sliced_bytes(rp) if kern[0] == m._lib.SPMV_SLICED else by_csr
+    spmv_gbs = by / t_spmv / 1e9
     traffic = load_traffic(traffic_key(m, kern))
-    if kern[0] == m._lib.SPMV_VECTOR:
+    if kern[0] == m._lib.SPMV_SLICED:
+        kname = "k_sell (sliced ELL, one lane per row, %d entries per chunk, %d workgroups/CU)" % (kern[2], kern[1])
+    elif kern[0] == m._lib.SPMV_VECTOR:
         kname = "k_csrv<%d lanes/row>" % kern[1]
     elif kern[0] == m._lib.SPMV_STREAM:
         kname = "k_csrp (persistent LDS tiles, %d workgroups/CU, %d rows/block)" % (kern[1], kern[2])
@@ -357,7 +377,9 @@ def main():
                 "config": traffic_key(m, kern),
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": by, "avg_launch_us": round(t_spmv * 1e6, 2)}
+                "bytes_per_launch": by, "csr_bytes_per_launch": by_csr,
+                "csr_equivalent_GBs": round(by_csr / t_spmv / 1e9, 1),
+                "avg_launch_us": round(t_spmv * 1e6, 2)}
     if args.only_spmv:   # the roofline kernel alone (profiling runs)
         if args.copy_calibration:   # a copy of known size for the PMC byte counters
             src = torch.rand(64 << 20, dtype=torch.float64, device="cuda")

@@ -153,7 +153,8 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
  * MTL4G_SPMV_SLICED: a sliced-ELL copy of the matrix (slices of 64 rows,
  * each padded to its longest row, stored column-major) built on the
  * device, one lane per row, a persistent grid of `group` workgroups per
- * compute unit (1..8); refused when the padding would exceed half the
+ * compute unit (1..8), `rows_per_block` = entries per lane and request
+ * chunk (4 or 8; 0 = 8); refused when the padding would exceed half the
  * non-zeros.
  * All sum every row in column order.  0 = library default. */
 #define MTL4G_SPMV_TILE   0

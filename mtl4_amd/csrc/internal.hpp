@@ -291,6 +291,8 @@ struct mtl4g_matrix_s {
     mtl4g::DevBuf<float> sl_val32;
     mtl4g::DevBuf<uint16_t> sl_valbf;
     int nsl = 0;
+    int sl_unroll = 8;                      // entries per lane per chunk (4 or 8)
+    int sl_unroll_lowp[2] = {8, 8};         // the same for FP32 / BF16 values
     int32_t max_row = 0;
     std::vector<int32_t> h_rowptr, h_col;   // host pattern (factorizations)
     std::vector<double> h_val;

@@ -485,8 +485,8 @@ __global__ __launch_bounds__(BS) void k_csrv(const int32_t* __restrict__ rowptr,
 // arrays carry (UNR * 64 entries) -- so the wait for the gathers never
 // waits for the next chunk; two register sets alternate between the
 // current and the next chunk.
-constexpr int kSellUnroll = 8;                     // entries per lane per chunk
-constexpr int kSellPad = kSellUnroll * kWave;      // padding entries after the arrays
+constexpr int kSellMaxUnroll = 8;                  // entries per lane per chunk: 4 or 8
+constexpr int kSellPad = kSellMaxUnroll * kWave;   // padding entries after the arrays
 
 template <int BS, int UNR, typename AT, typename MT, typename XT, typename Epi>
 __global__ __launch_bounds__(BS) void k_sell(const int64_t* __restrict__ soff,
@@ -814,13 +814,22 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
             set_error("sliced SpMV: no sliced copy of these values");
             return MTL4G_ERR_INVALID;
         }
-        int pcu = A->per_cu;
+        int pcu = A->per_cu, unr = A->sl_unroll;
         const int li = sizeof(MT) == 4 ? 0 : 1;
-        if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) pcu = A->per_cu_lowp[li];
+        if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) {
+            pcu = A->per_cu_lowp[li];
+            unr = A->sl_unroll_lowp[li];
+        }
         const int units = (A->nsl + 3) / 4;   // four slices per workgroup pass
-        const int g = persistent_grid<k_sell<256, kSellUnroll, AT, MT, XT, Epi>>(A->cus, pcu, units);
-        hipLaunchKernelGGL((k_sell<256, kSellUnroll, AT, MT, XT, Epi>), dim3(g), dim3(256), 0, s, A->sl_off.p, A->sl_col.p,
-                           sv, x, A->n, A->nsl, epi, partials, ticket);
+#define M4_LAUNCHS(U)                                                                               \
+    do {                                                                                            \
+        const int g = persistent_grid<k_sell<256, U, AT, MT, XT, Epi>>(A->cus, pcu, units);         \
+        hipLaunchKernelGGL((k_sell<256, U, AT, MT, XT, Epi>), dim3(g), dim3(256), 0, s, A->sl_off.p, \
+                           A->sl_col.p, sv, x, A->n, A->nsl, epi, partials, ticket);               \
+    } while (0)
+        if (unr == 4) M4_LAUNCHS(4);
+        else M4_LAUNCHS(8);
+#undef M4_LAUNCHS
         M4_HIP(hipGetLastError());
         return MTL4G_OK;
     }

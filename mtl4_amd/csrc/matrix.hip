@@ -305,18 +305,23 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     struct Cand {
         int f;
         bool wide;
+        int unr;
     };
     std::vector<Cand> cands;
-    if (A->kernel == MTL4G_SPMV_SLICED)
-        for (int f : {2, 4, 6, 8}) cands.push_back({f, false});
-    else
-        for (int f = 2; f <= 6; f++) cands.push_back({f, false});
+    int win_unr = A->sl_unroll;
+    if (A->kernel == MTL4G_SPMV_SLICED) {
+        for (int u : {4, 8})
+            for (int f : {2, 4, 6, 8}) cands.push_back({f, false, u});
+    } else {
+        for (int f = 2; f <= 6; f++) cands.push_back({f, false, 8});
+    }
     if (A->kernel == MTL4G_SPMV_STREAM && A->nb_wide > 0)
-        for (int f = 2; f <= 4; f++) cands.push_back({f, true});
+        for (int f = 2; f <= 4; f++) cands.push_back({f, true, 8});
     for (const Cand& cd : cands) {
         const int f = cd.f;
         A->per_cu_lowp[fmt - 1] = f;
         A->lowp_wide[fmt - 1] = cd.wide;
+        A->sl_unroll_lowp[fmt - 1] = cd.unr;
         int st = launch_spmv_f32(A, fmt, x, y, MTL4G_ASSIGN, s);
         if (!st) st = launch_spmv_f32(A, fmt, x, y, MTL4G_ASSIGN, s);
         if (st) continue;
@@ -331,6 +336,7 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
             best = ms;
             win = f;
             win_wide = cd.wide;
+            win_unr = cd.unr;
         }
     }
     (void)hipEventDestroy(e0);
@@ -339,6 +345,7 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     (void)hipFreeAsync(y, s);
     A->per_cu_lowp[fmt - 1] = win;
     A->lowp_wide[fmt - 1] = win_wide;
+    A->sl_unroll_lowp[fmt - 1] = win_unr;
     return MTL4G_OK;
 }
 
@@ -668,13 +675,16 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
     }
     if (kernel == MTL4G_SPMV_SLICED) {
         if (group == 0) group = A->per_cu;
-        if (group < 1 || group > 8) {
-            set_error("sliced kernel: 1 to 8 workgroups per compute unit");
+        if (rows_per_block == 0) rows_per_block = 8;
+        if (group < 1 || group > 8 || (rows_per_block != 4 && rows_per_block != 8)) {
+            set_error("sliced kernel: 1 to 8 workgroups per compute unit, 4 or 8 entries per lane and chunk");
             return MTL4G_ERR_INVALID;
         }
         M4_TRY(build_sliced(A));
         A->kernel = kernel;
         A->per_cu = group;
+        A->sl_unroll = rows_per_block;
+        A->sl_unroll_lowp[0] = A->sl_unroll_lowp[1] = rows_per_block;
         A->lowp_tuned[0] = A->lowp_tuned[1] = false;
         A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
         A->lowp_wide[0] = A->lowp_wide[1] = false;
@@ -714,7 +724,10 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
                      ? A->per_cu
                  : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
                                                   : 0;
-    if (rows_per_block) *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR ? A->vrows : A->rows_per_block;
+    if (rows_per_block)
+        *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR   ? A->vrows
+                          : A->kernel == MTL4G_SPMV_SLICED ? A->sl_unroll
+                                                           : A->rows_per_block;
     return MTL4G_OK;
 }
 
@@ -737,7 +750,8 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     const bool had_sliced = !A->h_sl_off.empty();
     const bool try_sliced = had_sliced || sliced_entries(A) <= A->nnz + A->nnz / 8 + kWave;
     if (try_sliced)
-        for (int f : {4, 6, 8}) cands.push_back({MTL4G_SPMV_SLICED, f, 0, 0, 0});
+        for (int u : {4, 8})
+            for (int f : {2, 4, 6, 8}) cands.push_back({MTL4G_SPMV_SLICED, f, u, 0, 0});
     // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
     const double avg = A->n ? (double)A->nnz / A->n : 0.0;
     for (int g : {4, 8, 16, 32, 64}) {

@@ -155,13 +155,14 @@ def _spmv32(m, A, fmt, x):
     return ys.cpu().numpy()
 
 
-@pytest.mark.parametrize("per_cu", [1, 4, 8])
+@pytest.mark.parametrize("shape", [(1, 8), (4, 8), (8, 8), (2, 4), (6, 4)])
 @pytest.mark.parametrize("fam", FAMILIES)
-def test_spmv_sliced_bitwise(m, fam, per_cu):
+def test_spmv_sliced_bitwise(m, fam, shape):
     """Sliced ELL (slices of 64 rows, one lane per row): same bits as MTL4's
     row sums for =, +=, -=; FP32 and BF16 values give the same bits as the
     stream kernel on those values; refused where padding exceeds half the
     non-zeros; CG on it within the CG bar of the tile kernel."""
+    per_cu, unroll = shape
     rp, ci, va = family(fam, seed=6)
     O, A = pair(rp, ci, va)
     n = A.num_rows()
@@ -171,10 +172,10 @@ def test_spmv_sliced_bitwise(m, fam, per_cu):
     ref32 = {f: _spmv32(m, A, f, x) for f in (m.F32, m.BF16)}
     if ent > len(ci) * 1.5 + 64:
         with pytest.raises(m.MtlError):
-            A.set_kernel(m._lib.SPMV_SLICED, per_cu, 0)
+            A.set_kernel(m._lib.SPMV_SLICED, per_cu, unroll)
         return
-    A.set_kernel(m._lib.SPMV_SLICED, per_cu, 0)
-    assert A.get_kernel()[:2] == (m._lib.SPMV_SLICED, per_cu)
+    A.set_kernel(m._lib.SPMV_SLICED, per_cu, unroll)
+    assert A.get_kernel() == (m._lib.SPMV_SLICED, per_cu, unroll)
     y0 = np.random.default_rng(9).uniform(-1, 1, A.num_rows())
     assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
     for mode in (m.PLUS, m.MINUS):
