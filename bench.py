@@ -49,6 +49,17 @@ def sliced_bytes(rowptr, vb=8, ib=4):
     return entries * (vb + ib) + (nsl + 1) * 8 + n * vb + n * vb
 
 
+def dia_bytes(rowptr, col, vb=8):
+    """Algorithmic bytes of y = A*x on the diagonal layout: one value per row
+    and diagonal (absent entries included), the diagonal offsets, x once, y
+    once (no column indices, no row pointers)."""
+    lens = np.diff(np.asarray(rowptr, dtype=np.int64))
+    rows = np.repeat(np.arange(len(lens), dtype=np.int64), lens)
+    ndiag = len(np.unique(np.asarray(col, dtype=np.int64) - rows))
+    n = len(lens)
+    return ndiag * n * vb + ndiag * 4 + n * vb + n * vb
+
+
 def timed_spmv(m, torch, A, x, y, reps):
     """Average duration of the SpMV kernel, HIP events on the launch stream."""
     for _ in range(3):
@@ -99,6 +110,8 @@ def traffic_key(m, kern):
         return "scalar/%d" % group
     if kind == m._lib.SPMV_SLICED:
         return "sliced/%d/%d" % (rows, group)
+    if kind == m._lib.SPMV_DIA:
+        return "dia/%d/%d" % (rows, group)
     return None
 
 
@@ -176,10 +189,11 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     # the FP32 recurrence may stagnate: at most 3x the FP64 iteration count
     itm = m.basic_iteration(bnorm, 3 * max(it.iterations(), 10), 1e-10)
     t0 = time.perf_counter()
+    failure = None
     try:
         errm, st = m.ir_solve(A, xm, b, itm, P32, method=method, value_format=m.F32, mode="reliable")
-    except m.nonfinite_error:
-        errm, st = -6, {"inner_total": itm.iterations(), "outer": -1}
+    except m.MtlError as e:   # the FP32 recurrence broke down or overflowed
+        errm, st, failure = -1, {"inner_total": itm.iterations(), "outer": -1}, "%s: %s" % (type(e).__name__, e)
     torch.cuda.synchronize()
     tm = time.perf_counter() - t0
     log("  mixed solve: %d FP32 iterations, error %d, %.3f s" % (st["inner_total"], errm, tm))
@@ -188,7 +202,7 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     return {"fp64": {"solves_per_s": round(1 / t64, 4), "iterations": it.iterations(), "error_code": err,
                      "relres": relres_of(m, A, x, b, bnorm)},
             "mixed_f32": {"solves_per_s": round(1 / tm, 4), "fp32_iterations": st["inner_total"],
-                          "fp64_refreshes": st["outer"], "error_code": errm,
+                          "fp64_refreshes": st["outer"], "error_code": errm, "failure": failure,
                           "relres": relres_of(m, A, xm, b, bnorm), "rel_err_vs_fp64": rel_err},
             "speedup": round(t64 / tm, 3), "steps": 1}
 
@@ -355,13 +369,21 @@ def main():
     y = m.dense_vector(n)
     t_spmv = timed_spmv(m, torch, A, xt, y, args.spmv_reps)
     kern = A.get_kernel()
-    by_csr = spmv_bytes(n, nnz)
-    # bytes the chosen layout moves once (the sliced copy has no row pointers
-    # but pads each slice of 64 rows to its longest row)
-    by = sliced_bytes(rp) if kern[0] == m._lib.SPMV_SLICED else by_csr
+    # algorithmic bytes of the CSR SpMV (the roofline figure), and the bytes
+    # the device layout the autotuner chose moves once: the sliced copy has no
+    # row pointers but pads slices of 64 rows, the diagonal copy has no
+    # column indices
+    by = spmv_bytes(n, nnz)
+    layout, by_layout = "csr", by
+    if kern[0] == m._lib.SPMV_SLICED:
+        layout, by_layout = "sliced_ell", sliced_bytes(rp)
+    elif kern[0] == m._lib.SPMV_DIA:
+        layout, by_layout = "diagonal", dia_bytes(rp, ci)
     spmv_gbs = by / t_spmv / 1e9
     traffic = load_traffic(traffic_key(m, kern))
-    if kern[0] == m._lib.SPMV_SLICED:
+    if kern[0] == m._lib.SPMV_DIA:
+        kname = "k_dia (diagonal layout, one lane per row, %d diagonals per chunk, %d workgroups/CU)" % (kern[2], kern[1])
+    elif kern[0] == m._lib.SPMV_SLICED:
         kname = "k_sell (sliced ELL, one lane per row, %d entries per chunk, %d workgroups/CU)" % (kern[2], kern[1])
     elif kern[0] == m._lib.SPMV_VECTOR:
         kname = "k_csrv<%d lanes/row>" % kern[1]
@@ -377,8 +399,9 @@ def main():
                 "config": traffic_key(m, kern),
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": by, "csr_bytes_per_launch": by_csr,
-                "csr_equivalent_GBs": round(by_csr / t_spmv / 1e9, 1),
+                "bytes_per_launch": by, "layout": layout, "layout_bytes_per_launch": by_layout,
+                "layout_GBs": round(by_layout / t_spmv / 1e9, 1),
+                "layout_frac": round(by_layout / t_spmv / 1e9 / HBM_PEAK_GBS, 4),
                 "avg_launch_us": round(t_spmv * 1e6, 2)}
     if args.only_spmv:   # the roofline kernel alone (profiling runs)
         if args.copy_calibration:   # a copy of known size for the PMC byte counters

@@ -155,7 +155,11 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
  * device, one lane per row, a persistent grid of `group` workgroups per
  * compute unit (1..8), `rows_per_block` = entries per lane and request
  * chunk (4 or 8; 0 = 8); refused when the padding would exceed half the
- * non-zeros.
+ * non-zeros;
+ * MTL4G_SPMV_DIA: a diagonal (banded) copy of the matrix -- one value per
+ * row and diagonal, no column indices -- for matrices whose entries lie on
+ * at most 32 diagonals filling at least 4/5 of their slots (stencils);
+ * `group` and `rows_per_block` as for MTL4G_SPMV_SLICED; refused otherwise.
  * All sum every row in column order.  0 = library default. */
 #define MTL4G_SPMV_TILE   0
 #define MTL4G_SPMV_VECTOR 1
@@ -163,6 +167,7 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
 #define MTL4G_SPMV_SCALAR 3
 #define MTL4G_SPMV_STREAM2 4
 #define MTL4G_SPMV_SLICED 5
+#define MTL4G_SPMV_DIA 6
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);

@@ -38,7 +38,7 @@ class IRStats(C.Structure):
     _fields_ = [("outer", C.c_int32), ("inner_total", C.c_int32), ("relres", C.c_double)]
 
 
-SPMV_TILE, SPMV_VECTOR, SPMV_STREAM, SPMV_SCALAR, SPMV_STREAM2, SPMV_SLICED = 0, 1, 2, 3, 4, 5
+SPMV_TILE, SPMV_VECTOR, SPMV_STREAM, SPMV_SCALAR, SPMV_STREAM2, SPMV_SLICED, SPMV_DIA = 0, 1, 2, 3, 4, 5, 6
 ORDER_TREE, ORDER_MTL4 = 0, 1
 
 

@@ -291,7 +291,14 @@ struct mtl4g_matrix_s {
     mtl4g::DevBuf<float> sl_val32;
     mtl4g::DevBuf<uint16_t> sl_valbf;
     int nsl = 0;
-    int sl_unroll = 8;                      // entries per lane per chunk (4 or 8)
+    // diagonal layout (MTL4G_SPMV_DIA): val[d * n + r] on ndiag diagonals, absent entries marked
+    std::vector<int32_t> h_dia_off;         // column - row of each diagonal, ascending
+    mtl4g::DevBuf<int32_t> dia_off;
+    mtl4g::DevBuf<double> dia_val;
+    mtl4g::DevBuf<float> dia_val32;
+    mtl4g::DevBuf<uint16_t> dia_valbf;
+    int ndiag = 0;
+    int sl_unroll = 8;                      // sliced / diagonal: entries per lane per chunk (4 or 8)
     int sl_unroll_lowp[2] = {8, 8};         // the same for FP32 / BF16 values
     int32_t max_row = 0;
     std::vector<int32_t> h_rowptr, h_col;   // host pattern (factorizations)

@@ -582,6 +582,114 @@ __global__ __launch_bounds__(BS) void k_sell(const int64_t* __restrict__ soff,
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
 }
 
+// Diagonal (banded) layout: a matrix whose entries lie on ndiag diagonals
+// (column - row = off[d], ascending) stores val[d * n + r] for every row r
+// and diagonal d, with a reserved NaN bit pattern where row r has no entry
+// on d.  No column indices are stored or read.  A lane sums its row over
+// the diagonals in ascending offset order -- ascending column order, as
+// every CSR kernel does -- skipping the absent entries, so the result is
+// the same bit for bit.  The walk, chunking and request order are those of
+// k_sell (slices of 64 rows per wavefront, UNR diagonals per chunk, the
+// next chunk's values in flight while the current chunk's x is gathered).
+constexpr uint64_t kDiaAbsent64 = 0x7ff4a5a5a5a5a5a5ull;   // signalling NaN: never produced by arithmetic
+constexpr uint32_t kDiaAbsent32 = 0x7fa5a5a5u;             // nor by a conversion (which quiets NaNs)
+constexpr uint16_t kDiaAbsent16 = 0x7fa5u;                 // bfloat16: quiet NaNs from rounding carry 0x0040
+constexpr int kDiaMax = 32;                                // diagonals at most
+constexpr int kDiaPad = kWave;                             // values of padding after the array
+
+__device__ __forceinline__ bool dia_absent(double v) { return (uint64_t)__double_as_longlong(v) == kDiaAbsent64; }
+__device__ __forceinline__ bool dia_absent(float v) { return __float_as_uint(v) == kDiaAbsent32; }
+__device__ __forceinline__ bool dia_absent(uint16_t v) { return v == kDiaAbsent16; }
+template <typename AT>
+__device__ __forceinline__ AT dia_value(double v) { return static_cast<AT>(v); }
+template <typename AT>
+__device__ __forceinline__ AT dia_value(float v) { return static_cast<AT>(v); }
+template <typename AT>
+__device__ __forceinline__ AT dia_value(uint16_t v) { return static_cast<AT>(bf16_to_f32(v)); }
+
+template <int BS, int UNR, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_dia(const int32_t* __restrict__ doff,
+                                            const MT* __restrict__ val,
+                                            const XT* __restrict__ x, int nrows, int ncols, int ndiag,
+                                            Epi epi, AT* partials, unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    constexpr int WPB = BS / kWave;
+    __shared__ AT sh[WPB];
+    // the offsets in LDS: read per chunk without waiting on vector memory
+    __shared__ int offs[kDiaMax];
+    if ((int)threadIdx.x < ndiag) offs[threadIdx.x] = doff[threadIdx.x];
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int xcd = blockIdx.x % kXcd;
+    const int wstep = (int)(gridDim.x / kXcd) * WPB;
+    const int nsl = (nrows + kWave - 1) / kWave;
+    const int per = (nsl + kXcd - 1) / kXcd;
+    const int hi = min(nsl, (xcd + 1) * per);
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+
+    struct Chunk {
+        MT v[UNR];
+        int off[UNR];   // wavefront-uniform
+        typename Epi::Pre pre;
+    };
+    int s = xcd * per + (int)(blockIdx.x / kXcd) * WPB + wv;
+    int k0 = 0;
+    auto fetch = [&](Chunk& ch) {
+        const int r = s * kWave + lane;
+        ch.pre = epi.preload(min(r, nrows - 1));
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int d = min(k0 + u, ndiag - 1);
+            ch.off[u] = __builtin_amdgcn_readfirstlane(offs[d]);
+            ch.v[u] = __builtin_nontemporal_load(val + (int64_t)d * nrows + r);   // rows past n read the padding
+        }
+    };
+    AT acc = AT(0);
+    auto step = [&](const Chunk& cur, Chunk& nxt) -> bool {
+        const int kc = k0, row = s * kWave + lane;
+        AT g[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) g[u] = static_cast<AT>(x[min(max(row + cur.off[u], 0), ncols - 1)]);
+        const bool last = __builtin_amdgcn_readfirstlane(kc + UNR >= ndiag) != 0;
+        const int sn = __builtin_amdgcn_readfirstlane(last ? s + wstep : s);
+        auto sum = [&]() {
+#pragma unroll
+            for (int u = 0; u < UNR; u++)
+                if (kc + u < ndiag && !dia_absent(cur.v[u])) acc += dia_value<AT>(cur.v[u]) * g[u];
+            if (last) {
+                if (row < nrows) epi(row, acc, cur.pre, part);
+                acc = AT(0);
+            }
+        };
+        if (__builtin_amdgcn_readfirstlane(sn >= hi)) {
+            sum();
+            return false;
+        }
+        s = sn;
+        k0 = last ? 0 : kc + UNR;
+        fetch(nxt);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) asm volatile("" : "+v"(g[u])::"memory");
+        sum();
+        return true;
+    };
+    if (s < hi) {
+        Chunk A, B;
+        fetch(A);
+        for (;;) {
+            if (!step(A, B)) break;
+            if (!step(B, A)) break;
+        }
+    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
+}
+
 // Element epilogues may split operator()(i, part) into In load(i) (all
 // reads of element i) and store(i, const In&, part) (the arithmetic, writes
 // and partial sums); k_vec then batches the loads of several elements.
@@ -830,6 +938,35 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
         if (unr == 4) M4_LAUNCHS(4);
         else M4_LAUNCHS(8);
 #undef M4_LAUNCHS
+        M4_HIP(hipGetLastError());
+        return MTL4G_OK;
+    }
+    if (A->kernel == MTL4G_SPMV_DIA) {
+        // the diagonal copy of the value array the caller passed
+        const MT* dv = nullptr;
+        if constexpr (std::is_same<MT, double>::value) dv = val == A->val.p ? A->dia_val.p : nullptr;
+        else if constexpr (std::is_same<MT, float>::value) dv = val == A->val32.p ? A->dia_val32.p : nullptr;
+        else dv = val == A->valbf.p ? A->dia_valbf.p : nullptr;
+        if (!dv || !A->dia_off.p) {
+            set_error("diagonal SpMV: no diagonal copy of these values");
+            return MTL4G_ERR_INVALID;
+        }
+        int pcu = A->per_cu, unr = A->sl_unroll;
+        const int li = sizeof(MT) == 4 ? 0 : 1;
+        if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) {
+            pcu = A->per_cu_lowp[li];
+            unr = A->sl_unroll_lowp[li];
+        }
+        const int units = ((A->n + kWave - 1) / kWave + 3) / 4;
+#define M4_LAUNCHD(U)                                                                               \
+    do {                                                                                            \
+        const int g = persistent_grid<k_dia<256, U, AT, MT, XT, Epi>>(A->cus, pcu, units);          \
+        hipLaunchKernelGGL((k_dia<256, U, AT, MT, XT, Epi>), dim3(g), dim3(256), 0, s, A->dia_off.p, dv, \
+                           x, A->n, A->m, A->ndiag, epi, partials, ticket);                        \
+    } while (0)
+        if (unr == 4) M4_LAUNCHD(4);
+        else M4_LAUNCHD(8);
+#undef M4_LAUNCHD
         M4_HIP(hipGetLastError());
         return MTL4G_OK;
     }

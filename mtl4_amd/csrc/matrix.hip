@@ -202,8 +202,112 @@ static int build_sliced(mtl4g_matrix_s* A) {
     return MTL4G_OK;
 }
 
+// ---- diagonal copy of the matrix
+// Distinct diagonals (column - row) in ascending order, or empty when there
+// are more than kDiaMax of them.
+static std::vector<int32_t> dia_offsets(const mtl4g_matrix_s* A) {
+    std::vector<int32_t> off;
+    for (int32_t r = 0; r < A->n; r++)
+        for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++) {
+            const int32_t o = A->h_col[j] - r;
+            auto it = std::lower_bound(off.begin(), off.end(), o);
+            if (it != off.end() && *it == o) continue;
+            if ((int)off.size() == kDiaMax) return {};
+            off.insert(it, o);
+        }
+    return off;
+}
+
+// out[d * n + r] = conv(src[j]) for the entry j of row r on diagonal d, absent elsewhere
+template <typename T, typename F>
+static void dia_fill(const mtl4g_matrix_s* A, T absent, F conv, std::vector<T>& out) {
+    const int nd = (int)A->h_dia_off.size();
+    out.assign((size_t)nd * A->n + kDiaPad, absent);
+    for (int32_t r = 0; r < A->n; r++) {
+        int d = 0;
+        for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++) {
+            while (A->h_dia_off[d] != A->h_col[j] - r) d++;   // columns ascend, so do the diagonals
+            out[(size_t)d * A->n + r] = conv(A->h_val[j]);
+        }
+    }
+}
+
+static void free_dia(mtl4g_matrix_s* A) {
+    A->h_dia_off.clear();
+    A->dia_off.free();
+    A->dia_val.free();
+    A->dia_val32.free();
+    A->dia_valbf.free();
+    A->ndiag = 0;
+}
+
+template <typename T>
+static bool bits_equal(T a, T b) {
+    return std::memcmp(&a, &b, sizeof(T)) == 0;
+}
+
+static double absent64() {
+    double d;
+    std::memcpy(&d, &kDiaAbsent64, 8);
+    return d;
+}
+static float absent32() {
+    float f;
+    std::memcpy(&f, &kDiaAbsent32, 4);
+    return f;
+}
+
+static int dia_values(mtl4g_matrix_s* A, int fmt) {
+    if (A->h_dia_off.empty()) return MTL4G_OK;
+    if (fmt == MTL4G_F64 && !A->dia_val.p) {
+        std::vector<double> v;
+        dia_fill(A, absent64(), [](double d) { return d; }, v);
+        M4_TRY(upload(A->dia_val, v));
+    }
+    if (fmt == MTL4G_F32 && !A->dia_val32.p) {
+        std::vector<float> v;
+        dia_fill(A, absent32(), [](double d) { return (float)d; }, v);
+        M4_TRY(upload(A->dia_val32, v));
+    }
+    if (fmt == MTL4G_BF16 && !A->dia_valbf.p) {
+        std::vector<uint16_t> v;
+        dia_fill(A, kDiaAbsent16, [](double d) { return f32_to_bf16((float)d); }, v);
+        M4_TRY(upload(A->dia_valbf, v));
+    }
+    return MTL4G_OK;
+}
+
+// Entries per slot of the diagonal layout: refused above kDiaMax diagonals,
+// below 4/5 fill, or when a value carries the reserved absent-entry bits.
+static int build_dia(mtl4g_matrix_s* A) {
+    if (!A->h_dia_off.empty()) return MTL4G_OK;
+    std::vector<int32_t> off = dia_offsets(A);
+    if (A->n == 0 || off.empty() || (double)off.size() * A->n * 0.8 > (double)A->nnz) {
+        set_error("diagonal layout: %zu diagonals for %lld non-zeros in %d rows (at most %d, filled 4/5)",
+                  off.size(), (long long)A->nnz, A->n, kDiaMax);
+        return MTL4G_ERR_UNSUPPORTED;
+    }
+    for (double v : A->h_val)
+        if (bits_equal(v, absent64())) {
+            set_error("diagonal layout: a value has the reserved absent-entry bits");
+            return MTL4G_ERR_UNSUPPORTED;
+        }
+    A->h_dia_off = off;
+    A->ndiag = (int)off.size();
+    int st;
+    if ((st = upload(A->dia_off, off)) || (st = dia_values(A, MTL4G_F64)) ||
+        (A->val32.p && (st = dia_values(A, MTL4G_F32))) || (A->valbf.p && (st = dia_values(A, MTL4G_BF16)))) {
+        free_dia(A);
+        return st;
+    }
+    return MTL4G_OK;
+}
+
 int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
-    if (fmt == MTL4G_F32 || fmt == MTL4G_BF16) M4_TRY(sliced_lowp_values(A, fmt));
+    if (fmt == MTL4G_F32 || fmt == MTL4G_BF16) {
+        M4_TRY(sliced_lowp_values(A, fmt));
+        M4_TRY(dia_values(A, fmt));
+    }
     if (fmt == MTL4G_F32 && !A->val32.p) {
         std::vector<float> v(A->h_val.begin(), A->h_val.end());
         M4_TRY(A->val32.alloc(v.size() + 2));   // padding for aligned pair loads
@@ -282,7 +386,8 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     A->lowp_tuned[fmt - 1] = true;
     A->per_cu_lowp[fmt - 1] = 0;
     A->lowp_wide[fmt - 1] = false;
-    if ((A->kernel != MTL4G_SPMV_STREAM && A->kernel != MTL4G_SPMV_STREAM2 && A->kernel != MTL4G_SPMV_SLICED) ||
+    if ((A->kernel != MTL4G_SPMV_STREAM && A->kernel != MTL4G_SPMV_STREAM2 && A->kernel != MTL4G_SPMV_SLICED &&
+         A->kernel != MTL4G_SPMV_DIA) ||
         A->n == 0 || A->m == 0)
         return MTL4G_OK;
     if (A->kernel == MTL4G_SPMV_STREAM && !A->desc_wide.p) {   // wide partition for the candidates below
@@ -309,7 +414,7 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     };
     std::vector<Cand> cands;
     int win_unr = A->sl_unroll;
-    if (A->kernel == MTL4G_SPMV_SLICED) {
+    if (A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA) {
         for (int u : {4, 8})
             for (int f : {2, 4, 6, 8}) cands.push_back({f, false, u});
     } else {
@@ -643,6 +748,17 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where) {
         sliced_fill(A, A->h_val.data(), 0.0, [](double d) { return d; }, v);
         M4_TRY(upload(A->sl_val, v));
     }
+    if (!A->h_dia_off.empty()) {
+        const bool dia = A->kernel == MTL4G_SPMV_DIA;
+        free_dia(A);
+        if (dia) {
+            const int st = build_dia(A);
+            if (st) {   // a value now carries the reserved bits: back to the CSR stream kernel
+                A->kernel = MTL4G_SPMV_STREAM;
+                if (!A->desc.p) M4_TRY(build_row_blocks(A, 0, 0));
+            }
+        }
+    }
     return MTL4G_OK;
 }
 
@@ -673,14 +789,14 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
         A->lowp_wide[0] = A->lowp_wide[1] = false;
         return MTL4G_OK;
     }
-    if (kernel == MTL4G_SPMV_SLICED) {
+    if (kernel == MTL4G_SPMV_SLICED || kernel == MTL4G_SPMV_DIA) {
         if (group == 0) group = A->per_cu;
         if (rows_per_block == 0) rows_per_block = 8;
         if (group < 1 || group > 8 || (rows_per_block != 4 && rows_per_block != 8)) {
-            set_error("sliced kernel: 1 to 8 workgroups per compute unit, 4 or 8 entries per lane and chunk");
+            set_error("sliced / diagonal kernel: 1 to 8 workgroups per compute unit, 4 or 8 entries per lane and chunk");
             return MTL4G_ERR_INVALID;
         }
-        M4_TRY(build_sliced(A));
+        M4_TRY(kernel == MTL4G_SPMV_SLICED ? build_sliced(A) : build_dia(A));
         A->kernel = kernel;
         A->per_cu = group;
         A->sl_unroll = rows_per_block;
@@ -720,13 +836,13 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
     if (group)
         *group = A->kernel == MTL4G_SPMV_VECTOR   ? A->gs
                  : A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2 ||
-                           A->kernel == MTL4G_SPMV_SLICED
+                           A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA
                      ? A->per_cu
                  : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
                                                   : 0;
     if (rows_per_block)
         *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR   ? A->vrows
-                          : A->kernel == MTL4G_SPMV_SLICED ? A->sl_unroll
+                          : A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA ? A->sl_unroll
                                                            : A->rows_per_block;
     return MTL4G_OK;
 }
@@ -752,6 +868,11 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     if (try_sliced)
         for (int u : {4, 8})
             for (int f : {2, 4, 6, 8}) cands.push_back({MTL4G_SPMV_SLICED, f, u, 0, 0});
+    // the diagonal layout where the entries lie on few, well-filled diagonals
+    const bool had_dia = !A->h_dia_off.empty();
+    if (had_dia || build_dia(A) == MTL4G_OK)
+        for (int u : {4, 8})
+            for (int f : {2, 4, 6, 8}) cands.push_back({MTL4G_SPMV_DIA, f, u, 0, 0});
     // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
     const double avg = A->n ? (double)A->nnz / A->n : 0.0;
     for (int g : {4, 8, 16, 32, 64}) {
@@ -793,6 +914,7 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     (void)hipFreeAsync(x, s);
     (void)hipFreeAsync(y, s);
     if (win.kernel != MTL4G_SPMV_SLICED && !had_sliced) free_sliced(A);
+    if (win.kernel != MTL4G_SPMV_DIA && !had_dia) free_dia(A);
     if (win.kernel == MTL4G_SPMV_STREAM || win.kernel == MTL4G_SPMV_STREAM2)
         M4_TRY(mtl4g_csr_tune(A, win.rows, win.nnz));
     M4_TRY(win.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, win.a, win.b)

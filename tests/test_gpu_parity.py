@@ -202,6 +202,99 @@ def test_spmv_sliced_bitwise(m, fam, shape):
         assert np.linalg.norm(b - orc.spmv(O, x32.numpy())) <= 1.01e-10 * np.linalg.norm(b)
 
 
+def _dia_eligible(rp, ci):
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(np.asarray(rp, np.int64)))
+    offs = np.unique(np.asarray(ci, np.int64) - rows)
+    return len(offs) <= 32 and len(offs) * (len(rp) - 1) * 0.8 <= len(ci)
+
+
+@pytest.mark.parametrize("shape", [(1, 8), (4, 8), (8, 8), (2, 4), (6, 4)])
+@pytest.mark.parametrize("fam", FAMILIES)
+def test_spmv_dia_bitwise(m, fam, shape):
+    """Diagonal layout (no column indices, absent entries skipped): same bits
+    as MTL4's row sums for =, +=, -=; FP32 and BF16 values give the stream
+    kernel's bits; refused for matrices off few well-filled diagonals; CG
+    and mixed precision on it within their bars."""
+    per_cu, unroll = shape
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    ref32 = {f: _spmv32(m, A, f, x) for f in (m.F32, m.BF16)}
+    if not _dia_eligible(rp, ci):
+        with pytest.raises(m.MtlError):
+            A.set_kernel(m._lib.SPMV_DIA, per_cu, unroll)
+        return
+    A.set_kernel(m._lib.SPMV_DIA, per_cu, unroll)
+    assert A.get_kernel() == (m._lib.SPMV_DIA, per_cu, unroll)
+    y0 = np.random.default_rng(9).uniform(-1, 1, A.num_rows())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+    for mode in (m.PLUS, m.MINUS):
+        w = dv(m, y0)
+        A.mult(dv(m, x), w, mode)
+        yr = y0.copy()
+        orc.spmv(O, x, yr, mode)
+        assert np.array_equal(w.numpy(), yr)
+    for f in (m.F32, m.BF16):
+        assert np.array_equal(_spmv32(m, A, f, x), ref32[f])
+    if fam in ("lap2d_64", "diff3d_16"):
+        b = orc.spmv(O, x)
+        xo, st, ito, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=3000, rtol=1e-10)
+        xs = m.dense_vector(n)
+        it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+        assert m.cg(A, xs, dv(m, b), m.pc.diagonal(A), it) == 0
+        assert abs(it.iterations() - ito.i) <= 2
+        assert np.linalg.norm(xs.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+        x32 = m.dense_vector(n)
+        it = m.basic_iteration(dv(m, b), 20000, 1e-10)
+        err, _ = m.ir_solve(A, x32, dv(m, b), it, m.pc.diagonal(A, m.F32), value_format=m.F32)
+        assert err == 0
+        assert np.linalg.norm(b - orc.spmv(O, x32.numpy())) <= 1.01e-10 * np.linalg.norm(b)
+
+
+def test_spmv_dia_rectangular_and_signed_zeros(m):
+    """Banded rectangular matrix; rows whose products are -0.0 keep the sign
+    an MTL4 row sum gives (absent entries are skipped, not added as 0*x)."""
+    n, ncols = 300, 307
+    rows, cols, vals = [], [], []
+    rng = np.random.default_rng(4)
+    for r in range(n):
+        for o in (-3, 0, 2, 7):
+            c = r + o
+            if 0 <= c < ncols and (r % 11 != 5 or o != 2):
+                rows.append(r)
+                cols.append(c)
+                vals.append(-0.0 if r % 7 == 3 else rng.uniform(-1, 1))
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, np.asarray(rows) + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    O, A = pair(rp, np.asarray(cols, np.int32), np.asarray(vals), ncols=ncols)
+    A.set_kernel(m._lib.SPMV_DIA, 2, 4)
+    x = rng.uniform(0.5, 1, ncols)
+    got, ref = (A * dv(m, x)).numpy(), orc.spmv(O, x)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_spmv_dia_refuses_reserved_bits(m):
+    rp, ci, va = family("lap2d_64")
+    va = np.array(va, dtype=np.float64)
+    va[5] = np.frombuffer(np.uint64(0x7ff4a5a5a5a5a5a5).tobytes(), dtype=np.float64)[0]
+    O, A = pair(rp, ci, va)
+    with pytest.raises(m.MtlError):
+        A.set_kernel(m._lib.SPMV_DIA, 4, 8)
+
+
+def test_spmv_dia_values_follow_set_values(m):
+    rp, ci, va = family("diff3d_16", seed=2)
+    O, A = pair(rp, ci, va)
+    A.set_kernel(m._lib.SPMV_DIA, 4, 8)
+    x = np.random.default_rng(3).uniform(-1, 1, A.num_cols())
+    va2 = np.asarray(va) * 1.5 - 0.25
+    A.set_values(va2)
+    O2, _ = pair(rp, ci, va2)
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O2, x))
+
+
 def test_spmv_sliced_values_follow_set_values(m):
     rp, ci, va = family("diff3d_16", seed=2)
     O, A = pair(rp, ci, va)
