@@ -501,23 +501,41 @@ struct EpiRu2 {
 
 __global__ void k_ctl_init(Ctl* c, Ctl init) { *c = init; }
 
+// zero `words` 32-bit words (a kernel, not a memset: stream-ordered like
+// the solve's own launches and safe inside a captured graph)
+__global__ void k_zero_words(unsigned* p, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+static int zero_on(void* p, size_t bytes, hipStream_t s) {
+    const size_t words = bytes / 4;
+    const int grid = (int)std::min<size_t>(4096, std::max<size_t>(1, (words + 255) / 256));
+    hipLaunchKernelGGL(k_zero_words, dim3(grid), dim3(256), 0, s, static_cast<unsigned*>(p), words);
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
 // ---------------------------------------------------------------- host driver
+// Buffers are (re)allocated here and cleared on the solve's stream s, so the
+// clearing is ordered before the solve's kernels whatever other streams do.
 template <typename VT>
 static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_pc_s* P,
-                            int32_t hist_cap) {
+                            int32_t hist_cap, hipStream_t s) {
+    static_assert(sizeof(VT) % 4 == 0 && sizeof(Ctl) % 4 == 0, "zeroed as 32-bit words");
     if (W.n != n || W.vecs.n < (size_t)n * nvec) {
         M4_TRY(W.vecs.alloc((size_t)std::max(n, 1) * nvec));
-        M4_HIP(hipMemset(W.vecs.p, 0, W.vecs.n * sizeof(VT)));
+        M4_TRY(zero_on(W.vecs.p, W.vecs.n * sizeof(VT), s));
         W.n = n;
     }
     if (!W.partials.p) M4_TRY(W.partials.alloc(kPartialWords));
     if (!W.tickets.p) {
         M4_TRY(W.tickets.alloc(kTicketWords));
-        M4_HIP(hipMemset(W.tickets.p, 0, kTicketWords * sizeof(unsigned)));
+        M4_TRY(zero_on(W.tickets.p, kTicketWords * sizeof(unsigned), s));
     }
     if (!W.ctl.p) {
         M4_TRY(W.ctl.alloc(1));
-        M4_HIP(hipMemset(W.ctl.p, 0, sizeof(Ctl)));
+        M4_TRY(zero_on(W.ctl.p, sizeof(Ctl), s));
     }
     if (!W.h_ctl) M4_HIP(hipHostMalloc((void**)&W.h_ctl, sizeof(Ctl), hipHostMallocDefault));
     if (hist_cap > 0 && W.hist.n < (size_t)hist_cap) M4_TRY(W.hist.alloc(hist_cap));
@@ -527,7 +545,7 @@ static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_p
     size_t words = P && (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) ? pc_counter_words(P) : 1;
     if (W.pc_counters.n < words) {
         M4_TRY(W.pc_counters.alloc(words));
-        M4_HIP(hipMemset(W.pc_counters.p, 0, words * sizeof(unsigned)));
+        M4_TRY(zero_on(W.pc_counters.p, words * sizeof(unsigned), s));
     }
     return MTL4G_OK;
 }
@@ -679,7 +697,7 @@ int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt
         return it->error;
     }
     const int nvec = method == MTL4G_CG ? 5 : 9;
-    M4_TRY(ensure_workspace(W, A->n, nvec, P, hist_cap));
+    M4_TRY(ensure_workspace(W, A->n, nvec, P, hist_cap, s));
     Ctl init;
     std::memset(&init, 0, sizeof init);
     init.norm_r0 = it->norm_r0;
@@ -794,7 +812,7 @@ int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, doubl
         return 0;
     }
     const int nvec = method == MTL4G_CG ? 5 : 9;
-    M4_TRY(ensure_workspace(W, A->n, nvec, P, 0));
+    M4_TRY(ensure_workspace(W, A->n, nvec, P, 0, s));
     M4_TRY(ensure_lowp_values(A, fmt, s));
     M4_HIP(hipMemsetAsync(x32, 0, (size_t)A->n * sizeof(float), s));
     Ctl init;

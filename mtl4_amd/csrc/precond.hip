@@ -205,6 +205,8 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
     }
     M4_TRY(S->counters.alloc(((size_t)nlev + 2) * kCounterStride));
     M4_HIP(hipMemset(S->counters.p, 0, ((size_t)nlev + 2) * kCounterStride * sizeof(unsigned)));
+    // complete before any stream (including non-blocking ones) uses the counters
+    M4_HIP(hipDeviceSynchronize());
     return MTL4G_OK;
 }
 
@@ -473,7 +475,8 @@ int mtl4g_pc_create(mtl4g_pc* out, mtl4g_matrix A, int kind, int fmt) {
     P->n = A->n;
     P->A = A;
     int st = P->err.alloc(1);
-    if (!st && hipMemset(P->err.p, 0, sizeof(int)) != hipSuccess) st = MTL4G_ERR_HIP;
+    if (!st && (hipMemset(P->err.p, 0, sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess))
+        st = MTL4G_ERR_HIP;   // cleared before any (non-blocking) stream reads it
     if (!st && kind == MTL4G_PC_DIAGONAL) {
         // diagonal(A): inv_diag[i] = reciprocal(A[i][i]) (diagonal.hpp:37-45)
         P->h_inv_diag.assign(A->n, 0.0);

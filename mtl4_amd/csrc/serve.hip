@@ -12,8 +12,9 @@
 //                slot's stream; launches on different slots overlap
 //   complete  -> an event per launch; its requests complete, memory returns
 //                to the arena, waiting requests can be admitted
-// Systems above big_rows rows go to a worker thread that runs the whole-GPU
-// Krylov engine on its own stream, overlapping with the batched launches.
+// Systems above big_rows rows go to `streams` worker threads that run the
+// whole-GPU Krylov engine (matrix autotuned at set-up), each on a stream of
+// its own, overlapping with each other and with the batched launches.
 // Right-hand sides are generated on the device from a per-request seed.
 #include <hip/hip_runtime.h>
 
@@ -139,7 +140,7 @@ struct mtl4g_server_s {
     // whole-GPU engine for large systems
     std::vector<mtl4g_matrix_s*> big_A;
     std::vector<mtl4g_pc_s*> big_P;
-    hipStream_t big_s = nullptr;
+    std::vector<hipStream_t> big_s;   // one per big-system worker
     ~mtl4g_server_s() {
         for (auto& slp : slots) {
             mtl4g::Slot& sl = *slp;
@@ -151,7 +152,7 @@ struct mtl4g_server_s {
         }
         for (auto* p : big_P) delete p;
         for (auto* a : big_A) delete a;
-        if (big_s) (void)hipStreamDestroy(big_s);
+        for (hipStream_t s : big_s) (void)hipStreamDestroy(s);
     }
 };
 
@@ -160,11 +161,6 @@ namespace mtl4g {
 static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-// Big-system worker: whole-GPU engine, one right-hand side after the other.
-struct BigJob {
-    Req r;
-};
 
 }  // namespace mtl4g
 
@@ -211,12 +207,17 @@ int mtl4g_server_create(mtl4g_server* out, mtl4g_batch B, const mtl4g_server_par
         mtl4g_matrix A = nullptr;
         if ((st = mtl4g_csr_create(&A, n, n, j1 - j0, rp.data(), ci.data(), B->h_val.data() + j0, 0))) return fail(st);
         S_->big_A[k] = A;
+        // kernel and low-precision copies settled here: the workers only read the matrix
+        if ((st = mtl4g_csr_autotune(A, nullptr, nullptr))) return fail(st);
+        if (prm->precision != MTL4G_F64 && (st = ensure_lowp_values(A, prm->precision, nullptr))) return fail(st);
         mtl4g_pc P = nullptr;
         const int fmt = prm->precision == MTL4G_F64 ? MTL4G_F64 : MTL4G_F32;
         if ((st = mtl4g_pc_create(&P, A, prm->pc_kind, fmt))) return fail(st);
         S_->big_P[k] = P;
     }
-    if (hipStreamCreateWithFlags(&S_->big_s, hipStreamNonBlocking)) return fail(MTL4G_ERR_HIP);
+    S_->big_s.assign(prm->streams, nullptr);
+    for (hipStream_t& bs : S_->big_s)
+        if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking)) return fail(MTL4G_ERR_HIP);
     *out = S_;
     return MTL4G_OK;
 }
@@ -262,7 +263,7 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
     for (int32_t r = 0; r < nreq; r++) order[r] = r;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return reqs[a].arrival < reqs[b].arrival; });
 
-    // big-system worker
+    // big-system workers
     std::mutex mu;
     std::condition_variable cv;
     std::deque<int32_t> big_todo;
@@ -270,11 +271,16 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
     std::atomic<bool> stop{false};
     std::atomic<int> worker_err{0};
     std::atomic<int32_t> big_errors{0}, big_solves{0};
-    Workspace<double> W64;
-    Workspace<float> W32;
-    DevBuf<float> x32;
-    std::thread worker([&]() {
+    auto worker_main = [&](hipStream_t bs) {
         (void)hipSetDevice(B->device);
+        Workspace<double> W64;
+        Workspace<float> W32;
+        DevBuf<float> x32;
+        DevBuf<GenItem> dg;
+        if (dg.alloc(1)) {
+            worker_err = MTL4G_ERR_OOM;
+            return;
+        }
         for (;;) {
             int32_t id;
             {
@@ -288,14 +294,11 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
             const int64_t pn = Arena::round(q.n);
             for (int32_t k = 0; k < q.nrhs; k++) {
                 GenItem g{q.off + k * 2 * pn, q.off + k * 2 * pn + pn, q.n, 0, q.seed * 131 + (uint64_t)k};
-                GenItem* dg = nullptr;
-                if (hipMallocAsync((void**)&dg, sizeof(GenItem), S_->big_s) ||
-                    hipMemcpyAsync(dg, &g, sizeof g, hipMemcpyHostToDevice, S_->big_s)) {
+                if (hipMemcpyAsync(dg.p, &g, sizeof g, hipMemcpyHostToDevice, bs)) {
                     worker_err = MTL4G_ERR_HIP;
                     break;
                 }
-                hipLaunchKernelGGL(k_gen_items, dim3(1), dim3(256), 0, S_->big_s, dg, 1, base);
-                (void)hipFreeAsync(dg, S_->big_s);
+                hipLaunchKernelGGL(k_gen_items, dim3(1), dim3(256), 0, bs, dg.p, 1, base);
                 double* bb = base + g.b_off;
                 double* xx = base + g.x_off;
                 mtl4g_iteration it{};
@@ -303,27 +306,29 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
                 it.rtol = prm.rtol;
                 // norm_r0 = ||b|| as basic_iteration(b, ...)
                 double nb = 0;
-                if (mtl4g_two_norm(q.n, bb, &nb, S_->big_s)) { worker_err = MTL4G_ERR_HIP; break; }
+                if (mtl4g_two_norm(q.n, bb, &nb, bs)) { worker_err = MTL4G_ERR_HIP; break; }
                 it.norm_r0 = nb;
                 int st;
                 if (prm.precision == MTL4G_F64) {
                     st = krylov_run<double>(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, MTL4G_F64, W64, xx, bb, &it,
-                                            nullptr, 0, S_->big_s);
+                                            nullptr, 0, bs);
                 } else {
                     if (x32.n < (size_t)q.n && x32.alloc(q.n)) { worker_err = MTL4G_ERR_OOM; break; }
                     int32_t rf = 0;
                     st = mixed_run(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, prm.precision, 0.1, 0, W32, x32.p, xx,
-                                   bb, &it, &rf, S_->big_s);
+                                   bb, &it, &rf, bs);
                 }
                 if (st != 0) big_errors++;
                 big_solves++;
                 if (iters) iters[iter_off[id] + k] = it.i;
             }
-            (void)hipStreamSynchronize(S_->big_s);
+            (void)hipStreamSynchronize(bs);
             std::lock_guard<std::mutex> lk(mu);
             big_done.emplace_back(id, now_s());
         }
-    });
+    };
+    std::vector<std::thread> workers;
+    for (hipStream_t bs : S_->big_s) workers.emplace_back(worker_main, bs);
 
     const double t0 = now_s();
     size_t next = 0;
@@ -462,7 +467,7 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
         stop = true;
         cv.notify_all();
     }
-    worker.join();
+    for (std::thread& w : workers) w.join();
     for (auto& slp : S_->slots) {
         if (slp->busy) (void)hipStreamSynchronize(slp->s);
         slp->busy = false;

@@ -100,3 +100,31 @@ def test_serving_admission_under_small_cap(m, setup):
     total = sum(int(nrhs[i]) * 5 * r64(int(sizes[mat[i]])) * 8 for i in range(30))
     assert total > cap
     assert st["launches"] >= int(np.ceil(total / cap))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_serving_concurrent_big_workers(m, prec):
+    """Several whole-GPU workers solving systems of different sizes at once
+    (each worker's workspace is re-sized and cleared on its own stream):
+    every request converges, with the same bits as when it runs alone."""
+    systems = [gen.laplacian3d(a, a, a - 1) for a in (14, 16, 18, 20, 22)]
+    B = m.Batch(systems)
+    rng = np.random.default_rng(9)
+    n = 40
+    mat = rng.integers(0, len(systems), n)
+    nrhs = rng.integers(1, 4, n)
+    arr = np.zeros(n)                         # all at once
+    seeds = rng.integers(1, 2 ** 62, n, dtype=np.uint64)
+    S = m.Server(B, precision=m.F32 if prec == "f32" else m.F64, rtol=1e-10, big_rows=1000,
+                 streams=4, max_items=64, cap_bytes=2e9)
+    lat, its, xs, st = S.run(mat, nrhs, arr, seeds, want_x=True)
+    assert st["errors"] == 0 and st["big_solves"] == nrhs.sum(), st
+    for r in range(n):
+        rp, ci, va = systems[mat[r]]
+        for k in range(nrhs[r]):
+            b = splitmix_uniform(int(seeds[r]) * 131 + k, len(rp) - 1)
+            res = np.linalg.norm(b - gen.csr_matvec(rp, ci, va, xs[r][k])) / np.linalg.norm(b)
+            assert res <= 1.0e-10 * 1.01, (r, k, res)
+    for r in (0, 13, 27, 39):
+        _, _, x1, _ = S.run(mat[r:r + 1], nrhs[r:r + 1], np.zeros(1), seeds[r:r + 1], want_x=True)
+        assert np.array_equal(x1[0], xs[r]), r
