@@ -175,6 +175,12 @@ class compressed2D:
         check(_L().mtl4g_csr_get_kernel(self.h, C.byref(k), C.byref(g), C.byref(r)))
         return k.value, g.value, r.value
 
+    def grid(self, value_format=F64):
+        """Workgroups of the persistent SpMV grid (0: not a persistent kernel)."""
+        g = C.c_int32()
+        check(_L().mtl4g_csr_grid(self.h, value_format, C.byref(g)))
+        return g.value
+
     def autotune(self):
         """Time the CSR kernel candidates on this matrix, keep the fastest (us)."""
         us = C.c_double()

@@ -454,6 +454,40 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     return MTL4G_OK;
 }
 
+// Workgroups of the persistent grid y = A*x launches with on these values
+// (the persistent kernels clamp the requested per-CU count to what one CU
+// holds of that instantiation); 0 for the one-block-per-tile kernels.
+template <typename AT, typename MT>
+static int assign_grid(const mtl4g_matrix_s* A) {
+    using Epi = EpiAssign<AT, 0>;
+    int pcu = A->per_cu, unr = A->sl_unroll;
+    const int li = sizeof(MT) == 4 ? 0 : 1;
+    if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) {
+        pcu = A->per_cu_lowp[li];
+        unr = A->sl_unroll_lowp[li];
+    }
+    const int units = ((A->n + kWave - 1) / kWave + 3) / 4;
+    switch (A->kernel) {
+    case MTL4G_SPMV_SLICED:
+        return unr == 4 ? persistent_grid<k_sell<256, 4, AT, MT, AT, Epi>>(A->cus, pcu, units)
+                        : persistent_grid<k_sell<256, 8, AT, MT, AT, Epi>>(A->cus, pcu, units);
+    case MTL4G_SPMV_DIA:
+        return unr == 4 ? persistent_grid<k_dia<256, 4, AT, MT, AT, Epi>>(A->cus, pcu, units)
+                        : persistent_grid<k_dia<256, 8, AT, MT, AT, Epi>>(A->cus, pcu, units);
+    case MTL4G_SPMV_STREAM:
+        if (sizeof(MT) < 8 && A->lowp_wide[li] && A->nb_wide > 0)
+            return persistent_grid<k_csrp<256, 4096, AT, MT, AT, Epi, 2>>(A->cus, pcu, A->nb_wide);
+        return A->nnz_per_block <= 1024 ? persistent_grid<k_csrp<256, 1024, AT, MT, AT, Epi>>(A->cus, pcu, A->nb)
+                                        : persistent_grid<k_csrp<256, 2048, AT, MT, AT, Epi>>(A->cus, pcu, A->nb);
+    case MTL4G_SPMV_STREAM2:
+        return A->nnz_per_block <= 1024
+                   ? persistent_grid<k_csrp2<256, 1024, 2, AT, MT, AT, Epi>>(A->cus, pcu, A->nb)
+                   : persistent_grid<k_csrp2<256, 2048, 2, AT, MT, AT, Epi>>(A->cus, pcu, A->nb);
+    default:
+        return 0;
+    }
+}
+
 int launch_spmv_f32(const mtl4g_matrix_s* A, int fmt, const float* x, float* y, int assign,
                     hipStream_t s) {
     switch (assign) {
@@ -708,7 +742,21 @@ int mtl4g_csr_create(mtl4g_matrix* out, int32_t nrows, int32_t ncols, int64_t nn
     const double avg = nrows ? (double)nnz / nrows : 0.0;
     A->gs = avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
     A->vrows = std::max(4, 2048 / A->gs);
-    A->kernel = MTL4G_SPMV_VECTOR;
+    // default kernel, chosen from the structure alone (so it is the same on
+    // every run; mtl4g_csr_autotune may refine it): the diagonal layout for
+    // matrices on few well-filled diagonals, the sliced layout where slices
+    // of 64 rows pad little, else the CSR stream kernel
+    A->sl_unroll = A->sl_unroll_lowp[0] = A->sl_unroll_lowp[1] = 8;
+    if (nnz > 0 && build_dia(A) == MTL4G_OK) {
+        A->kernel = MTL4G_SPMV_DIA;
+        A->per_cu = 4;
+    } else if (nnz > 0 && sliced_entries(A) <= nnz + nnz / 8 + kWave && build_sliced(A) == MTL4G_OK) {
+        A->kernel = MTL4G_SPMV_SLICED;
+        A->per_cu = 2;
+    } else {
+        A->kernel = MTL4G_SPMV_STREAM;
+        A->per_cu = 4;
+    }
     *out = A;
     return MTL4G_OK;
 }
@@ -827,6 +875,14 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
     A->kernel = MTL4G_SPMV_VECTOR;
     A->gs = group;
     A->vrows = rows_per_block;
+    return MTL4G_OK;
+}
+
+int mtl4g_csr_grid(mtl4g_matrix A, int value_format, int32_t* workgroups) {
+    if (!A || !workgroups || value_format < MTL4G_F64 || value_format > MTL4G_BF16) return MTL4G_ERR_INVALID;
+    *workgroups = value_format == MTL4G_F64   ? assign_grid<double, double>(A)
+                  : value_format == MTL4G_F32 ? assign_grid<float, float>(A)
+                                              : assign_grid<float, uint16_t>(A);
     return MTL4G_OK;
 }
 

@@ -167,8 +167,9 @@ def test_spmv_sliced_bitwise(m, fam, shape):
     O, A = pair(rp, ci, va)
     n = A.num_rows()
     ent = _sliced_entries(rp, n)
-    # FP32 / BF16 reference bits from the stream kernel, before the switch
+    # FP32 / BF16 reference bits from the CSR stream kernel, before the switch
     x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    A.set_kernel(m._lib.SPMV_STREAM, 4, 0)
     ref32 = {f: _spmv32(m, A, f, x) for f in (m.F32, m.BF16)}
     if ent > len(ci) * 1.5 + 64:
         with pytest.raises(m.MtlError):
@@ -220,6 +221,7 @@ def test_spmv_dia_bitwise(m, fam, shape):
     O, A = pair(rp, ci, va)
     n = A.num_rows()
     x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    A.set_kernel(m._lib.SPMV_STREAM, 4, 0)
     ref32 = {f: _spmv32(m, A, f, x) for f in (m.F32, m.BF16)}
     if not _dia_eligible(rp, ci):
         with pytest.raises(m.MtlError):
@@ -250,6 +252,25 @@ def test_spmv_dia_bitwise(m, fam, shape):
         err, _ = m.ir_solve(A, x32, dv(m, b), it, m.pc.diagonal(A, m.F32), value_format=m.F32)
         assert err == 0
         assert np.linalg.norm(b - orc.spmv(O, x32.numpy())) <= 1.01e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("fam", FAMILIES)
+def test_default_kernel_by_structure(m, fam):
+    """Without the autotuner the kernel is chosen from the structure alone:
+    diagonal layout, else sliced ELL where it pads little, else the CSR
+    stream kernel."""
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    n = len(rp) - 1
+    if _dia_eligible(rp, ci):
+        want = m._lib.SPMV_DIA
+    elif _sliced_entries(rp, n) <= len(ci) + len(ci) // 8 + 64:
+        want = m._lib.SPMV_SLICED
+    else:
+        want = m._lib.SPMV_STREAM
+    assert A.get_kernel()[0] == want
+    x = np.random.default_rng(1).uniform(-1, 1, A.num_cols())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
 
 
 def test_spmv_dia_rectangular_and_signed_zeros(m):
