@@ -57,7 +57,8 @@ def dia_bytes(rowptr, col, vb=8):
     rows = np.repeat(np.arange(len(lens), dtype=np.int64), lens)
     ndiag = len(np.unique(np.asarray(col, dtype=np.int64) - rows))
     n = len(lens)
-    return ndiag * n * vb + ndiag * 4 + n * vb + n * vb
+    padded_rows = (n + 63) // 64 * 64   # slice-major: the last slice of 64 rows is whole
+    return ndiag * padded_rows * vb + ndiag * 4 + n * vb + n * vb
 
 
 def timed_spmv(m, torch, A, x, y, reps):
@@ -97,9 +98,12 @@ def hbm_context(torch):
     return out
 
 
-def traffic_key(m, kern):
-    """Name of a CSR kernel configuration as in the PMC traffic table."""
+def traffic_key(m, kern, grid=0):
+    """Name of a CSR kernel configuration as in the PMC traffic table (the
+    persistent kernels by the workgroups per CU they actually launch)."""
     kind, group, rows = kern
+    if grid:
+        group = grid // 256
     if kind == m._lib.SPMV_STREAM:
         return "stream/%d/%d" % (rows, group)
     if kind == m._lib.SPMV_STREAM2:
@@ -380,7 +384,8 @@ def main():
     elif kern[0] == m._lib.SPMV_DIA:
         layout, by_layout = "diagonal", dia_bytes(rp, ci)
     spmv_gbs = by / t_spmv / 1e9
-    traffic = load_traffic(traffic_key(m, kern))
+    grid = A.grid()
+    traffic = load_traffic(traffic_key(m, kern, grid))
     if kern[0] == m._lib.SPMV_DIA:
         kname = "k_dia (diagonal layout, one lane per row, %d diagonals per chunk, %d workgroups/CU)" % (kern[2], kern[1])
     elif kern[0] == m._lib.SPMV_SLICED:
@@ -396,7 +401,7 @@ def main():
     else:
         kname = "k_csr (LDS tiles, %d rows/block)" % kern[2]
     roofline = {"bound": "hbm", "kernel": "%s<EpiAssign> (y = A*x, FP64, diffusion3d %d^3)" % (kname, g),
-                "config": traffic_key(m, kern),
+                "config": traffic_key(m, kern, grid), "workgroups": grid,
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": by, "layout": layout, "layout_bytes_per_launch": by_layout,

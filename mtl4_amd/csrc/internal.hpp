@@ -291,7 +291,7 @@ struct mtl4g_matrix_s {
     mtl4g::DevBuf<float> sl_val32;
     mtl4g::DevBuf<uint16_t> sl_valbf;
     int nsl = 0;
-    // diagonal layout (MTL4G_SPMV_DIA): val[d * n + r] on ndiag diagonals, absent entries marked
+    // diagonal layout (MTL4G_SPMV_DIA): ndiag values per row, slice-major, absent entries marked
     std::vector<int32_t> h_dia_off;         // column - row of each diagonal, ascending
     mtl4g::DevBuf<int32_t> dia_off;
     mtl4g::DevBuf<double> dia_val;

@@ -583,9 +583,11 @@ __global__ __launch_bounds__(BS) void k_sell(const int64_t* __restrict__ soff,
 }
 
 // Diagonal (banded) layout: a matrix whose entries lie on ndiag diagonals
-// (column - row = off[d], ascending) stores val[d * n + r] for every row r
-// and diagonal d, with a reserved NaN bit pattern where row r has no entry
-// on d.  No column indices are stored or read.  A lane sums its row over
+// (column - row = off[d], ascending) stores one value for every row r and
+// diagonal d, slice-major -- at ((r / 64) * ndiag + d) * 64 + r % 64, so a
+// wavefront's 64 rows read one contiguous block -- with a reserved NaN bit
+// pattern where row r has no entry on d (and for the rows past n of the last
+// slice).  No column indices are stored or read.  A lane sums its row over
 // the diagonals in ascending offset order -- ascending column order, as
 // every CSR kernel does -- skipping the absent entries, so the result is
 // the same bit for bit.  The walk, chunking and request order are those of
@@ -595,7 +597,6 @@ constexpr uint64_t kDiaAbsent64 = 0x7ff4a5a5a5a5a5a5ull;   // signalling NaN: ne
 constexpr uint32_t kDiaAbsent32 = 0x7fa5a5a5u;             // nor by a conversion (which quiets NaNs)
 constexpr uint16_t kDiaAbsent16 = 0x7fa5u;                 // bfloat16: quiet NaNs from rounding carry 0x0040
 constexpr int kDiaMax = 32;                                // diagonals at most
-constexpr int kDiaPad = kWave;                             // values of padding after the array
 
 __device__ __forceinline__ bool dia_absent(double v) { return (uint64_t)__double_as_longlong(v) == kDiaAbsent64; }
 __device__ __forceinline__ bool dia_absent(float v) { return __float_as_uint(v) == kDiaAbsent32; }
@@ -646,7 +647,7 @@ __global__ __launch_bounds__(BS) void k_dia(const int32_t* __restrict__ doff,
         for (int u = 0; u < UNR; u++) {
             const int d = min(k0 + u, ndiag - 1);
             ch.off[u] = __builtin_amdgcn_readfirstlane(offs[d]);
-            ch.v[u] = __builtin_nontemporal_load(val + (int64_t)d * nrows + r);   // rows past n read the padding
+            ch.v[u] = __builtin_nontemporal_load(val + ((int64_t)s * ndiag + d) * kWave + lane);
         }
     };
     AT acc = AT(0);

@@ -218,16 +218,21 @@ static std::vector<int32_t> dia_offsets(const mtl4g_matrix_s* A) {
     return off;
 }
 
-// out[d * n + r] = conv(src[j]) for the entry j of row r on diagonal d, absent elsewhere
+// Slice-major diagonal layout: the entry of row r on diagonal d at
+// ((r / 64) * ndiag + d) * 64 + r % 64 (a wavefront's rows read one
+// contiguous block per slice); absent entries and the rows past n of the
+// last slice hold the reserved pattern.
 template <typename T, typename F>
 static void dia_fill(const mtl4g_matrix_s* A, T absent, F conv, std::vector<T>& out) {
     const int nd = (int)A->h_dia_off.size();
-    out.assign((size_t)nd * A->n + kDiaPad, absent);
+    const size_t nsl = ((size_t)A->n + kWave - 1) / kWave;
+    out.assign(nsl * nd * kWave, absent);
     for (int32_t r = 0; r < A->n; r++) {
         int d = 0;
+        const size_t base = (size_t)(r / kWave) * nd * kWave + r % kWave;
         for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++) {
             while (A->h_dia_off[d] != A->h_col[j] - r) d++;   // columns ascend, so do the diagonals
-            out[(size_t)d * A->n + r] = conv(A->h_val[j]);
+            out[base + (size_t)d * kWave] = conv(A->h_val[j]);
         }
     }
 }
