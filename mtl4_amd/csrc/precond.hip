@@ -43,6 +43,28 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
     __syncthreads();
     const int b = s_b;
     const int L = S.blk_level[b];
+    // The row's own data (position, bounds, right-hand side, diagonal and its
+    // first eight entries) does not depend on earlier levels: request it
+    // before waiting, so only the reads of earlier results follow the wait.
+    const int k = S.lev_ptr[L] + (b - S.lev_blk[L]) * BS + (int)threadIdx.x;
+    const bool mine = k < S.lev_ptr[L + 1];
+    int r = 0, a = 0, e = 0;
+    VT rr = VT(0), dk = VT(1);
+    int c[8];
+    VT m[8];
+    if (mine) {
+        r = S.perm[k];
+        a = S.rowptr[k];
+        e = S.rowptr[k + 1];
+        rr = v[r];
+        if (!S.unit) dk = static_cast<VT>(S.dia[k]);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const bool in = a + u < e;
+            c[u] = in ? S.col[a + u] : 0;
+            m[u] = in ? static_cast<VT>(S.val[a + u]) : VT(0);
+        }
+    }
     if (threadIdx.x == 0) {
         int ok = 1;
         if (L > 0) {
@@ -61,33 +83,27 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
     // Values of earlier levels are read with agent-scope loads and results
     // are published with agent-scope atomics, so no L2 write-back fences are
     // needed between workgroups of this kernel.
-    if (s_ok) {
-        const int k = S.lev_ptr[L] + (b - S.lev_blk[L]) * BS + (int)threadIdx.x;
-        if (k < S.lev_ptr[L + 1]) {
-            const int r = S.perm[k];
-            VT rr = v[r];
-            // eight entries at a time: indices and values first, then the
-            // eight solution values they name (one round trip each), then
-            // the subtractions in the row's column order
-            const int a = S.rowptr[k], e = S.rowptr[k + 1];
-            for (int j0 = a; j0 < e; j0 += 8) {
-                int c[8];
-                VT m[8], wv[8];
+    if (s_ok && mine) {
+        // eight entries at a time: the eight solution values they name (one
+        // round trip), then the subtractions in the row's column order
+        for (int j0 = a; j0 < e; j0 += 8) {
+            if (j0 > a) {
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     const bool in = j0 + u < e;
                     c[u] = in ? S.col[j0 + u] : 0;
                     m[u] = in ? static_cast<VT>(S.val[j0 + u]) : VT(0);
                 }
-#pragma unroll
-                for (int u = 0; u < 8; u++) wv[u] = j0 + u < e ? ld_agent(w + c[u]) : VT(0);
-#pragma unroll
-                for (int u = 0; u < 8; u++)
-                    if (j0 + u < e) rr -= m[u] * wv[u];
             }
-            if (!S.unit) rr = rr * static_cast<VT>(S.dia[k]);
-            st_agent(w + r, rr);   // the L1 writes through; wait_memory() below waits for it
+            VT wv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) wv[u] = j0 + u < e ? ld_agent(w + c[u]) : VT(0);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (j0 + u < e) rr -= m[u] * wv[u];
         }
+        if (!S.unit) rr = rr * dk;
+        st_agent(w + r, rr);   // the L1 writes through; wait_memory() below waits for it
     }
     wait_memory();
     __syncthreads();
