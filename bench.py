@@ -190,8 +190,8 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     t64 = time.perf_counter() - t0
     log("  FP64 solve: %d iterations, %.3f s" % (it.iterations(), t64))
     xm = m.dense_vector(n)
-    # the FP32 recurrence may stagnate: at most 3x the FP64 iteration count
-    itm = m.basic_iteration(bnorm, 3 * max(it.iterations(), 10), 1e-10)
+    # the FP32 recurrence may stagnate or diverge: at most 2x the FP64 iteration count
+    itm = m.basic_iteration(bnorm, 2 * max(it.iterations(), 10), 1e-10)
     t0 = time.perf_counter()
     failure = None
     try:
@@ -292,7 +292,7 @@ def serving_section(m, torch, gen, nreq):
     """Continuous batching over 512 resident matrices (the batched families
     plus 3D stencils up to 96^3), Poisson arrivals, 1-8 right-hand sides per
     request, mixed precision to 1e-10, against plain FP64 per-request
-    launches on one stream."""
+    launches on one stream (on the first quarter of each trace)."""
     log("serving: generating 512 matrices")
     systems = batch_systems(gen, 480, seed=1)
     rng = np.random.default_rng(3)
@@ -309,6 +309,10 @@ def serving_section(m, torch, gen, nreq):
                         np.cumsum(rg.exponential(1.0 / rate, nreq)),
                         rg.integers(1, 2 ** 62, nreq, dtype=np.uint64))
         out["%dk_req_s" % int(rate / 1000)] = {}
+    # the plain baseline replays the first quarter of each trace (it is
+    # saturated at every rate; a shorter queue only flatters its latencies)
+    nbase = max(1, nreq // 4)
+    out["baseline_requests_per_rate"] = nbase
     for name, kw in (("mixed", dict(precision=m.F32, streams=4, max_items=1024)),
                      ("fp64_per_request", dict(precision=m.F64, streams=1, max_items=8,
                                                 per_request=True))):
@@ -316,6 +320,8 @@ def serving_section(m, torch, gen, nreq):
         for rate in rates:
             log("serving: %s at %d req/s" % (name, rate))
             mat, nrhs, arr, seeds = traces[rate]
+            if name == "fp64_per_request":
+                mat, nrhs, arr, seeds = mat[:nbase], nrhs[:nbase], arr[:nbase], seeds[:nbase]
             lat, its, _, st = Sv.run(mat, nrhs, arr, seeds)
             out["%dk_req_s" % int(rate / 1000)][name] = {
                 "p50_ms": round(1000 * float(np.percentile(lat, 50)), 3),
