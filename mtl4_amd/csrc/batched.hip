@@ -51,13 +51,29 @@ __device__ __forceinline__ void block_sum2(T& a, T& b, T* sh) {
     b = rb;
 }
 
-// one row of the product, column order (mat_vec_mult.hpp:656-682)
+// one row of the product, column order (mat_vec_mult.hpp:656-682); the
+// indices and values of eight entries are requested together, then their
+// eight gathers, so a row costs two memory round trips per eight entries
 template <typename AT, typename MT, typename VT>
 __device__ __forceinline__ AT brow(const BatchDev& M, const MT* __restrict__ val, int grow, int r0,
                                    const VT* __restrict__ v) {
     AT s = 0;
-    const int j1 = M.rowptr[grow + 1];
-    for (int j = M.rowptr[grow]; j < j1; j++) s += load_val<AT>(val, j) * static_cast<AT>(v[M.col[j] - r0]);
+    const int j0 = M.rowptr[grow], j1 = M.rowptr[grow + 1];
+    for (int j = j0; j < j1; j += 8) {
+        int c[8];
+        AT m[8], g[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const bool in = j + u < j1;
+            c[u] = in ? M.col[j + u] - r0 : 0;
+            m[u] = in ? load_val<AT>(val, j + u) : AT(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) g[u] = j + u < j1 ? static_cast<AT>(v[c[u]]) : AT(0);
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (j + u < j1) s += m[u] * g[u];
+    }
     return s;
 }
 
