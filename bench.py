@@ -291,8 +291,10 @@ def batched_section(m, torch, gen, count):
 def serving_section(m, torch, gen, nreq):
     """Continuous batching over 512 resident matrices (the batched families
     plus 3D stencils up to 96^3), Poisson arrivals, 1-8 right-hand sides per
-    request, mixed precision to 1e-10, against plain FP64 per-request
-    launches on one stream (on the first quarter of each trace)."""
+    request, mixed precision to 1e-10 (systems above 16384 rows on 8
+    whole-GPU workers, the rest continuously batched on 8 streams), against
+    plain FP64 per-request launches on one stream (on the first quarter of
+    each trace)."""
     log("serving: generating 512 matrices")
     systems = batch_systems(gen, 480, seed=1)
     rng = np.random.default_rng(3)
@@ -313,10 +315,10 @@ def serving_section(m, torch, gen, nreq):
     # saturated at every rate; a shorter queue only flatters its latencies)
     nbase = max(1, nreq // 4)
     out["baseline_requests_per_rate"] = nbase
-    for name, kw in (("mixed", dict(precision=m.F32, streams=4, max_items=1024)),
+    for name, kw in (("mixed", dict(precision=m.F32, streams=8, max_items=1024)),
                      ("fp64_per_request", dict(precision=m.F64, streams=1, max_items=8,
                                                 per_request=True))):
-        Sv = m.Server(B, rtol=1e-10, big_rows=65536, cap_bytes=200e9, **kw)
+        Sv = m.Server(B, rtol=1e-10, big_rows=16384, cap_bytes=200e9, **kw)
         for rate in rates:
             log("serving: %s at %d req/s" % (name, rate))
             mat, nrhs, arr, seeds = traces[rate]
