@@ -68,6 +68,7 @@ extern "C" {
 /* Krylov methods */
 #define MTL4G_CG       0   /* itl::cg,       itl/krylov/cg.hpp:38-111 */
 #define MTL4G_BICGSTAB 1   /* itl::bicgstab, itl/krylov/bicgstab.hpp:27-78 */
+#define MTL4G_CGS      2   /* itl::cgs,      itl/krylov/cgs.hpp:24-62 (FP64 only) */
 
 typedef struct mtl4g_matrix_s* mtl4g_matrix;
 typedef struct mtl4g_pc_s* mtl4g_pc;
@@ -171,6 +172,11 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);
+/* Workgroups of the persistent grid that y = A*x on these values
+ * (MTL4G_F64 / F32 / BF16) launches with: `group` per CU clamped to what one
+ * CU holds of the chosen kernel; 0 for the non-persistent kernels.  No MTL4
+ * counterpart (a launch-shape diagnostic). */
+int mtl4g_csr_grid(mtl4g_matrix A, int value_format, int32_t* workgroups);
 /* Autotuner: times y = A*x for the candidate kernels and shapes on this
  * matrix and keeps the fastest; h_best_us (may be NULL) gets its time. */
 int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream);
@@ -230,6 +236,10 @@ int mtl4g_cg(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
              mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
 int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
                    mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
+/* itl::cgs(A, x, b, P, iter) (cgs.hpp:24-62): same conventions; rho_1 == 0
+ * sets error code 2 and the loop goes on, as MTL4's iter.fail(2) does. */
+int mtl4g_cgs(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
+              mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
 
 /* Reusable solver: the workspace (MTL4 allocates it per call, cg.hpp:90)
  * stays with the handle, so repeated solves allocate nothing. */

@@ -23,7 +23,7 @@ from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, ORDER_MTL4, ORDE
                    missing_diagonal, nonfinite_error, unexpected_orthogonality)
 
 __all__ = ["compressed2D", "dense_vector", "pc", "solve", "adjoint_solve", "basic_iteration",
-           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "dot", "unary_dot",
+           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "dot", "unary_dot",
            "two_norm", "num_rows", "num_cols", "size", "laplacian_setup", "ir_solve",
            "MtlError", "matrix_not_square", "incompatible_size", "missing_diagonal",
            "unexpected_orthogonality", "nonfinite_error", "logic_error"]
@@ -402,7 +402,7 @@ class cyclic_iteration(basic_iteration):
         for k in range(len(self.history)):
             if method == _lib.BICGSTAB and k > 0 and k % 2 == 1:
                 i += 1
-            elif method == _lib.CG and k > 0:
+            elif method in (_lib.CG, _lib.CGS) and k > 0:
                 i += 1
             counts.append(i)
         last = -1
@@ -445,7 +445,7 @@ def _krylov(fn, method, A, x, b, P, it):
     h = hist[:cap]
     h = h[~np.isnan(h)] if cap else None
     it._update(st, h, method)
-    check(status, "cg" if method == CG else "bicgstab")
+    check(status, {CG: "cg", BICGSTAB: "bicgstab"}.get(method, "cgs"))
     return it.error_code()
 
 
@@ -461,6 +461,13 @@ def bicgstab(A, x, b, P=None, it=None):
     if isinstance(P, basic_iteration):
         P, it = None, P
     return _krylov(_L().mtl4g_bicgstab, BICGSTAB, A, x, b, P, it)
+
+
+def cgs(A, x, b, P=None, it=None):
+    """itl::cgs(A, x, b, P, iter) (itl/krylov/cgs.hpp:24-62)."""
+    if isinstance(P, basic_iteration):
+        P, it = None, P
+    return _krylov(_L().mtl4g_cgs, _lib.CGS, A, x, b, P, it)
 
 
 class Solver:

@@ -451,6 +451,131 @@ struct EpiBiXR {
     }
 };
 
+// ---------------------------------------------------------------- CGS epilogues
+// itl::cgs, itl/krylov/cgs.hpp:26-62.  Vectors p, phat, q, vhat, u, uhat,
+// r, rtilde; r = b - A*x, rtilde = r and the loop head finished(r) are
+// EpiBiInit's.
+
+// rho_1 = dot(rtilde, r); rho_1 == 0 is iter.fail(2) -- the error code
+// is set and the loop goes on, as in MTL4; beta = rho_1/rho_2  (cgs.hpp:35-42)
+template <typename VT>
+struct EpiCgsRho {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
+    const VT *rt, *r;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, VT* part) const { part[0] += rt[i] * r[i]; }
+    __device__ void finish(const VT* tot) const {
+        const VT rho_1 = tot[0];
+        c->rho_1 = (double)rho_1;
+        if (rho_1 == VT(0)) c->error = 2;
+        if (!c->first) c->beta = (double)(rho_1 / static_cast<VT>(c->rho_2));
+    }
+};
+
+// p = u = r, or u = r + beta*q; p = u + beta*(q + beta*p)  (cgs.hpp:37-43)
+template <typename VT>
+struct EpiCgsUP {
+    static constexpr int NP = 0;
+    const VT *r, *q;
+    VT *u, *p;
+    Ctl* c;
+    VT beta;
+    int first;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        beta = static_cast<VT>(c->beta);
+        first = c->first;
+    }
+    __device__ void operator()(int64_t i, VT*) const {
+        if (first) {
+            u[i] = r[i];
+            p[i] = r[i];
+        } else {
+            const VT qi = q[i];
+            const VT ui = r[i] + beta * qi;
+            u[i] = ui;
+            p[i] = ui + beta * (qi + beta * p[i]);
+        }
+    }
+    __device__ void finish(const VT*) const {}
+};
+
+// vhat = A*phat; alpha = rho_1 / dot(rtilde, vhat)  (cgs.hpp:45-46; no
+// breakdown test there: a zero dot yields a non-finite alpha)
+template <typename VT>
+struct EpiCgsV {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
+    struct Pre {
+        VT rt;
+    };
+    const VT* rt;
+    VT* v;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ Pre preload(int row) const { return Pre{rt[row]}; }
+    __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
+        v[row] = s;
+        part[0] += pre.rt * s;
+    }
+    __device__ void finish(const VT* tot) const {
+        c->gamma = (double)tot[0];
+        c->alpha = (double)(static_cast<VT>(c->rho_1) / tot[0]);
+    }
+};
+
+// q = u - alpha*vhat; u += q  (cgs.hpp:47-49)
+template <typename VT>
+struct EpiCgsQU {
+    static constexpr int NP = 0;
+    const VT* v;
+    VT *q, *u;
+    Ctl* c;
+    VT alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
+    __device__ void operator()(int64_t i, VT*) const {
+        const VT ui = u[i];
+        const VT qi = ui - alpha * v[i];
+        q[i] = qi;
+        u[i] = ui + qi;
+    }
+    __device__ void finish(const VT*) const {}
+};
+
+// x += alpha*uhat; qhat = A*uhat; r -= alpha*qhat; rho_2 = rho_1; loop
+// head finished(r)  (cgs.hpp:52-57, 32)
+template <typename VT>
+struct EpiCgsXR {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_NORM4; }
+    struct Pre {
+        VT x, uh, r;
+    };
+    VT *x, *r;
+    const VT* uh;
+    Ctl* c;
+    double* hist;
+    VT alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { alpha = static_cast<VT>(c->alpha); }
+    __device__ Pre preload(int row) const { return Pre{x[row], uh[row], r[row]}; }
+    __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
+        x[row] = pre.x + alpha * pre.uh;
+        const VT v = pre.r - alpha * s;
+        r[row] = v;
+        part[0] += v * v;
+    }
+    __device__ void finish(const VT* tot) const {
+        c->rho_2 = c->rho_1;
+        iter_finished(c, (double)std::sqrt(tot[0]), hist);
+    }
+};
+
 // ---------------------------------------------------------------- FP64 refresh
 // x += double(x32); x32 = 0
 struct EpiRu1 {
@@ -626,6 +751,32 @@ static int bicg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc
 }
 
 template <typename VT, typename MT>
+static int cgs_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P,
+                         Workspace<VT>& W, VT* x, hipStream_t s) {
+    const int64_t n = A->n;
+    VT* base = W.vecs.p;
+    VT *p = base, *phat = base + n, *q = base + 2 * n, *vhat = base + 3 * n, *u = base + 4 * n,
+       *uhat = base + 5 * n, *r = base + 6 * n, *rt = base + 7 * n, *y0 = base + 8 * n;
+    const bool ident = !P || P->kind == MTL4G_PC_IDENTITY;
+    if (ident) {   // solve(identity, .) is a copy: use p and u directly
+        phat = p;
+        uhat = u;
+    }
+    Ctl* c = W.ctl.p;
+    VT* part = reinterpret_cast<VT*>(W.partials.p);
+    VT* tm = reinterpret_cast<VT*>(W.terms.p);
+    unsigned* tk = W.tickets.p;
+    M4_TRY(launch_vec<VT>(n, EpiCgsRho<VT>{rt, r, c}, part, tk, s, tm));
+    M4_TRY(launch_vec<VT>(n, EpiCgsUP<VT>{r, q, u, p, c, VT(0), 0}, part, tk, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s));
+    M4_TRY(launch_csr<VT>(A, val, phat, EpiCgsV<VT>{rt, vhat, c}, part, tk, s, tm));
+    M4_TRY(launch_vec<VT>(n, EpiCgsQU<VT>{vhat, q, u, c, VT(0)}, part, tk, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, u, uhat, y0, &c->done, W.pc_counters.p, s));
+    M4_TRY(launch_csr<VT>(A, val, uhat, EpiCgsXR<VT>{x, r, uhat, c, W.hist.p, VT(0)}, part, tk, s, tm));
+    return MTL4G_OK;
+}
+
+template <typename VT, typename MT>
 static int run_typed(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P, int method,
                      Workspace<VT>& W, VT* x, const VT* b, hipStream_t s, int32_t budget) {
     const int pcm = (!P || P->kind == MTL4G_PC_IDENTITY) ? PCM_NONE
@@ -639,6 +790,7 @@ static int run_typed(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P
     }
     auto iteration = [&]() -> int {
         if (method == MTL4G_BICGSTAB) return bicg_iteration<VT, MT>(A, val, P, W, x, s);
+        if (method == MTL4G_CGS) return cgs_iteration<VT, MT>(A, val, P, W, x, s);
         if (pcm == PCM_NONE) return cg_iteration<VT, MT, VT, PCM_NONE>(A, val, P, d, W, x, s);
         if (pcm == PCM_DIAG) {
             if (d) return cg_iteration<VT, MT, VT, PCM_DIAG>(A, val, P, d, W, x, s);
@@ -648,7 +800,7 @@ static int run_typed(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P
         }
         return cg_iteration<VT, MT, VT, PCM_GENERAL>(A, val, P, d, W, x, s);
     };
-    if (method == MTL4G_BICGSTAB) {
+    if (method == MTL4G_BICGSTAB || method == MTL4G_CGS) {   // same r, rtilde slots and loop head
         M4_TRY((bicg_init<VT, MT>(A, val, W, x, b, s)));
     } else if (pcm == PCM_NONE) {
         M4_TRY((cg_init<VT, MT, VT, PCM_NONE>(A, val, d, W, x, b, s)));
@@ -689,7 +841,7 @@ int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt
         set_error("preconditioner size %d != matrix size %d", P->n, A->n);
         return MTL4G_ERR_SIZE;
     }
-    if (method != MTL4G_CG && method != MTL4G_BICGSTAB) return MTL4G_ERR_INVALID;
+    if (method != MTL4G_CG && method != MTL4G_BICGSTAB && method != MTL4G_CGS) return MTL4G_ERR_INVALID;
     if (A->n == 0) {   // r is empty: finished(0) holds at once (basic_iteration.hpp:85-93)
         it->resid = 0.0;
         it->error = (it->norm_r0 == 0 || 0.0 <= it->rtol * it->norm_r0) ? 0 : 1;
@@ -804,6 +956,10 @@ int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, doubl
               int32_t max_refresh, Workspace<float>& W, float* x32, double* x, const double* b,
               mtl4g_iteration* it, int32_t* refreshes, hipStream_t s) {
     (void)max_refresh;
+    if (method != MTL4G_CG && method != MTL4G_BICGSTAB) {
+        set_error("mixed precision: cg and bicgstab only");
+        return MTL4G_ERR_UNSUPPORTED;
+    }
     if (A->n != A->m) return MTL4G_ERR_NOT_SQUARE;
     if (P && P->n != A->n) return MTL4G_ERR_SIZE;
     if (A->n == 0) {
@@ -859,7 +1015,8 @@ using namespace mtl4g;
 extern "C" {
 
 int mtl4g_solver_create(mtl4g_solver* out, mtl4g_matrix A, mtl4g_pc P, int method) {
-    if (!out || !A || (method != MTL4G_CG && method != MTL4G_BICGSTAB)) return MTL4G_ERR_INVALID;
+    if (!out || !A || (method != MTL4G_CG && method != MTL4G_BICGSTAB && method != MTL4G_CGS))
+        return MTL4G_ERR_INVALID;
     mtl4g_solver_s* S_ = new (std::nothrow) mtl4g_solver_s;
     if (!S_) return MTL4G_ERR_OOM;
     S_->A = A;
@@ -885,6 +1042,12 @@ int mtl4g_cg(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_itera
              double* h_hist, int32_t hist_cap, void* stream) {
     Workspace<double> W;
     return krylov_run<double>(A, P, MTL4G_CG, MTL4G_F64, W, x, b, it, h_hist, hist_cap, S(stream));
+}
+
+int mtl4g_cgs(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,
+              double* h_hist, int32_t hist_cap, void* stream) {
+    Workspace<double> W;
+    return krylov_run<double>(A, P, MTL4G_CGS, MTL4G_F64, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,

@@ -1069,7 +1069,7 @@ class cyclic_iteration : public basic_iteration<Real> {
     void replay_(const std::vector<double>& hist, int method, int i0) override {
         int i = i0;
         for (std::size_t k = 0; k < hist.size(); k++) {
-            if (k > 0 && (method == MTL4G_CG || k % 2 == 1)) i++;
+            if (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)) i++;   // cg, cgs: one finished() per iteration
             this->i = i;
             this->resid_ = hist[k];
             print_resid();
@@ -1263,8 +1263,10 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
     mtl4g_matrix h = A.device_();
     const double* db = b.device_read_();
     double* dx = x.device_readwrite_();
-    int st = method == MTL4G_CG ? mtl4g_cg(h, P, dx, db, &s, cap ? hist.data() : nullptr, cap, nullptr)
-                                : mtl4g_bicgstab(h, P, dx, db, &s, cap ? hist.data() : nullptr, cap, nullptr);
+    double* hp = cap ? hist.data() : nullptr;
+    int st = method == MTL4G_CG    ? mtl4g_cg(h, P, dx, db, &s, hp, cap, nullptr)
+             : method == MTL4G_CGS ? mtl4g_cgs(h, P, dx, db, &s, hp, cap, nullptr)
+                                   : mtl4g_bicgstab(h, P, dx, db, &s, hp, cap, nullptr);
     std::vector<double> used;
     for (double v : hist)
         if (v == v) used.push_back(v);
@@ -1306,8 +1308,15 @@ int bicgstab(const LinearOperator& A, HilbertSpaceX& x, const HilbertSpaceB& b, 
     return iter;
 }
 
-// itl::base_solver, cg_solver, bicgstab_solver (krylov/base_solver.hpp:24-84,
-// cg.hpp:122-156, bicgstab.hpp:80-112)
+// itl::cgs (krylov/cgs.hpp:24-62)
+template <typename LinearOperator, typename Vector, typename Preconditioner, typename Iteration>
+int cgs(const LinearOperator& A, Vector& x, const Vector& b, const Preconditioner& M, Iteration& iter) {
+    detail::run(MTL4G_CGS, A, x, b, pc::is_identity(M) ? nullptr : detail::pc_handle(M), iter);
+    return iter;
+}
+
+// itl::base_solver, cg_solver, bicgstab_solver, cgs_solver (krylov/base_solver.hpp:24-84,
+// cg.hpp:122-156, bicgstab.hpp:80-112, cgs.hpp:64-95)
 template <typename Solver, typename LinearOperator>
 struct base_solver {
     typedef double magnitude_type;
@@ -1380,6 +1389,29 @@ class bicgstab_solver
     }
 
   private:
+    Preconditioner L;
+};
+
+// right preconditioner ignored, with a warning (cgs.hpp:64-95)
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class cgs_solver
+    : public base_solver<cgs_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<cgs_solver, LinearOperator> base;
+
+  public:
+    explicit cgs_solver(const LinearOperator& A) : base(A), L(A) { warn(); }
+    cgs_solver(const LinearOperator& A, const Preconditioner& L) : base(A), L(L) { warn(); }
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const {
+        return cgs(this->A, x, b, L, iter);
+    }
+
+  private:
+    static void warn() {
+        if (!std::is_same<RightPreconditioner, pc::identity<LinearOperator>>::value)
+            std::cerr << "Right Preconditioner ignored!" << std::endl;
+    }
     Preconditioner L;
 };
 

@@ -630,3 +630,53 @@ int orc_bicgstab(const orc_csr* A, const orc_pc* P, double* x, const double* b,
     free(w);
     return st != ORC_OK ? st : it->error;
 }
+
+/* itl::cgs, itl/krylov/cgs.hpp:24-62.  rho_1 == 0 is iter.fail(2, ...)
+ * (basic_iteration.hpp:129-136): the error code is set and the loop goes
+ * on; there is no test on dot(rtilde, vhat).                           */
+int orc_cgs(const orc_csr* A, const orc_pc* P, double* x, const double* b,
+            orc_iter* it, double* hist, int32_t hist_cap)
+{
+    int32_t n = A->n, nh = 0;
+    if (A->n != A->ncols) return ORC_NOT_SQUARE;
+    size_t sn = (size_t)(n ? n : 1);
+    double* w = (double*)malloc(sn * 9 * sizeof(double));
+    if (!w) return ORC_OOM;
+    double *p = w, *phat = w + sn, *q = w + 2 * sn, *qhat = w + 3 * sn, *vhat = w + 4 * sn,
+           *u = w + 5 * sn, *uhat = w + 6 * sn, *r = w + 7 * sn, *rtilde = w + 8 * sn;
+    double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
+    int st = ORC_OK;
+    it->error = 0;
+    memset(p, 0, sn * sizeof(double));
+    memset(q, 0, sn * sizeof(double));
+
+    residual(A, x, b, r, qhat);                          /* r(b - A * x), rtilde = r */
+    memcpy(rtilde, r, (size_t)n * sizeof(double));
+    while (!finished(it, orc_two_norm(n, r), hist, hist_cap, &nh)) {
+        ++it->i;
+        rho_1 = orc_dot(n, rtilde, r, 4);
+        if (rho_1 == 0.0) it->error = 2;                 /* iter.fail(2, "cgs breakdown") */
+        if (it->i <= 1) {
+            memcpy(u, r, (size_t)n * sizeof(double));
+            memcpy(p, u, (size_t)n * sizeof(double));
+        } else {
+            beta = rho_1 / rho_2;
+            for (int32_t i = 0; i < n; i++) u[i] = r[i] + beta * q[i];
+            for (int32_t i = 0; i < n; i++) p[i] = u[i] + beta * (q[i] + beta * p[i]);
+        }
+        if (P) { st = orc_pc_solve(P, p, phat); if (st) break; }
+        else memcpy(phat, p, (size_t)n * sizeof(double));
+        orc_spmv(A, phat, vhat, ORC_ASSIGN);
+        alpha = rho_1 / orc_dot(n, rtilde, vhat, 4);
+        for (int32_t i = 0; i < n; i++) q[i] = u[i] - alpha * vhat[i];
+        for (int32_t i = 0; i < n; i++) u[i] += q[i];
+        if (P) { st = orc_pc_solve(P, u, uhat); if (st) break; }
+        else memcpy(uhat, u, (size_t)n * sizeof(double));
+        for (int32_t i = 0; i < n; i++) x[i] += alpha * uhat[i];
+        orc_spmv(A, uhat, qhat, ORC_ASSIGN);
+        for (int32_t i = 0; i < n; i++) r[i] -= alpha * qhat[i];
+        rho_2 = rho_1;
+    }
+    free(w);
+    return st != ORC_OK ? st : it->error;
+}

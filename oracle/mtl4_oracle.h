@@ -82,6 +82,8 @@ void orc_upper_invdiag_first(const orc_csr* U, const double* v, double* w); /* v
  * receives the residual passed to each finished() call, hist_cap entries. */
 int orc_cg(const orc_csr* A, const orc_pc* P, double* x, const double* b,
            orc_iter* it, double* hist, int32_t hist_cap);
+int orc_cgs(const orc_csr* A, const orc_pc* P, double* x, const double* b,
+            orc_iter* it, double* hist, int32_t hist_cap);
 int orc_bicgstab(const orc_csr* A, const orc_pc* P, double* x, const double* b,
                  orc_iter* it, double* hist, int32_t hist_cap);
 

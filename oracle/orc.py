@@ -83,6 +83,8 @@ def lib():
         L.orc_cg.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
         L.orc_bicgstab.restype = C.c_int
         L.orc_bicgstab.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
+        L.orc_cgs.restype = C.c_int
+        L.orc_cgs.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
         _lib = L
     return _lib
 
@@ -241,4 +243,11 @@ def bicgstab(A, b, P=None, x0=None, it=None, hist_cap=0, **kw):
     """itl::bicgstab(A, x, b, P, iter), itl/krylov/bicgstab.hpp:27-78."""
     it = iteration(b, **kw) if it is None else it
     x, st, hist = _solve(lib().orc_bicgstab, A, b, P, x0, it, hist_cap)
+    return x, st, it, hist
+
+
+def cgs(A, b, P=None, x0=None, it=None, hist_cap=0, **kw):
+    """itl::cgs(A, x, b, P, iter), itl/krylov/cgs.hpp:24-62."""
+    it = iteration(b, **kw) if it is None else it
+    x, st, hist = _solve(lib().orc_cgs, A, b, P, x0, it, hist_cap)
     return x, st, it, hist

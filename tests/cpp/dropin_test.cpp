@@ -144,6 +144,29 @@ int main() {
         CHECK(two_norm(r) <= 1e-6 * two_norm(b));
     });
 
+    // libs/numeric/itl/test/ic_0_cgs_test.cpp: 10x10 laplacian, ic_0, cgs
+    run("ic_0_cgs_test", [] {
+        const int size = 10, N = size * size;
+        matrix_type A(N, N);
+        laplacian_setup(A, size, size);
+        itl::pc::ic_0<matrix_type> P(A);
+        vector_type x(N, 1.0), b(N);
+        b = A * x;
+        x = 0;
+        std::ostringstream log;
+        itl::cyclic_iteration<double, std::ostringstream> iter(b, N, 1.e-6, 0.0, 5, log);
+        cgs(A, x, b, P, iter);
+        CHECK(iter.is_converged());
+        vector_type r(b);
+        r -= A * x;
+        CHECK(two_norm(r) <= 1e-6 * two_norm(b));
+        itl::cgs_solver<matrix_type, itl::pc::ic_0<matrix_type>> S(A, P);
+        vector_type y(N, 0.0);
+        itl::basic_iteration<double> it2(b, N, 1.e-6);
+        CHECK(S.solve(y, b, it2) == 0);
+        CHECK(it2.iterations() == iter.iterations());
+    });
+
     // libs/numeric/itl/test/ilu_0_cg_test.cpp:20-33
     run("ilu_0_cg_test", [] {
         const int size = 3, N = size * size;

@@ -138,6 +138,20 @@ def test_bicgstab_bitwise(m, mtl4_order, fam, kind):
     assert np.array_equal(x, xo)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ilu_0", "ic_0"])
+def test_cgs_bitwise(m, mtl4_order, fam, kind):
+    """itl::cgs in MTL4's reduction order: iteration count, final residual
+    and solution bit for bit with the oracle's restatement of cgs.hpp."""
+    if kind == "ic_0" and fam != "lap2d_64":
+        pytest.skip("ic_0 needs a symmetric matrix")
+    (err, it, x), (st, ito, xo) = _solve_both(m, "cgs", fam, kind, seed=5, max_iter=2000)
+    assert err == st == 0
+    assert it.iterations() == ito.i
+    assert it.resid() == ito.resid
+    assert np.array_equal(x, xo)
+
+
 def test_cg_config1_bitwise(m, mtl4_order):
     """CG + diagonal on the 64x64 Laplacian, the reference's CPU test case."""
     rp, ci, va = family("lap2d_64")

@@ -600,6 +600,50 @@ def test_bicgstab_families(m, fam, kind):
     assert res <= 1e-9 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000", "diff3d_16"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ilu_0"])
+def test_cgs_families(m, fam, kind):
+    """itl::cgs in the default (tree) reduction order against the oracle:
+    CGS squares the BiCG polynomial, so its residual is even less monotone
+    than BiCGSTAB's; bound 2 + 15 % on the iteration count, the solution to
+    within ten times the oracle's own error, and the true residual."""
+    rp, ci, va = family(fam, seed=4)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(3).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OP = None if kind == "identity" else orc.Preconditioner(kind, O)
+    xo, st, ito, _ = orc.cgs(O, b, OP, max_iter=3000, rtol=1e-10)
+    P = getattr(m.pc, kind)(A)
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+    err = m.cgs(A, x, dv(m, b), P, it)
+    assert err == st == 0
+    assert abs(it.iterations() - ito.i) <= 2 + 0.15 * ito.i, (it.iterations(), ito.i)
+    own = max(np.linalg.norm(xo - xt), 1e-12 * np.linalg.norm(xt))
+    assert np.linalg.norm(x.numpy() - xo) <= 10 * own
+    res = np.linalg.norm(b - orc.spmv(O, x.numpy()))
+    assert res <= 1e-9 * np.linalg.norm(b)
+
+
+def test_cgs_solver_handle_and_mixed_refusal(m):
+    rp, ci, va = family("convdiff_40", seed=1)
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.ones(A.num_rows()))
+    S = m.Solver(A, m.pc.ilu_0(A), method=m._lib.CGS)
+    out = []
+    for _ in range(2):
+        x = m.dense_vector(A.num_rows())
+        it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+        assert S.solve(x, dv(m, b), it) == 0
+        out.append((it.iterations(), x.numpy()))
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+    with pytest.raises(m.MtlError):
+        m.ir_solve(A, x, dv(m, b), it, m.pc.diagonal(A, m.F32), method=m._lib.CGS, value_format=m.F32)
+
+
 def test_bicgstab_breakdown(m):
     A = m.compressed2D(2, 2, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.zeros(2))
     x = m.dense_vector(2)

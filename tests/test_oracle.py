@@ -464,3 +464,46 @@ def test_bicgstab_breakdown():
     A = orc.Matrix([0, 1, 2], [0, 1], [0.0, 0.0])
     x, st, it, _ = orc.bicgstab(A, np.ones(2), None, max_iter=10, rtol=1e-8)
     assert st == orc.BREAKDOWN
+
+
+# ---------------------------------------------------------------- CGS
+def test_cgs_ic0_reference_case():
+    """The reference's ic_0_cgs_test.cpp case: 10x10 laplacian_setup, ic_0,
+    x = 1, b = A*x, cyclic_iteration(b, N, 1e-6): converges, and the
+    recurrence residual it stops on is the true one within rounding."""
+    A = orc.laplacian_setup(10, 10)
+    b = orc.spmv(A, np.ones(A.n))
+    x, st, it, hist = orc.cgs(A, b, orc.Preconditioner("ic_0", A), max_iter=A.n, rtol=1e-6, hist_cap=200)
+    assert st == 0 and it.i < A.n
+    true = np.linalg.norm(b - orc.spmv(A, x))
+    assert it.resid <= 1e-6 * np.linalg.norm(b)
+    assert abs(true - it.resid) <= 1e-9 * np.linalg.norm(b)
+    assert np.allclose(x, np.ones(A.n), atol=1e-4)
+
+
+def test_cgs_finite_termination():
+    """In exact arithmetic CGS, like BiCG, ends after as many steps as A has
+    distinct eigenvalues: a diagonal matrix with three distinct values needs
+    three iterations (a wrong sign or a dropped term in any update breaks
+    this)."""
+    vals = np.repeat([1.0, 2.0, 5.0], 7)
+    n = len(vals)
+    A = orc.Matrix(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), vals)
+    b = np.random.default_rng(3).uniform(0.5, 1.5, n)
+    x, st, it, _ = orc.cgs(A, b, None, max_iter=10, rtol=1e-12)
+    assert st == 0 and it.i == 3
+    assert np.allclose(x, b / vals, rtol=1e-12)
+
+
+@pytest.mark.parametrize("pc", ["identity", "diagonal", "ilu_0"])
+def test_cgs_nonsymmetric(pc):
+    rng = np.random.default_rng(12)
+    n = 60
+    D = rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < 0.1)
+    np.fill_diagonal(D, np.abs(D).sum(1) + 1.0)
+    A = from_dense(D)
+    xt = rng.standard_normal(n)
+    b = D @ xt
+    P = None if pc == "identity" else orc.Preconditioner(pc, A)
+    x, st, it, _ = orc.cgs(A, b, P, max_iter=200, rtol=1e-12)
+    assert st == 0 and np.allclose(x, xt, rtol=1e-8, atol=1e-9)
