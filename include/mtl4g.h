@@ -69,6 +69,7 @@ extern "C" {
 #define MTL4G_CG       0   /* itl::cg,       itl/krylov/cg.hpp:38-111 */
 #define MTL4G_BICGSTAB 1   /* itl::bicgstab, itl/krylov/bicgstab.hpp:27-78 */
 #define MTL4G_CGS      2   /* itl::cgs,      itl/krylov/cgs.hpp:24-62 (FP64 only) */
+#define MTL4G_TFQMR    3   /* itl::tfqmr,    itl/krylov/tfqmr.hpp:32-96 (FP64; mtl4g_tfqmr) */
 
 typedef struct mtl4g_matrix_s* mtl4g_matrix;
 typedef struct mtl4g_pc_s* mtl4g_pc;
@@ -240,6 +241,12 @@ int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
  * sets error code 2 and the loop goes on, as MTL4's iter.fail(2) does. */
 int mtl4g_cgs(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
               mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
+/* itl::tfqmr(A, x, b, L, R, iter) (tfqmr.hpp:32-96): left preconditioner L
+ * and right preconditioner R (either may be NULL = identity); x is shifted
+ * back by solve(R, x) after the loop as MTL4 does.  A zero sigma or rho is
+ * iter.fail(1): returns 1.  An empty system is MTL4's logic_error. */
+int mtl4g_tfqmr(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* d_x, const double* d_b,
+                mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
 
 /* Reusable solver: the workspace (MTL4 allocates it per call, cg.hpp:90)
  * stays with the handle, so repeated solves allocate nothing. */

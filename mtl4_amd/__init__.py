@@ -23,7 +23,7 @@ from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, ORDER_MTL4, ORDE
                    missing_diagonal, nonfinite_error, unexpected_orthogonality)
 
 __all__ = ["compressed2D", "dense_vector", "pc", "solve", "adjoint_solve", "basic_iteration",
-           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "dot", "unary_dot",
+           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "tfqmr", "dot", "unary_dot",
            "two_norm", "num_rows", "num_cols", "size", "laplacian_setup", "ir_solve",
            "MtlError", "matrix_not_square", "incompatible_size", "missing_diagonal",
            "unexpected_orthogonality", "nonfinite_error", "logic_error"]
@@ -404,6 +404,8 @@ class cyclic_iteration(basic_iteration):
                 i += 1
             elif method in (_lib.CG, _lib.CGS) and k > 0:
                 i += 1
+            elif method == _lib.TFQMR and k > 1:   # finished(rt) before the loop, then tau
+                i += 1
             counts.append(i)
         last = -1
         for i, r in zip(counts, self.history):
@@ -431,7 +433,7 @@ class noisy_iteration(cyclic_iteration):
 
 
 # ---------------------------------------------------------------- solvers
-def _krylov(fn, method, A, x, b, P, it):
+def _krylov(fn, method, A, x, b, P, it, R=None):
     if A.num_rows() != A.num_cols():
         raise matrix_not_square("Krylov solve")
     if x.size() != A.num_cols() or b.size() != A.num_rows():
@@ -440,12 +442,15 @@ def _krylov(fn, method, A, x, b, P, it):
     cap = it._wants_history()
     hist = np.full(max(cap, 1), np.nan)
     ph = P.h if P is not None else None
-    status = fn(A.h, ph, x.ptr, b.ptr, C.byref(st),
-                hist.ctypes.data_as(C.c_void_p) if cap else None, cap, _stream())
+    hp = hist.ctypes.data_as(C.c_void_p) if cap else None
+    if method == _lib.TFQMR:
+        status = fn(A.h, ph, R.h if R is not None else None, x.ptr, b.ptr, C.byref(st), hp, cap, _stream())
+    else:
+        status = fn(A.h, ph, x.ptr, b.ptr, C.byref(st), hp, cap, _stream())
     h = hist[:cap]
     h = h[~np.isnan(h)] if cap else None
     it._update(st, h, method)
-    check(status, {CG: "cg", BICGSTAB: "bicgstab"}.get(method, "cgs"))
+    check(status, {CG: "cg", BICGSTAB: "bicgstab", _lib.CGS: "cgs"}.get(method, "tfqmr"))
     return it.error_code()
 
 
@@ -468,6 +473,12 @@ def cgs(A, x, b, P=None, it=None):
     if isinstance(P, basic_iteration):
         P, it = None, P
     return _krylov(_L().mtl4g_cgs, _lib.CGS, A, x, b, P, it)
+
+
+def tfqmr(A, x, b, L=None, R=None, it=None):
+    """itl::tfqmr(A, x, b, L, R, iter) (itl/krylov/tfqmr.hpp:32-96): left
+    preconditioner L, right preconditioner R (None: identity)."""
+    return _krylov(_L().mtl4g_tfqmr, _lib.TFQMR, A, x, b, L, it, R)
 
 
 class Solver:

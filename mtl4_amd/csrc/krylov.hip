@@ -31,7 +31,8 @@ enum { PCM_NONE = 0, PCM_DIAG = 1, PCM_GENERAL = 2 };
 // basic_iteration::finished(Real) (basic_iteration.hpp:60-65,85-93,43-48)
 // followed by ++iter when the loop goes on.  NaN/Inf residuals stop the
 // solve with MTL4G_ERR_NONFINITE instead of running to max_iter.
-__device__ __forceinline__ void iter_finished(Ctl* c, double resid, double* hist) {
+// finished() alone (no ++iter): sets done when the loop must stop
+__device__ __forceinline__ void iter_check(Ctl* c, double resid, double* hist) {
     if (c->hist_n < c->hist_cap) hist[c->hist_n] = resid;
     c->hist_n++;
     c->resid = resid;
@@ -40,6 +41,11 @@ __device__ __forceinline__ void iter_finished(Ctl* c, double resid, double* hist
     if (conv) { c->done = 1; return; }
     if (c->i >= c->max_iter) { c->error = 1; c->done = 1; return; }
     if (!isfinite(resid)) { c->status = MTL4G_ERR_NONFINITE; c->done = 1; return; }
+}
+
+__device__ __forceinline__ void iter_finished(Ctl* c, double resid, double* hist) {
+    iter_check(c, resid, hist);
+    if (c->done) return;
     c->i++;
     c->first = c->i <= 1;
 }
@@ -576,6 +582,207 @@ struct EpiCgsXR {
     }
 };
 
+// ---------------------------------------------------------------- TFQMR epilogues
+// itl::tfqmr, itl/krylov/tfqmr.hpp:32-96.  Vectors rt, r, u1, u2, y1, y2,
+// w, d, v, a temporary for solve(R, .) and the preconditioners' scratch.
+
+// rt = b - A*solve(R, x); the check finished(rt) before the loop (:44-49)
+template <typename VT>
+struct EpiTfInit {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_NORM4; }
+    struct Pre {
+        VT b;
+    };
+    const VT* b;
+    VT* rt;
+    Ctl* c;
+    double* hist;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ Pre preload(int row) const { return Pre{b[row]}; }
+    __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
+        const VT v = pre.b - s;
+        rt[row] = v;
+        part[0] += v * v;
+    }
+    __device__ void finish(const VT* tot) const { iter_check(c, (double)std::sqrt(tot[0]), hist); }
+};
+
+// y = A*v, skipped once the solve is done
+template <typename VT>
+struct EpiTfAv {
+    static constexpr int NP = 0;
+    struct Pre {};
+    VT* y;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ Pre preload(int) const { return Pre{}; }
+    __device__ void operator()(int row, VT s, const Pre&, VT*) const { y[row] = s; }
+    __device__ void finish(const VT*) const {}
+};
+
+// a = src (and b = src when b is given)
+template <typename VT>
+struct EpiTfCopy {
+    static constexpr int NP = 0;
+    const VT* src;
+    VT *a, *b;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, VT*) const {
+        const VT v = src[i];
+        a[i] = v;
+        if (b) b[i] = v;
+    }
+    __device__ void finish(const VT*) const {}
+};
+
+// tau = two_norm(r); rho = tau*tau; loop head finished(tau)  (:51-55)
+template <typename VT>
+struct EpiTfTau {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_NORM4; }
+    const VT* r;
+    Ctl* c;
+    double* hist;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, VT* part) const { part[0] += r[i] * r[i]; }
+    __device__ void finish(const VT* tot) const {
+        const VT tau = std::sqrt(tot[0]);
+        c->tau = (double)tau;
+        c->rho = (double)(tau * tau);
+        c->looped = 1;
+        iter_finished(c, (double)tau, hist);
+    }
+};
+
+// sigma = dot(r, v); sigma == 0 is iter.fail(1) and returns; alpha  (:58-62)
+template <typename VT>
+struct EpiTfSigma {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
+    const VT *r, *v;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, VT* part) const { part[0] += r[i] * v[i]; }
+    __device__ void finish(const VT* tot) const {
+        c->gamma = (double)tot[0];
+        if (tot[0] == VT(0)) { c->error = 1; c->done = 1; return; }
+        c->alpha = (double)(static_cast<VT>(c->rho) / tot[0]);
+    }
+};
+
+// w -= alpha*u; d = y + (theta*theta*eta/alpha)*d; theta = two_norm(w)/tau;
+// c = 1/sqrt(1 + theta*theta); tau *= theta*c; eta = c*c*alpha  (:66-81)
+template <typename VT>
+struct EpiTfW {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_NORM4; }
+    const VT *u, *y;
+    VT *w, *d;
+    Ctl* c;
+    VT alpha, coef;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        alpha = static_cast<VT>(c->alpha);
+        const VT theta = static_cast<VT>(c->theta);
+        coef = theta * theta * static_cast<VT>(c->eta) / alpha;
+    }
+    __device__ void operator()(int64_t i, VT* part) const {
+        const VT wi = w[i] - alpha * u[i];
+        w[i] = wi;
+        d[i] = y[i] + coef * d[i];
+        part[0] += wi * wi;
+    }
+    __device__ void finish(const VT* tot) const {
+        const VT tau = static_cast<VT>(c->tau);
+        const VT theta = std::sqrt(tot[0]) / tau;
+        const VT cc = VT(1) / std::sqrt(VT(1) + theta * theta);
+        c->theta = (double)theta;
+        c->tau = (double)(tau * (theta * cc));
+        c->eta = (double)(cc * cc * static_cast<VT>(c->alpha));
+    }
+};
+
+// x += eta*d (end of the first inner step); y2 = y1 - alpha*v  (:71, :68)
+template <typename VT>
+struct EpiTfXY2 {
+    static constexpr int NP = 0;
+    VT *x, *y2;
+    const VT *d, *y1, *v;
+    Ctl* c;
+    VT eta, alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        eta = static_cast<VT>(c->eta);
+        alpha = static_cast<VT>(c->alpha);
+    }
+    __device__ void operator()(int64_t i, VT*) const {
+        x[i] = x[i] + eta * d[i];
+        y2[i] = y1[i] - alpha * v[i];
+    }
+    __device__ void finish(const VT*) const {}
+};
+
+// x += eta*d (second inner step); rho == 0 is iter.fail(1); rhon =
+// dot(r, w); beta = rhon/rho; rho = rhon; loop head finished(tau)  (:81-87, :57)
+template <typename VT>
+struct EpiTfRho {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int k) { return RO_DOT4; }
+    VT* x;
+    const VT *d, *r, *w;
+    Ctl* c;
+    double* hist;
+    VT eta;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { eta = static_cast<VT>(c->eta); }
+    __device__ void operator()(int64_t i, VT* part) const {
+        x[i] = x[i] + eta * d[i];
+        part[0] += r[i] * w[i];
+    }
+    __device__ void finish(const VT* tot) const {
+        const VT rho = static_cast<VT>(c->rho);
+        if (rho == VT(0)) { c->error = 1; c->done = 1; return; }
+        c->beta = (double)(tot[0] / rho);
+        c->rho = (double)tot[0];
+        iter_finished(c, c->tau, hist);
+    }
+};
+
+// y1 = w + beta*y2  (:89)
+template <typename VT>
+struct EpiTfY1 {
+    static constexpr int NP = 0;
+    VT* y1;
+    const VT *w, *y2;
+    Ctl* c;
+    VT beta;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { beta = static_cast<VT>(c->beta); }
+    __device__ void operator()(int64_t i, VT*) const { y1[i] = w[i] + beta * y2[i]; }
+    __device__ void finish(const VT*) const {}
+};
+
+// v = u1 + beta*(u2 + beta*v)  (:92)
+template <typename VT>
+struct EpiTfV {
+    static constexpr int NP = 0;
+    VT* v;
+    const VT *u1, *u2;
+    Ctl* c;
+    VT beta;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { beta = static_cast<VT>(c->beta); }
+    __device__ void operator()(int64_t i, VT*) const { v[i] = u1[i] + beta * (u2[i] + beta * v[i]); }
+    __device__ void finish(const VT*) const {}
+};
+
 // ---------------------------------------------------------------- FP64 refresh
 // x += double(x32); x32 = 0
 struct EpiRu1 {
@@ -892,6 +1099,129 @@ int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt
     return c.error;
 }
 
+// itl::tfqmr (tfqmr.hpp:32-96) with every scalar on the device.  The line
+// rt = A*x - b at the end of each iteration (:93) computes a value nothing
+// reads before it is overwritten, so it is not run.
+int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R, Workspace<double>& W,
+              double* x, const double* b, mtl4g_iteration* it, double* h_hist, int32_t hist_cap,
+              hipStream_t s) {
+    using VT = double;
+    if (!A || !it || (A->n && (!x || !b))) return MTL4G_ERR_INVALID;
+    if (A->n != A->m) {
+        set_error("Krylov solve needs a square matrix");
+        return MTL4G_ERR_NOT_SQUARE;
+    }
+    if ((L && L->n != A->n) || (R && R->n != A->n)) {
+        set_error("preconditioner size != matrix size %d", A->n);
+        return MTL4G_ERR_SIZE;
+    }
+    if (A->n == 0) {   // tfqmr.hpp:40
+        set_error("empty rhs vector");
+        return MTL4G_ERR_INVALID;
+    }
+    if (L && L->kind == MTL4G_PC_IDENTITY) L = nullptr;
+    if (R && R->kind == MTL4G_PC_IDENTITY) R = nullptr;
+    M4_TRY(ensure_workspace(W, A->n, 12, L, hist_cap, s));
+    M4_TRY(ensure_workspace(W, A->n, 12, R, hist_cap, s));   // sweep counters for both
+    const int64_t n = A->n;
+    VT* base = W.vecs.p;
+    VT *rt = base, *r = base + n, *u1 = base + 2 * n, *u2 = base + 3 * n, *y1 = base + 4 * n,
+       *y2 = base + 5 * n, *w = base + 6 * n, *d = base + 7 * n, *v = base + 8 * n, *tmp = base + 9 * n,
+       *y0 = base + 10 * n;
+    Ctl init;
+    std::memset(&init, 0, sizeof init);
+    init.norm_r0 = it->norm_r0;
+    init.rtol = it->rtol;
+    init.atol = it->atol;
+    init.max_iter = it->max_iter;
+    init.i = it->i;
+    init.hist_cap = h_hist ? hist_cap : 0;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
+    M4_HIP(hipGetLastError());
+    Ctl* c = W.ctl.p;
+    VT* part = reinterpret_cast<VT*>(W.partials.p);
+    VT* tm = reinterpret_cast<VT*>(W.terms.p);
+    unsigned* tk = W.tickets.p;
+    const double* val = A->val.p;
+    // y = A * Vector(solve(R, in))
+    auto a_r = [&](const VT* in, VT* y) -> int {
+        const VT* src = in;
+        if (R) {
+            M4_TRY(pc_apply<VT>(R, in, tmp, y0, &c->done, W.pc_counters.p, s));
+            src = tmp;
+        }
+        return launch_csr<VT>(A, val, src, EpiTfAv<VT>{y, c}, part, tk, s);
+    };
+    // out = solve(L, A * Vector(solve(R, in))), through rt
+    auto l_a_r = [&](const VT* in, VT* out) -> int {
+        if (!L) return a_r(in, out);
+        M4_TRY(a_r(in, rt));
+        return pc_apply<VT>(L, rt, out, y0, &c->done, W.pc_counters.p, s);
+    };
+    M4_TRY(zero_on(d, (size_t)n * sizeof(VT), s));   // d(resource(x), zero)
+    // rt = b - A*solve(R, x); finished(rt); r = solve(L, rt)  (:44-49)
+    const VT* xs = x;
+    if (R) {
+        M4_TRY(pc_apply<VT>(R, x, tmp, y0, nullptr, W.pc_counters.p, s));
+        xs = tmp;
+    }
+    M4_TRY(launch_csr<VT>(A, val, xs, EpiTfInit<VT>{b, rt, c, W.hist.p}, part, tk, s, tm));
+    if (L) M4_TRY(pc_apply<VT>(L, rt, r, y0, &c->done, W.pc_counters.p, s));
+    else M4_TRY(launch_vec<VT>(n, EpiTfCopy<VT>{rt, r, nullptr, c}, part, tk, s));
+    // y1 = w = r; u1 = v = solve(L, A*solve(R, y1)); tau, rho; loop head  (:50-57)
+    M4_TRY(launch_vec<VT>(n, EpiTfCopy<VT>{r, y1, w, c}, part, tk, s));
+    M4_TRY(l_a_r(y1, u1));
+    M4_TRY(launch_vec<VT>(n, EpiTfCopy<VT>{u1, v, nullptr, c}, part, tk, s));
+    M4_TRY(launch_vec<VT>(n, EpiTfTau<VT>{r, c, W.hist.p}, part, tk, s, tm));
+    auto iteration = [&]() -> int {
+        M4_TRY(launch_vec<VT>(n, EpiTfSigma<VT>{r, v, c}, part, tk, s, tm));
+        M4_TRY(launch_vec<VT>(n, EpiTfW<VT>{u1, y1, w, d, c, VT(0), VT(0)}, part, tk, s, tm));
+        M4_TRY(launch_vec<VT>(n, EpiTfXY2<VT>{x, y2, d, y1, v, c, VT(0), VT(0)}, part, tk, s));
+        M4_TRY(l_a_r(y2, u2));
+        M4_TRY(launch_vec<VT>(n, EpiTfW<VT>{u2, y2, w, d, c, VT(0), VT(0)}, part, tk, s, tm));
+        M4_TRY(launch_vec<VT>(n, EpiTfRho<VT>{x, d, r, w, c, W.hist.p, VT(0)}, part, tk, s, tm));
+        M4_TRY(launch_vec<VT>(n, EpiTfY1<VT>{y1, w, y2, c, VT(0)}, part, tk, s));
+        M4_TRY(l_a_r(y1, u1));
+        return launch_vec<VT>(n, EpiTfV<VT>{v, u1, u2, c, VT(0)}, part, tk, s);
+    };
+    const int32_t budget = std::max(0, it->max_iter - it->i) + 8;
+    int32_t launched = 0, chunk = 2;
+    for (;;) {
+        M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        M4_HIP(hipStreamSynchronize(s));
+        if (W.h_ctl->done) break;
+        if (launched > budget) {
+            set_error("solver did not stop after %d launched iterations", launched);
+            return MTL4G_ERR_INVALID;
+        }
+        for (int k = 0; k < chunk; k++) M4_TRY(iteration());
+        launched += chunk;
+        chunk = std::min(chunk * 2, 32);
+    }
+    const Ctl& hc = *W.h_ctl;
+    // shift back: x = solve(R, x)  (:95), not after the early return at :48
+    if (R && hc.looped) {
+        M4_TRY(pc_apply<VT>(R, x, tmp, y0, nullptr, W.pc_counters.p, s));
+        M4_HIP(hipMemcpyAsync(x, tmp, (size_t)n * sizeof(VT), hipMemcpyDeviceToDevice, s));
+    }
+    if (L) M4_TRY(pc_check_error(L, s));
+    if (R) M4_TRY(pc_check_error(R, s));
+    M4_HIP(hipStreamSynchronize(s));
+    it->i = hc.i;
+    it->error = hc.error;
+    it->resid = hc.resid;
+    if (h_hist && hist_cap > 0) {
+        int32_t nh = std::min(hc.hist_n, hist_cap);
+        if (nh > 0) M4_HIP(hipMemcpy(h_hist, W.hist.p, (size_t)nh * sizeof(double), hipMemcpyDeviceToHost));
+        for (int32_t k = nh; k < hist_cap; k++) h_hist[k] = NAN;
+    }
+    if (hc.status != 0) {
+        set_error("non-finite residual");
+        return hc.status;
+    }
+    return hc.error;
+}
+
 template int krylov_run<double>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<double>&,
                                 double*, const double*, mtl4g_iteration*, double*, int32_t, hipStream_t);
 template int krylov_run<float>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<float>&,
@@ -1042,6 +1372,12 @@ int mtl4g_cg(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_itera
              double* h_hist, int32_t hist_cap, void* stream) {
     Workspace<double> W;
     return krylov_run<double>(A, P, MTL4G_CG, MTL4G_F64, W, x, b, it, h_hist, hist_cap, S(stream));
+}
+
+int mtl4g_tfqmr(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* x, const double* b, mtl4g_iteration* it,
+                double* h_hist, int32_t hist_cap, void* stream) {
+    Workspace<double> W;
+    return tfqmr_run(A, L, R, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_cgs(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,

@@ -9,6 +9,8 @@ namespace mtl4g {
 // Scalars of the recurrences and of basic_iteration, kept on the device.
 struct Ctl {
     double rho, rho_1, rr, alpha, alpha_1, beta, omega, gamma, rho_2;
+    double theta, eta, tau;  // tfqmr
+    int32_t looped;          // tfqmr: the loop was entered (x is shifted back)
     double norm_r0, rtol, atol, resid;
     double rmax, delta;      // mixed precision: residual at the last FP64 refresh, trigger
     int32_t i, max_iter, error, done, first, half, status;
@@ -42,6 +44,11 @@ template <typename VT>
 int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int value_format,
                Workspace<VT>& W, VT* x, const VT* b, mtl4g_iteration* it, double* h_hist,
                int32_t hist_cap, hipStream_t s);
+
+// itl::tfqmr(A, x, b, L, R, iter) on the device (L, R may be NULL = identity).
+int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R, Workspace<double>& W,
+              double* x, const double* b, mtl4g_iteration* it, double* h_hist, int32_t hist_cap,
+              hipStream_t s);
 
 // Mixed precision with reliable updates: one FP32 recurrence (values in
 // value_format), solution accumulated in FP64, the true residual b - A*x

@@ -1069,7 +1069,9 @@ class cyclic_iteration : public basic_iteration<Real> {
     void replay_(const std::vector<double>& hist, int method, int i0) override {
         int i = i0;
         for (std::size_t k = 0; k < hist.size(); k++) {
-            if (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)) i++;   // cg, cgs: one finished() per iteration
+            if (method == MTL4G_TFQMR ? k > 1   // finished(rt) before the loop, then tau
+                                      : (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)))   // cg, cgs: one per iteration
+                i++;
             this->i = i;
             this->resid_ = hist[k];
             print_resid();
@@ -1252,7 +1254,8 @@ template <typename PC>
 inline mtl4g_pc pc_handle(const PC& P) { return P.handle_(); }
 
 template <typename Matrix, typename VX, typename VB, typename Iter>
-inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter& iter) {
+inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter& iter,
+               mtl4g_pc R = nullptr) {
     MTL_THROW_IF(A.num_rows() != A.num_cols(), mtl::matrix_not_square());
     MTL_THROW_IF(x.size() != A.num_cols() || b.size() != A.num_rows(), mtl::incompatible_size());
     mtl4g_iteration s = iteration_access::state(iter);
@@ -1264,9 +1267,10 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
     const double* db = b.device_read_();
     double* dx = x.device_readwrite_();
     double* hp = cap ? hist.data() : nullptr;
-    int st = method == MTL4G_CG    ? mtl4g_cg(h, P, dx, db, &s, hp, cap, nullptr)
-             : method == MTL4G_CGS ? mtl4g_cgs(h, P, dx, db, &s, hp, cap, nullptr)
-                                   : mtl4g_bicgstab(h, P, dx, db, &s, hp, cap, nullptr);
+    int st = method == MTL4G_CG      ? mtl4g_cg(h, P, dx, db, &s, hp, cap, nullptr)
+             : method == MTL4G_CGS   ? mtl4g_cgs(h, P, dx, db, &s, hp, cap, nullptr)
+             : method == MTL4G_TFQMR ? mtl4g_tfqmr(h, P, R, dx, db, &s, hp, cap, nullptr)
+                                     : mtl4g_bicgstab(h, P, dx, db, &s, hp, cap, nullptr);
     std::vector<double> used;
     for (double v : hist)
         if (v == v) used.push_back(v);
@@ -1315,8 +1319,20 @@ int cgs(const LinearOperator& A, Vector& x, const Vector& b, const Preconditione
     return iter;
 }
 
-// itl::base_solver, cg_solver, bicgstab_solver, cgs_solver (krylov/base_solver.hpp:24-84,
-// cg.hpp:122-156, bicgstab.hpp:80-112, cgs.hpp:64-95)
+// itl::tfqmr (krylov/tfqmr.hpp:32-96): left and right preconditioners
+template <typename Matrix, typename Vector, typename LeftPreconditioner, typename RightPreconditioner,
+          typename Iteration>
+int tfqmr(const Matrix& A, Vector& x, const Vector& b, const LeftPreconditioner& L, const RightPreconditioner& R,
+          Iteration& iter) {
+    if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
+    detail::run(MTL4G_TFQMR, A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
+                pc::is_identity(R) ? nullptr : detail::pc_handle(R));
+    return iter;
+}
+
+// itl::base_solver, cg_solver, bicgstab_solver, cgs_solver, tfqmr_solver
+// (krylov/base_solver.hpp:24-84, cg.hpp:122-156, bicgstab.hpp:80-112,
+// cgs.hpp:64-95, tfqmr.hpp:98-127)
 template <typename Solver, typename LinearOperator>
 struct base_solver {
     typedef double magnitude_type;
@@ -1413,6 +1429,27 @@ class cgs_solver
             std::cerr << "Right Preconditioner ignored!" << std::endl;
     }
     Preconditioner L;
+};
+
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class tfqmr_solver
+    : public base_solver<tfqmr_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<tfqmr_solver, LinearOperator> base;
+
+  public:
+    explicit tfqmr_solver(const LinearOperator& A) : base(A), L(A), R(A) {}
+    tfqmr_solver(const LinearOperator& A, const Preconditioner& L) : base(A), L(L), R(A) {}
+    tfqmr_solver(const LinearOperator& A, const Preconditioner& L, const RightPreconditioner& R)
+        : base(A), L(L), R(R) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const {
+        return tfqmr(this->A, x, b, L, R, iter);
+    }
+
+  private:
+    Preconditioner L;
+    RightPreconditioner R;
 };
 
 }  // namespace itl

@@ -22,6 +22,7 @@
 #define ORC_MISSING_DIAG (-3)
 #define ORC_BREAKDOWN (-4)
 #define ORC_OOM (-5)
+#define ORC_INVALID (-6)   /* MTL4's logic_error */
 
 /* ------------------------------------------------------------------ */
 /* laplacian_setup: mtl/matrix/laplacian_setup.hpp:25-44.  The inserter
@@ -678,5 +679,89 @@ int orc_cgs(const orc_csr* A, const orc_pc* P, double* x, const double* b,
         rho_2 = rho_1;
     }
     free(w);
+    return st != ORC_OK ? st : it->error;
+}
+
+/* itl::tfqmr, itl/krylov/tfqmr.hpp:32-96, with left preconditioner L and
+ * right preconditioner R (NULL: identity).  sigma == 0 and rho == 0 are
+ * iter.fail(1, ...) returned at once; rt = A*x - b at the end of each
+ * iteration (:93) is computed as there, though nothing reads it.        */
+static int tf_solve(const orc_pc* P, const double* in, double* out, int32_t n)
+{
+    if (P) return orc_pc_solve(P, in, out);
+    memcpy(out, in, (size_t)n * sizeof(double));
+    return ORC_OK;
+}
+
+int orc_tfqmr(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
+              orc_iter* it, double* hist, int32_t hist_cap)
+{
+    int32_t n = A->n, nh = 0;
+    if (A->n != A->ncols) return ORC_NOT_SQUARE;
+    if (n == 0) return ORC_INVALID;                       /* "empty rhs vector" */
+    size_t sn = (size_t)n;
+    double* wk = (double*)calloc(sn * 11, sizeof(double));
+    if (!wk) return ORC_OOM;
+    double *rt = wk, *r = wk + sn, *u1 = wk + 2 * sn, *u2 = wk + 3 * sn, *y1 = wk + 4 * sn,
+           *y2 = wk + 5 * sn, *w = wk + 6 * sn, *d = wk + 7 * sn, *v = wk + 8 * sn,
+           *tmp = wk + 9 * sn, *tmp2 = wk + 10 * sn;
+    double theta = 0, eta = 0, tau, rho, rhon, sigma, alpha, beta, c;
+    int st = ORC_OK;
+    it->error = 0;
+#define TF_TRY(e) do { st = (e); if (st != ORC_OK) goto out; } while (0)
+    TF_TRY(tf_solve(R, x, tmp, n));                      /* rt(b - A*Vector(solve(R, x))) */
+    residual(A, tmp, b, rt, tmp2);
+    TF_TRY(tf_solve(L, rt, r, n));                       /* r(solve(L, rt)) */
+    if (finished(it, orc_two_norm(n, rt), hist, hist_cap, &nh)) goto out;
+    memcpy(w, r, sn * sizeof(double));                   /* y1 = w = r */
+    memcpy(y1, w, sn * sizeof(double));
+    TF_TRY(tf_solve(R, y1, tmp, n));
+    orc_spmv(A, tmp, rt, ORC_ASSIGN);
+    TF_TRY(tf_solve(L, rt, v, n));                       /* u1 = v = solve(L, rt) */
+    memcpy(u1, v, sn * sizeof(double));
+    tau = orc_two_norm(n, r);
+    rho = tau * tau;
+    while (!finished(it, tau, hist, hist_cap, &nh)) {
+        ++it->i;
+        sigma = orc_dot(n, r, v, 4);
+        if (sigma == 0.0) { it->error = 1; goto out; }
+        alpha = rho / sigma;
+        for (int j = 1; j < 3; j++) {
+            if (j == 1) {
+                for (int32_t i = 0; i < n; i++) w[i] -= alpha * u1[i];
+                double coef = theta * theta * eta / alpha;
+                for (int32_t i = 0; i < n; i++) d[i] = y1[i] + coef * d[i];
+            } else {
+                for (int32_t i = 0; i < n; i++) y2[i] = y1[i] - alpha * v[i];
+                TF_TRY(tf_solve(R, y2, tmp, n));
+                orc_spmv(A, tmp, rt, ORC_ASSIGN);
+                TF_TRY(tf_solve(L, rt, u2, n));
+                for (int32_t i = 0; i < n; i++) w[i] -= alpha * u2[i];
+                double coef = theta * theta * eta / alpha;
+                for (int32_t i = 0; i < n; i++) d[i] = y2[i] + coef * d[i];
+            }
+            theta = orc_two_norm(n, w) / tau;
+            c = 1.0 / sqrt(1.0 + theta * theta);
+            tau *= theta * c;
+            eta = c * c * alpha;
+            for (int32_t i = 0; i < n; i++) x[i] += eta * d[i];
+        }
+        if (rho == 0.0) { it->error = 1; goto out; }
+        rhon = orc_dot(n, r, w, 4);
+        beta = rhon / rho;
+        rho = rhon;
+        for (int32_t i = 0; i < n; i++) y1[i] = w[i] + beta * y2[i];
+        TF_TRY(tf_solve(R, y1, tmp, n));
+        orc_spmv(A, tmp, rt, ORC_ASSIGN);
+        TF_TRY(tf_solve(L, rt, u1, n));
+        for (int32_t i = 0; i < n; i++) v[i] = u1[i] + beta * (u2[i] + beta * v[i]);
+        orc_spmv(A, x, rt, ORC_ASSIGN);                  /* rt = A*x - b */
+        for (int32_t i = 0; i < n; i++) rt[i] -= b[i];
+    }
+    TF_TRY(tf_solve(R, x, tmp, n));                      /* shift back: x = solve(R, x) */
+    memcpy(x, tmp, sn * sizeof(double));
+out:
+#undef TF_TRY
+    free(wk);
     return st != ORC_OK ? st : it->error;
 }

@@ -84,6 +84,8 @@ int orc_cg(const orc_csr* A, const orc_pc* P, double* x, const double* b,
            orc_iter* it, double* hist, int32_t hist_cap);
 int orc_cgs(const orc_csr* A, const orc_pc* P, double* x, const double* b,
             orc_iter* it, double* hist, int32_t hist_cap);
+int orc_tfqmr(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
+              orc_iter* it, double* hist, int32_t hist_cap);
 int orc_bicgstab(const orc_csr* A, const orc_pc* P, double* x, const double* b,
                  orc_iter* it, double* hist, int32_t hist_cap);
 

@@ -19,7 +19,7 @@ ASSIGN, PLUS, MINUS = 0, 1, 2
 PC_IDENTITY, PC_DIAGONAL, PC_ILU0, PC_IC0 = 0, 1, 2, 3
 PC_KINDS = {"identity": 0, "diagonal": 1, "ilu_0": 2, "ic_0": 3}
 
-OK, NOT_SQUARE, SIZE_MISMATCH, MISSING_DIAG, BREAKDOWN, OOM = 0, -1, -2, -3, -4, -5
+OK, NOT_SQUARE, SIZE_MISMATCH, MISSING_DIAG, BREAKDOWN, OOM, INVALID = 0, -1, -2, -3, -4, -5, -6
 
 
 class CSR(C.Structure):
@@ -85,6 +85,8 @@ def lib():
         L.orc_bicgstab.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
         L.orc_cgs.restype = C.c_int
         L.orc_cgs.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
+        L.orc_tfqmr.restype = C.c_int
+        L.orc_tfqmr.argtypes = [P, P, P, P, P, C.POINTER(Iter), P, I32]
         _lib = L
     return _lib
 
@@ -251,3 +253,15 @@ def cgs(A, b, P=None, x0=None, it=None, hist_cap=0, **kw):
     it = iteration(b, **kw) if it is None else it
     x, st, hist = _solve(lib().orc_cgs, A, b, P, x0, it, hist_cap)
     return x, st, it, hist
+
+
+def tfqmr(A, b, L=None, R=None, x0=None, it=None, hist_cap=0, **kw):
+    """itl::tfqmr(A, x, b, L, R, iter), itl/krylov/tfqmr.hpp:32-96."""
+    it = iteration(b, **kw) if it is None else it
+    b = _f64(b)
+    x = np.zeros(A.n) if x0 is None else _f64(x0).copy()
+    hist = np.full(max(hist_cap, 1), np.nan)
+    st = lib().orc_tfqmr(A.ref(), L.h if L is not None else None, R.h if R is not None else None,
+                         _p(x), _p(b), C.byref(it), _p(hist), hist_cap)
+    nh = int(np.argmax(np.isnan(hist))) if np.isnan(hist).any() else len(hist)
+    return x, st, it, hist[:min(nh, hist_cap)]

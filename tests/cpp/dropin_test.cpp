@@ -167,6 +167,39 @@ int main() {
         CHECK(it2.iterations() == iter.iterations());
     });
 
+    // libs/numeric/itl/test/tfqmr_test.cpp: 10x10 laplacian, b = 1, the
+    // same preconditioner left and right
+    run("tfqmr_test", [] {
+        const int N = 10;
+        matrix_type A(N, N);
+        vector_type b(N * N, 1.0), x(N * N), r(x);
+        laplacian_setup(A, N, N);
+        itl::pc::identity<matrix_type> Ident(A);
+        itl::pc::ilu_0<matrix_type> ilu(A);
+        itl::pc::diagonal<matrix_type> diag(A);
+        x = 0.5;
+        itl::basic_iteration<double> iter_1(b, 100 * N, 1.e-8), iter_2(b, 100 * N, 1.e-8), iter_4(b, 100 * N, 1.e-8);
+        tfqmr(A, x, b, Ident, Ident, iter_1);
+        r = b;
+        r -= A * x;   // |b - A x| = |A x - b|
+        CHECK(two_norm(r) <= 0.000001);
+        x = 0.5;
+        tfqmr(A, x, b, ilu, ilu, iter_2);
+        r = b;
+        r -= A * x;   // |b - A x| = |A x - b|
+        CHECK(two_norm(r) <= 0.000001);
+        x = 10.5;
+        tfqmr(A, x, b, diag, diag, iter_4);
+        r = b;
+        r -= A * x;   // |b - A x| = |A x - b|
+        CHECK(two_norm(r) <= 0.000001);
+        itl::tfqmr_solver<matrix_type, itl::pc::ilu_0<matrix_type>, itl::pc::ilu_0<matrix_type>> S(A, ilu, ilu);
+        vector_type y(N * N, 0.5);
+        itl::basic_iteration<double> it2(b, 100 * N, 1.e-8);
+        CHECK(S.solve(y, b, it2) == 0);
+        CHECK(it2.iterations() == iter_2.iterations());
+    });
+
     // libs/numeric/itl/test/ilu_0_cg_test.cpp:20-33
     run("ilu_0_cg_test", [] {
         const int size = 3, N = size * size;

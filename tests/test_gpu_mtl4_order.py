@@ -152,6 +152,30 @@ def test_cgs_bitwise(m, mtl4_order, fam, kind):
     assert np.array_equal(x, xo)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000"])
+@pytest.mark.parametrize("lr", [("identity", "identity"), ("ilu_0", "identity"), ("identity", "ilu_0"),
+                                ("diagonal", "diagonal")])
+def test_tfqmr_bitwise(m, mtl4_order, fam, lr):
+    """itl::tfqmr with left and right preconditioners in MTL4's reduction
+    order: iteration count, final quasi-residual and solution (after the
+    shift back by solve(R, x)) bit for bit with the oracle."""
+    rp, ci, va = family(fam, seed=7)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(107).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OL, OR = [None if k == "identity" else orc.Preconditioner(k, O) for k in lr]
+    xo, st, ito, _ = orc.tfqmr(O, b, OL, OR, max_iter=2000, rtol=1e-10)
+    L, R = [getattr(m.pc, k)(A) for k in lr]
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+    err = m.tfqmr(A, x, dv(m, b), L, R, it)
+    assert err == st == 0
+    assert it.iterations() == ito.i
+    assert it.resid() == ito.resid
+    assert np.array_equal(x.numpy(), xo)
+
+
 def test_cg_config1_bitwise(m, mtl4_order):
     """CG + diagonal on the 64x64 Laplacian, the reference's CPU test case."""
     rp, ci, va = family("lap2d_64")

@@ -626,6 +626,45 @@ def test_cgs_families(m, fam, kind):
     assert res <= 1e-9 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000", "diff3d_16"])
+@pytest.mark.parametrize("lr", [("identity", "identity"), ("ilu_0", "identity"), ("diagonal", "ilu_0")])
+def test_tfqmr_families(m, fam, lr):
+    """itl::tfqmr in tree order against the oracle: iterations within
+    2 + 15 %, the solution within ten times the oracle's own error."""
+    rp, ci, va = family(fam, seed=4)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(3).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OL, OR = [None if k == "identity" else orc.Preconditioner(k, O) for k in lr]
+    xo, st, ito, _ = orc.tfqmr(O, b, OL, OR, max_iter=3000, rtol=1e-10)
+    L, R = [getattr(m.pc, k)(A) for k in lr]
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+    err = m.tfqmr(A, x, dv(m, b), L, R, it)
+    assert err == st == 0
+    assert abs(it.iterations() - ito.i) <= 2 + 0.15 * ito.i, (it.iterations(), ito.i)
+    own = max(np.linalg.norm(xo - xt), 1e-12 * np.linalg.norm(xt))
+    assert np.linalg.norm(x.numpy() - xo) <= 10 * own
+
+
+def test_tfqmr_reference_case(m):
+    """tfqmr_test.cpp: 10x10 laplacian_setup, b = 1, x = 0.5, the same ilu_0
+    left and right, cyclic_iteration(b, 1000, 1e-8): |A x - b| <= 1e-6."""
+    rp, ci, va = family("lap2d_5")
+    A = m.laplacian_setup(10, 10)
+    P = m.pc.ilu_0(A)
+    x = m.dense_vector(np.full(100, 0.5))
+    b = m.dense_vector(np.ones(100))
+    it = m.basic_iteration(b, 1000, 1e-8)
+    assert m.tfqmr(A, x, b, P, P, it) == 0
+    r = (A * x).numpy() - 1.0
+    assert np.linalg.norm(r) <= 1e-6
+    with pytest.raises(m.MtlError):
+        E = m.compressed2D(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        m.tfqmr(E, m.dense_vector(0), m.dense_vector(0), None, None, m.basic_iteration(0.0, 3, 1e-8))
+
+
 def test_cgs_solver_handle_and_mixed_refusal(m):
     rp, ci, va = family("convdiff_40", seed=1)
     O, A = pair(rp, ci, va)

@@ -507,3 +507,54 @@ def test_cgs_nonsymmetric(pc):
     P = None if pc == "identity" else orc.Preconditioner(pc, A)
     x, st, it, _ = orc.cgs(A, b, P, max_iter=200, rtol=1e-12)
     assert st == 0 and np.allclose(x, xt, rtol=1e-8, atol=1e-9)
+
+
+# ---------------------------------------------------------------- TFQMR
+@pytest.mark.parametrize("pc", ["identity", "ilu_0", "ic_0", "diagonal"])
+def test_tfqmr_reference_case(pc):
+    """The reference's tfqmr_test.cpp: 10x10 laplacian_setup, b = 1, x = 0.5
+    (10.5 for the diagonal case), the same preconditioner left and right,
+    cyclic_iteration(b, 1000, 1e-8); it requires |A x - b| <= 1e-6."""
+    A = orc.laplacian_setup(10, 10)
+    b = np.ones(A.n)
+    P = None if pc == "identity" else orc.Preconditioner(pc, A)
+    x0 = np.full(A.n, 10.5 if pc == "diagonal" else 0.5)
+    x, st, it, _ = orc.tfqmr(A, b, P, P, x0=x0, max_iter=1000, rtol=1e-8)
+    assert st == 0
+    assert np.linalg.norm(orc.spmv(A, x) - b) <= 1e-6
+
+
+def test_tfqmr_finite_termination():
+    """TFQMR's iterates span the same Krylov space as CGS's: on a diagonal
+    matrix with two distinct eigenvalues the quasi-residual tau reaches zero
+    (to rounding) within two outer iterations and x is exact."""
+    vals = np.repeat([1.0, 3.0], 9)
+    n = len(vals)
+    A = orc.Matrix(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), vals)
+    b = np.random.default_rng(4).uniform(0.5, 1.5, n)
+    x, st, it, _ = orc.tfqmr(A, b, None, None, max_iter=10, rtol=1e-12)
+    assert st == 0 and it.i <= 2
+    assert np.allclose(x, b / vals, rtol=1e-10)
+
+
+@pytest.mark.parametrize("lr", [("ilu_0", None), (None, "ilu_0"), ("diagonal", "ilu_0")])
+def test_tfqmr_nonsymmetric_left_right(lr):
+    rng = np.random.default_rng(13)
+    n = 60
+    D = rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < 0.1)
+    np.fill_diagonal(D, np.abs(D).sum(1) + 1.0)
+    A = from_dense(D)
+    xt = rng.standard_normal(n)
+    b = D @ xt
+    L = orc.Preconditioner(lr[0], A) if lr[0] else None
+    R = orc.Preconditioner(lr[1], A) if lr[1] else None
+    x, st, it, _ = orc.tfqmr(A, b, L, R, max_iter=400, rtol=1e-13)
+    assert st == 0
+    # with R the loop iterates on R x; x = solve(R, x) at the end recovers x
+    assert np.allclose(x, xt, rtol=1e-7, atol=1e-8)
+
+
+def test_tfqmr_empty_is_logic_error():
+    A = orc.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    x, st, it, _ = orc.tfqmr(A, np.zeros(0), None, None, max_iter=3, rtol=1e-8)
+    assert st == orc.INVALID
