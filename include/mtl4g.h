@@ -70,6 +70,7 @@ extern "C" {
 #define MTL4G_BICGSTAB 1   /* itl::bicgstab, itl/krylov/bicgstab.hpp:27-78 */
 #define MTL4G_CGS      2   /* itl::cgs,      itl/krylov/cgs.hpp:24-62 (FP64 only) */
 #define MTL4G_TFQMR    3   /* itl::tfqmr,    itl/krylov/tfqmr.hpp:32-96 (FP64; mtl4g_tfqmr) */
+#define MTL4G_BICGSTAB_2 4 /* itl::bicgstab_2, itl/krylov/bicgstab_2.hpp:37-130 (FP64) */
 
 typedef struct mtl4g_matrix_s* mtl4g_matrix;
 typedef struct mtl4g_pc_s* mtl4g_pc;
@@ -247,6 +248,14 @@ int mtl4g_cgs(mtl4g_matrix A, mtl4g_pc P, double* d_x, const double* d_b,
  * iter.fail(1): returns 1.  An empty system is MTL4's logic_error. */
 int mtl4g_tfqmr(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* d_x, const double* d_b,
                 mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
+/* itl::bicgstab_2(A, x, b, L, iter) (bicgstab_2.hpp:37-130): BiCGSTAB(l)
+ * with l = 2 and left preconditioner L (NULL = identity).  finished() runs
+ * at the loop head on r_hat[0] and after each inner step on r_hat[j]; an
+ * exit there returns before that step's x += alpha*u_hat[0], as in MTL4.
+ * omega == 0 is iter.fail(3): returns 3 and leaves x unshifted (no x +=
+ * x0).  An empty system is MTL4's logic_error. */
+int mtl4g_bicgstab_2(mtl4g_matrix A, mtl4g_pc L, double* d_x, const double* d_b,
+                     mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
 
 /* Reusable solver: the workspace (MTL4 allocates it per call, cg.hpp:90)
  * stays with the handle, so repeated solves allocate nothing. */

@@ -23,7 +23,8 @@ from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, ORDER_MTL4, ORDE
                    missing_diagonal, nonfinite_error, unexpected_orthogonality)
 
 __all__ = ["compressed2D", "dense_vector", "pc", "solve", "adjoint_solve", "basic_iteration",
-           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "tfqmr", "dot", "unary_dot",
+           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "tfqmr", "bicgstab_2", "dot",
+           "unary_dot",
            "two_norm", "num_rows", "num_cols", "size", "laplacian_setup", "ir_solve",
            "MtlError", "matrix_not_square", "incompatible_size", "missing_diagonal",
            "unexpected_orthogonality", "nonfinite_error", "logic_error"]
@@ -384,9 +385,18 @@ class cyclic_iteration(basic_iteration):
         self.cycle = int(cycle)
         self.out = out or sys.stdout
         self._i0 = 0
+        self.multi_print = False
+
+    def set_multi_print(self, m):
+        """Print every finished() call of a printing iteration, also several
+        with one iteration number (cyclic_iteration.hpp:44-50)."""
+        self.multi_print = bool(m)
+
+    def is_multi_print(self):
+        return self.multi_print
 
     def _wants_history(self):
-        return 2 * max(self.max_iter - self.i, 0) + 4
+        return 3 * max(self.max_iter - self.i, 0) + 4
 
     def _state(self):
         self._i0 = self.i
@@ -397,10 +407,13 @@ class cyclic_iteration(basic_iteration):
         if self.quite:
             return
         # finished() calls: CG one per iteration; BiCGSTAB r at the loop head,
-        # then s (after ++iter) and r again per iteration
+        # then s (after ++iter) and r again per iteration; BiCGSTAB(2) the
+        # loop head, then r_hat[0] and r_hat[1] after ++iter
         counts, i = [], self._i0
         for k in range(len(self.history)):
             if method == _lib.BICGSTAB and k > 0 and k % 2 == 1:
+                i += 1
+            elif method == _lib.BICGSTAB_2 and k % 3 == 1:
                 i += 1
             elif method in (_lib.CG, _lib.CGS) and k > 0:
                 i += 1
@@ -409,7 +422,7 @@ class cyclic_iteration(basic_iteration):
             counts.append(i)
         last = -1
         for i, r in zip(counts, self.history):
-            if i % self.cycle == 0 and i != last:
+            if i % self.cycle == 0 and (self.multi_print or i != last):
                 print("iteration %d: resid %g" % (i, r), file=self.out)
                 last = i
 
@@ -450,7 +463,8 @@ def _krylov(fn, method, A, x, b, P, it, R=None):
     h = hist[:cap]
     h = h[~np.isnan(h)] if cap else None
     it._update(st, h, method)
-    check(status, {CG: "cg", BICGSTAB: "bicgstab", _lib.CGS: "cgs"}.get(method, "tfqmr"))
+    check(status, {CG: "cg", BICGSTAB: "bicgstab", _lib.CGS: "cgs", _lib.BICGSTAB_2: "bicgstab_2"}
+          .get(method, "tfqmr"))
     return it.error_code()
 
 
@@ -479,6 +493,15 @@ def tfqmr(A, x, b, L=None, R=None, it=None):
     """itl::tfqmr(A, x, b, L, R, iter) (itl/krylov/tfqmr.hpp:32-96): left
     preconditioner L, right preconditioner R (None: identity)."""
     return _krylov(_L().mtl4g_tfqmr, _lib.TFQMR, A, x, b, L, it, R)
+
+
+def bicgstab_2(A, x, b, L=None, it=None):
+    """itl::bicgstab_2(A, x, b, L, iter) (itl/krylov/bicgstab_2.hpp:37-130):
+    BiCGSTAB(l) with l = 2, left preconditioner L (None: identity).  Returns
+    3 (iter.fail(3)) on omega == 0."""
+    if isinstance(L, basic_iteration):
+        L, it = None, L
+    return _krylov(_L().mtl4g_bicgstab_2, _lib.BICGSTAB_2, A, x, b, L, it)
 
 
 class Solver:

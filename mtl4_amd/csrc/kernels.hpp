@@ -34,7 +34,8 @@ constexpr int kFanIn = 128;                                  // workgroups per t
 constexpr int kMaxGroups = kMaxPartials / kFanIn;            // 512
 constexpr int kTicketStride = 32;                            // one ticket per 128-byte line
 constexpr size_t kTicketWords = (size_t)(1 + kMaxGroups) * kTicketStride;
-constexpr size_t kPartialWords = 2 * (size_t)(kMaxPartials + kMaxGroups);
+constexpr int kMaxNP = 3;                                    // reductions fused into one launch
+constexpr size_t kPartialWords = kMaxNP * (size_t)(kMaxPartials + kMaxGroups);
 
 // Two-level combine of per-workgroup partial sums.  Workgroup b publishes its
 // NP partials, then takes the ticket of its group of kFanIn workgroups; the
@@ -48,6 +49,7 @@ template <int BS, typename AT, typename Epi>
 __device__ __forceinline__ void combine_partials(const AT* part, AT* partials, unsigned* tickets,
                                                  int b, int G, const Epi& epi, AT* sh) {
     constexpr int NP = Epi::NP;
+    static_assert(NP <= kMaxNP, "partials are sized for kMaxNP reductions per launch");
     const int ngroups = (G + kFanIn - 1) / kFanIn;
     const int g = b / kFanIn;
     const int gsize = min(kFanIn, G - g * kFanIn);

@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
+#include <utility>
 
 #include "kernels.hpp"
 #include "krylov.hpp"
@@ -783,6 +785,289 @@ struct EpiTfV {
     __device__ void finish(const VT*) const {}
 };
 
+// ---------------------------------------------------------------- BiCGSTAB(2) epilogues
+// itl::bicgstab_2 with l = 2, itl/krylov/bicgstab_2.hpp:37-130.  Vectors
+// r_hat[0..2], u_hat[0..2], r0_tilde, y, x0 and the preconditioner's
+// scratch.  Every reduction is fused into the pass that produces (or last
+// reads) its operands; with an identity preconditioner the dots on
+// u_hat[j+1] = A*u_hat[j] and r_hat[j+1] = A*r_hat[j] run in the product's
+// epilogue, otherwise in a pass after solve(L, y).
+
+// two_norm(x) != 0 decides the shift of the problem  (:52-59)
+struct EpiB2XNorm {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_NORM4; }
+    const double* x;
+    Ctl* c;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double* part) const { part[0] += x[i] * x[i]; }
+    __device__ void finish(const double* tot) const { c->shift = std::sqrt(tot[0]) != 0.0; }
+};
+
+// r_hat[0] = b (-= A*x and x0 = x when shifted); two_norm(r_hat[0])  (:53-61)
+struct EpiB2Init {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_NORM4; }
+    struct Pre {
+        double b, x;
+    };
+    const double *b, *x;
+    double *r0, *x0;
+    Ctl* c;
+    int shift;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() { shift = c->shift; }
+    __device__ Pre preload(int row) const { return Pre{b[row], x[row]}; }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
+        const double r = shift ? pre.b - s : pre.b;
+        r0[row] = r;
+        x0[row] = shift ? pre.x : 0.0;
+        part[0] += r * r;
+    }
+    __device__ void finish(const double* tot) const { c->nr = std::sqrt(tot[0]); }
+};
+
+// x = 0 when shifted (after the product above has read it); r0_tilde =
+// r_hat[0] / two_norm(r_hat[0]); u_hat[0] = 0  (:58, :61, :64)
+struct EpiB2Start {
+    static constexpr int NP = 0;
+    double *x, *rt, *u0;
+    const double* r0;
+    Ctl* c;
+    int shift;
+    double nr;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {
+        shift = c->shift;
+        nr = c->nr;
+    }
+    __device__ void operator()(int64_t i, double*) const {
+        if (shift) x[i] = 0.0;
+        rt[i] = r0[i] / nr;
+        u0[i] = 0.0;
+    }
+    __device__ void finish(const double*) const {}
+};
+
+// Loop head: finished(r_hat[0]) and ++iter; rho_0 = -omega*rho_0; then the
+// first inner step's rho_1 = dot(r0_tilde, r_hat[0]); beta = alpha*rho_1/rho_0;
+// rho_0 = rho_1  (:70-76)
+__device__ __forceinline__ void b2_head(Ctl* c, const double* tot, double* hist) {
+    iter_finished(c, std::sqrt(tot[0]), hist);
+    if (c->done) return;
+    c->rho = -c->omega * c->rho;
+    c->rho_1 = tot[1];
+    c->beta = c->alpha * tot[1] / c->rho;
+    c->rho = tot[1];
+}
+
+// UPD: the minimal-residual updates of x, r_hat[0] and u_hat[0] (:119-126)
+// in their order per element, then the loop head on the new r_hat[0]
+template <bool UPD>
+struct EpiB2Head {
+    static constexpr int NP = 2;
+    __host__ __device__ static constexpr int order(int k) { return k == 0 ? RO_NORM4 : RO_DOT4; }
+    double *x, *r0, *u0;
+    const double *r1, *r2, *u1, *u2, *rt;
+    Ctl* c;
+    double* hist;
+    double g1, g2, ga1, ga2, gaa1;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        if (UPD) {
+            g1 = c->g1;
+            g2 = c->g2;
+            ga1 = c->ga1;
+            ga2 = c->ga2;
+            gaa1 = c->gaa1;
+        }
+    }
+    __device__ void operator()(int64_t i, double* part) const {
+        double r = r0[i];
+        if (UPD) {
+            const double a = r1[i];
+            x[i] = (x[i] + g1 * r) + gaa1 * a;
+            r = (r - ga2 * r2[i]) - ga1 * a;
+            r0[i] = r;
+            u0[i] = (u0[i] - g2 * u2[i]) - g1 * u1[i];
+        }
+        part[0] += r * r;
+        part[1] += rt[i] * r;
+    }
+    __device__ void finish(const double* tot) const { b2_head(c, tot, hist); }
+};
+
+// u_hat[i] = r_hat[i] - beta*u_hat[i] for i <= j  (:78-79)
+struct EpiB2U {
+    static constexpr int NP = 0;
+    const double *r0, *r1;
+    double *u0, *u1;   // u1 == nullptr: j = 0
+    Ctl* c;
+    double beta;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { beta = c->beta; }
+    __device__ void operator()(int64_t i, double*) const {
+        u0[i] = r0[i] - beta * u0[i];
+        if (u1) u1[i] = r1[i] - beta * u1[i];
+    }
+    __device__ void finish(const double*) const {}
+};
+
+// Reductions that finish with a product (or with the pass after solve(L, .)):
+//   GAMMA: Gamma = dot(r0_tilde, u_hat[j+1]); alpha = rho_0/Gamma  (:83-84)
+//   RHO:   the next inner step's rho_1 = dot(r0_tilde, r_hat[1]); beta;
+//          rho_0 = rho_1  (:75-76)
+//   TAU:   tau[1][1] = dot(r_hat[1], r_hat[1]); tau[1][2] =
+//          dot(r_hat[1], r_hat[2]) / tau[1][1]  (orth.hpp:75, :87-89)
+enum { B2_NONE = 0, B2_GAMMA = 1, B2_RHO = 2, B2_TAU = 3 };
+__host__ __device__ constexpr int b2_np(int mode) { return mode == B2_NONE ? 0 : mode == B2_TAU ? 2 : 1; }
+
+__device__ __forceinline__ void b2_finish(int mode, Ctl* c, const double* tot) {
+    if (mode == B2_GAMMA) {
+        c->gamma = tot[0];
+        c->alpha = c->rho / tot[0];
+    } else if (mode == B2_RHO) {
+        c->rho_1 = tot[0];
+        c->beta = c->alpha * tot[0] / c->rho;
+        c->rho = tot[0];
+    } else if (mode == B2_TAU) {
+        c->t11 = tot[0];
+        c->t12 = tot[1] / tot[0];
+    }
+}
+
+// out = A*in; XUP: x += alpha*u_hat[0] (:96, rows of x are not gathered);
+// a = r0_tilde (GAMMA, RHO) or r_hat[1] (TAU)
+template <int MODE, bool XUP>
+struct EpiB2Mv {
+    static constexpr int NP = b2_np(MODE);
+    __host__ __device__ static constexpr int order(int) { return RO_DOT4; }
+    struct Pre {
+        double x, u, a;
+    };
+    const double* a;
+    double* out;
+    double* x;
+    const double* u0;
+    Ctl* c;
+    double alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { alpha = c->alpha; }
+    __device__ Pre preload(int row) const {
+        return Pre{XUP ? x[row] : 0.0, XUP ? u0[row] : 0.0, NP ? a[row] : 0.0};
+    }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
+        out[row] = s;
+        if (XUP) x[row] = pre.x + alpha * pre.u;
+        if (MODE == B2_TAU) {
+            part[0] += pre.a * pre.a;
+            part[1] += pre.a * s;
+        } else if (NP) {
+            part[0] += pre.a * s;
+        }
+    }
+    __device__ void finish(const double* tot) const { b2_finish(MODE, c, tot); }
+};
+
+// The same reductions over v = solve(L, y)
+template <int MODE>
+struct EpiB2Dot {
+    static constexpr int NP = b2_np(MODE);
+    __host__ __device__ static constexpr int order(int) { return RO_DOT4; }
+    const double *a, *v;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double* part) const {
+        const double ai = a[i], vi = v[i];
+        if (MODE == B2_TAU) {
+            part[0] += ai * ai;
+            part[1] += ai * vi;
+        } else {
+            part[0] += ai * vi;
+        }
+    }
+    __device__ void finish(const double* tot) const { b2_finish(MODE, c, tot); }
+};
+
+// r_hat[i] -= alpha*u_hat[i+1] for i <= j; finished(r_hat[j]) without
+// ++iter (:86-92); r1 == nullptr: j = 0
+struct EpiB2R {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_NORM4; }
+    double *r0, *r1;
+    const double *u1, *u2;
+    Ctl* c;
+    double* hist;
+    double alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { alpha = c->alpha; }
+    __device__ void operator()(int64_t i, double* part) const {
+        const double v0 = r0[i] - alpha * u1[i];
+        r0[i] = v0;
+        if (r1) {
+            const double v1 = r1[i] - alpha * u2[i];
+            r1[i] = v1;
+            part[0] += v1 * v1;
+        } else {
+            part[0] += v0 * v0;
+        }
+    }
+    __device__ void finish(const double* tot) const { iter_check(c, std::sqrt(tot[0]), hist); }
+};
+
+// Minimal-residual part: r_hat[2] -= tau[1][2]*r_hat[1]; tau[2][2] =
+// dot(r_hat[2], r_hat[2]) (orth.hpp:87-93); gamma_a[j] = dot(r_hat[j],
+// r_hat[0]) / tau[j][j]; omega = gamma[2] = gamma_a[2]; omega == 0 is
+// iter.fail(3) (returned before the shift back); gamma[1] = gamma_a[1] -
+// tau[1][2]*gamma[2]; gamma_aa[1] = 0 + gamma[2] (the strictly upper part
+// of a 1x1 matrix is empty)  (bicgstab_2.hpp:100-117)
+struct EpiB2MR {
+    static constexpr int NP = 3;
+    __host__ __device__ static constexpr int order(int) { return RO_DOT4; }
+    double* r2;
+    const double *r1, *r0;
+    Ctl* c;
+    double t12;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { t12 = c->t12; }
+    __device__ void operator()(int64_t i, double* part) const {
+        const double a = r1[i];
+        const double v = r2[i] - t12 * a;
+        r2[i] = v;
+        const double z = r0[i];
+        part[0] += v * v;
+        part[1] += a * z;
+        part[2] += v * z;
+    }
+    __device__ void finish(const double* tot) const {
+        c->t22 = tot[0];
+        c->ga1 = tot[1] / c->t11;
+        c->ga2 = tot[2] / tot[0];
+        c->g2 = c->ga2;
+        c->omega = c->g2;
+        if (c->omega == 0.0) {
+            c->error = 3;
+            c->done = 1;
+            return;
+        }
+        c->g1 = c->ga1 - (0.0 + c->t12 * c->g2);
+        c->gaa1 = 0.0 + c->g2;
+    }
+};
+
+// x += x0: undo the shift (:90, :128)
+struct EpiB2Shift {
+    static constexpr int NP = 0;
+    double* x;
+    const double* x0;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double*) const { x[i] = x[i] + x0[i]; }
+    __device__ void finish(const double*) const {}
+};
+
 // ---------------------------------------------------------------- FP64 refresh
 // x += double(x32); x32 = 0
 struct EpiRu1 {
@@ -872,8 +1157,8 @@ static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_p
     if (!W.h_ctl) M4_HIP(hipHostMalloc((void**)&W.h_ctl, sizeof(Ctl), hipHostMallocDefault));
     if (hist_cap > 0 && W.hist.n < (size_t)hist_cap) M4_TRY(W.hist.alloc(hist_cap));
     if (!W.hist.p) M4_TRY(W.hist.alloc(1));
-    if (sizeof(VT) == 8 && reduction_order() == MTL4G_ORDER_MTL4 && W.terms.n < 2 * (size_t)n)
-        M4_TRY(W.terms.alloc(2 * (size_t)n));
+    if (sizeof(VT) == 8 && reduction_order() == MTL4G_ORDER_MTL4 && W.terms.n < kMaxNP * (size_t)n)
+        M4_TRY(W.terms.alloc(kMaxNP * (size_t)n));
     size_t words = P && (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) ? pc_counter_words(P) : 1;
     if (W.pc_counters.n < words) {
         M4_TRY(W.pc_counters.alloc(words));
@@ -1222,6 +1507,122 @@ int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
     return hc.error;
 }
 
+// itl::bicgstab_2 (bicgstab_2.hpp:37-130) with every scalar on the device.
+// With L = identity solve(L, y) is a copy: the products write u_hat[j+1]
+// and r_hat[j+1] directly.  Ten launches per iteration with four products.
+int bicgstab2_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, Workspace<double>& W, double* x,
+                  const double* b, mtl4g_iteration* it, double* h_hist, int32_t hist_cap,
+                  hipStream_t s) {
+    if (!A || !it || (A->n && (!x || !b))) return MTL4G_ERR_INVALID;
+    if (A->n != A->m) {
+        set_error("Krylov solve needs a square matrix");
+        return MTL4G_ERR_NOT_SQUARE;
+    }
+    if (L && L->n != A->n) {
+        set_error("preconditioner size %d != matrix size %d", L->n, A->n);
+        return MTL4G_ERR_SIZE;
+    }
+    if (A->n == 0) {   // bicgstab_2.hpp:45
+        set_error("empty rhs vector");
+        return MTL4G_ERR_INVALID;
+    }
+    if (L && L->kind == MTL4G_PC_IDENTITY) L = nullptr;
+    M4_TRY(ensure_workspace(W, A->n, 10, L, hist_cap, s));
+    const int64_t n = A->n;
+    double* base = W.vecs.p;
+    double *r0 = base, *r1 = base + n, *r2 = base + 2 * n, *u0 = base + 3 * n, *u1 = base + 4 * n,
+           *u2 = base + 5 * n, *rt = base + 6 * n, *y = base + 7 * n, *x0 = base + 8 * n,
+           *y0 = base + 9 * n;
+    Ctl init;
+    std::memset(&init, 0, sizeof init);
+    init.norm_r0 = it->norm_r0;
+    init.rtol = it->rtol;
+    init.atol = it->atol;
+    init.max_iter = it->max_iter;
+    init.i = it->i;
+    init.hist_cap = h_hist ? hist_cap : 0;
+    init.rho = 1.0;     // rho_0
+    init.omega = 1.0;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
+    M4_HIP(hipGetLastError());
+    Ctl* c = W.ctl.p;
+    double* part = W.partials.p;
+    double* tm = W.terms.p;
+    unsigned* tk = W.tickets.p;
+    const double* val = A->val.p;
+    // shift problem; r0_tilde; r_hat[0] = solve(L, r_hat[0]); u_hat[0] = 0  (:52-64)
+    M4_TRY(launch_vec<double>(n, EpiB2XNorm{x, c}, part, tk, s, tm));
+    M4_TRY(launch_csr<double>(A, val, x, EpiB2Init{b, x, r0, x0, c, 0}, part, tk, s, tm));
+    M4_TRY(launch_vec<double>(n, EpiB2Start{x, rt, u0, r0, c, 0, 0.0}, part, tk, s));
+    if (L) {   // y = solve(L, r_hat[0]); r_hat[0] = y: the two buffers swap roles
+        M4_TRY(pc_apply<double>(L, r0, y, y0, nullptr, W.pc_counters.p, s));
+        std::swap(r0, y);
+    }
+    M4_TRY(launch_vec<double>(n, EpiB2Head<false>{x, r0, u0, r1, r2, u1, u2, rt, c, W.hist.p}, part, tk, s, tm));
+    // out = solve(L, A*in) with the reductions of MODE on out; XUP adds alpha*u_hat[0] to x
+    auto product = [&](const double* in, double* out, auto mode, auto xup, const double* a) -> int {
+        constexpr int MODE = decltype(mode)::value;
+        constexpr bool XUP = decltype(xup)::value;
+        if (!L) return launch_csr<double>(A, val, in, EpiB2Mv<MODE, XUP>{a, out, x, u0, c, 0.0}, part, tk, s, tm);
+        M4_TRY((launch_csr<double>(A, val, in, EpiB2Mv<B2_NONE, XUP>{a, y, x, u0, c, 0.0}, part, tk, s)));
+        M4_TRY(pc_apply<double>(L, y, out, y0, &c->done, W.pc_counters.p, s));
+        return launch_vec<double>(n, EpiB2Dot<MODE>{a, out, c}, part, tk, s, tm);
+    };
+    using Gamma = std::integral_constant<int, B2_GAMMA>;
+    using Rho = std::integral_constant<int, B2_RHO>;
+    using Tau = std::integral_constant<int, B2_TAU>;
+    using NoX = std::false_type;
+    using AddX = std::true_type;
+    auto iteration = [&]() -> int {
+        // j = 0  (:75-96; rho_1 and beta came with the loop head)
+        M4_TRY(launch_vec<double>(n, EpiB2U{r0, nullptr, u0, nullptr, c, 0.0}, part, tk, s));
+        M4_TRY(product(u0, u1, Gamma{}, NoX{}, rt));
+        M4_TRY(launch_vec<double>(n, EpiB2R{r0, nullptr, u1, nullptr, c, W.hist.p, 0.0}, part, tk, s, tm));
+        M4_TRY(product(r0, r1, Rho{}, AddX{}, rt));
+        // j = 1
+        M4_TRY(launch_vec<double>(n, EpiB2U{r0, r1, u0, u1, c, 0.0}, part, tk, s));
+        M4_TRY(product(u1, u2, Gamma{}, NoX{}, rt));
+        M4_TRY(launch_vec<double>(n, EpiB2R{r0, r1, u1, u2, c, W.hist.p, 0.0}, part, tk, s, tm));
+        M4_TRY(product(r1, r2, Tau{}, AddX{}, r1));
+        // minimal-residual part, then the loop head
+        M4_TRY(launch_vec<double>(n, EpiB2MR{r2, r1, r0, c, 0.0}, part, tk, s, tm));
+        return launch_vec<double>(n, EpiB2Head<true>{x, r0, u0, r1, r2, u1, u2, rt, c, W.hist.p},
+                                  part, tk, s, tm);
+    };
+    const int32_t budget = std::max(0, it->max_iter - it->i) + 8;
+    int32_t launched = 0, chunk = 2;
+    for (;;) {
+        M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        M4_HIP(hipStreamSynchronize(s));
+        if (W.h_ctl->done) break;
+        if (launched > budget) {
+            set_error("solver did not stop after %d launched iterations", launched);
+            return MTL4G_ERR_INVALID;
+        }
+        for (int k = 0; k < chunk; k++) M4_TRY(iteration());
+        launched += chunk;
+        chunk = std::min(chunk * 2, 32);
+    }
+    const Ctl& hc = *W.h_ctl;
+    if (hc.error != 3) M4_TRY(launch_vec<double>(n, EpiB2Shift{x, x0}, part, tk, s));
+    if (L) M4_TRY(pc_check_error(L, s));
+    M4_HIP(hipStreamSynchronize(s));
+    it->i = hc.i;
+    it->error = hc.error;
+    it->resid = hc.resid;
+    if (h_hist && hist_cap > 0) {
+        int32_t nh = std::min(hc.hist_n, hist_cap);
+        if (nh > 0) M4_HIP(hipMemcpy(h_hist, W.hist.p, (size_t)nh * sizeof(double), hipMemcpyDeviceToHost));
+        for (int32_t k = nh; k < hist_cap; k++) h_hist[k] = NAN;
+    }
+    if (hc.status != 0) {
+        set_error("non-finite residual");
+        return hc.status;
+    }
+    if (hc.error == 3) set_error("bicg breakdown #2");
+    return hc.error;
+}
+
 template int krylov_run<double>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<double>&,
                                 double*, const double*, mtl4g_iteration*, double*, int32_t, hipStream_t);
 template int krylov_run<float>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<float>&,
@@ -1345,7 +1746,8 @@ using namespace mtl4g;
 extern "C" {
 
 int mtl4g_solver_create(mtl4g_solver* out, mtl4g_matrix A, mtl4g_pc P, int method) {
-    if (!out || !A || (method != MTL4G_CG && method != MTL4G_BICGSTAB && method != MTL4G_CGS))
+    if (!out || !A ||
+        (method != MTL4G_CG && method != MTL4G_BICGSTAB && method != MTL4G_CGS && method != MTL4G_BICGSTAB_2))
         return MTL4G_ERR_INVALID;
     mtl4g_solver_s* S_ = new (std::nothrow) mtl4g_solver_s;
     if (!S_) return MTL4G_ERR_OOM;
@@ -1364,6 +1766,8 @@ int mtl4g_solver_destroy(mtl4g_solver S_) {
 int mtl4g_solver_solve(mtl4g_solver S_, double* x, const double* b, mtl4g_iteration* it,
                        double* h_hist, int32_t hist_cap, void* stream) {
     if (!S_) return MTL4G_ERR_INVALID;
+    if (S_->method == MTL4G_BICGSTAB_2)
+        return bicgstab2_run(S_->A, S_->P, S_->W, x, b, it, h_hist, hist_cap, S(stream));
     return krylov_run<double>(S_->A, S_->P, S_->method, MTL4G_F64, S_->W, x, b, it, h_hist,
                               hist_cap, S(stream));
 }
@@ -1384,6 +1788,12 @@ int mtl4g_cgs(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iter
               double* h_hist, int32_t hist_cap, void* stream) {
     Workspace<double> W;
     return krylov_run<double>(A, P, MTL4G_CGS, MTL4G_F64, W, x, b, it, h_hist, hist_cap, S(stream));
+}
+
+int mtl4g_bicgstab_2(mtl4g_matrix A, mtl4g_pc L, double* x, const double* b, mtl4g_iteration* it,
+                     double* h_hist, int32_t hist_cap, void* stream) {
+    Workspace<double> W;
+    return bicgstab2_run(A, L, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,

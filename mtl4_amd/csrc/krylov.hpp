@@ -11,6 +11,9 @@ struct Ctl {
     double rho, rho_1, rr, alpha, alpha_1, beta, omega, gamma, rho_2;
     double theta, eta, tau;  // tfqmr
     int32_t looped;          // tfqmr: the loop was entered (x is shifted back)
+    // bicgstab_2: tau of the minimal-residual part, gamma, gamma_a, gamma_aa
+    double t11, t12, t22, ga1, ga2, g1, g2, gaa1, nr;
+    int32_t shift;           // bicgstab_2: x was nonzero, the problem is shifted
     double norm_r0, rtol, atol, resid;
     double rmax, delta;      // mixed precision: residual at the last FP64 refresh, trigger
     int32_t i, max_iter, error, done, first, half, status;
@@ -31,7 +34,7 @@ struct Workspace {
     DevBuf<Ctl> ctl;
     DevBuf<double> hist;
     DevBuf<unsigned> pc_counters;
-    DevBuf<double> terms;     // 2n, reductions in MTL4's order (FP64 only)
+    DevBuf<double> terms;     // 3n, reductions in MTL4's order (FP64 only)
     Ctl* h_ctl = nullptr;     // pinned mirror for polling
     ~Workspace() {
         if (h_ctl) (void)hipHostFree(h_ctl);
@@ -49,6 +52,11 @@ int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int val
 int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R, Workspace<double>& W,
               double* x, const double* b, mtl4g_iteration* it, double* h_hist, int32_t hist_cap,
               hipStream_t s);
+
+// itl::bicgstab_2(A, x, b, L, iter) on the device (L may be NULL = identity).
+int bicgstab2_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, Workspace<double>& W, double* x,
+                  const double* b, mtl4g_iteration* it, double* h_hist, int32_t hist_cap,
+                  hipStream_t s);
 
 // Mixed precision with reliable updates: one FP32 recurrence (values in
 // value_format), solution accumulated in FP64, the true residual b - A*x

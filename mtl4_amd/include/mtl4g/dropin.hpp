@@ -1069,8 +1069,9 @@ class cyclic_iteration : public basic_iteration<Real> {
     void replay_(const std::vector<double>& hist, int method, int i0) override {
         int i = i0;
         for (std::size_t k = 0; k < hist.size(); k++) {
-            if (method == MTL4G_TFQMR ? k > 1   // finished(rt) before the loop, then tau
-                                      : (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)))   // cg, cgs: one per iteration
+            if (method == MTL4G_TFQMR        ? k > 1        // finished(rt) before the loop, then tau
+                : method == MTL4G_BICGSTAB_2 ? k % 3 == 1   // loop head, then r_hat[0] and r_hat[1]
+                                             : (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)))   // cg, cgs: one per iteration
                 i++;
             this->i = i;
             this->resid_ = hist[k];
@@ -1261,7 +1262,7 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
     mtl4g_iteration s = iteration_access::state(iter);
     const int i0 = s.i;
     const bool want = iteration_access::wants_history(iter);
-    const int cap = want ? 2 * std::max(s.max_iter - s.i, 0) + 4 : 0;
+    const int cap = want ? 3 * std::max(s.max_iter - s.i, 0) + 4 : 0;
     std::vector<double> hist(cap);
     mtl4g_matrix h = A.device_();
     const double* db = b.device_read_();
@@ -1270,6 +1271,7 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
     int st = method == MTL4G_CG      ? mtl4g_cg(h, P, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_CGS   ? mtl4g_cgs(h, P, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_TFQMR ? mtl4g_tfqmr(h, P, R, dx, db, &s, hp, cap, nullptr)
+             : method == MTL4G_BICGSTAB_2 ? mtl4g_bicgstab_2(h, P, dx, db, &s, hp, cap, nullptr)
                                      : mtl4g_bicgstab(h, P, dx, db, &s, hp, cap, nullptr);
     std::vector<double> used;
     for (double v : hist)
@@ -1330,9 +1332,18 @@ int tfqmr(const Matrix& A, Vector& x, const Vector& b, const LeftPreconditioner&
     return iter;
 }
 
-// itl::base_solver, cg_solver, bicgstab_solver, cgs_solver, tfqmr_solver
-// (krylov/base_solver.hpp:24-84, cg.hpp:122-156, bicgstab.hpp:80-112,
-// cgs.hpp:64-95, tfqmr.hpp:98-127)
+// itl::bicgstab_2 (krylov/bicgstab_2.hpp:37-130): BiCGSTAB(l), l = 2, left
+// preconditioner; omega == 0 returns iter.fail(3)
+template <typename LinearOperator, typename Vector, typename Preconditioner, typename Iteration>
+int bicgstab_2(const LinearOperator& A, Vector& x, const Vector& b, const Preconditioner& L, Iteration& iter) {
+    if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
+    detail::run(MTL4G_BICGSTAB_2, A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter);
+    return iter;
+}
+
+// itl::base_solver, cg_solver, bicgstab_solver, cgs_solver, tfqmr_solver,
+// bicgstab_2_solver (krylov/base_solver.hpp:24-84, cg.hpp:122-156,
+// bicgstab.hpp:80-112, cgs.hpp:64-95, tfqmr.hpp:98-127, bicgstab_2.hpp:132-165)
 template <typename Solver, typename LinearOperator>
 struct base_solver {
     typedef double magnitude_type;
@@ -1450,6 +1461,29 @@ class tfqmr_solver
   private:
     Preconditioner L;
     RightPreconditioner R;
+};
+
+// right preconditioner ignored, with a warning (bicgstab_2.hpp:132-165)
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class bicgstab_2_solver
+    : public base_solver<bicgstab_2_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<bicgstab_2_solver, LinearOperator> base;
+
+  public:
+    explicit bicgstab_2_solver(const LinearOperator& A) : base(A), L(A) { warn(); }
+    bicgstab_2_solver(const LinearOperator& A, const Preconditioner& L) : base(A), L(L) { warn(); }
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const {
+        return bicgstab_2(this->A, x, b, L, iter);
+    }
+
+  private:
+    static void warn() {
+        if (!std::is_same<RightPreconditioner, pc::identity<LinearOperator>>::value)
+            std::cerr << "Right Preconditioner ignored!" << std::endl;
+    }
+    Preconditioner L;
 };
 
 }  // namespace itl

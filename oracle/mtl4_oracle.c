@@ -765,3 +765,86 @@ out:
     free(wk);
     return st != ORC_OK ? st : it->error;
 }
+
+/* itl::bicgstab_2 (l = 2), itl/krylov/bicgstab_2.hpp:37-130, with left
+ * preconditioner L (NULL: identity).  The minimal-residual part runs
+ * orthogonalize_factors (operation/orth.hpp:61-96, the loop without
+ * MTL_WITH_FUSED_ORTHOGONALIZATION) on r_hat[1..2] in place: r_hat_tail
+ * is constructed from the sub-vector r_hat[irange(1, imax)], which holds
+ * r_hat's own elements (dense_vector.hpp:326-338) and is taken over, not
+ * copied.  gamma_aa[1] = strict_upper(1x1) * gamma[2] + gamma[2] is the
+ * empty product (zero) plus gamma[2].  omega == 0 is iter.fail(3) and
+ * returns before the shift back x += x0; every other exit adds x0.     */
+int orc_bicgstab_2(const orc_csr* A, const orc_pc* L, double* x, const double* b,
+                   orc_iter* it, double* hist, int32_t hist_cap)
+{
+    int32_t n = A->n, nh = 0;
+    if (A->n != A->ncols) return ORC_NOT_SQUARE;
+    if (n == 0) return ORC_INVALID;                       /* "empty rhs vector" */
+    size_t sn = (size_t)n;
+    double* wk = (double*)calloc(sn * 9, sizeof(double));
+    if (!wk) return ORC_OOM;
+    double* r[3] = {wk, wk + sn, wk + 2 * sn};
+    double* u[3] = {wk + 3 * sn, wk + 4 * sn, wk + 5 * sn};
+    double *rt = wk + 6 * sn, *y = wk + 7 * sn, *x0 = wk + 8 * sn;   /* u[0] = x0 = 0 */
+    double rho_0 = 1.0, rho_1, alpha = 0.0, Gamma, beta, omega = 1.0;
+    double t11, t12, t22, ga1, ga2, g1, g2, gaa1, nr;
+    int st = ORC_OK;
+    it->error = 0;
+#define B2_TRY(e) do { st = (e); if (st != ORC_OK) goto out; } while (0)
+    memcpy(r[0], b, sn * sizeof(double));                 /* r_hat[0] = b */
+    if (orc_two_norm(n, x) != 0.0) {                      /* shift problem */
+        orc_spmv(A, x, r[0], ORC_MINUS);
+        memcpy(x0, x, sn * sizeof(double));
+        memset(x, 0, sn * sizeof(double));
+    }
+    nr = orc_two_norm(n, r[0]);
+    for (int32_t i = 0; i < n; i++) rt[i] = r[0][i] / nr; /* r0_tilde */
+    B2_TRY(tf_solve(L, r[0], y, n));
+    memcpy(r[0], y, sn * sizeof(double));
+    while (!finished(it, orc_two_norm(n, r[0]), hist, hist_cap, &nh)) {
+        ++it->i;
+        rho_0 = -omega * rho_0;
+        for (int j = 0; j < 2; j++) {
+            rho_1 = orc_dot(n, rt, r[j], 4);
+            beta = alpha * rho_1 / rho_0;
+            rho_0 = rho_1;
+            for (int k = 0; k <= j; k++)
+                for (int32_t i = 0; i < n; i++) u[k][i] = r[k][i] - beta * u[k][i];
+            orc_spmv(A, u[j], y, ORC_ASSIGN);
+            B2_TRY(tf_solve(L, y, u[j + 1], n));
+            Gamma = orc_dot(n, rt, u[j + 1], 4);
+            alpha = rho_0 / Gamma;
+            for (int k = 0; k <= j; k++)
+                for (int32_t i = 0; i < n; i++) r[k][i] -= alpha * u[k + 1][i];
+            if (finished(it, orc_two_norm(n, r[j]), hist, hist_cap, &nh)) goto shift_back;
+            orc_spmv(A, r[j], y, ORC_ASSIGN);
+            B2_TRY(tf_solve(L, y, r[j + 1], n));
+            for (int32_t i = 0; i < n; i++) x[i] += alpha * u[0][i];
+        }
+        /* orthogonalize_factors(r_hat[1..2]) */
+        t11 = orc_dot(n, r[1], r[1], 4);
+        t12 = orc_dot(n, r[1], r[2], 4) / t11;
+        for (int32_t i = 0; i < n; i++) r[2][i] -= t12 * r[1][i];
+        t22 = orc_dot(n, r[2], r[2], 4);
+        ga1 = orc_dot(n, r[1], r[0], 4) / t11;
+        ga2 = orc_dot(n, r[2], r[0], 4) / t22;
+        g2 = ga2;
+        omega = g2;
+        if (omega == 0.0) { it->error = 3; goto out; }    /* "bicg breakdown #2" */
+        g1 = ga1 - (0.0 + t12 * g2);
+        gaa1 = 0.0 + g2;
+        for (int32_t i = 0; i < n; i++) x[i] += g1 * r[0][i];
+        for (int32_t i = 0; i < n; i++) r[0][i] -= ga2 * r[2][i];
+        for (int32_t i = 0; i < n; i++) u[0][i] -= g2 * u[2][i];
+        for (int32_t i = 0; i < n; i++) u[0][i] -= g1 * u[1][i];
+        for (int32_t i = 0; i < n; i++) x[i] += gaa1 * r[1][i];
+        for (int32_t i = 0; i < n; i++) r[0][i] -= ga1 * r[1][i];
+    }
+shift_back:
+    for (int32_t i = 0; i < n; i++) x[i] += x0[i];
+out:
+#undef B2_TRY
+    free(wk);
+    return st != ORC_OK ? st : it->error;
+}

@@ -88,6 +88,8 @@ int orc_tfqmr(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, con
               orc_iter* it, double* hist, int32_t hist_cap);
 int orc_bicgstab(const orc_csr* A, const orc_pc* P, double* x, const double* b,
                  orc_iter* it, double* hist, int32_t hist_cap);
+int orc_bicgstab_2(const orc_csr* A, const orc_pc* L, double* x, const double* b,
+                   orc_iter* it, double* hist, int32_t hist_cap);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,8 @@ def lib():
         L.orc_cgs.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
         L.orc_tfqmr.restype = C.c_int
         L.orc_tfqmr.argtypes = [P, P, P, P, P, C.POINTER(Iter), P, I32]
+        L.orc_bicgstab_2.restype = C.c_int
+        L.orc_bicgstab_2.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
         _lib = L
     return _lib
 
@@ -265,3 +267,10 @@ def tfqmr(A, b, L=None, R=None, x0=None, it=None, hist_cap=0, **kw):
                          _p(x), _p(b), C.byref(it), _p(hist), hist_cap)
     nh = int(np.argmax(np.isnan(hist))) if np.isnan(hist).any() else len(hist)
     return x, st, it, hist[:min(nh, hist_cap)]
+
+
+def bicgstab_2(A, b, L=None, x0=None, it=None, hist_cap=0, **kw):
+    """itl::bicgstab_2(A, x, b, L, iter), itl/krylov/bicgstab_2.hpp:37-130."""
+    it = iteration(b, **kw) if it is None else it
+    x, st, hist = _solve(lib().orc_bicgstab_2, A, b, L, x0, it, hist_cap)
+    return x, st, it, hist

@@ -167,6 +167,39 @@ int main() {
         CHECK(it2.iterations() == iter.iterations());
     });
 
+    // libs/numeric/itl/test/ic_0_bicgstab_2_test.cpp: 10x10 laplacian, ic_0,
+    // cyclic_iteration(b, 200, 1e-6, 0, 1) with multi_print
+    run("ic_0_bicgstab_2_test", [] {
+        const int size = 10, N = size * size;
+        matrix_type A(N, N);
+        laplacian_setup(A, size, size);
+        itl::pc::ic_0<matrix_type> P(A);
+        vector_type x(N, 1.0), b(N);
+        b = A * x;
+        x = 0;
+        std::ostringstream log;
+        itl::cyclic_iteration<double, std::ostringstream> iter(b, 200, 1.e-6, 0.0, 1, log);
+        iter.set_multi_print(true);
+        bicgstab_2(A, x, b, P, iter);
+        CHECK(iter.is_converged());
+        CHECK(iter.iterations() >= 2);
+        // iteration 1: finished() after both inner steps and at the loop head
+        std::string text = log.str();
+        int ones = 0;
+        for (std::size_t at = text.find("iteration 1: "); at != std::string::npos;
+             at = text.find("iteration 1: ", at + 1))
+            ones++;
+        CHECK(ones == 3);
+        vector_type r(b);
+        r -= A * x;
+        CHECK(two_norm(r) <= 1e-4 * two_norm(b));
+        itl::bicgstab_2_solver<matrix_type, itl::pc::ic_0<matrix_type>> S(A, P);
+        vector_type y(N, 0.0);
+        itl::basic_iteration<double> it2(b, 200, 1.e-6);
+        CHECK(S.solve(y, b, it2) == 0);
+        CHECK(it2.iterations() == iter.iterations());
+    });
+
     // libs/numeric/itl/test/tfqmr_test.cpp: 10x10 laplacian, b = 1, the
     // same preconditioner left and right
     run("tfqmr_test", [] {

@@ -176,6 +176,57 @@ def test_tfqmr_bitwise(m, mtl4_order, fam, lr):
     assert np.array_equal(x.numpy(), xo)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ilu_0", "ic_0"])
+@pytest.mark.parametrize("start", ["zero", "nonzero"])
+def test_bicgstab_2_bitwise(m, mtl4_order, fam, kind, start):
+    """itl::bicgstab_2 in MTL4's reduction order: iteration count, every
+    residual handed to finished() (loop head and both inner steps), and the
+    solution after the shift back bit for bit with the oracle; a nonzero
+    start takes the shifted-problem branch (bicgstab_2.hpp:55-59)."""
+    if kind == "ic_0" and fam != "lap2d_64":
+        pytest.skip("ic_0 needs a symmetric matrix")
+    rp, ci, va = family(fam, seed=9)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    rng = np.random.default_rng(109)
+    xt = rng.uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    x0 = np.zeros(n) if start == "zero" else rng.uniform(-1, 1, n)
+    OP = None if kind == "identity" else orc.Preconditioner(kind, O)
+    xo, st, ito, ho = orc.bicgstab_2(O, b, OP, x0=x0, max_iter=2000, rtol=1e-10, hist_cap=7000)
+    x = dv(m, x0)
+    it = m.cyclic_iteration(dv(m, b), 2000, 1e-10, 0.0, 10**9)
+    it.set_quite(True)
+    it.suppress_resume(True)
+    err = m.bicgstab_2(A, x, dv(m, b), getattr(m.pc, kind)(A), it)
+    assert err == st == 0
+    assert it.iterations() == ito.i
+    assert it.resid() == ito.resid
+    assert np.array_equal(np.asarray(it.history), ho)
+    assert np.array_equal(x.numpy(), xo)
+
+
+def test_bicgstab_2_omega_breakdown(m, mtl4_order):
+    """omega == 0 is iter.fail(3) (bicgstab_2.hpp:107): status 3 after one
+    iteration and x returned without the shift back x += x0, bit for bit
+    with the oracle (whose own breakdown case is pinned in the CPU tests)."""
+    D = np.array([[1.0, 0.0, -2.0], [0.0, -2.0, 1.0], [2.0, 2.0, -1.0]])
+    x_init = np.array([1.0, 0.0, 0.0])
+    b = np.array([-2.0, 0.0, 0.0]) + D @ x_init
+    rows, cols = np.nonzero(D)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=3))]).astype(np.int32)
+    O = orc.Matrix(rp, cols.astype(np.int32), D[rows, cols].copy())
+    A = m.compressed2D(3, 3, rp, cols.astype(np.int32), D[rows, cols].copy())
+    xo, st, ito, _ = orc.bicgstab_2(O, b, None, x0=x_init, max_iter=5, rtol=1e-14)
+    assert st == 3 and ito.i == 1
+    x = dv(m, x_init)
+    it = m.basic_iteration(dv(m, b), 5, 1e-14)
+    assert m.bicgstab_2(A, x, dv(m, b), None, it) == 3
+    assert it.iterations() == 1 and it.error_code() == 3
+    assert np.array_equal(x.numpy(), xo)
+
+
 def test_cg_config1_bitwise(m, mtl4_order):
     """CG + diagonal on the 64x64 Laplacian, the reference's CPU test case."""
     rp, ci, va = family("lap2d_64")

@@ -683,6 +683,85 @@ def test_cgs_solver_handle_and_mixed_refusal(m):
         m.ir_solve(A, x, dv(m, b), it, m.pc.diagonal(A, m.F32), method=m._lib.CGS, value_format=m.F32)
 
 
+def _pc_residual(O, OP, b, x):
+    r = b - orc.spmv(O, x)
+    return np.linalg.norm(OP.solve(r) if OP is not None else r)
+
+
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000", "diff3d_16"])
+@pytest.mark.parametrize("kind", ["identity", "diagonal", "ilu_0"])
+def test_bicgstab_2_families(m, fam, kind):
+    """itl::bicgstab_2 in tree order against the oracle: iterations within
+    2 + 15 %.  The returned x carries the preconditioned residual that the
+    loop last checked at its head, or -- after an exit at an inner step,
+    which returns before that step's x update -- the one checked before it;
+    the device's x must satisfy that relation with its own residuals."""
+    rp, ci, va = family(fam, seed=4)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(3).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OP = None if kind == "identity" else orc.Preconditioner(kind, O)
+    xo, st, ito, ho = orc.bicgstab_2(O, b, OP, max_iter=3000, rtol=1e-10, hist_cap=10000)
+    x = m.dense_vector(n)
+    it = m.cyclic_iteration(dv(m, b), 3000, 1e-10)
+    it.set_quite(True)
+    it.suppress_resume(True)
+    err = m.bicgstab_2(A, x, dv(m, b), getattr(m.pc, kind)(A), it)
+    assert err == st == 0
+    assert abs(it.iterations() - ito.i) <= 2 + 0.15 * ito.i, (it.iterations(), ito.i)
+    for xs, h in ((xo, ho), (x.numpy(), np.asarray(it.history))):
+        want = h[-1] if len(h) % 3 == 1 else h[-2]
+        got = _pc_residual(O, OP, b, xs)
+        assert abs(got - want) <= 1e-6 * want + 1e-13 * np.linalg.norm(b), (got, want, len(h))
+
+
+def test_bicgstab_2_reference_case(m):
+    """ic_0_bicgstab_2_test.cpp: 10x10 laplacian_setup, ic_0, x = 1, b = A*x,
+    x = 0, cyclic_iteration(b, 200, 1e-6, 0, 1) with multi_print: every
+    finished() call prints, three of them with iteration number 1."""
+    import io
+    A = m.laplacian_setup(10, 10)
+    P = m.pc.ic_0(A)
+    b = A * m.dense_vector(np.ones(100))
+    x = m.dense_vector(100)
+    out = io.StringIO()
+    it = m.cyclic_iteration(b, 200, 1e-6, 0.0, 1, out=out)
+    it.set_multi_print(True)
+    assert m.bicgstab_2(A, x, b, P, it) == 0
+    lines = [l for l in out.getvalue().splitlines() if l.startswith("iteration ")]
+    assert len(lines) == len(it.history) and it.iterations() >= 2
+    assert sum(l.startswith("iteration 1: ") for l in lines) == 3
+    O = orc.laplacian_setup(10, 10)
+    xo, st, ito, _ = orc.bicgstab_2(O, orc.spmv(O, np.ones(100)), orc.Preconditioner("ic_0", O),
+                                    max_iter=200, rtol=1e-6)
+    assert st == 0 and abs(it.iterations() - ito.i) <= 1
+    with pytest.raises(m.MtlError):
+        E = m.compressed2D(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        m.bicgstab_2(E, m.dense_vector(0), m.dense_vector(0), None, m.basic_iteration(0.0, 3, 1e-8))
+
+
+def test_bicgstab_2_solver_handle(m):
+    """A solver handle for BiCGSTAB(2) keeps its workspace: two solves give
+    the same bits as each other and as the one-shot call."""
+    rp, ci, va = family("convdiff_40", seed=1)
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.ones(A.num_rows()))
+    S = m.Solver(A, m.pc.ilu_0(A), method=m._lib.BICGSTAB_2)
+    out = []
+    for _ in range(2):
+        x = m.dense_vector(A.num_rows())
+        it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+        assert S.solve(x, dv(m, b), it) == 0
+        out.append((it.iterations(), x.numpy()))
+    x = m.dense_vector(A.num_rows())
+    it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+    assert m.bicgstab_2(A, x, dv(m, b), m.pc.ilu_0(A), it) == 0
+    out.append((it.iterations(), x.numpy()))
+    assert out[0][0] == out[1][0] == out[2][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][1], out[2][1])
+
+
 def test_bicgstab_breakdown(m):
     A = m.compressed2D(2, 2, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.zeros(2))
     x = m.dense_vector(2)

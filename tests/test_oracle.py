@@ -558,3 +558,136 @@ def test_tfqmr_empty_is_logic_error():
     A = orc.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
     x, st, it, _ = orc.tfqmr(A, np.zeros(0), None, None, max_iter=3, rtol=1e-8)
     assert st == orc.INVALID
+
+
+# ---------------------------------------------------------------- BiCGSTAB(2)
+def _pg_residual(M, r0, rt, k):
+    """The BiCG residual after k steps from its characterization: r in r0 +
+    span{M r0, ..., M^k r0} with r orthogonal to rt, M^T rt, ..., (M^T)^(k-1) rt
+    (Petrov-Galerkin), solved densely."""
+    V = np.column_stack([np.linalg.matrix_power(M, p) @ r0 for p in range(1, k + 1)])
+    W = np.column_stack([np.linalg.matrix_power(M.T, p) @ rt for p in range(k)])
+    return r0 - V @ np.linalg.solve(W.T @ V, W.T @ r0)
+
+
+def _b2_pc_residual(A, P, b, x):
+    r = b - orc.spmv(A, x)
+    return np.linalg.norm(P.solve(r) if P is not None else r)
+
+
+def _nonsym(n, seed, density=0.2):
+    rng = np.random.default_rng(seed)
+    D = rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < density)
+    np.fill_diagonal(D, np.abs(D).sum(1) + 1.0)
+    return D, rng
+
+
+@pytest.mark.parametrize("pc", ["identity", "diagonal"])
+def test_bicgstab_2_checked_residuals(pc):
+    """Every residual bicgstab_2 hands to finished() in its first one and a
+    half iterations, from the Krylov characterizations instead of the
+    recurrences: with M = L^-1 A, r0 = L^-1 b and the shadow residual
+    rtilde = b/|b| (taken before solve(L, .), bicgstab_2.hpp:61-63),
+      head:        |r0|
+      j=0 check:   |r_1|,   r_1 the Petrov-Galerkin BiCG residual of step 1
+      j=1 check:   |M r_2|  (the check is on r_hat[1] = M r_hat[0])
+      next head:   min over g of |r_2 - g1 M r_2 - g2 M^2 r_2| (the MR part)
+      next j=0:    |Q(M) r_3| with Q(t) = 1 - g1 t - g2 t^2 from that minimum.
+    A wrong sign in rho_0 = -omega*rho_0, a dropped term of the MR
+    coefficients or a copy instead of the in-place orthogonalization each
+    moves one of these."""
+    n = 40
+    D, rng = _nonsym(n, 21)
+    A = from_dense(D)
+    b = rng.standard_normal(n)
+    P = orc.Preconditioner(pc, A) if pc != "identity" else None
+    Minv = np.diag(1.0 / np.diag(D)) if P is not None else np.eye(n)
+    M = Minv @ D
+    x, st, it, h = orc.bicgstab_2(A, b, P, max_iter=2, rtol=1e-14, hist_cap=20)
+    assert st == 1 and it.i == 2 and len(h) == 5   # stops at the j=0 check of iteration 2
+    r0 = Minv @ b
+    rt = b / np.linalg.norm(b)
+    r2 = _pg_residual(M, r0, rt, 2)
+    G = np.column_stack([M @ r2, M @ M @ r2])
+    g = np.linalg.lstsq(G, r2, rcond=None)[0]
+    r3 = _pg_residual(M, r0, rt, 3)
+    q3 = r3 - g[0] * (M @ r3) - g[1] * (M @ (M @ r3))
+    want = [np.linalg.norm(r0), np.linalg.norm(_pg_residual(M, r0, rt, 1)), np.linalg.norm(M @ r2),
+            np.linalg.norm(r2 - G @ g), np.linalg.norm(q3)]
+    assert np.allclose(h, want, rtol=1e-10, atol=0)
+    # the exit at an inner check returns before that step's x update: x has
+    # the residual of the previous check, the loop head
+    assert _b2_pc_residual(A, P, b, x) == pytest.approx(h[3], rel=1e-10)
+
+
+@pytest.mark.parametrize("k,iters,calls", [(2, 1, 3), (3, 2, 5), (4, 2, 6), (5, 3, 8)])
+def test_bicgstab_2_finite_termination(k, iters, calls):
+    """With k distinct eigenvalues the BiCG residual of step k is zero, and
+    the first check after that step stops the solve: step s is the inner
+    step j = (s-1) % 2 of iteration (s+1)//2 (head, j=0, j=1 per iteration)."""
+    vals = np.repeat([1.0, 2.0, 5.0, 3.0, 7.0][:k], 6)
+    n = len(vals)
+    A = orc.Matrix(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), vals)
+    b = np.random.default_rng(k).uniform(0.5, 1.5, n)
+    x, st, it, h = orc.bicgstab_2(A, b, None, max_iter=10, rtol=1e-11, hist_cap=40)
+    assert st == 0 and it.i == iters and len(h) == calls
+    assert h[-1] <= 1e-11 * np.linalg.norm(b) and min(h[:-1]) > 1e-6
+    assert _b2_pc_residual(A, None, b, x) == pytest.approx(h[-2], rel=1e-8)
+
+
+@pytest.mark.parametrize("pc", ["identity", "diagonal", "ilu_0"])
+@pytest.mark.parametrize("start", ["zero", "nonzero"])
+def test_bicgstab_2_solution_tracks_residual(pc, start):
+    """On a nonsymmetric system solved to 1e-12: the returned x (after the
+    shift back by x0) has the preconditioned residual of the loop head when
+    the head stopped, of the check before otherwise; and it solves A x = b."""
+    n = 80
+    D, rng = _nonsym(n, 31, 0.1)
+    A = from_dense(D)
+    xt = rng.standard_normal(n)
+    b = D @ xt
+    x0 = np.zeros(n) if start == "zero" else rng.standard_normal(n)
+    P = orc.Preconditioner(pc, A) if pc != "identity" else None
+    x, st, it, h = orc.bicgstab_2(A, b, P, x0=x0, max_iter=200, rtol=1e-12, hist_cap=700)
+    assert st == 0
+    want = h[-1] if len(h) % 3 == 1 else h[-2]
+    assert _b2_pc_residual(A, P, b, x) == pytest.approx(want, rel=1e-6, abs=1e-14 * np.linalg.norm(b))
+    assert np.allclose(x, xt, rtol=1e-6, atol=1e-7)
+
+
+def test_bicgstab_2_reference_case():
+    """The reference's ic_0_bicgstab_2_test.cpp case: 10x10 laplacian_setup,
+    ic_0, x = 1, b = A*x, x = 0, cyclic_iteration(b, 200, 1e-6)."""
+    A = orc.laplacian_setup(10, 10)
+    b = orc.spmv(A, np.ones(A.n))
+    P = orc.Preconditioner("ic_0", A)
+    x, st, it, h = orc.bicgstab_2(A, b, P, max_iter=200, rtol=1e-6, hist_cap=700)
+    assert st == 0 and 2 <= it.i < 20
+    assert h[-1] <= 1e-6 * np.linalg.norm(b)
+    assert np.allclose(x, 1.0, atol=1e-4)
+
+
+def test_bicgstab_2_omega_breakdown():
+    """A 3x3 integer system where the minimal-residual step's last
+    coefficient is zero: omega == 0 is iter.fail(3), returned before the
+    shift back.  Pinned by least squares: min |r_2 - g1 A r_2 - g2 A^2 r_2|
+    has g2 = 0, and the returned x is the shifted problem's BiCG iterate
+    (A x = r0 - r_2) without x0 added."""
+    D = np.array([[1.0, 0.0, -2.0], [0.0, -2.0, 1.0], [2.0, 2.0, -1.0]])
+    x_init = np.array([1.0, 0.0, 0.0])
+    r0 = np.array([-2.0, 0.0, 0.0])
+    b = r0 + D @ x_init
+    x, st, it, h = orc.bicgstab_2(from_dense(D), b, None, x0=x_init, max_iter=5, rtol=1e-14, hist_cap=10)
+    assert st == 3 and it.i == 1 and len(h) == 3
+    r2 = _pg_residual(D, r0, r0 / np.linalg.norm(r0), 2)
+    G = np.column_stack([D @ r2, D @ D @ r2])
+    g = np.linalg.lstsq(G, r2, rcond=None)[0]
+    assert abs(g[1]) <= 1e-12 * abs(g[0])
+    assert np.allclose(D @ x, r0 - r2, atol=1e-12)   # x0 = x_init not added
+    assert np.linalg.norm(D @ x_init) > 1.0
+
+
+def test_bicgstab_2_empty_is_logic_error():
+    A = orc.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    x, st, it, _ = orc.bicgstab_2(A, np.zeros(0), None, max_iter=3, rtol=1e-8)
+    assert st == orc.INVALID
