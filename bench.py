@@ -178,7 +178,17 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     residuals in FP64 and the mixed solution's error against the FP64 one."""
     n = A.num_rows()
     log("%s: preconditioners (%s, n=%d)" % ("cg" if method == m.CG else "bicgstab", pc, n))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     P64 = getattr(m.pc, pc)(A)
+    t_setup = time.perf_counter() - t0
+    P64.refactor()   # warm: the factorization kernels' first launch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P64.refactor()   # the device factorization alone, with the plan made at construction
+    torch.cuda.synchronize()
+    t_refactor = time.perf_counter() - t0
+    log("  setup %.3f s (pattern plan, device factorization, sweeps), refactor %.4f s" % (t_setup, t_refactor))
     P32 = getattr(m.pc, pc)(A, m.F32)
     S = m.Solver(A, P64, method=method)
     x = m.dense_vector(n)
@@ -208,7 +218,8 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
             "mixed_f32": {"solves_per_s": round(1 / tm, 4), "fp32_iterations": st["inner_total"],
                           "fp64_refreshes": st["outer"], "error_code": errm, "failure": failure,
                           "relres": relres_of(m, A, xm, b, bnorm), "rel_err_vs_fp64": rel_err},
-            "speedup": round(t64 / tm, 3), "steps": 1}
+            "speedup": round(t64 / tm, 3), "steps": 1,
+            "pc_setup_s": round(t_setup, 4), "pc_refactor_s": round(t_refactor, 5)}
 
 
 def c3_section(m, torch, gen):

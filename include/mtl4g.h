@@ -210,9 +210,11 @@ int mtl4g_axpy_unary_dot(int64_t n, double alpha, const double* d_q, double* d_r
 /* ---- preconditioners ------------------------------------------------ */
 /* itl::pc::identity(A) (identity.hpp:32), diagonal(A) (diagonal.hpp:37-45),
  * ilu_0<Matrix,Value>(A) (ilu_0.hpp:34-92 with ilu.hpp:54-56), ic_0<Matrix,
- * Value>(A) (ic_0.hpp:57,135-186).  Factorization runs on the host in FP64
- * in MTL4's loop order; value_format selects the stored factor precision.
- * Sweeps run level-scheduled on the device. */
+ * Value>(A) (ic_0.hpp:57,135-186).  The ILU(0)/IC(0) factorization runs on
+ * the device in FP64, level-scheduled, each row in MTL4's loop order (the
+ * factors equal MTL4's bit for bit); value_format selects the stored factor
+ * precision.  Sweeps run level-scheduled on the device.  The handle keeps
+ * A: destroy it before A. */
 int mtl4g_pc_create(mtl4g_pc* P, mtl4g_matrix A, int kind, int value_format);
 int mtl4g_pc_destroy(mtl4g_pc P);
 /* solve(P, x) into y: P.solve(x, y) (pc/solver.hpp:23-52) */
@@ -226,6 +228,13 @@ int mtl4g_pc_factor(mtl4g_pc P, int which, int32_t* n_out, int64_t* nnz_out,
                     int32_t* h_rowptr, int32_t* h_col, double* h_val);
 /* Number of dependency levels of the lower / upper sweep. */
 int mtl4g_pc_levels(mtl4g_pc P, int32_t* lower_levels, int32_t* upper_levels);
+/* Factor again from A's current values (after mtl4g_csr_set_values), same
+ * pattern: ILU(0)/IC(0) rerun the device factorization on `stream` with
+ * the plan made at creation (levels, positions), nothing on the host; the
+ * diagonal PC recomputes its reciprocals.  No MTL4 counterpart (MTL4
+ * factors only in the constructor); the result equals a new
+ * mtl4g_pc_create(A) bit for bit.  Not concurrent with solves using P. */
+int mtl4g_pc_refactor(mtl4g_pc P, void* stream);
 
 /* ---- Krylov solvers ------------------------------------------------- */
 /* itl::cg(A, x, b, P, iter) (cg.hpp:74-111; P == NULL or identity takes

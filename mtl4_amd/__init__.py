@@ -297,6 +297,12 @@ class _PC:
         check(_L().mtl4g_pc_levels(self.h, C.byref(lo), C.byref(up)))
         return lo.value, up.value
 
+    def refactor(self):
+        """Factor again from the matrix's current values (same pattern), on
+        the device with the plan made at construction."""
+        check(_L().mtl4g_pc_refactor(self.h, _stream()), "refactor")
+        return self
+
 
 class _PCNamespace:
     class identity(_PC):

@@ -100,6 +100,7 @@ _SIGS = {
     "mtl4g_pc_adjoint_solve": (_INT, [_P, _P, _P, _P]),
     "mtl4g_pc_factor": (_INT, [_P, _INT, C.POINTER(_I32), C.POINTER(_I64), _P, _P, _P]),
     "mtl4g_pc_levels": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32)]),
+    "mtl4g_pc_refactor": (_INT, [_P, _P]),
     "mtl4g_cg": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_bicgstab": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_cgs": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),

@@ -1,7 +1,8 @@
 // precond.hip -- itl::pc::identity / diagonal / ilu_0 / ic_0 on the device.
 //
-// Factorizations run on the host in FP64 in MTL4's loop order (so the
-// stored factors are the ones MTL4 computes); the triangular sweeps run on
+// ILU(0)/IC(0) factor on the GPU in FP64, each row in MTL4's loop order
+// (so the stored factors are the ones MTL4 computes), level scheduled like
+// the sweeps; the host plans only the pattern.  The triangular sweeps run on
 // the GPU, level scheduled.  A sweep is one persistent kernel: workgroups
 // take tickets in order, each owns up to kSweepBS rows of one dependency
 // level, waits until every workgroup of the previous level has published
@@ -172,17 +173,20 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
     }
     std::vector<int32_t> rowptr(n + 1, 0);
     int64_t nnz = R.rowptr[n];
-    std::vector<int32_t> col(nnz);
-    std::vector<double> val(nnz), dia(n, 1.0);
+    const bool gathered = !R.dsrc.empty();   // dsrc has one entry per row on that path
+    std::vector<int32_t> col(nnz), src(gathered ? nnz : 0), dsrc(gathered ? n : 0);
+    std::vector<double> val(gathered ? 0 : nnz), dia(gathered ? 0 : n, 1.0);
     for (int32_t k = 0; k < n; k++) {
         int32_t r = perm[k];
         int32_t len = R.rowptr[r + 1] - R.rowptr[r];
         rowptr[k + 1] = rowptr[k] + len;
         for (int32_t t = 0; t < len; t++) {
             col[rowptr[k] + t] = R.col[R.rowptr[r] + t];
-            val[rowptr[k] + t] = R.val[R.rowptr[r] + t];
+            if (gathered) src[rowptr[k] + t] = R.src[R.rowptr[r] + t];
+            else val[rowptr[k] + t] = R.val[R.rowptr[r] + t];
         }
-        if (!R.unit) dia[k] = R.dia[r];
+        if (gathered) dsrc[k] = R.unit ? -1 : R.dsrc[r];
+        else if (!R.unit) dia[k] = R.dia[r];
     }
     // workgroups: ceil(rows / kSweepBS) per level
     std::vector<int32_t> lev_blk(nlev + 1, 0), blk_level;
@@ -193,6 +197,7 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
         for (int32_t i = 0; i < nb; i++) blk_level.push_back(l);
     }
     S->n = n;
+    S->nnz = nnz;
     S->nlev = nlev;
     S->nblocks = lev_blk[nlev];
     S->unit = R.unit;
@@ -209,7 +214,17 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
     M4_TRY(up(S->lev_ptr, lev_ptr));
     M4_TRY(up(S->lev_blk, lev_blk));
     M4_TRY(up(S->blk_level, blk_level));
-    if (fmt == MTL4G_F64) {
+    if (gathered) {   // values come later from the device factor (fill_sweep)
+        M4_TRY(up(S->src, src));
+        M4_TRY(up(S->dsrc, dsrc));
+        if (fmt == MTL4G_F64) {
+            M4_TRY(S->val64.alloc(std::max<int64_t>(nnz, 1)));
+            M4_TRY(S->dia64.alloc(std::max(n, 1)));
+        } else {
+            M4_TRY(S->val32.alloc(std::max<int64_t>(nnz, 1)));
+            M4_TRY(S->dia32.alloc(std::max(n, 1)));
+        }
+    } else if (fmt == MTL4G_F64) {
         M4_TRY(up(S->val64, val));
         M4_TRY(up(S->dia64, dia));
     } else {
@@ -224,6 +239,143 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
     // complete before any stream (including non-blocking ones) uses the counters
     M4_HIP(hipDeviceSynchronize());
     return MTL4G_OK;
+}
+
+// ------------------------------------------------------------------ device factorization
+// ILU(0) and IC(0) factor row by row in the lower sweep's level order, with
+// the sweep's ticket and level-counter protocol: a row starts once every
+// row its strictly lower entries name is final.  Values other rows write
+// are read with agent-scope loads and written with agent-scope stores.
+struct FactorView {
+    const int32_t *perm, *lev_ptr, *lev_blk, *blk_level;
+    const int32_t *lrp, *lcol, *lsrc;   // IC(0): lower sweep rows (level order), U positions
+    int32_t nlev;
+};
+
+// ilu_0_factorizer::factorize (itl/pc/ilu_0.hpp:45-82), row r: each
+// strictly lower entry k in column order becomes a_rk * inv_dia[k], then
+// every later entry j of the row loses a_rk * a_kj (a_kj looked up in row
+// k by column, 0 where row k has none); inv_dia[r] = 1 / a_rr.
+__device__ __forceinline__ void ilu0_row(int r, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                         const int32_t* __restrict__ dpos, double* lu, double* finv) {
+    const int a = rp[r], e = rp[r + 1];
+    for (int kc = a; kc < e; kc++) {
+        const int k = ci[kc];
+        if (k == r) break;
+        const double ark = ld_agent(lu + kc) * ld_agent(finv + k);
+        st_agent(lu + kc, ark);
+        const int ka = rp[k], ke = rp[k + 1];
+        for (int jc = kc + 1; jc < e; jc++) {
+            const int j = ci[jc];
+            int lo = ka, hi = ke;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ci[mid] < j) lo = mid + 1;
+                else hi = mid;
+            }
+            const double akj = (lo < ke && ci[lo] == j) ? ld_agent(lu + lo) : 0.0;
+            st_agent(lu + jc, ld_agent(lu + jc) - ark * akj);
+        }
+    }
+    st_agent(finv + r, 1.0 / ld_agent(lu + dpos[r]));
+}
+
+// ic_0::factorizer::factorize (itl/pc/ic_0.hpp:135-186) gathered by row:
+// U[r][r] loses d*d for every d = U[k][r] * inv_dia[k], k ascending (the
+// order the row-k steps reach it); inv_dia[r] = 1 / sqrt(U[r][r]).
+__device__ __forceinline__ void ic0_row(int q, int r, const FactorView& F, const int32_t* __restrict__ urp,
+                                        double* u, double* finv) {
+    double dg = ld_agent(u + urp[r]);
+    for (int t = F.lrp[q]; t < F.lrp[q + 1]; t++) {
+        const double d = ld_agent(u + F.lsrc[t]) * ld_agent(finv + F.lcol[t]);
+        dg = dg - d * d;
+    }
+    st_agent(finv + r, 1.0 / sqrt(dg));
+}
+
+// pos: ILU(0) diagonal positions in A; IC(0) row pointers of U
+template <int BS, bool IC>
+__global__ __launch_bounds__(BS) void k_factor(FactorView F, const int32_t* __restrict__ rp,
+                                               const int32_t* __restrict__ ci, const int32_t* __restrict__ pos,
+                                               double* fv, double* finv, unsigned* counters, int* err) {
+    __shared__ int s_b, s_ok, s_final;
+    if (threadIdx.x == 0)
+        s_b = (int)__hip_atomic_fetch_add(sweep_counter(counters, 0), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int b = s_b;
+    const int L = F.blk_level[b];
+    const int q = F.lev_ptr[L] + (b - F.lev_blk[L]) * BS + (int)threadIdx.x;
+    const bool mine = q < F.lev_ptr[L + 1];
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        if (L > 0) {
+            const unsigned need = (unsigned)(F.lev_blk[L] - F.lev_blk[L - 1]);
+            unsigned it = 0;
+            const unsigned* lev = sweep_counter(counters, 2 + L - 1);
+            while (ld_agent(lev) < need) {
+                if ((++it & 63u) == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) { ok = 0; break; }
+                if (it > kSpinLimit) { st_agent(sweep_counter(counters, 1), 1u); ok = 0; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    if (s_ok && mine) {
+        const int r = F.perm[q];
+        if (IC) ic0_row(q, r, F, pos, fv, finv);
+        else ilu0_row(r, rp, ci, pos, fv, finv);
+    }
+    wait_memory();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned need = (unsigned)(F.lev_blk[L + 1] - F.lev_blk[L]);
+        const unsigned done =
+            __hip_atomic_fetch_add(sweep_counter(counters, 2 + L), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        s_final = (L == F.nlev - 1 && done == need);
+    }
+    __syncthreads();
+    if (s_final) {   // every other workgroup has published: reset for the next launch
+        if (threadIdx.x == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) atomicOr(err, 1);
+        for (int i = threadIdx.x; i < F.nlev + 2; i += BS) st_agent(sweep_counter(counters, i), 0u);
+    }
+}
+
+// U's entries from A: u[p] = a[src[p]]
+__global__ void k_gather_values(int64_t nnz, const int32_t* __restrict__ src, const double* __restrict__ a,
+                                double* __restrict__ u) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        u[i] = a[src[i]];
+}
+
+// ILU(0) invert_diagonal (ilu_0.hpp:78-79): a_rr <- 1/a_rr, the inv_dia of the row
+__global__ void k_ilu0_finish(int32_t n, const int32_t* __restrict__ dpos, const double* __restrict__ finv,
+                              double* __restrict__ lu) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        lu[dpos[r]] = finv[r];
+}
+
+// IC(0) rows of U: U[r][r] <- inv_dia[r], U[r][i] <- U[r][i] * inv_dia[r]  (ic_0.hpp:143-150)
+__global__ void k_ic0_finish(int32_t n, const int32_t* __restrict__ urp, const double* __restrict__ finv,
+                             double* __restrict__ u) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double f = finv[r];
+        const int a = urp[r], e = urp[r + 1];
+        u[a] = f;
+        for (int p = a + 1; p < e; p++) u[p] = u[p] * f;
+    }
+}
+
+// sweep values from the factor: val[k] = fv[src[k]], dia[k] = fv[dsrc[k]] (1 for a unit diagonal)
+template <typename MT>
+__global__ void k_fill_sweep(int64_t nnz, int32_t n, const int32_t* __restrict__ src,
+                             const int32_t* __restrict__ dsrc, const double* __restrict__ fv,
+                             MT* __restrict__ val, MT* __restrict__ dia) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += stride)
+        val[i] = static_cast<MT>(fv[src[i]]);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+        dia[i] = dsrc[i] < 0 ? MT(1) : static_cast<MT>(fv[dsrc[i]]);
 }
 
 template <typename VT>
@@ -301,109 +453,205 @@ int pc_check_error(const mtl4g_pc_s* P, hipStream_t s) {
     return MTL4G_OK;
 }
 
-// ---- factorizations (host, FP64, MTL4's loop order)
+// ---- factorizations: the pattern is planned on the host, the values are
+// computed on the device (k_factor) in MTL4's loop order per row, so the
+// stored factors are the ones MTL4 computes bit for bit.
 
-// ilu_0_factorizer::factorize, itl/pc/ilu_0.hpp:45-82: IKJ elimination in
-// place on a copy of A, entries of row k looked up by column (a missing
-// entry reads 0), then invert_diagonal and the strict_lower / upper split.
-static int factor_ilu0(const mtl4g_matrix_s* A, mtl4g_pc_s* P) {
+int fill_sweep(Sweep& S, int fmt, const double* fval, hipStream_t s) {
+    if (S.n == 0) return MTL4G_OK;
+    const int grid = grid_for(std::max<int64_t>(S.nnz, S.n));
+    if (fmt == MTL4G_F64)
+        hipLaunchKernelGGL(k_fill_sweep<double>, dim3(grid), dim3(256), 0, s, S.nnz, S.n, S.src.p, S.dsrc.p, fval,
+                           S.val64.p, S.dia64.p);
+    else
+        hipLaunchKernelGGL(k_fill_sweep<float>, dim3(grid), dim3(256), 0, s, S.nnz, S.n, S.src.p, S.dsrc.p, fval,
+                           S.val32.p, S.dia32.p);
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
+// ilu_0 (itl/pc/ilu_0.hpp:45-82): L = strictly lower part (unit diagonal),
+// U = diagonal (inverted) and upper part, both in A's positions; every row
+// needs its diagonal entry.
+static int plan_ilu0(mtl4g_pc_s* P, SweepRows& lo, SweepRows& up, std::vector<int32_t>& dpos) {
+    const mtl4g_matrix_s* A = P->A;
     const int32_t n = A->n;
-    std::vector<double> lu = A->h_val;
     const auto& rp = A->h_rowptr;
     const auto& ci = A->h_col;
-    std::vector<double> inv_dia(n, 0.0);
-    std::vector<int64_t> dpos(n);
+    dpos.assign(n, 0);
     for (int32_t i = 0; i < n; i++) {
-        dpos[i] = find_entry(rp, ci, i, i);
-        if (dpos[i] < 0) {
+        const int64_t d = find_entry(rp, ci, i, i);
+        if (d < 0) {
             set_error("ilu_0: row %d has no diagonal entry", i);
             return MTL4G_ERR_MISSING_DIAG;
         }
-        for (int32_t kc = rp[i]; kc < rp[i + 1]; kc++) {
-            const int32_t k = ci[kc];
-            if (k == i) break;
-            const double aik = lu[kc] * inv_dia[k];
-            lu[kc] = aik;
-            for (int32_t jc = kc + 1; jc < rp[i + 1]; jc++) {
-                const int64_t kj = find_entry(rp, ci, k, ci[jc]);
-                lu[jc] = lu[jc] - aik * (kj >= 0 ? lu[kj] : 0.0);
-            }
-        }
-        inv_dia[i] = 1.0 / lu[dpos[i]];
+        dpos[i] = (int32_t)d;
     }
-    for (int32_t i = 0; i < n; i++) lu[dpos[i]] = 1.0 / lu[dpos[i]];
-    const bool f32 = P->fmt != MTL4G_F64;
-    auto store = [&](double v) { return f32 ? (double)(float)v : v; };
     P->L_rowptr.assign(n + 1, 0);
     P->U_rowptr.assign(n + 1, 0);
     for (int32_t r = 0; r < n; r++) {
         for (int32_t j = rp[r]; j < rp[r + 1]; j++) {
-            if (ci[j] < r) { P->L_col.push_back(ci[j]); P->L_val.push_back(store(lu[j])); }
-            else { P->U_col.push_back(ci[j]); P->U_val.push_back(store(lu[j])); }
+            if (ci[j] < r) { P->L_col.push_back(ci[j]); P->L_src.push_back(j); }
+            else { P->U_col.push_back(ci[j]); P->U_src.push_back(j); }
         }
         P->L_rowptr[r + 1] = (int32_t)P->L_col.size();
         P->U_rowptr[r + 1] = (int32_t)P->U_col.size();
     }
+    lo.n = up.n = n;
+    lo.lower = true;
+    lo.unit = true;
+    lo.rowptr = P->L_rowptr;
+    lo.col = P->L_col;
+    lo.src = P->L_src;
+    lo.dsrc.assign(n, -1);
+    up.lower = false;
+    up.unit = false;
+    up.rowptr.assign(n + 1, 0);
+    up.dsrc.assign(n, 0);
+    for (int32_t r = 0; r < n; r++) {   // inverse diagonal first (upper_trisolve v3)
+        up.dsrc[r] = P->U_src[P->U_rowptr[r]];
+        for (int32_t j = P->U_rowptr[r] + 1; j < P->U_rowptr[r + 1]; j++) {
+            up.col.push_back(P->U_col[j]);
+            up.src.push_back(P->U_src[j]);
+        }
+        up.rowptr[r + 1] = (int32_t)up.col.size();
+    }
     return MTL4G_OK;
 }
 
-// ic_0::factorizer::factorize, itl/pc/ic_0.hpp:135-186, on U = upper(A):
-// row k: U[k][k] <- 1/sqrt(U[k][k]); each later entry U[k][i] <- d =
-// U[k][i]*U[k][k]; then entries of row i with column in (k, i] get
-// U(j,i) -= d*U[k][j] -- in an upper factor only j = i can match.
-// Rounded to float afterwards for ic_0<Matrix,float> (ic_0.hpp:122-130).
-static int factor_ic0(const mtl4g_matrix_s* A, mtl4g_pc_s* P) {
+// ic_0 (itl/pc/ic_0.hpp:135-186) on U = upper(A), L = trans(U) (:57) with
+// the inverse diagonal last; values in U's order.  usrc: A position of
+// each U entry.
+static int plan_ic0(mtl4g_pc_s* P, SweepRows& lo, SweepRows& up, std::vector<int32_t>& usrc) {
+    const mtl4g_matrix_s* A = P->A;
     const int32_t n = A->n;
     auto& urp = P->U_rowptr;
     auto& uci = P->U_col;
-    auto& uva = P->U_val;
     urp.assign(n + 1, 0);
     for (int32_t r = 0; r < n; r++) {
         for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++)
-            if (A->h_col[j] >= r) { uci.push_back(A->h_col[j]); uva.push_back(A->h_val[j]); }
+            if (A->h_col[j] >= r) { uci.push_back(A->h_col[j]); usrc.push_back(j); }
         urp[r + 1] = (int32_t)uci.size();
     }
-    for (int32_t k = 0; k < n; k++) {
-        int32_t ic = urp[k];
-        const int32_t iend = urp[k + 1];
-        if (ic == iend || uci[ic] != k) {
+    for (int32_t k = 0; k < n; k++)
+        if (urp[k] == urp[k + 1] || uci[urp[k]] != k) {
             set_error("ic_0: row %d has no diagonal entry", k);
             return MTL4G_ERR_MISSING_DIAG;
         }
-        const double inv_dia = 1.0 / std::sqrt(uva[ic]);
-        uva[ic] = inv_dia;
-        for (++ic; ic < iend; ++ic) {
-            const double d = uva[ic] * inv_dia;
-            uva[ic] = d;
-            const int32_t i = uci[ic];
-            int32_t jc = urp[i], jend = urp[i + 1];
-            while (jc < jend && uci[jc] <= k) ++jc;
-            while (jend > jc && uci[jend - 1] > i) --jend;
-            for (; jc < jend; ++jc) {
-                const int32_t j = uci[jc];
-                const int64_t ji = find_entry(urp, uci, j, i);
-                const int64_t kj = find_entry(urp, uci, k, j);
-                if (ji >= 0) uva[ji] -= d * (kj >= 0 ? uva[kj] : 0.0);
-            }
-        }
-    }
-    if (P->fmt != MTL4G_F64)
-        for (auto& v : uva) v = (double)(float)v;
-    // L = trans(U) (ic_0.hpp:57): row r of L lists columns ascending, the
-    // (inverse) diagonal last.
+    P->U_src.resize(uci.size());
+    for (size_t p = 0; p < uci.size(); p++) P->U_src[p] = (int32_t)p;
     auto& lrp = P->L_rowptr;
     lrp.assign(n + 1, 0);
     for (int32_t c : uci) lrp[c + 1]++;
     for (int32_t r = 0; r < n; r++) lrp[r + 1] += lrp[r];
     P->L_col.resize(uci.size());
-    P->L_val.resize(uci.size());
+    P->L_src.resize(uci.size());
     std::vector<int32_t> fill(lrp.begin(), lrp.end() - 1);
     for (int32_t r = 0; r < n; r++)
         for (int32_t j = urp[r]; j < urp[r + 1]; j++) {
-            int32_t d = fill[uci[j]]++;
+            const int32_t d = fill[uci[j]]++;
             P->L_col[d] = r;
-            P->L_val[d] = uva[j];
+            P->L_src[d] = j;
         }
+    lo.n = up.n = n;
+    lo.lower = true;
+    lo.unit = false;
+    lo.rowptr.assign(n + 1, 0);
+    lo.dsrc.assign(n, 0);
+    for (int32_t r = 0; r < n; r++) {   // inverse diagonal last (lower_trisolve v6)
+        const int32_t a = lrp[r], e = lrp[r + 1] - 1;
+        lo.dsrc[r] = P->L_src[e];
+        for (int32_t j = a; j < e; j++) {
+            lo.col.push_back(P->L_col[j]);
+            lo.src.push_back(P->L_src[j]);
+        }
+        lo.rowptr[r + 1] = (int32_t)lo.col.size();
+    }
+    up.lower = false;
+    up.unit = false;
+    up.rowptr.assign(n + 1, 0);
+    up.dsrc.assign(n, 0);
+    for (int32_t r = 0; r < n; r++) {   // inverse diagonal first (upper_trisolve v3)
+        up.dsrc[r] = urp[r];
+        for (int32_t j = urp[r] + 1; j < urp[r + 1]; j++) {
+            up.col.push_back(uci[j]);
+            up.src.push_back(j);
+        }
+        up.rowptr[r + 1] = (int32_t)up.col.size();
+    }
+    return MTL4G_OK;
+}
+
+// Factor values from A's current device values, then the sweeps' values.
+static int factor_device(mtl4g_pc_s* P, hipStream_t s) {
+    const mtl4g_matrix_s* A = P->A;
+    const int32_t n = P->n;
+    if (n == 0) return MTL4G_OK;
+    const bool ic = P->kind == MTL4G_PC_IC0;
+    const int64_t nf = ic ? (int64_t)P->U_col.size() : A->nnz;
+    if (ic)
+        hipLaunchKernelGGL(k_gather_values, dim3(grid_for(nf)), dim3(256), 0, s, nf, P->fpos.p, A->val.p, P->fval.p);
+    else
+        M4_HIP(hipMemcpyAsync(P->fval.p, A->val.p, (size_t)nf * sizeof(double), hipMemcpyDeviceToDevice, s));
+    M4_HIP(hipGetLastError());
+    const Sweep& Lo = P->lower;
+    FactorView F{Lo.perm.p, Lo.lev_ptr.p, Lo.lev_blk.p, Lo.blk_level.p, Lo.rowptr.p, Lo.col.p, Lo.src.p, Lo.nlev};
+    if (ic) {
+        hipLaunchKernelGGL((k_factor<kSweepBS, true>), dim3(Lo.nblocks), dim3(kSweepBS), 0, s, F, A->rowptr.p,
+                           A->col.p, P->urowptr.p, P->fval.p, P->finv.p, P->fcounters.p, P->err.p);
+        hipLaunchKernelGGL(k_ic0_finish, dim3(grid_for(n)), dim3(256), 0, s, n, P->urowptr.p, P->finv.p, P->fval.p);
+    } else {
+        hipLaunchKernelGGL((k_factor<kSweepBS, false>), dim3(Lo.nblocks), dim3(kSweepBS), 0, s, F, A->rowptr.p,
+                           A->col.p, P->fpos.p, P->fval.p, P->finv.p, P->fcounters.p, P->err.p);
+        hipLaunchKernelGGL(k_ilu0_finish, dim3(grid_for(n)), dim3(256), 0, s, n, P->fpos.p, P->finv.p, P->fval.p);
+    }
+    M4_HIP(hipGetLastError());
+    M4_TRY(fill_sweep(P->lower, P->fmt, P->fval.p, s));
+    M4_TRY(fill_sweep(P->upper, P->fmt, P->fval.p, s));
+    P->host_values = false;
+    P->has_adjoint = false;
+    return MTL4G_OK;
+}
+
+static int setup_factor(mtl4g_pc_s* P) {
+    SweepRows lo, up;
+    std::vector<int32_t> pos;
+    const bool ic = P->kind == MTL4G_PC_IC0;
+    M4_TRY(ic ? plan_ic0(P, lo, up, pos) : plan_ilu0(P, lo, up, pos));
+    M4_TRY(upload_sweep(lo, P->fmt, &P->lower));
+    M4_TRY(upload_sweep(up, P->fmt, &P->upper));
+    auto up32 = [](DevBuf<int32_t>& buf, const std::vector<int32_t>& v) -> int {
+        M4_TRY(buf.alloc(v.size()));
+        if (!v.empty()) M4_HIP(hipMemcpy(buf.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+        return MTL4G_OK;
+    };
+    M4_TRY(up32(P->fpos, pos));
+    if (ic) M4_TRY(up32(P->urowptr, P->U_rowptr));
+    M4_TRY(P->fval.alloc(ic ? P->U_col.size() : (size_t)P->A->nnz));
+    M4_TRY(P->finv.alloc((size_t)P->n));
+    const size_t words = ((size_t)P->lower.nlev + 2) * kCounterStride;
+    M4_TRY(P->fcounters.alloc(words));
+    M4_HIP(hipMemset(P->fcounters.p, 0, words * sizeof(unsigned)));
+    M4_HIP(hipDeviceSynchronize());
+    M4_TRY(factor_device(P, nullptr));
+    M4_HIP(hipDeviceSynchronize());
+    return pc_check_error(P, nullptr);
+}
+
+// Host copies of the factor values (mtl4g_pc_factor, the adjoint sweeps),
+// rounded as the sweeps store them.
+static int ensure_host_values(mtl4g_pc_s* P) {
+    if (P->host_values || (P->kind != MTL4G_PC_ILU0 && P->kind != MTL4G_PC_IC0)) return MTL4G_OK;
+    std::vector<double> fv(P->fval.n);
+    M4_HIP(hipDeviceSynchronize());
+    if (!fv.empty()) M4_HIP(hipMemcpy(fv.data(), P->fval.p, fv.size() * 8, hipMemcpyDeviceToHost));
+    const bool f32 = P->fmt != MTL4G_F64;
+    auto store = [&](double v) { return f32 ? (double)(float)v : v; };
+    P->L_val.resize(P->L_src.size());
+    P->U_val.resize(P->U_src.size());
+    for (size_t j = 0; j < P->L_src.size(); j++) P->L_val[j] = store(fv[P->L_src[j]]);
+    for (size_t j = 0; j < P->U_src.size(); j++) P->U_val[j] = store(fv[P->U_src[j]]);
+    P->host_values = true;
     return MTL4G_OK;
 }
 
@@ -435,6 +683,7 @@ static SweepRows rows_from(int32_t n, const std::vector<int32_t>& rp, const std:
 // are visited: ascending for the lower, descending for the upper one.
 static int build_adjoint(mtl4g_pc_s* P) {
     const int32_t n = P->n;
+    M4_TRY(ensure_host_values(P));
     auto transpose = [n](const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
                          const std::vector<double>& va, std::vector<int32_t>& trp,
                          std::vector<int32_t>& tci, std::vector<double>& tva) {
@@ -511,14 +760,8 @@ int mtl4g_pc_create(mtl4g_pc* out, mtl4g_matrix A, int kind, int fmt) {
             if (!st && A->n)
                 st = hipMemcpy(P->inv_diag32.p, f.data(), A->n * 4, hipMemcpyHostToDevice) ? MTL4G_ERR_HIP : 0;
         }
-    } else if (!st && kind == MTL4G_PC_ILU0) {
-        st = factor_ilu0(A, P);
-        if (!st) st = upload_sweep(rows_from(A->n, P->L_rowptr, P->L_col, P->L_val, 0), fmt, &P->lower);
-        if (!st) st = upload_sweep(rows_from(A->n, P->U_rowptr, P->U_col, P->U_val, 2), fmt, &P->upper);
-    } else if (!st && kind == MTL4G_PC_IC0) {
-        st = factor_ic0(A, P);
-        if (!st) st = upload_sweep(rows_from(A->n, P->L_rowptr, P->L_col, P->L_val, 1), fmt, &P->lower);
-        if (!st) st = upload_sweep(rows_from(A->n, P->U_rowptr, P->U_col, P->U_val, 2), fmt, &P->upper);
+    } else if (!st && (kind == MTL4G_PC_ILU0 || kind == MTL4G_PC_IC0)) {
+        st = setup_factor(P);
     }
     if (st) {
         delete P;
@@ -573,6 +816,7 @@ int mtl4g_pc_factor(mtl4g_pc P, int which, int32_t* n_out, int64_t* nnz_out, int
             std::memcpy(val, P->h_inv_diag.data(), P->h_inv_diag.size() * 8);
         return MTL4G_OK;
     }
+    M4_TRY(ensure_host_values(P));
     const auto& rp = which == 1 ? P->L_rowptr : P->U_rowptr;
     const auto& ci = which == 1 ? P->L_col : P->U_col;
     const auto& va = which == 1 ? P->L_val : P->U_val;
@@ -584,6 +828,31 @@ int mtl4g_pc_factor(mtl4g_pc P, int which, int32_t* n_out, int64_t* nnz_out, int
     if (rowptr) std::memcpy(rowptr, rp.data(), rp.size() * 4);
     if (col && !ci.empty()) std::memcpy(col, ci.data(), ci.size() * 4);
     if (val && !va.empty()) std::memcpy(val, va.data(), va.size() * 8);
+    return MTL4G_OK;
+}
+
+int mtl4g_pc_refactor(mtl4g_pc P, void* stream) {
+    if (!P || !P->A || P->A->n != P->n) return MTL4G_ERR_INVALID;
+    hipStream_t s = S(stream);
+    if (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) {
+        M4_TRY(factor_device(P, s));
+        return pc_check_error(P, s);
+    }
+    if (P->kind == MTL4G_PC_DIAGONAL) {   // n reciprocals, as at creation (diagonal.hpp:37-45)
+        const mtl4g_matrix_s* A = P->A;
+        for (int32_t r = 0; r < A->n; r++) {
+            const int64_t d = find_entry(A->h_rowptr, A->h_col, r, r);
+            P->h_inv_diag[r] = 1.0 / (d >= 0 ? A->h_val[d] : 0.0);
+        }
+        M4_HIP(hipStreamSynchronize(s));
+        if (P->fmt == MTL4G_F64) {
+            if (A->n) M4_HIP(hipMemcpy(P->inv_diag64.p, P->h_inv_diag.data(), (size_t)A->n * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(P->h_inv_diag.begin(), P->h_inv_diag.end());
+            for (size_t i = 0; i < f.size(); i++) P->h_inv_diag[i] = f[i];
+            if (A->n) M4_HIP(hipMemcpy(P->inv_diag32.p, f.data(), (size_t)A->n * 4, hipMemcpyHostToDevice));
+        }
+    }
     return MTL4G_OK;
 }
 

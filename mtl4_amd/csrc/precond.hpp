@@ -18,8 +18,10 @@ __host__ __device__ inline unsigned* sweep_counter(unsigned* base, int i) {
 // off-diagonal entries of that row follow in the order MTL4 subtracts them.
 struct Sweep {
     int32_t n = 0, nlev = 0, nblocks = 0;
+    int64_t nnz = 0;
     bool unit = false;
     DevBuf<int32_t> perm, rowptr, col, lev_ptr, lev_blk, blk_level;
+    DevBuf<int32_t> src, dsrc;   // values gathered from a device factor: entry / diagonal sources
     DevBuf<double> val64, dia64;
     DevBuf<float> val32, dia32;
     DevBuf<unsigned> counters;   // default counters: 0 ticket, 1 abort, 2..2+nlev level done counts
@@ -34,9 +36,15 @@ struct SweepRows {
     std::vector<int32_t> rowptr, col;   // off-diagonal entries in subtraction order
     std::vector<double> val;
     std::vector<double> dia;            // stored (inverse) diagonal, unused if unit
+    // instead of val / dia: positions in a device array of factor values
+    // (filled later by fill_sweep); dsrc -1 for a unit diagonal
+    std::vector<int32_t> src, dsrc;
 };
 
 int upload_sweep(const SweepRows& rows, int value_format, Sweep* S);
+// val / dia of a sweep uploaded with src positions: gathered from fval
+// (rounded to float for MTL4G_F32 storage)
+int fill_sweep(Sweep& S, int value_format, const double* fval, hipStream_t s);
 
 // counters: (nlev+2) * kCounterStride zeroed words owned by the caller (NULL =
 // the sweep's own); each launch leaves them zeroed again.
@@ -56,11 +64,21 @@ struct mtl4g_pc_s {
     mtl4g::Sweep lower, upper;            // solve: lower then upper
     mtl4g::Sweep adj_lower, adj_upper;    // adjoint solve (built on demand)
     bool has_adjoint = false;
-    // host copies of the factors as MTL4 stores them (natural row order)
+    // host copies of the factors as MTL4 stores them (natural row order);
+    // the values of ILU(0)/IC(0) come from the device factor on demand
     std::vector<double> h_inv_diag;
     std::vector<int32_t> L_rowptr, L_col, U_rowptr, U_col;
     std::vector<double> L_val, U_val;
+    bool host_values = false;
     mtl4g::DevBuf<int> err;               // device error word of the sweeps
+    // device factorization (ILU(0): values in A's CSR order; IC(0): in the
+    // order of the upper factor U), kept for mtl4g_pc_refactor
+    mtl4g::DevBuf<double> fval, finv;     // factor values, inverse diagonal per row
+    mtl4g::DevBuf<int32_t> fpos;          // ILU(0): diagonal position per row; IC(0): A position per U entry
+    mtl4g::DevBuf<int32_t> urowptr;       // IC(0): row pointers of U
+    mtl4g::DevBuf<unsigned> fcounters;    // level counters of the factorization kernel
+    std::vector<int32_t> L_src, U_src;    // positions in fval of the L / U entries (natural order)
+    mtl4g::DevBuf<int32_t> dpos;          // diagonal: position of A[i][i] or -1
 };
 
 namespace mtl4g {
