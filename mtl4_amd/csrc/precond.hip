@@ -33,6 +33,42 @@ struct SweepView {
     int unit;
 };
 
+// A result not yet written reads as this signalling-NaN pattern: every
+// output of a sweep is preset to it (k_fill_pending), and a row reads the
+// values it depends on until none is pending.  Arithmetic never produces a
+// signalling NaN, so the pattern cannot be a result -- except when a unit
+// row with no entries copies an input carrying it; that copy is quieted.
+constexpr uint64_t kPending64 = 0x7ff5c3c3c3c3c3c3ull;
+constexpr uint32_t kPending32 = 0x7f93c3c3u;
+__device__ __forceinline__ bool pending(double v) { return (uint64_t)__double_as_longlong(v) == kPending64; }
+__device__ __forceinline__ bool pending(float v) { return __float_as_uint(v) == kPending32; }
+__device__ __forceinline__ double quiet_if_pending(double v) {
+    return pending(v) ? __longlong_as_double((long long)(kPending64 | 0x0008000000000000ull)) : v;
+}
+__device__ __forceinline__ float quiet_if_pending(float v) {
+    return pending(v) ? __uint_as_float(kPending32 | 0x00400000u) : v;
+}
+
+template <typename VT>
+__global__ void k_fill_pending(int64_t n, VT* w, const int* skip) {
+    if (skip && *skip) return;
+    VT p;
+    if constexpr (sizeof(VT) == 8) p = __longlong_as_double((long long)kPending64);
+    else p = __uint_as_float(kPending32);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = p;
+}
+
+// Counters: [0] ticket, [1] abort flag, [2] finished workgroups, [3 + l]
+// finished workgroups of level l; the last workgroup to finish resets them,
+// so a sweep needs no memset between launches.  Workgroups take tickets in
+// level order.  A workgroup of level l first waits (one lane polling) until
+// level l-2 is complete, then each row reads the values it depends on until
+// none is pending: the rows of level l-1 are seen as soon as they are
+// stored, without a barrier per level, and only the next level's rows poll
+// values.  Every row depended on belongs to a workgroup that took its
+// ticket earlier and is running, so every wait ends -- or gives up after
+// kSpinLimit polls and raises the abort flag.
 template <int BS, typename MT, typename VT>
 __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __restrict__ v,
                                               VT* w, unsigned* counters, int* err,
@@ -46,7 +82,7 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
     const int L = S.blk_level[b];
     // The row's own data (position, bounds, right-hand side, diagonal and its
     // first eight entries) does not depend on earlier levels: request it
-    // before waiting, so only the reads of earlier results follow the wait.
+    // before waiting.
     const int k = S.lev_ptr[L] + (b - S.lev_blk[L]) * BS + (int)threadIdx.x;
     const bool mine = k < S.lev_ptr[L + 1];
     int r = 0, a = 0, e = 0;
@@ -68,10 +104,10 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
     }
     if (threadIdx.x == 0) {
         int ok = 1;
-        if (L > 0) {
-            const unsigned need = (unsigned)(S.lev_blk[L] - S.lev_blk[L - 1]);
+        if (L > 1) {
+            const unsigned need = (unsigned)(S.lev_blk[L - 1] - S.lev_blk[L - 2]);
             unsigned it = 0;
-            const unsigned* lev = sweep_counter(counters, 2 + L - 1);
+            const unsigned* lev = sweep_counter(counters, 3 + L - 2);
             while (ld_agent(lev) < need) {
                 if ((++it & 63u) == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) { ok = 0; break; }
                 if (it > kSpinLimit) { st_agent(sweep_counter(counters, 1), 1u); ok = 0; break; }
@@ -81,13 +117,12 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
         s_ok = ok;
     }
     __syncthreads();
-    // Values of earlier levels are read with agent-scope loads and results
-    // are published with agent-scope atomics, so no L2 write-back fences are
-    // needed between workgroups of this kernel.
     if (s_ok && mine) {
-        // eight entries at a time: the eight solution values they name (one
-        // round trip), then the subtractions in the row's column order
-        for (int j0 = a; j0 < e; j0 += 8) {
+        bool ok = true;
+        // eight entries at a time: the eight values they name (one round
+        // trip, repeated for those still pending), then the subtractions in
+        // the row's column order
+        for (int j0 = a; j0 < e && ok; j0 += 8) {
             if (j0 > a) {
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
@@ -99,25 +134,45 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
             VT wv[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) wv[u] = j0 + u < e ? ld_agent(w + c[u]) : VT(0);
+            unsigned it = 0;
+            for (;;) {
+                bool wait = false;
+#pragma unroll
+                for (int u = 0; u < 8; u++) wait |= pending(wv[u]);
+                if (!wait) break;
+                if (++it > kSpinLimit) {
+                    st_agent(sweep_counter(counters, 1), 1u);
+                    ok = false;
+                    break;
+                }
+                if ((it & 63u) == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) {
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (pending(wv[u])) wv[u] = ld_agent(w + c[u]);
+            }
 #pragma unroll
             for (int u = 0; u < 8; u++)
                 if (j0 + u < e) rr -= m[u] * wv[u];
         }
         if (!S.unit) rr = rr * dk;
-        st_agent(w + r, rr);   // the L1 writes through; wait_memory() below waits for it
+        st_agent(w + r, quiet_if_pending(rr));   // the L1 writes through; wait_memory() below waits for it
     }
     wait_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned need = (unsigned)(S.lev_blk[L + 1] - S.lev_blk[L]);
+        __hip_atomic_fetch_add(sweep_counter(counters, 3 + L), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned done =
-            __hip_atomic_fetch_add(sweep_counter(counters, 2 + L), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        s_final = (L == S.nlev - 1 && done == need);
+            __hip_atomic_fetch_add(sweep_counter(counters, 2), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        s_final = done == (unsigned)gridDim.x;
     }
     __syncthreads();
-    if (s_final) {   // every other workgroup has published: reset for the next launch
+    if (s_final) {   // every other workgroup has finished: reset for the next launch
         if (threadIdx.x == 0 && ld_agent(sweep_counter(counters, 1)) != 0u) atomicOr(err, 1);
-        for (int i = threadIdx.x; i < S.nlev + 2; i += BS) st_agent(sweep_counter(counters, i), 0u);
+        for (int i = threadIdx.x; i < S.nlev + 3; i += BS) st_agent(sweep_counter(counters, i), 0u);
     }
 }
 
@@ -234,8 +289,8 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
         M4_TRY(up(S->val32, v32));
         M4_TRY(up(S->dia32, d32));
     }
-    M4_TRY(S->counters.alloc(((size_t)nlev + 2) * kCounterStride));
-    M4_HIP(hipMemset(S->counters.p, 0, ((size_t)nlev + 2) * kCounterStride * sizeof(unsigned)));
+    M4_TRY(S->counters.alloc(sweep_counter_words(nlev)));
+    M4_HIP(hipMemset(S->counters.p, 0, sweep_counter_words(nlev) * sizeof(unsigned)));
     // complete before any stream (including non-blocking ones) uses the counters
     M4_HIP(hipDeviceSynchronize());
     return MTL4G_OK;
@@ -378,11 +433,17 @@ __global__ void k_fill_sweep(int64_t nnz, int32_t n, const int32_t* __restrict__
         dia[i] = dsrc[i] < 0 ? MT(1) : static_cast<MT>(fv[dsrc[i]]);
 }
 
+static int grid_for(int64_t n) {
+    int64_t g = (n + 1023) / 1024;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
 template <typename VT>
 int launch_sweep(const Sweep& S, int fmt, const VT* v, VT* w, const int* skip, int* err,
                  unsigned* counters, hipStream_t s) {
     if (S.n == 0) return MTL4G_OK;
     if (!counters) counters = S.counters.p;
+    hipLaunchKernelGGL(k_fill_pending<VT>, dim3(grid_for(S.n)), dim3(256), 0, s, (int64_t)S.n, w, skip);
     if (fmt == MTL4G_F64) {
         SweepView<double, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
                                 S.blk_level.p, S.val64.p, S.dia64.p, S.nlev, S.unit ? 1 : 0};
@@ -401,13 +462,9 @@ int launch_sweep(const Sweep& S, int fmt, const VT* v, VT* w, const int* skip, i
 template int launch_sweep<double>(const Sweep&, int, const double*, double*, const int*, int*, unsigned*, hipStream_t);
 template int launch_sweep<float>(const Sweep&, int, const float*, float*, const int*, int*, unsigned*, hipStream_t);
 
-static int grid_for(int64_t n) {
-    int64_t g = (n + 1023) / 1024;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
-}
 
 size_t pc_counter_words(const mtl4g_pc_s* P) {
-    return ((size_t)P->lower.nlev + 2 + (size_t)P->upper.nlev + 2) * kCounterStride;
+    return sweep_counter_words(P->lower.nlev) + sweep_counter_words(P->upper.nlev);
 }
 
 template <typename VT>
@@ -430,7 +487,7 @@ int pc_apply(const mtl4g_pc_s* P, const VT* r, VT* z, VT* y0, const int* skip,
     default:
         M4_TRY(launch_sweep<VT>(P->lower, P->fmt, r, y0, skip, P->err.p, counters, s));
         M4_TRY(launch_sweep<VT>(P->upper, P->fmt, y0, z, skip, P->err.p,
-                                counters ? counters + ((size_t)P->lower.nlev + 2) * kCounterStride : nullptr, s));
+                                counters ? counters + sweep_counter_words(P->lower.nlev) : nullptr, s));
         return MTL4G_OK;
     }
     M4_HIP(hipGetLastError());

@@ -12,6 +12,9 @@ constexpr int kCounterStride = 32;
 __host__ __device__ inline unsigned* sweep_counter(unsigned* base, int i) {
     return base + (size_t)i * kCounterStride;
 }
+// counter words of a sweep with nlev levels: ticket, abort flag, finished
+// workgroups, then one per level
+inline size_t sweep_counter_words(int32_t nlev) { return ((size_t)nlev + 3) * kCounterStride; }
 
 // One triangular sweep  w[r] = (v[r] - sum_j T[r][c_j] * w[c_j]) * dia[r]
 // stored in level order: position k of the sweep handles row perm[k]; the
@@ -24,7 +27,7 @@ struct Sweep {
     DevBuf<int32_t> src, dsrc;   // values gathered from a device factor: entry / diagonal sources
     DevBuf<double> val64, dia64;
     DevBuf<float> val32, dia32;
-    DevBuf<unsigned> counters;   // default counters: 0 ticket, 1 abort, 2..2+nlev level done counts
+    DevBuf<unsigned> counters;   // default counters (sweep_counter_words)
     std::vector<int32_t> h_lev_ptr;
 };
 
@@ -46,7 +49,7 @@ int upload_sweep(const SweepRows& rows, int value_format, Sweep* S);
 // (rounded to float for MTL4G_F32 storage)
 int fill_sweep(Sweep& S, int value_format, const double* fval, hipStream_t s);
 
-// counters: (nlev+2) * kCounterStride zeroed words owned by the caller (NULL =
+// counters: sweep_counter_words(nlev) zeroed words owned by the caller (NULL =
 // the sweep's own); each launch leaves them zeroed again.
 template <typename VT>
 int launch_sweep(const Sweep& S, int value_format, const VT* v, VT* w, const int* skip, int* err,
