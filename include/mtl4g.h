@@ -71,6 +71,7 @@ extern "C" {
 #define MTL4G_CGS      2   /* itl::cgs,      itl/krylov/cgs.hpp:24-62 (FP64 only) */
 #define MTL4G_TFQMR    3   /* itl::tfqmr,    itl/krylov/tfqmr.hpp:32-96 (FP64; mtl4g_tfqmr) */
 #define MTL4G_BICGSTAB_2 4 /* itl::bicgstab_2, itl/krylov/bicgstab_2.hpp:37-130 (FP64) */
+#define MTL4G_BICGSTAB_ELL 5 /* itl::bicgstab_ell, itl/krylov/bicgstab_ell.hpp:37-134 (FP64; mtl4g_bicgstab_ell) */
 
 typedef struct mtl4g_matrix_s* mtl4g_matrix;
 typedef struct mtl4g_pc_s* mtl4g_pc;
@@ -265,6 +266,14 @@ int mtl4g_tfqmr(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* d_x, const doubl
  * x0).  An empty system is MTL4's logic_error. */
 int mtl4g_bicgstab_2(mtl4g_matrix A, mtl4g_pc L, double* d_x, const double* d_b,
                      mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
+/* itl::bicgstab_ell(A, x, b, L, R, iter, l) (bicgstab_ell.hpp:37-134):
+ * BiCGSTAB(l), 1 <= l <= 8, left preconditioner L and right preconditioner
+ * R (NULL = identity).  finished() runs at the loop head on r_hat[0] and
+ * after each inner step j on r_hat[j]; every exit but iter.fail(3) (omega
+ * == 0, returns 3) sets x = solve(R, x) + x0 as MTL4 does.  An empty
+ * system is MTL4's logic_error; l outside 1..8 is MTL4G_ERR_INVALID. */
+int mtl4g_bicgstab_ell(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* d_x, const double* d_b, int32_t l,
+                       mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
 
 /* Reusable solver: the workspace (MTL4 allocates it per call, cg.hpp:90)
  * stays with the handle, so repeated solves allocate nothing. */

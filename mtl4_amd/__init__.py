@@ -23,7 +23,8 @@ from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, ORDER_MTL4, ORDE
                    missing_diagonal, nonfinite_error, unexpected_orthogonality)
 
 __all__ = ["compressed2D", "dense_vector", "pc", "solve", "adjoint_solve", "basic_iteration",
-           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "tfqmr", "bicgstab_2", "dot",
+           "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "tfqmr", "bicgstab_2",
+           "bicgstab_ell", "dot",
            "unary_dot",
            "two_norm", "num_rows", "num_cols", "size", "laplacian_setup", "ir_solve",
            "MtlError", "matrix_not_square", "incompatible_size", "missing_diagonal",
@@ -373,12 +374,12 @@ class basic_iteration:
     def _state(self):
         return _lib.Iteration(self.norm_r0_, self.max_iter, self.rtol_, self.atol_, self.i, 0, 0.0)
 
-    def _update(self, st, hist, method):
+    def _update(self, st, hist, method, ell=2):
         self.i, self.error, self.resid_ = st.i, st.error, st.resid
         self.is_finished = True
         self.history = list(hist) if hist is not None else []
 
-    def _wants_history(self):
+    def _wants_history(self, per_iteration=3):
         return 0
 
 
@@ -401,25 +402,28 @@ class cyclic_iteration(basic_iteration):
     def is_multi_print(self):
         return self.multi_print
 
-    def _wants_history(self):
-        return 3 * max(self.max_iter - self.i, 0) + 4
+    def _wants_history(self, per_iteration=3):
+        return per_iteration * max(self.max_iter - self.i, 0) + 4
 
     def _state(self):
         self._i0 = self.i
         return super()._state()
 
-    def _update(self, st, hist, method):
+    def _update(self, st, hist, method, ell=2):
         super()._update(st, hist, method)
         if self.quite:
             return
         # finished() calls: CG one per iteration; BiCGSTAB r at the loop head,
         # then s (after ++iter) and r again per iteration; BiCGSTAB(2) the
-        # loop head, then r_hat[0] and r_hat[1] after ++iter
+        # loop head, then r_hat[0] and r_hat[1] after ++iter; BiCGSTAB(l) the
+        # loop head, then r_hat[0..l-1]
         counts, i = [], self._i0
         for k in range(len(self.history)):
             if method == _lib.BICGSTAB and k > 0 and k % 2 == 1:
                 i += 1
             elif method == _lib.BICGSTAB_2 and k % 3 == 1:
+                i += 1
+            elif method == _lib.BICGSTAB_ELL and k % (ell + 1) == 1:
                 i += 1
             elif method in (_lib.CG, _lib.CGS) and k > 0:
                 i += 1
@@ -452,25 +456,28 @@ class noisy_iteration(cyclic_iteration):
 
 
 # ---------------------------------------------------------------- solvers
-def _krylov(fn, method, A, x, b, P, it, R=None):
+def _krylov(fn, method, A, x, b, P, it, R=None, ell=2):
     if A.num_rows() != A.num_cols():
         raise matrix_not_square("Krylov solve")
     if x.size() != A.num_cols() or b.size() != A.num_rows():
         raise incompatible_size("Krylov solve")
     st = it._state()
-    cap = it._wants_history()
+    cap = it._wants_history(ell + 1 if method == _lib.BICGSTAB_ELL else 3)
     hist = np.full(max(cap, 1), np.nan)
     ph = P.h if P is not None else None
     hp = hist.ctypes.data_as(C.c_void_p) if cap else None
     if method == _lib.TFQMR:
         status = fn(A.h, ph, R.h if R is not None else None, x.ptr, b.ptr, C.byref(st), hp, cap, _stream())
+    elif method == _lib.BICGSTAB_ELL:
+        status = fn(A.h, ph, R.h if R is not None else None, x.ptr, b.ptr, int(ell), C.byref(st), hp, cap,
+                    _stream())
     else:
         status = fn(A.h, ph, x.ptr, b.ptr, C.byref(st), hp, cap, _stream())
     h = hist[:cap]
     h = h[~np.isnan(h)] if cap else None
-    it._update(st, h, method)
-    check(status, {CG: "cg", BICGSTAB: "bicgstab", _lib.CGS: "cgs", _lib.BICGSTAB_2: "bicgstab_2"}
-          .get(method, "tfqmr"))
+    it._update(st, h, method, ell)
+    check(status, {CG: "cg", BICGSTAB: "bicgstab", _lib.CGS: "cgs", _lib.BICGSTAB_2: "bicgstab_2",
+                   _lib.BICGSTAB_ELL: "bicgstab_ell"}.get(method, "tfqmr"))
     return it.error_code()
 
 
@@ -508,6 +515,13 @@ def bicgstab_2(A, x, b, L=None, it=None):
     if isinstance(L, basic_iteration):
         L, it = None, L
     return _krylov(_L().mtl4g_bicgstab_2, _lib.BICGSTAB_2, A, x, b, L, it)
+
+
+def bicgstab_ell(A, x, b, L, R, it, l):
+    """itl::bicgstab_ell(A, x, b, L, R, iter, l) (itl/krylov/bicgstab_ell.hpp:
+    37-134): BiCGSTAB(l), 1 <= l <= 8, left preconditioner L and right
+    preconditioner R (None: identity).  Returns 3 (iter.fail(3)) on omega == 0."""
+    return _krylov(_L().mtl4g_bicgstab_ell, _lib.BICGSTAB_ELL, A, x, b, L, it, R, ell=l)
 
 
 class Solver:

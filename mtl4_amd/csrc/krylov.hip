@@ -1021,8 +1021,9 @@ struct EpiB2R {
 // dot(r_hat[2], r_hat[2]) (orth.hpp:87-93); gamma_a[j] = dot(r_hat[j],
 // r_hat[0]) / tau[j][j]; omega = gamma[2] = gamma_a[2]; omega == 0 is
 // iter.fail(3) (returned before the shift back); gamma[1] = gamma_a[1] -
-// tau[1][2]*gamma[2]; gamma_aa[1] = 0 + gamma[2] (the strictly upper part
-// of a 1x1 matrix is empty)  (bicgstab_2.hpp:100-117)
+// tau[1][2]*gamma[2]; gamma_aa[1] = (0 + 0*gamma[2]) + gamma[2]: the naive
+// dense mat-vec over the strictly upper view, which reads 0 off the band
+// (mat_vec_mult.hpp:259-298)  (bicgstab_2.hpp:100-117)
 struct EpiB2MR {
     static constexpr int NP = 3;
     __host__ __device__ static constexpr int order(int) { return RO_DOT4; }
@@ -1053,7 +1054,7 @@ struct EpiB2MR {
             return;
         }
         c->g1 = c->ga1 - (0.0 + c->t12 * c->g2);
-        c->gaa1 = 0.0 + c->g2;
+        c->gaa1 = (0.0 + 0.0 * c->g2) + c->g2;
     }
 };
 
@@ -1065,6 +1066,172 @@ struct EpiB2Shift {
     __device__ bool skip() const { return false; }
     __device__ void prepare() {}
     __device__ void operator()(int64_t i, double*) const { x[i] = x[i] + x0[i]; }
+    __device__ void finish(const double*) const {}
+};
+
+// ---------------------------------------------------------------- BiCGSTAB(l) epilogues
+// itl::bicgstab_ell, itl/krylov/bicgstab_ell.hpp:37-134: BiCGSTAB(2)'s
+// steps for any l <= kMaxEll, with the scalars of the minimal-residual part
+// (tau, gamma, gamma_a, gamma_aa) in a small device array.
+constexpr int kMaxEll = 8;
+struct EllVecs {
+    double* r[kMaxEll + 1];
+    double* u[kMaxEll + 1];
+};
+struct EllScalars {   // offsets into the small array: tau[(l+1)^2], gamma, gamma_a, gamma_aa
+    double* sm;
+    int l;
+    __host__ __device__ double& tau(int i, int j) const { return sm[i * (l + 1) + j]; }
+    __host__ __device__ double* gam() const { return sm + (l + 1) * (l + 1); }
+    __host__ __device__ double* ga() const { return gam() + (l + 1); }
+    __host__ __device__ double* gaa() const { return ga() + (l + 1); }
+};
+
+// UPD: the minimal-residual updates of x, r_hat[0], u_hat[0] (:119-131) in
+// their order per element, then the loop head on the new r_hat[0]
+template <bool UPD>
+struct EpiEllHead {
+    static constexpr int NP = 2;
+    __host__ __device__ static constexpr int order(int k) { return k == 0 ? RO_NORM4 : RO_DOT4; }
+    EllVecs V;
+    double* x;
+    const double* rt;
+    EllScalars E;
+    Ctl* c;
+    double* hist;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double* part) const {
+        double r0 = V.r[0][i];
+        if (UPD) {
+            const int l = E.l;
+            const double *gam = E.gam(), *ga = E.ga(), *gaa = E.gaa();
+            double xi = x[i] + gam[1] * r0;
+            double rn = r0 - ga[l] * V.r[l][i];
+            double ui = V.u[0][i] - gam[l] * V.u[l][i];
+            for (int j = 1; j < l; j++) {
+                ui = ui - gam[j] * V.u[j][i];
+                const double rj = V.r[j][i];
+                xi = xi + gaa[j] * rj;
+                rn = rn - ga[j] * rj;
+            }
+            x[i] = xi;
+            V.r[0][i] = rn;
+            V.u[0][i] = ui;
+            r0 = rn;
+        }
+        part[0] += r0 * r0;
+        part[1] += rt[i] * r0;
+    }
+    __device__ void finish(const double* tot) const { b2_head(c, tot, hist); }
+};
+
+// u_hat[k] = r_hat[k] - beta*u_hat[k] for k <= j  (:78-79)
+struct EpiEllU {
+    static constexpr int NP = 0;
+    EllVecs V;
+    int j;
+    Ctl* c;
+    double beta;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { beta = c->beta; }
+    __device__ void operator()(int64_t i, double*) const {
+        for (int k = 0; k <= j; k++) V.u[k][i] = V.r[k][i] - beta * V.u[k][i];
+    }
+    __device__ void finish(const double*) const {}
+};
+
+// r_hat[k] -= alpha*u_hat[k+1] for k <= j; finished(r_hat[j]) without
+// ++iter  (:86-92)
+struct EpiEllR {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_NORM4; }
+    EllVecs V;
+    int j;
+    Ctl* c;
+    double* hist;
+    double alpha;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { alpha = c->alpha; }
+    __device__ void operator()(int64_t i, double* part) const {
+        double v = 0.0;
+        for (int k = 0; k <= j; k++) {
+            v = V.r[k][i] - alpha * V.u[k + 1][i];
+            V.r[k][i] = v;
+        }
+        part[0] += v * v;
+    }
+    __device__ void finish(const double* tot) const { iter_check(c, std::sqrt(tot[0]), hist); }
+};
+
+// One step (i, j) of orthogonalize_factors on r_hat[1..l] (orth.hpp:75-93),
+// fused with the axpy of the step before: r_hat[j] -= tau[i-1][j] *
+// r_hat[i-1] (i > 1), then tau[i][j] = dot(r_hat[i], r_hat[j]) / tau[i][i]
+// (i < j), or tau[j][j] = dot(r_hat[j], r_hat[j]) and gamma_a[j] =
+// dot(r_hat[j], r_hat[0]) / tau[j][j] (i == j; bicgstab_ell.hpp:103-104).
+// After the last step (l, l): gamma, omega and gamma_aa (:106-117); omega
+// == 0 is iter.fail(3).
+struct EpiEllOrth {
+    static constexpr int NP = 2;
+    __host__ __device__ static constexpr int order(int) { return RO_DOT4; }
+    EllVecs V;
+    EllScalars E;
+    int i, j;
+    Ctl* c;
+    double t;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { t = i > 1 ? E.tau(i - 1, j) : 0.0; }
+    __device__ void operator()(int64_t e, double* part) const {
+        double rj = V.r[j][e];
+        if (i > 1) {
+            rj = rj - t * V.r[i - 1][e];
+            V.r[j][e] = rj;
+        }
+        if (i == j) {
+            part[0] += rj * rj;
+            part[1] += rj * V.r[0][e];
+        } else {
+            part[0] += V.r[i][e] * rj;
+        }
+    }
+    __device__ void finish(const double* tot) const {
+        if (i < j) {
+            E.tau(i, j) = tot[0] / E.tau(i, i);
+            return;
+        }
+        E.tau(j, j) = tot[0];
+        E.ga()[j] = tot[1] / tot[0];
+        if (j < E.l) return;
+        const int l = E.l;
+        double *gam = E.gam(), *ga = E.ga(), *gaa = E.gaa();
+        gam[l] = ga[l];
+        c->omega = gam[l];
+        if (c->omega == 0.0) {
+            c->error = 3;
+            c->done = 1;
+            return;
+        }
+        for (int jj = l - 1; jj > 0; --jj) {
+            double sum = 0.0;
+            for (int ii = jj + 1; ii <= l; ++ii) sum += E.tau(jj, ii) * gam[ii];
+            gam[jj] = ga[jj] - sum;
+        }
+        for (int p = 0; p < l - 1; p++) {   // naive dense mat-vec over strict_upper (mat_vec_mult.hpp:259-298)
+            double s = 0.0;
+            for (int q = 0; q < l - 1; q++) s += (q > p ? E.tau(p + 1, q + 1) : 0.0) * gam[q + 2];
+            gaa[p + 1] = s + gam[p + 2];
+        }
+    }
+};
+
+// x = solve(R, x) + x0 (src: the solve's output, or x itself)  (:90-91, :133)
+struct EpiEllShift {
+    static constexpr int NP = 0;
+    double* x;
+    const double *src, *x0;
+    __device__ bool skip() const { return false; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double*) const { x[i] = src[i] + x0[i]; }
     __device__ void finish(const double*) const {}
 };
 
@@ -1623,6 +1790,147 @@ int bicgstab2_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, Workspace<double
     return hc.error;
 }
 
+// itl::bicgstab_ell (bicgstab_ell.hpp:37-134) with every scalar on the
+// device.  Per iteration: l products A*solve(R, u_hat[j]) and l products
+// A*solve(R, r_hat[j]) (each followed by solve(L, .)), l(l+1)/2 passes of
+// the orthogonalization, and the loop head fused with the minimal-residual
+// updates.
+int bicgstab_ell_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R, int32_t l,
+                     Workspace<double>& W, double* x, const double* b, mtl4g_iteration* it, double* h_hist,
+                     int32_t hist_cap, hipStream_t s) {
+    if (!A || !it || (A->n && (!x || !b))) return MTL4G_ERR_INVALID;
+    if (l < 1 || l > kMaxEll) {
+        set_error("bicgstab_ell: l = %d outside 1..%d", l, kMaxEll);
+        return MTL4G_ERR_INVALID;
+    }
+    if (A->n != A->m) {
+        set_error("Krylov solve needs a square matrix");
+        return MTL4G_ERR_NOT_SQUARE;
+    }
+    if ((L && L->n != A->n) || (R && R->n != A->n)) {
+        set_error("preconditioner size != matrix size %d", A->n);
+        return MTL4G_ERR_SIZE;
+    }
+    if (A->n == 0) {   // bicgstab_ell.hpp:48
+        set_error("empty rhs vector");
+        return MTL4G_ERR_INVALID;
+    }
+    if (L && L->kind == MTL4G_PC_IDENTITY) L = nullptr;
+    if (R && R->kind == MTL4G_PC_IDENTITY) R = nullptr;
+    const int nvec = 2 * (l + 1) + 5;
+    M4_TRY(ensure_workspace(W, A->n, nvec, L, hist_cap, s));
+    M4_TRY(ensure_workspace(W, A->n, nvec, R, hist_cap, s));   // sweep counters for both
+    if (!W.small.p) M4_TRY(W.small.alloc(128));
+    const int64_t n = A->n;
+    double* base = W.vecs.p;
+    EllVecs V;
+    for (int j = 0; j <= kMaxEll; j++) {
+        V.r[j] = j <= l ? base + (size_t)j * n : nullptr;
+        V.u[j] = j <= l ? base + (size_t)(l + 1 + j) * n : nullptr;
+    }
+    double *rt = base + (size_t)(2 * (l + 1)) * n, *y = rt + n, *x0 = y + n, *tmp = x0 + n, *y0 = tmp + n;
+    Ctl init;
+    std::memset(&init, 0, sizeof init);
+    init.norm_r0 = it->norm_r0;
+    init.rtol = it->rtol;
+    init.atol = it->atol;
+    init.max_iter = it->max_iter;
+    init.i = it->i;
+    init.hist_cap = h_hist ? hist_cap : 0;
+    init.rho = 1.0;     // rho_0
+    init.omega = 1.0;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
+    M4_HIP(hipGetLastError());
+    Ctl* c = W.ctl.p;
+    double* part = W.partials.p;
+    double* tm = W.terms.p;
+    unsigned* tk = W.tickets.p;
+    const double* val = A->val.p;
+    const EllScalars E{W.small.p, l};
+    // shift problem; r0_tilde; r_hat[0] = solve(L, r_hat[0]); u_hat[0] = 0  (:52-64)
+    M4_TRY(launch_vec<double>(n, EpiB2XNorm{x, c}, part, tk, s, tm));
+    M4_TRY(launch_csr<double>(A, val, x, EpiB2Init{b, x, V.r[0], x0, c, 0}, part, tk, s, tm));
+    M4_TRY(launch_vec<double>(n, EpiB2Start{x, rt, V.u[0], V.r[0], c, 0, 0.0}, part, tk, s));
+    if (L) {   // y = solve(L, r_hat[0]); r_hat[0] = y: the two buffers swap roles
+        M4_TRY(pc_apply<double>(L, V.r[0], y, y0, nullptr, W.pc_counters.p, s));
+        std::swap(V.r[0], y);
+    }
+    M4_TRY(launch_vec<double>(n, EpiEllHead<false>{V, x, rt, E, c, W.hist.p}, part, tk, s, tm));
+    // out = solve(L, A * solve(R, in)) with the reductions of MODE on out;
+    // XUP adds alpha*u_hat[0] to x
+    auto product = [&](const double* in, double* out, auto mode, auto xup) -> int {
+        constexpr int MODE = decltype(mode)::value;
+        constexpr bool XUP = decltype(xup)::value;
+        if (R) {
+            M4_TRY(pc_apply<double>(R, in, tmp, y0, &c->done, W.pc_counters.p, s));
+            in = tmp;
+        }
+        if (!L) return launch_csr<double>(A, val, in, EpiB2Mv<MODE, XUP>{rt, out, x, V.u[0], c, 0.0}, part, tk, s, tm);
+        M4_TRY((launch_csr<double>(A, val, in, EpiB2Mv<B2_NONE, XUP>{rt, y, x, V.u[0], c, 0.0}, part, tk, s)));
+        M4_TRY(pc_apply<double>(L, y, out, y0, &c->done, W.pc_counters.p, s));
+        if constexpr (MODE == B2_NONE) return MTL4G_OK;
+        else return launch_vec<double>(n, EpiB2Dot<MODE>{rt, out, c}, part, tk, s, tm);
+    };
+    using Gamma = std::integral_constant<int, B2_GAMMA>;
+    using Rho = std::integral_constant<int, B2_RHO>;
+    using None = std::integral_constant<int, B2_NONE>;
+    using NoX = std::false_type;
+    using AddX = std::true_type;
+    auto iteration = [&]() -> int {
+        for (int j = 0; j < l; j++) {   // (:74-97; rho_1 and beta of j = 0 came with the loop head)
+            M4_TRY(launch_vec<double>(n, EpiEllU{V, j, c, 0.0}, part, tk, s));
+            M4_TRY(product(V.u[j], V.u[j + 1], Gamma{}, NoX{}));
+            M4_TRY(launch_vec<double>(n, EpiEllR{V, j, c, W.hist.p, 0.0}, part, tk, s, tm));
+            if (j + 1 < l) M4_TRY(product(V.r[j], V.r[j + 1], Rho{}, AddX{}));
+            else M4_TRY(product(V.r[j], V.r[j + 1], None{}, AddX{}));
+        }
+        for (int j = 1; j <= l; j++)   // minimal-residual part (:100-117)
+            for (int i = 1; i <= j; i++)
+                M4_TRY(launch_vec<double>(n, EpiEllOrth{V, E, i, j, c, 0.0}, part, tk, s, tm));
+        return launch_vec<double>(n, EpiEllHead<true>{V, x, rt, E, c, W.hist.p}, part, tk, s, tm);
+    };
+    const int32_t budget = std::max(0, it->max_iter - it->i) + 8;
+    int32_t launched = 0, chunk = 2;
+    for (;;) {
+        M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        M4_HIP(hipStreamSynchronize(s));
+        if (W.h_ctl->done) break;
+        if (launched > budget) {
+            set_error("solver did not stop after %d launched iterations", launched);
+            return MTL4G_ERR_INVALID;
+        }
+        for (int k = 0; k < chunk; k++) M4_TRY(iteration());
+        launched += chunk;
+        chunk = std::min(chunk * 2, 32);
+    }
+    const Ctl& hc = *W.h_ctl;
+    if (hc.error != 3) {   // x = solve(R, x); x += x0
+        const double* src = x;
+        if (R) {
+            M4_TRY(pc_apply<double>(R, x, tmp, y0, nullptr, W.pc_counters.p, s));
+            src = tmp;
+        }
+        M4_TRY(launch_vec<double>(n, EpiEllShift{x, src, x0}, part, tk, s));
+    }
+    if (L) M4_TRY(pc_check_error(L, s));
+    if (R) M4_TRY(pc_check_error(R, s));
+    M4_HIP(hipStreamSynchronize(s));
+    it->i = hc.i;
+    it->error = hc.error;
+    it->resid = hc.resid;
+    if (h_hist && hist_cap > 0) {
+        int32_t nh = std::min(hc.hist_n, hist_cap);
+        if (nh > 0) M4_HIP(hipMemcpy(h_hist, W.hist.p, (size_t)nh * sizeof(double), hipMemcpyDeviceToHost));
+        for (int32_t k = nh; k < hist_cap; k++) h_hist[k] = NAN;
+    }
+    if (hc.status != 0) {
+        set_error("non-finite residual");
+        return hc.status;
+    }
+    if (hc.error == 3) set_error("bicg breakdown #2");
+    return hc.error;
+}
+
 template int krylov_run<double>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<double>&,
                                 double*, const double*, mtl4g_iteration*, double*, int32_t, hipStream_t);
 template int krylov_run<float>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<float>&,
@@ -1794,6 +2102,12 @@ int mtl4g_bicgstab_2(mtl4g_matrix A, mtl4g_pc L, double* x, const double* b, mtl
                      double* h_hist, int32_t hist_cap, void* stream) {
     Workspace<double> W;
     return bicgstab2_run(A, L, W, x, b, it, h_hist, hist_cap, S(stream));
+}
+
+int mtl4g_bicgstab_ell(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* x, const double* b, int32_t l,
+                       mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream) {
+    Workspace<double> W;
+    return bicgstab_ell_run(A, L, R, l, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,

@@ -35,6 +35,7 @@ struct Workspace {
     DevBuf<double> hist;
     DevBuf<unsigned> pc_counters;
     DevBuf<double> terms;     // 3n, reductions in MTL4's order (FP64 only)
+    DevBuf<double> small;     // bicgstab_ell: tau, gamma, gamma_a, gamma_aa
     Ctl* h_ctl = nullptr;     // pinned mirror for polling
     ~Workspace() {
         if (h_ctl) (void)hipHostFree(h_ctl);
@@ -57,6 +58,12 @@ int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
 int bicgstab2_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, Workspace<double>& W, double* x,
                   const double* b, mtl4g_iteration* it, double* h_hist, int32_t hist_cap,
                   hipStream_t s);
+
+// itl::bicgstab_ell(A, x, b, L, R, iter, l) on the device, 1 <= l <= 8
+// (L, R may be NULL = identity).
+int bicgstab_ell_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R, int32_t l,
+                     Workspace<double>& W, double* x, const double* b, mtl4g_iteration* it, double* h_hist,
+                     int32_t hist_cap, hipStream_t s);
 
 // Mixed precision with reliable updates: one FP32 recurrence (values in
 // value_format), solution accumulated in FP64, the true residual b - A*x

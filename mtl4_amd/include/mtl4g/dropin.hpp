@@ -1071,6 +1071,7 @@ class cyclic_iteration : public basic_iteration<Real> {
         for (std::size_t k = 0; k < hist.size(); k++) {
             if (method == MTL4G_TFQMR        ? k > 1        // finished(rt) before the loop, then tau
                 : method == MTL4G_BICGSTAB_2 ? k % 3 == 1   // loop head, then r_hat[0] and r_hat[1]
+                : method >= 100              ? k % (method - 100 + 1) == 1   // BiCGSTAB(l), l = method - 100
                                              : (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)))   // cg, cgs: one per iteration
                 i++;
             this->i = i;
@@ -1254,6 +1255,7 @@ namespace detail {
 template <typename PC>
 inline mtl4g_pc pc_handle(const PC& P) { return P.handle_(); }
 
+// method: MTL4G_* or 100 + l for BiCGSTAB(l)
 template <typename Matrix, typename VX, typename VB, typename Iter>
 inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter& iter,
                mtl4g_pc R = nullptr) {
@@ -1262,7 +1264,8 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
     mtl4g_iteration s = iteration_access::state(iter);
     const int i0 = s.i;
     const bool want = iteration_access::wants_history(iter);
-    const int cap = want ? 3 * std::max(s.max_iter - s.i, 0) + 4 : 0;
+    const int per = method >= 100 ? method - 100 + 1 : 3;   // finished() calls per iteration, at most
+    const int cap = want ? per * std::max(s.max_iter - s.i, 0) + 4 : 0;
     std::vector<double> hist(cap);
     mtl4g_matrix h = A.device_();
     const double* db = b.device_read_();
@@ -1272,6 +1275,7 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
              : method == MTL4G_CGS   ? mtl4g_cgs(h, P, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_TFQMR ? mtl4g_tfqmr(h, P, R, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_BICGSTAB_2 ? mtl4g_bicgstab_2(h, P, dx, db, &s, hp, cap, nullptr)
+             : method >= 100 ? mtl4g_bicgstab_ell(h, P, R, dx, db, method - 100, &s, hp, cap, nullptr)
                                      : mtl4g_bicgstab(h, P, dx, db, &s, hp, cap, nullptr);
     std::vector<double> used;
     for (double v : hist)
@@ -1341,8 +1345,21 @@ int bicgstab_2(const LinearOperator& A, Vector& x, const Vector& b, const Precon
     return iter;
 }
 
+// itl::bicgstab_ell (krylov/bicgstab_ell.hpp:37-134): BiCGSTAB(l), left and
+// right preconditioners, 1 <= l <= 8 here
+template <typename LinearOperator, typename Vector, typename LeftPreconditioner, typename RightPreconditioner,
+          typename Iteration>
+int bicgstab_ell(const LinearOperator& A, Vector& x, const Vector& b, const LeftPreconditioner& L,
+                 const RightPreconditioner& R, Iteration& iter, std::size_t l) {
+    if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
+    MTL_THROW_IF(l < 1 || l > 8, mtl::logic_error("bicgstab_ell: l must be in 1..8"));
+    detail::run(100 + int(l), A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
+                pc::is_identity(R) ? nullptr : detail::pc_handle(R));
+    return iter;
+}
+
 // itl::base_solver, cg_solver, bicgstab_solver, cgs_solver, tfqmr_solver,
-// bicgstab_2_solver (krylov/base_solver.hpp:24-84, cg.hpp:122-156,
+// bicgstab_2_solver, bicgstab_ell_solver (bicgstab_ell.hpp:137-167) (krylov/base_solver.hpp:24-84, cg.hpp:122-156,
 // bicgstab.hpp:80-112, cgs.hpp:64-95, tfqmr.hpp:98-127, bicgstab_2.hpp:132-165)
 template <typename Solver, typename LinearOperator>
 struct base_solver {
@@ -1459,6 +1476,28 @@ class tfqmr_solver
     }
 
   private:
+    Preconditioner L;
+    RightPreconditioner R;
+};
+
+template <typename LinearOperator, typename Preconditioner, typename RightPreconditioner>
+class bicgstab_ell_solver
+    : public base_solver<bicgstab_ell_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<bicgstab_ell_solver, LinearOperator> base;
+
+  public:
+    explicit bicgstab_ell_solver(const LinearOperator& A, std::size_t l = 8) : base(A), l(l), L(A), R(A) {}
+    bicgstab_ell_solver(const LinearOperator& A, std::size_t l, const Preconditioner& L)
+        : base(A), l(l), L(L), R(A) {}
+    bicgstab_ell_solver(const LinearOperator& A, std::size_t l, const Preconditioner& L, const RightPreconditioner& R)
+        : base(A), l(l), L(L), R(R) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const {
+        return bicgstab_ell(this->A, x, b, L, R, iter, l);
+    }
+
+  private:
+    std::size_t l;
     Preconditioner L;
     RightPreconditioner R;
 };

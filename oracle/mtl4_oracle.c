@@ -772,8 +772,10 @@ out:
  * MTL_WITH_FUSED_ORTHOGONALIZATION) on r_hat[1..2] in place: r_hat_tail
  * is constructed from the sub-vector r_hat[irange(1, imax)], which holds
  * r_hat's own elements (dense_vector.hpp:326-338) and is taken over, not
- * copied.  gamma_aa[1] = strict_upper(1x1) * gamma[2] + gamma[2] is the
- * empty product (zero) plus gamma[2].  omega == 0 is iter.fail(3) and
+ * copied.  gamma_aa[1] = strict_upper(1x1) * gamma[2] + gamma[2]: the
+ * naive dense mat-vec (mat_vec_mult.hpp:259-298) sums 0 * gamma[2] from
+ * zero (the view reads 0 off the band), then gamma[2] is added.
+ * omega == 0 is iter.fail(3) and
  * returns before the shift back x += x0; every other exit adds x0.     */
 int orc_bicgstab_2(const orc_csr* A, const orc_pc* L, double* x, const double* b,
                    orc_iter* it, double* hist, int32_t hist_cap)
@@ -833,7 +835,7 @@ int orc_bicgstab_2(const orc_csr* A, const orc_pc* L, double* x, const double* b
         omega = g2;
         if (omega == 0.0) { it->error = 3; goto out; }    /* "bicg breakdown #2" */
         g1 = ga1 - (0.0 + t12 * g2);
-        gaa1 = 0.0 + g2;
+        gaa1 = (0.0 + 0.0 * g2) + g2;                       /* naive dense mat-vec, then + gamma[2] */
         for (int32_t i = 0; i < n; i++) x[i] += g1 * r[0][i];
         for (int32_t i = 0; i < n; i++) r[0][i] -= ga2 * r[2][i];
         for (int32_t i = 0; i < n; i++) u[0][i] -= g2 * u[2][i];
@@ -846,5 +848,110 @@ shift_back:
 out:
 #undef B2_TRY
     free(wk);
+    return st != ORC_OK ? st : it->error;
+}
+
+/* itl::bicgstab_ell, itl/krylov/bicgstab_ell.hpp:37-134, with left
+ * preconditioner L and right preconditioner R (NULL: identity), 1 <= l.
+ * As bicgstab_2 but with l inner steps, y = A * solve(R, u_hat[j]),
+ * r_hat[j+1] = solve(L, A * solve(R, r_hat[j])), and x = solve(R, x)
+ * before x += x0 on every exit but iter.fail(3).  The minimal-residual part
+ * orthogonalizes r_hat[1..l] in place (orth.hpp:61-96); gamma_aa[j] is the
+ * naive dense mat-vec over the strictly upper view (mat_vec_mult.hpp:
+ * 259-298: every column, 0 off the band, summed from zero) plus gamma[j+1]. */
+int orc_bicgstab_ell(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
+                     int32_t l, orc_iter* it, double* hist, int32_t hist_cap)
+{
+    int32_t n = A->n, nh = 0;
+    if (A->n != A->ncols) return ORC_NOT_SQUARE;
+    if (n == 0 || l < 1) return ORC_INVALID;              /* "empty rhs vector" */
+    size_t sn = (size_t)n, L1 = (size_t)l + 1;
+    double* wk = (double*)calloc(sn * (2 * L1 + 4), sizeof(double));
+    double* sc = (double*)calloc(L1 * L1 + 3 * L1, sizeof(double));
+    double** r = (double**)malloc(2 * L1 * sizeof(double*));
+    if (!wk || !sc || !r) { free(wk); free(sc); free(r); return ORC_OOM; }
+    double** u = r + L1;
+    for (size_t j = 0; j < L1; j++) { r[j] = wk + j * sn; u[j] = wk + (L1 + j) * sn; }
+    double *rt = wk + 2 * L1 * sn, *y = rt + sn, *x0 = y + sn, *tmp = x0 + sn;
+    double *tau = sc, *gam = sc + L1 * L1, *ga = gam + L1, *gaa = ga + L1;
+#define TAU(i, j) tau[(size_t)(i) * L1 + (j)]
+    double rho_0 = 1.0, rho_1, alpha = 0.0, Gamma, beta, omega = 1.0, nr;
+    int st = ORC_OK;
+    it->error = 0;
+#define BL_TRY(e) do { st = (e); if (st != ORC_OK) goto out; } while (0)
+    memcpy(r[0], b, sn * sizeof(double));
+    if (orc_two_norm(n, x) != 0.0) {
+        orc_spmv(A, x, r[0], ORC_MINUS);
+        memcpy(x0, x, sn * sizeof(double));
+        memset(x, 0, sn * sizeof(double));
+    }
+    nr = orc_two_norm(n, r[0]);
+    for (int32_t i = 0; i < n; i++) rt[i] = r[0][i] / nr;
+    BL_TRY(tf_solve(L, r[0], y, n));
+    memcpy(r[0], y, sn * sizeof(double));
+    while (!finished(it, orc_two_norm(n, r[0]), hist, hist_cap, &nh)) {
+        ++it->i;
+        rho_0 = -omega * rho_0;
+        for (int32_t j = 0; j < l; j++) {
+            rho_1 = orc_dot(n, rt, r[j], 4);
+            beta = alpha * rho_1 / rho_0;
+            rho_0 = rho_1;
+            for (int32_t k = 0; k <= j; k++)
+                for (int32_t i = 0; i < n; i++) u[k][i] = r[k][i] - beta * u[k][i];
+            BL_TRY(tf_solve(R, u[j], tmp, n));
+            orc_spmv(A, tmp, y, ORC_ASSIGN);
+            BL_TRY(tf_solve(L, y, u[j + 1], n));
+            Gamma = orc_dot(n, rt, u[j + 1], 4);
+            alpha = rho_0 / Gamma;
+            for (int32_t k = 0; k <= j; k++)
+                for (int32_t i = 0; i < n; i++) r[k][i] -= alpha * u[k + 1][i];
+            if (finished(it, orc_two_norm(n, r[j]), hist, hist_cap, &nh)) goto shift_back;
+            BL_TRY(tf_solve(R, r[j], r[j + 1], n));
+            orc_spmv(A, r[j + 1], y, ORC_ASSIGN);
+            BL_TRY(tf_solve(L, y, r[j + 1], n));
+            for (int32_t i = 0; i < n; i++) x[i] += alpha * u[0][i];
+        }
+        /* orthogonalize_factors(r_hat[1..l]) into tau[1..l][1..l] */
+        TAU(1, 1) = orc_dot(n, r[1], r[1], 4);
+        for (int32_t j = 2; j <= l; j++) {
+            for (int32_t i = 1; i < j; i++) {
+                double t = orc_dot(n, r[i], r[j], 4) / TAU(i, i);
+                TAU(i, j) = t;
+                for (int32_t e = 0; e < n; e++) r[j][e] -= t * r[i][e];
+            }
+            TAU(j, j) = orc_dot(n, r[j], r[j], 4);
+        }
+        for (int32_t j = 1; j <= l; j++) ga[j] = orc_dot(n, r[j], r[0], 4) / TAU(j, j);
+        gam[l] = ga[l];
+        omega = gam[l];
+        if (omega == 0.0) { it->error = 3; goto out; }      /* "bicg breakdown #2" */
+        for (int32_t j = l - 1; j > 0; --j) {
+            double sum = 0.0;
+            for (int32_t i = j + 1; i <= l; ++i) sum += TAU(j, i) * gam[i];
+            gam[j] = ga[j] - sum;
+        }
+        for (int32_t p = 0; p < l - 1; p++) {                /* gamma_aa[p+1] */
+            double t = 0.0;
+            for (int32_t q = 0; q < l - 1; q++) t += (q > p ? TAU(p + 1, q + 1) : 0.0) * gam[q + 2];
+            gaa[p + 1] = t + gam[p + 2];
+        }
+        for (int32_t i = 0; i < n; i++) x[i] += gam[1] * r[0][i];
+        for (int32_t i = 0; i < n; i++) r[0][i] -= ga[l] * r[l][i];
+        for (int32_t i = 0; i < n; i++) u[0][i] -= gam[l] * u[l][i];
+        for (int32_t j = 1; j < l; ++j) {
+            for (int32_t i = 0; i < n; i++) u[0][i] -= gam[j] * u[j][i];
+            for (int32_t i = 0; i < n; i++) x[i] += gaa[j] * r[j][i];
+            for (int32_t i = 0; i < n; i++) r[0][i] -= ga[j] * r[j][i];
+        }
+    }
+shift_back:
+    BL_TRY(tf_solve(R, x, tmp, n));                       /* x = solve(R, x); x += x0 */
+    for (int32_t i = 0; i < n; i++) x[i] = tmp[i] + x0[i];
+out:
+#undef BL_TRY
+#undef TAU
+    free(wk);
+    free(sc);
+    free(r);
     return st != ORC_OK ? st : it->error;
 }

@@ -90,6 +90,8 @@ int orc_bicgstab(const orc_csr* A, const orc_pc* P, double* x, const double* b,
                  orc_iter* it, double* hist, int32_t hist_cap);
 int orc_bicgstab_2(const orc_csr* A, const orc_pc* L, double* x, const double* b,
                    orc_iter* it, double* hist, int32_t hist_cap);
+int orc_bicgstab_ell(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
+                     int32_t l, orc_iter* it, double* hist, int32_t hist_cap);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,8 @@ def lib():
         L.orc_tfqmr.argtypes = [P, P, P, P, P, C.POINTER(Iter), P, I32]
         L.orc_bicgstab_2.restype = C.c_int
         L.orc_bicgstab_2.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
+        L.orc_bicgstab_ell.restype = C.c_int
+        L.orc_bicgstab_ell.argtypes = [P, P, P, P, P, I32, C.POINTER(Iter), P, I32]
         _lib = L
     return _lib
 
@@ -274,3 +276,15 @@ def bicgstab_2(A, b, L=None, x0=None, it=None, hist_cap=0, **kw):
     it = iteration(b, **kw) if it is None else it
     x, st, hist = _solve(lib().orc_bicgstab_2, A, b, L, x0, it, hist_cap)
     return x, st, it, hist
+
+
+def bicgstab_ell(A, b, L=None, R=None, l=2, x0=None, it=None, hist_cap=0, **kw):
+    """itl::bicgstab_ell(A, x, b, L, R, iter, l), itl/krylov/bicgstab_ell.hpp:37-134."""
+    it = iteration(b, **kw) if it is None else it
+    b = _f64(b)
+    x = np.zeros(A.n) if x0 is None else _f64(x0).copy()
+    hist = np.full(max(hist_cap, 1), np.nan)
+    st = lib().orc_bicgstab_ell(A.ref(), L.h if L is not None else None, R.h if R is not None else None,
+                                _p(x), _p(b), int(l), C.byref(it), _p(hist), hist_cap)
+    nh = int(np.argmax(np.isnan(hist))) if np.isnan(hist).any() else len(hist)
+    return x, st, it, hist[:min(nh, hist_cap)]

@@ -200,6 +200,39 @@ int main() {
         CHECK(it2.iterations() == iter.iterations());
     });
 
+    // libs/numeric/itl/test/bicgstab_ell_output_test.cpp: 10x10 laplacian,
+    // b = 1, x = 0.5, cyclic_iteration(b, 100, 0, 1e-8, 10), l = 8, no
+    // preconditioner: |A x - b| <= 1e-6; then left / right ilu_0 with l = 4
+    run("bicgstab_ell_output_test", [] {
+        const int N = 10;
+        matrix_type A(N, N);
+        vector_type b(N * N, 1.0), x(N * N), r(x);
+        laplacian_setup(A, N, N);
+        itl::pc::identity<matrix_type> Ident(A);
+        itl::pc::ilu_0<matrix_type> P(A);
+        x = 0.5;
+        std::ostringstream log;
+        itl::cyclic_iteration<double, std::ostringstream> iter_1(b, 10 * N, 0, 1.e-8, 10, log);
+        bicgstab_ell(A, x, b, Ident, Ident, iter_1, 8);
+        r = b;
+        r -= A * x;
+        CHECK(two_norm(r) <= 0.000001);
+        for (int side = 0; side < 2; side++) {
+            x = 0.5;
+            itl::basic_iteration<double> it(b, 10 * N, 1.e-8);
+            if (side == 0) bicgstab_ell(A, x, b, P, Ident, it, 4);
+            else bicgstab_ell(A, x, b, Ident, P, it, 4);
+            CHECK(it.is_converged());
+            r = b;
+            r -= A * x;
+            CHECK(two_norm(r) <= 1e-6 * two_norm(b));
+        }
+        itl::bicgstab_ell_solver<matrix_type, itl::pc::identity<matrix_type>, itl::pc::ilu_0<matrix_type>> S(A, 4, Ident, P);
+        vector_type y(N * N, 0.5);
+        itl::basic_iteration<double> it2(b, 10 * N, 1.e-8);
+        CHECK(S.solve(y, b, it2) == 0);
+    });
+
     // libs/numeric/itl/test/tfqmr_test.cpp: 10x10 laplacian, b = 1, the
     // same preconditioner left and right
     run("tfqmr_test", [] {

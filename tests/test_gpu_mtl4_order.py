@@ -207,6 +207,36 @@ def test_bicgstab_2_bitwise(m, mtl4_order, fam, kind, start):
     assert np.array_equal(x.numpy(), xo)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000"])
+@pytest.mark.parametrize("l", [1, 3, 8])
+@pytest.mark.parametrize("lr", [("identity", "identity"), ("ilu_0", "identity"), ("identity", "ilu_0"),
+                                ("diagonal", "ilu_0")])
+def test_bicgstab_ell_bitwise(m, mtl4_order, fam, l, lr):
+    """itl::bicgstab_ell in MTL4's reduction order, l = 1, 3, 8, left and
+    right preconditioners, nonzero start: iteration count, every residual
+    handed to finished() and the solution (after x = solve(R, x) + x0) bit
+    for bit with the oracle."""
+    rp, ci, va = family(fam, seed=11)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    rng = np.random.default_rng(111)
+    b = orc.spmv(O, rng.uniform(-1, 1, n))
+    x0 = rng.uniform(-1, 1, n)
+    OL, OR = [None if k == "identity" else orc.Preconditioner(k, O) for k in lr]
+    xo, st, ito, ho = orc.bicgstab_ell(O, b, OL, OR, l, x0=x0, max_iter=1000, rtol=1e-10, hist_cap=10000)
+    L, R = [getattr(m.pc, k)(A) for k in lr]
+    x = dv(m, x0)
+    it = m.cyclic_iteration(dv(m, b), 1000, 1e-10, 0.0, 10**9)
+    it.set_quite(True)
+    it.suppress_resume(True)
+    err = m.bicgstab_ell(A, x, dv(m, b), L, R, it, l)
+    assert err == st
+    assert it.iterations() == ito.i
+    assert it.resid() == ito.resid
+    assert np.array_equal(np.asarray(it.history), ho)
+    assert np.array_equal(x.numpy(), xo)
+
+
 def test_bicgstab_2_omega_breakdown(m, mtl4_order):
     """omega == 0 is iter.fail(3) (bicgstab_2.hpp:107): status 3 after one
     iteration and x returned without the shift back x += x0, bit for bit

@@ -786,6 +786,50 @@ def test_bicgstab_2_reference_case(m):
         m.bicgstab_2(E, m.dense_vector(0), m.dense_vector(0), None, m.basic_iteration(0.0, 3, 1e-8))
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000", "diff3d_16"])
+@pytest.mark.parametrize("l", [1, 4, 8])
+@pytest.mark.parametrize("lr", [("identity", "identity"), ("ilu_0", "identity"), ("identity", "ilu_0")])
+def test_bicgstab_ell_families(m, fam, l, lr):
+    """itl::bicgstab_ell in tree order against the oracle: iterations within
+    2 + 15 %; the solution solves A x = b to the tolerance."""
+    rp, ci, va = family(fam, seed=4)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(3).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OL, OR = [None if k == "identity" else orc.Preconditioner(k, O) for k in lr]
+    xo, st, ito, _ = orc.bicgstab_ell(O, b, OL, OR, l, max_iter=3000, rtol=1e-10)
+    L, R = [getattr(m.pc, k)(A) for k in lr]
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+    err = m.bicgstab_ell(A, x, dv(m, b), L, R, it, l)
+    assert err == st == 0
+    assert abs(it.iterations() - ito.i) <= 2 + 0.15 * ito.i, (it.iterations(), ito.i)
+    res = np.linalg.norm(b - orc.spmv(O, x.numpy()))
+    ores = np.linalg.norm(b - orc.spmv(O, xo))
+    assert res <= max(1e-8 * np.linalg.norm(b), 100 * ores)
+
+
+def test_bicgstab_ell_reference_case(m):
+    """bicgstab_ell_output_test.cpp: 10x10 laplacian_setup, b = 1, x = 0.5,
+    cyclic_iteration(b, 100, 0, 1e-8, 10), l = 8: |A x - b| <= 1e-6; l
+    outside 1..8 and an empty system are errors."""
+    import io
+    A = m.laplacian_setup(10, 10)
+    b = m.dense_vector(np.ones(100))
+    x = m.dense_vector(np.full(100, 0.5))
+    out = io.StringIO()
+    it = m.cyclic_iteration(b, 100, 0.0, 1e-8, 10, out=out)
+    assert m.bicgstab_ell(A, x, b, None, None, it, 8) == 0
+    assert np.linalg.norm((A * x).numpy() - 1.0) <= 1e-6
+    assert "iteration 0: resid" in out.getvalue()
+    with pytest.raises(m.MtlError):
+        m.bicgstab_ell(A, x, b, None, None, m.basic_iteration(b, 10, 1e-8), 9)
+    with pytest.raises(m.MtlError):
+        E = m.compressed2D(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        m.bicgstab_ell(E, m.dense_vector(0), m.dense_vector(0), None, None, m.basic_iteration(0.0, 3, 1e-8), 2)
+
+
 def test_bicgstab_2_solver_handle(m):
     """A solver handle for BiCGSTAB(2) keeps its workspace: two solves give
     the same bits as each other and as the one-shot call."""

@@ -691,3 +691,108 @@ def test_bicgstab_2_empty_is_logic_error():
     A = orc.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
     x, st, it, _ = orc.bicgstab_2(A, np.zeros(0), None, max_iter=3, rtol=1e-8)
     assert st == orc.INVALID
+
+
+# ---------------------------------------------------------------- BiCGSTAB(l)
+@pytest.mark.parametrize("pc", ["identity", "diagonal", "ilu_0"])
+def test_bicgstab_ell_2_is_bicgstab_2(pc):
+    """bicgstab_ell with l = 2 and no right preconditioner runs
+    bicgstab_2's arithmetic step for step: same bits, same history."""
+    D, rng = _nonsym(80, 31, 0.1)
+    A = from_dense(D)
+    b = D @ rng.standard_normal(80)
+    P = orc.Preconditioner(pc, A) if pc != "identity" else None
+    x2, s2, i2, h2 = orc.bicgstab_2(A, b, P, max_iter=100, rtol=1e-12, hist_cap=400)
+    xe, se, ie, he = orc.bicgstab_ell(A, b, P, None, 2, max_iter=100, rtol=1e-12, hist_cap=400)
+    assert s2 == se == 0 and i2.i == ie.i
+    assert np.array_equal(x2, xe) and np.array_equal(h2, he)
+
+
+@pytest.mark.parametrize("l", [1, 3, 4])
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_bicgstab_ell_checked_residuals(l, side):
+    """The residuals bicgstab_ell hands to finished() in its first iteration
+    and the next head and inner check, from Krylov characterizations: with
+    M = L^-1 A and r0 = L^-1 b (left) or M = A R^-1 and r0 = b (right, the
+    iteration runs on R x), rtilde = b/|b|:
+      head |r0|; inner check j: |M^j r_(j+1)| (r_hat[j] = M^j r_hat[0]);
+      next head: min over g of |r_l - sum_i g_i M^i r_l|;
+      next j=0 check: |Q(M) r_(l+1)|, Q(t) = 1 - sum_i g_i t^i,
+    r_k the Petrov-Galerkin BiCG residual of step k."""
+    n = 40
+    D, rng = _nonsym(n, 21)
+    A = from_dense(D)
+    b = rng.standard_normal(n)
+    P = orc.Preconditioner("diagonal", A)
+    Dinv = np.diag(1.0 / np.diag(D))
+    if side == "left":
+        M, r0 = Dinv @ D, Dinv @ b
+        x, st, it, h = orc.bicgstab_ell(A, b, P, None, l, max_iter=2, rtol=1e-14, hist_cap=40)
+    else:
+        M, r0 = D @ Dinv, b.copy()
+        x, st, it, h = orc.bicgstab_ell(A, b, None, P, l, max_iter=2, rtol=1e-14, hist_cap=40)
+    assert st == 1 and it.i == 2 and len(h) == l + 3
+    rt = b / np.linalg.norm(b)
+    want = [np.linalg.norm(r0)]
+    for j in range(l):
+        want.append(np.linalg.norm(np.linalg.matrix_power(M, j) @ _pg_residual(M, r0, rt, j + 1)))
+    rl = _pg_residual(M, r0, rt, l)
+    G = np.column_stack([np.linalg.matrix_power(M, i) @ rl for i in range(1, l + 1)])
+    g = np.linalg.lstsq(G, rl, rcond=None)[0]
+    want.append(np.linalg.norm(rl - G @ g))
+    rn = _pg_residual(M, r0, rt, l + 1)
+    want.append(np.linalg.norm(rn - sum(g[i - 1] * (np.linalg.matrix_power(M, i) @ rn) for i in range(1, l + 1))))
+    assert np.allclose(h, want, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("l", [1, 3, 4, 8])
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_bicgstab_ell_solution(l, side):
+    """Solved to 1e-12 on a nonsymmetric system with ILU(0) on either side:
+    x (after x = solve(R, x) + x0) solves A x = b; when the loop head or the
+    first inner check stopped it, x carries exactly the preconditioned
+    residual that check (or the loop head before it) measured."""
+    n = 80
+    D, rng = _nonsym(n, 33, 0.1)
+    A = from_dense(D)
+    xt = rng.standard_normal(n)
+    b = D @ xt
+    x0 = rng.standard_normal(n)
+    P = orc.Preconditioner("ilu_0", A)
+    L, R = (P, None) if side == "left" else (None, P)
+    x, st, it, h = orc.bicgstab_ell(A, b, L, R, l, x0=x0, max_iter=300, rtol=1e-12, hist_cap=3000)
+    assert st == 0
+    assert np.allclose(x, xt, rtol=1e-6, atol=1e-7)
+    k = (len(h) - 1) % (l + 1)   # 0: loop head, else inner check k-1
+    if k <= 1:
+        # the recurrence drifts from the true residual by rounding, more so
+        # the more vectors the minimal-residual step combines
+        r = b - orc.spmv(A, x)
+        got = np.linalg.norm(P.solve(r) if side == "left" else r)
+        assert got == pytest.approx(h[-1] if k == 0 else h[-2], rel=1e-6, abs=1e-11 * l * np.linalg.norm(b))
+
+
+def test_bicgstab_ell_reference_case():
+    """bicgstab_ell_output_test.cpp: 10x10 laplacian_setup, b = 1, x = 0.5,
+    cyclic_iteration(b, 100, 0, 1e-8), l = 8: |A x - b| <= 1e-6."""
+    A = orc.laplacian_setup(10, 10)
+    b = np.ones(A.n)
+    x, st, it, _ = orc.bicgstab_ell(A, b, None, None, 8, x0=np.full(A.n, 0.5), max_iter=100, rtol=0.0,
+                                    atol=1e-8)
+    assert st == 0
+    assert np.linalg.norm(orc.spmv(A, x) - b) <= 1e-6
+
+
+def test_bicgstab_ell_breakdown_and_bad_l():
+    D = np.array([[1.0, 0.0, -2.0], [0.0, -2.0, 1.0], [2.0, 2.0, -1.0]])
+    x_init = np.array([1.0, 0.0, 0.0])
+    b = np.array([-2.0, 0.0, 0.0]) + D @ x_init
+    x2, s2, _, _ = orc.bicgstab_2(from_dense(D), b, None, x0=x_init, max_iter=5, rtol=1e-14)
+    xe, se, it, _ = orc.bicgstab_ell(from_dense(D), b, None, None, 2, x0=x_init, max_iter=5, rtol=1e-14)
+    assert s2 == se == 3 and it.i == 1 and np.array_equal(x2, xe)
+    A = orc.laplacian_setup(3, 3)
+    _, st, _, _ = orc.bicgstab_ell(A, np.ones(9), None, None, 0, max_iter=5, rtol=1e-8)
+    assert st == orc.INVALID
+    E = orc.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    _, st, _, _ = orc.bicgstab_ell(E, np.zeros(0), None, None, 2, max_iter=3, rtol=1e-8)
+    assert st == orc.INVALID
