@@ -72,6 +72,7 @@ extern "C" {
 #define MTL4G_TFQMR    3   /* itl::tfqmr,    itl/krylov/tfqmr.hpp:32-96 (FP64; mtl4g_tfqmr) */
 #define MTL4G_BICGSTAB_2 4 /* itl::bicgstab_2, itl/krylov/bicgstab_2.hpp:37-130 (FP64) */
 #define MTL4G_BICGSTAB_ELL 5 /* itl::bicgstab_ell, itl/krylov/bicgstab_ell.hpp:37-134 (FP64; mtl4g_bicgstab_ell) */
+#define MTL4G_GMRES    6   /* itl::gmres,    itl/krylov/gmres.hpp:37-137 (FP64; mtl4g_gmres) */
 
 typedef struct mtl4g_matrix_s* mtl4g_matrix;
 typedef struct mtl4g_pc_s* mtl4g_pc;
@@ -274,6 +275,17 @@ int mtl4g_bicgstab_2(mtl4g_matrix A, mtl4g_pc L, double* d_x, const double* d_b,
  * system is MTL4's logic_error; l outside 1..8 is MTL4G_ERR_INVALID. */
 int mtl4g_bicgstab_ell(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* d_x, const double* d_b, int32_t l,
                        mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
+/* itl::gmres(A, x, b, L, R, iter, restart) (gmres.hpp:122-137) for restart
+ * > 0: cycles of gmres_full on a copy of iter limited to restart more
+ * iterations, each followed by iter.update_progress; restart == 0 is
+ * itl::gmres_full(A, x, b, L, R, iter) (:37-119) alone.  A cycle always runs
+ * min(n, iterations left) Arnoldi steps, as MTL4's does.  h_hist receives
+ * the residuals a cycle hands to finished(): two_norm(solve(L, b - A*x))
+ * at its start, two_norm(b - A*x) at its end.  iter.fail(2) (no direction)
+ * returns 2.  At most 256 Krylov vectors per cycle (MTL4G_ERR_INVALID
+ * beyond); an empty system is MTL4's logic_error. */
+int mtl4g_gmres(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* d_x, const double* d_b, int32_t restart,
+                mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream);
 
 /* Reusable solver: the workspace (MTL4 allocates it per call, cg.hpp:90)
  * stays with the handle, so repeated solves allocate nothing. */

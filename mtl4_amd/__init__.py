@@ -24,7 +24,7 @@ from ._lib import (ASSIGN, BF16, BICGSTAB, CG, F32, F64, MINUS, ORDER_MTL4, ORDE
 
 __all__ = ["compressed2D", "dense_vector", "pc", "solve", "adjoint_solve", "basic_iteration",
            "cyclic_iteration", "noisy_iteration", "cg", "bicgstab", "cgs", "tfqmr", "bicgstab_2",
-           "bicgstab_ell", "dot",
+           "bicgstab_ell", "gmres", "gmres_full", "dot",
            "unary_dot",
            "two_norm", "num_rows", "num_cols", "size", "laplacian_setup", "ir_solve",
            "MtlError", "matrix_not_square", "incompatible_size", "missing_diagonal",
@@ -425,6 +425,8 @@ class cyclic_iteration(basic_iteration):
                 i += 1
             elif method == _lib.BICGSTAB_ELL and k % (ell + 1) == 1:
                 i += 1
+            elif method == _lib.GMRES and k % 2 == 1:   # start, end of each cycle of `ell` steps
+                i = min(i + (ell if ell > 0 else self.max_iter), self.max_iter)
             elif method in (_lib.CG, _lib.CGS) and k > 0:
                 i += 1
             elif method == _lib.TFQMR and k > 1:   # finished(rt) before the loop, then tau
@@ -468,7 +470,7 @@ def _krylov(fn, method, A, x, b, P, it, R=None, ell=2):
     hp = hist.ctypes.data_as(C.c_void_p) if cap else None
     if method == _lib.TFQMR:
         status = fn(A.h, ph, R.h if R is not None else None, x.ptr, b.ptr, C.byref(st), hp, cap, _stream())
-    elif method == _lib.BICGSTAB_ELL:
+    elif method in (_lib.BICGSTAB_ELL, _lib.GMRES):
         status = fn(A.h, ph, R.h if R is not None else None, x.ptr, b.ptr, int(ell), C.byref(st), hp, cap,
                     _stream())
     else:
@@ -477,7 +479,7 @@ def _krylov(fn, method, A, x, b, P, it, R=None, ell=2):
     h = h[~np.isnan(h)] if cap else None
     it._update(st, h, method, ell)
     check(status, {CG: "cg", BICGSTAB: "bicgstab", _lib.CGS: "cgs", _lib.BICGSTAB_2: "bicgstab_2",
-                   _lib.BICGSTAB_ELL: "bicgstab_ell"}.get(method, "tfqmr"))
+                   _lib.BICGSTAB_ELL: "bicgstab_ell", _lib.GMRES: "gmres"}.get(method, "tfqmr"))
     return it.error_code()
 
 
@@ -522,6 +524,22 @@ def bicgstab_ell(A, x, b, L, R, it, l):
     37-134): BiCGSTAB(l), 1 <= l <= 8, left preconditioner L and right
     preconditioner R (None: identity).  Returns 3 (iter.fail(3)) on omega == 0."""
     return _krylov(_L().mtl4g_bicgstab_ell, _lib.BICGSTAB_ELL, A, x, b, L, it, R, ell=l)
+
+
+def gmres(A, x, b, L, R, it, restart):
+    """itl::gmres(A, x, b, L, R, iter, restart) (itl/krylov/gmres.hpp:122-137):
+    restarted GMRES, left preconditioner L and right preconditioner R (None:
+    identity), at most 256 Krylov vectors per cycle.  Returns 2
+    (iter.fail(2)) when no direction was found."""
+    if int(restart) < 1:
+        raise logic_error("gmres: restart must be positive")
+    return _krylov(_L().mtl4g_gmres, _lib.GMRES, A, x, b, L, it, R, ell=int(restart))
+
+
+def gmres_full(A, x, b, L, R, it):
+    """itl::gmres_full(A, x, b, L, R, iter) (itl/krylov/gmres.hpp:37-119): one
+    cycle of min(n, iterations left) Arnoldi steps."""
+    return _krylov(_L().mtl4g_gmres, _lib.GMRES, A, x, b, L, it, R, ell=0)
 
 
 class Solver:

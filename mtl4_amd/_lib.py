@@ -17,7 +17,7 @@ ERR_BREAKDOWN, ERR_NONFINITE, ERR_OOM, ERR_HIP, ERR_UNSUPPORTED, ERR_TIMEOUT = -
 ASSIGN, PLUS, MINUS = 0, 1, 2
 PC_IDENTITY, PC_DIAGONAL, PC_ILU0, PC_IC0 = 0, 1, 2, 3
 F64, F32, BF16 = 0, 1, 2
-CG, BICGSTAB, CGS, TFQMR, BICGSTAB_2, BICGSTAB_ELL = 0, 1, 2, 3, 4, 5
+CG, BICGSTAB, CGS, TFQMR, BICGSTAB_2, BICGSTAB_ELL, GMRES = 0, 1, 2, 3, 4, 5, 6
 
 
 class Iteration(C.Structure):
@@ -107,6 +107,7 @@ _SIGS = {
     "mtl4g_tfqmr": (_INT, [_P, _P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_bicgstab_2": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_bicgstab_ell": (_INT, [_P, _P, _P, _P, _P, _I32, C.POINTER(Iteration), _P, _I32, _P]),
+    "mtl4g_gmres": (_INT, [_P, _P, _P, _P, _P, _I32, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_solver_create": (_INT, [_PP, _P, _P, _INT]),
     "mtl4g_solver_destroy": (_INT, [_P]),
     "mtl4g_solver_solve": (_INT, [_P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),

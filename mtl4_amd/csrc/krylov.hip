@@ -1235,6 +1235,355 @@ struct EpiEllShift {
     __device__ void finish(const double*) const {}
 };
 
+// ---------------------------------------------------------------- GMRES epilogues
+// itl::gmres_full / itl::gmres, itl/krylov/gmres.hpp:37-137.  The Krylov
+// basis V (kmax + 1 vectors) and the vectors r0, r, va0 live in the
+// workspace; H ((kmax+1) x kmax, row-major), g, the rotations, y and the
+// reorthogonalization coefficients live in a small device array.
+constexpr int kMaxGmres = 256;   // Krylov vectors per cycle
+struct GmScalars {
+    double* sm;
+    int kmax;
+    __host__ __device__ double& H(int i, int j) const { return sm[(size_t)i * kmax + j]; }
+    __host__ __device__ double* g() const { return sm + (size_t)(kmax + 1) * kmax; }
+    __host__ __device__ double* cs() const { return g() + (kmax + 1); }
+    __host__ __device__ double* sn() const { return cs() + (kmax + 1); }
+    __host__ __device__ double* y() const { return sn() + (kmax + 1); }
+    __host__ __device__ double* z() const { return y() + (kmax + 1); }
+    __host__ __device__ double* hr() const { return z() + (kmax + 1); }
+    __host__ __device__ double* perm() const { return hr() + (kmax + 1); }   // pivots (small integers)
+    __host__ __device__ double& m() const { return perm()[kmax + 1]; }      // columns solved for
+    static size_t words(int kmax) { return (size_t)(kmax + 1) * kmax + 7 * (size_t)(kmax + 1) + 1; }
+};
+
+// r0 = b - A*x (compute_summand: the product first, then b - it); with
+// NORM: two_norm of it handed to finished() -- the terminate(r) that ends
+// a cycle (gmres.hpp:117-118); the next cycle's r0 is the same vector
+template <bool NORM>
+struct EpiGmRes {
+    static constexpr int NP = NORM ? 1 : 0;
+    __host__ __device__ static constexpr int order(int) { return RO_NORM4; }
+    struct Pre {
+        double b;
+    };
+    const double* b;
+    double* r0;
+    Ctl* c;
+    double* hist;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ Pre preload(int row) const { return Pre{b[row]}; }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
+        const double v = pre.b - s;
+        r0[row] = v;
+        if (NORM) part[0] += v * v;
+    }
+    __device__ void finish(const double* tot) const {
+        if (NORM) iter_check(c, std::sqrt(tot[0]), hist);
+    }
+};
+
+// rho = g[0] = two_norm(r); finished(rho) at the start of a cycle (:56-58)
+struct EpiGmRho {
+    static constexpr int NP = 1;
+    __host__ __device__ static constexpr int order(int) { return RO_NORM4; }
+    const double* r;
+    GmScalars G;
+    Ctl* c;
+    double* hist;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double* part) const { part[0] += r[i] * r[i]; }
+    __device__ void finish(const double* tot) const {
+        const double rho = std::sqrt(tot[0]);
+        G.g()[0] = rho;
+        c->gamma = rho;
+        iter_check(c, rho, hist);
+    }
+};
+
+// V.vector(0) = r / rho  (:59)
+struct EpiGmV0 {
+    static constexpr int NP = 0;
+    const double* r;
+    double* v0;
+    Ctl* c;
+    double rho;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { rho = c->gamma; }
+    __device__ void operator()(int64_t i, double*) const { v0[i] = r[i] / rho; }
+    __device__ void finish(const double*) const {}
+};
+
+// V_(k+1) = A * in (L = identity), with DOT: H[0][k] = dot(V_0, V_(k+1))
+template <bool DOT>
+struct EpiGmMv {
+    static constexpr int NP = DOT ? 1 : 0;
+    __host__ __device__ static constexpr int order(int) { return RO_DOT4; }
+    struct Pre {
+        double v0;
+    };
+    const double* v0;
+    double* out;
+    GmScalars G;
+    int k;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ Pre preload(int row) const { return Pre{DOT ? v0[row] : 0.0}; }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
+        out[row] = s;
+        if (DOT) part[0] += pre.v0 * s;
+    }
+    __device__ void finish(const double* tot) const {
+        if (DOT) G.H(0, k) = tot[0];
+    }
+};
+
+// One pass of the orthogonalization of w = V_(k+1) (:68-79), fused with the
+// axpy of the pass before.  STEP:
+//   0: H[0][k] = dot(V_0, w)                               (no axpy; L path)
+//   1: w -= H[j-1][k] V_(j-1); H[j][k] = dot(V_j, w)       (MGS, 1 <= j <= k)
+//   2: w -= H[k][k] V_k; H[k+1][k] = two_norm(w); hr[0] = dot(w, V_0);
+//      H[0][k] += hr[0]                                    (end of MGS, reorth j = 0)
+//   3: w -= hr[j-1] V_(j-1); hr[j] = dot(w, V_j); H[j][k] += hr[j]  (reorth, 1 <= j <= k)
+//   4: w -= hr[k] V_k; H[k+1][k] = two_norm(w)
+template <int STEP>
+struct EpiGmOrth {
+    static constexpr int NP = STEP == 2 ? 2 : 1;
+    __host__ __device__ static constexpr int order(int q) {
+        return (STEP == 4 || (STEP == 2 && q == 0)) ? RO_NORM4 : RO_DOT4;
+    }
+    double* w;
+    const double *va, *vb;   // va: V_(j-1) or V_k for the axpy; vb: V_j for the dot
+    GmScalars G;
+    int j, k;
+    Ctl* c;
+    double t;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        t = STEP == 1 ? G.H(j - 1, k) : STEP == 2 ? G.H(k, k) : STEP == 3 ? G.hr()[j - 1] : STEP == 4 ? G.hr()[k] : 0.0;
+    }
+    __device__ void operator()(int64_t i, double* part) const {
+        double v = w[i];
+        if (STEP != 0) {
+            v = v - t * va[i];
+            w[i] = v;
+        }
+        if (STEP == 0 || STEP == 1) part[0] += vb[i] * v;
+        else if (STEP == 2) {
+            part[0] += v * v;
+            part[1] += v * vb[i];
+        } else if (STEP == 3) part[0] += v * vb[i];
+        else part[0] += v * v;
+    }
+    __device__ void finish(const double* tot) const {
+        if (STEP == 0 || STEP == 1) G.H(j, k) = tot[0];
+        else if (STEP == 2) {
+            G.H(k + 1, k) = std::sqrt(tot[0]);
+            G.hr()[0] = tot[1];
+            G.H(0, k) += tot[1];
+        } else if (STEP == 3) {
+            G.hr()[j] = tot[0];
+            G.H(j, k) += tot[0];
+        } else {
+            G.H(k + 1, k) = std::sqrt(tot[0]);
+        }
+    }
+};
+
+// V_(k+1) *= 1. / H[k+1][k] unless H[k+1][k] == 0  (:80-81)
+struct EpiGmScale {
+    static constexpr int NP = 0;
+    double* w;
+    GmScalars G;
+    int k;
+    Ctl* c;
+    double f;
+    int on;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {
+        const double h = G.H(k + 1, k);
+        on = h != 0.0;
+        f = on ? 1. / h : 0.0;
+    }
+    __device__ void operator()(int64_t i, double*) const {
+        if (on) w[i] = w[i] * f;
+    }
+    __device__ void finish(const double*) const {}
+};
+
+// Step k's rotations (:84-94) and ++iter: for i < k the rotation of
+// (H[i][k-1], H[i+1][k-1]) applied as operation/givens.hpp does -- rows
+// i, i+1 of H <- trans(G) * rows, then columns i, i+1 <- columns * G, each
+// 2-term product summed from zero -- then the new rotation (c_k, s_k) of
+// column k and of g.  One workgroup: a rotation's row and column updates
+// run across the lanes, rotations one after the other.
+__global__ __launch_bounds__(256) void k_gm_givens(GmScalars G, int k, Ctl* c) {
+    if (c->done) return;
+    const int kmax = G.kmax;
+    for (int i = 0; i < k; i++) {
+        const double a = G.H(i, k - 1), b = G.H(i + 1, k - 1);
+        double cc, ss, t;
+        if (b == 0.0) { cc = 1.0; ss = 0.0; }
+        else if (fabs(b) > fabs(a)) { t = -a / b; ss = 1.0 / sqrt(1.0 + t * t); cc = ss * t; }
+        else { t = -b / a; cc = 1.0 / sqrt(1.0 + t * t); ss = cc * t; }
+        const double ms = -ss;
+        __syncthreads();   // every lane has read a and b
+        for (int jj = threadIdx.x; jj < kmax; jj += blockDim.x) {
+            const double b0 = G.H(i, jj), b1 = G.H(i + 1, jj);
+            G.H(i, jj) = (0.0 + cc * b0) + ms * b1;
+            G.H(i + 1, jj) = (0.0 + ss * b0) + cc * b1;
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r <= kmax; r += blockDim.x) {
+            const double m0 = G.H(r, i), m1 = G.H(r, i + 1);
+            G.H(r, i) = (0.0 + m0 * cc) + m1 * ms;
+            G.H(r, i + 1) = (0.0 + m0 * ss) + m1 * cc;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double hkk = G.H(k, k), hk1 = G.H(k + 1, k);
+        const double nu = sqrt(hkk * hkk + hk1 * hk1);
+        if (nu != 0.0) {
+            double* cs = G.cs();
+            double* sn = G.sn();
+            double* g = G.g();
+            cs[k] = hkk / nu;
+            sn[k] = -hk1 / nu;
+            G.H(k, k) = cs[k] * hkk - sn[k] * hk1;
+            G.H(k + 1, k) = 0;
+            const double w1 = cs[k] * g[k] - sn[k] * g[k + 1], w2 = sn[k] * g[k] + cs[k] * g[k + 1];
+            g[k] = w1;
+            g[k + 1] = w2;
+        }
+        c->i++;
+        c->first = c->i <= 1;
+    }
+}
+
+// After the cycle (:98-114): drop trailing columns with |g[k-1]| <= atol,
+// then lu_solve(H[range][range], g[range]) shrinking the range while a
+// pivot is zero.  The LU runs in place on H, as in MTL4 (the sub-matrix is
+// a view and the copy of a view shares its memory): partial pivoting with
+// the first row of largest magnitude, rows swapped within the block;
+// permute(P, g) scatters (g'[P[i]] = g[i]); then the unit lower and the
+// upper sweep (division by the diagonal).  No direction: iter.fail(2).
+__global__ __launch_bounds__(256) void k_gm_lsq(GmScalars G, int kmax_run, double atol, Ctl* c) {
+    if (c->done) return;
+    __shared__ int s_m, s_rmax, s_sing;
+    if (threadIdx.x == 0) {
+        int k = kmax_run;
+        while (k > 0 && fabs(G.g()[k - 1]) <= atol) k--;
+        s_m = k;
+    }
+    __syncthreads();
+    double* P = G.perm();
+    for (int m = s_m; m > 0; m--) {
+        for (int i = threadIdx.x; i < m; i += blockDim.x) P[i] = i;
+        __syncthreads();
+        if (threadIdx.x == 0) s_sing = 0;
+        for (int i = 0; i + 1 < m; i++) {
+            if (threadIdx.x == 0) {
+                int rmax = i;
+                double best = fabs(G.H(i, i));
+                for (int r = i + 1; r < m; r++)
+                    if (fabs(G.H(r, i)) > best) { best = fabs(G.H(r, i)); rmax = r; }
+                s_rmax = rmax;
+                if (rmax != i) {
+                    const double t = P[i];
+                    P[i] = P[rmax];
+                    P[rmax] = t;
+                }
+            }
+            __syncthreads();
+            const int rmax = s_rmax;
+            if (rmax != i)
+                for (int cc = threadIdx.x; cc < m; cc += blockDim.x) {
+                    const double t = G.H(i, cc);
+                    G.H(i, cc) = G.H(rmax, cc);
+                    G.H(rmax, cc) = t;
+                }
+            __syncthreads();
+            if (fabs(G.H(i, i)) <= 0.0) {   // every lane reads the same pivot
+                if (threadIdx.x == 0) s_sing = 1;
+                break;
+            }
+            const double piv = G.H(i, i);
+            for (int r = i + 1 + threadIdx.x; r < m; r += blockDim.x) {
+                const double l = G.H(r, i) / piv;
+                G.H(r, i) = l;
+                for (int cc = i + 1; cc < m; cc++) G.H(r, cc) -= l * G.H(i, cc);
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        if (s_sing) continue;   // the next (smaller) block starts from H as left here
+        if (threadIdx.x == 0) {
+            double *y = G.y(), *z = G.z();
+            const double* g = G.g();
+            for (int i = 0; i < m; i++) y[(int)P[i]] = g[i];
+            for (int r = 0; r < m; r++) {
+                double rr = y[r];
+                for (int cc = 0; cc < r; cc++) rr -= G.H(r, cc) * z[cc];
+                z[r] = rr;
+            }
+            for (int r = m; r-- > 0;) {
+                double rr = z[r];
+                for (int cc = r + 1; cc < m; cc++) rr -= G.H(r, cc) * y[cc];
+                y[r] = rr / G.H(r, r);
+            }
+            G.m() = m;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) {   // "GMRES did not find any direction to correct x"
+        G.m() = 0;
+        c->error = 2;
+        c->done = 1;
+    }
+}
+
+// out = V.vector(range) * y[range]: zero, then += V_j * y_j column by
+// column (mat_vec_mult.hpp:352-360)
+struct EpiGmComb {
+    static constexpr int NP = 0;
+    const double* v;   // V_0; V_j = v + j*n
+    int64_t n;
+    double* out;
+    GmScalars G;
+    Ctl* c;
+    int m;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() { m = (int)G.m(); }
+    __device__ void operator()(int64_t i, double*) const {
+        double t = 0.0;
+        const double* y = G.y();
+        for (int j = 0; j < m; j++) t += v[(size_t)j * n + i] * y[j];
+        out[i] = t;
+    }
+    __device__ void finish(const double*) const {}
+};
+
+// x += solve(R, V*y)  (:115)
+struct EpiGmX {
+    static constexpr int NP = 0;
+    double* x;
+    const double* t;
+    Ctl* c;
+    __device__ bool skip() const { return c->done != 0; }
+    __device__ void prepare() {}
+    __device__ void operator()(int64_t i, double*) const { x[i] = x[i] + t[i]; }
+    __device__ void finish(const double*) const {}
+};
+
+__global__ void k_gm_zero(double* p, size_t words, const Ctl* c) {
+    if (c->done) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
 // ---------------------------------------------------------------- FP64 refresh
 // x += double(x32); x32 = 0
 struct EpiRu1 {
@@ -1931,6 +2280,148 @@ int bicgstab_ell_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_p
     return hc.error;
 }
 
+// itl::gmres (restart > 0, gmres.hpp:122-137) or itl::gmres_full (restart
+// == 0, :37-119).  A cycle runs kmax Arnoldi steps whatever the residual
+// (as gmres_full does), then the small least-squares solve, the update of
+// x and the true residual; the host reads the control block once per cycle.
+int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R, int32_t restart,
+              Workspace<double>& W, double* x, const double* b, mtl4g_iteration* it, double* h_hist,
+              int32_t hist_cap, hipStream_t s) {
+    if (!A || !it || (A->n && (!x || !b)) || restart < 0) return MTL4G_ERR_INVALID;
+    if (A->n != A->m) {
+        set_error("Krylov solve needs a square matrix");
+        return MTL4G_ERR_NOT_SQUARE;
+    }
+    if ((L && L->n != A->n) || (R && R->n != A->n)) {
+        set_error("preconditioner size != matrix size %d", A->n);
+        return MTL4G_ERR_SIZE;
+    }
+    if (A->n == 0) {   // gmres.hpp:45
+        set_error("empty rhs vector");
+        return MTL4G_ERR_INVALID;
+    }
+    if (L && L->kind == MTL4G_PC_IDENTITY) L = nullptr;
+    if (R && R->kind == MTL4G_PC_IDENTITY) R = nullptr;
+    const int64_t n = A->n;
+    // Krylov vectors per cycle: min(size(x), iterations left), at most restart
+    auto cycle_len = [&](int32_t i) -> int64_t {
+        const int64_t lim = restart > 0 ? std::min<int64_t>((int64_t)i + restart, it->max_iter) : it->max_iter;
+        const int64_t rem = lim - i;
+        return (rem < 0 || rem > n) ? n : rem;
+    };
+    const int64_t kcap = restart > 0 ? std::min<int64_t>(restart, n) : cycle_len(it->i);
+    if (kcap > kMaxGmres) {
+        set_error("gmres: %lld Krylov vectors per cycle, at most %d here", (long long)kcap, kMaxGmres);
+        return MTL4G_ERR_INVALID;
+    }
+    const int kvec = (int)std::max<int64_t>(kcap, 1);
+    const int nvec = kvec + 1 + 4;
+    M4_TRY(ensure_workspace(W, A->n, nvec, L, hist_cap, s));
+    M4_TRY(ensure_workspace(W, A->n, nvec, R, hist_cap, s));
+    const size_t words = GmScalars::words(kvec);
+    if (W.small.n < words) M4_TRY(W.small.alloc(words));
+    double* base = W.vecs.p;
+    double *r0 = base, *rv = base + n, *tmp = base + 2 * n, *y0 = base + 3 * n, *V = base + 4 * n;
+    auto Vj = [&](int j) { return V + (size_t)j * n; };
+    Ctl init;
+    std::memset(&init, 0, sizeof init);
+    init.norm_r0 = it->norm_r0;
+    init.rtol = it->rtol;
+    init.atol = it->atol;
+    init.max_iter = it->max_iter;
+    init.i = it->i;
+    init.hist_cap = h_hist ? hist_cap : 0;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
+    M4_HIP(hipGetLastError());
+    Ctl* c = W.ctl.p;
+    double* part = W.partials.p;
+    double* tm = W.terms.p;
+    unsigned* tk = W.tickets.p;
+    const double* val = A->val.p;
+    // r0 = b - A*x of the first cycle; later cycles reuse terminate's r
+    M4_TRY(launch_csr<double>(A, val, x, EpiGmRes<false>{b, r0, c, W.hist.p}, part, tk, s));
+    int32_t i_now = it->i;
+    const int64_t max_cycles = (restart > 0 ? (std::max(0, it->max_iter - it->i) / restart + 2) : 1);
+    for (int64_t cyc = 0; cyc < max_cycles; cyc++) {
+        // <= kvec whenever the cycle runs (a cycle that starts at or past
+        // max_iter stops at its first finished() and launches nothing that runs)
+        const int kmax = (int)std::min<int64_t>(cycle_len(i_now), kvec);
+        const GmScalars G{W.small.p, kmax};
+        hipLaunchKernelGGL(k_gm_zero, dim3(1), dim3(256), 0, s, W.small.p, GmScalars::words(kmax), (const Ctl*)c);
+        M4_HIP(hipGetLastError());
+        // r = solve(L, r0); rho = g[0] = two_norm(r); finished(rho); V_0 = r / rho  (:50-59)
+        const double* r = r0;
+        if (L) {
+            M4_TRY(pc_apply<double>(L, r0, rv, y0, &c->done, W.pc_counters.p, s));
+            r = rv;
+        }
+        M4_TRY(launch_vec<double>(n, EpiGmRho{r, G, c, W.hist.p}, part, tk, s, tm));
+        M4_TRY(launch_vec<double>(n, EpiGmV0{r, Vj(0), c, 0.0}, part, tk, s));
+        for (int k = 0; k < kmax; k++) {   // (:63-96)
+            const double* in = Vj(k);
+            if (R) {
+                M4_TRY(pc_apply<double>(R, in, tmp, y0, &c->done, W.pc_counters.p, s));
+                in = tmp;
+            }
+            double* w = Vj(k + 1);
+            if (!L) {
+                M4_TRY((launch_csr<double>(A, val, in, EpiGmMv<true>{Vj(0), w, G, k, c}, part, tk, s, tm)));
+            } else {
+                M4_TRY((launch_csr<double>(A, val, in, EpiGmMv<false>{Vj(0), rv, G, k, c}, part, tk, s)));
+                M4_TRY(pc_apply<double>(L, rv, w, y0, &c->done, W.pc_counters.p, s));
+                M4_TRY(launch_vec<double>(n, EpiGmOrth<0>{w, nullptr, Vj(0), G, 0, k, c, 0.0}, part, tk, s, tm));
+            }
+            for (int j = 1; j <= k; j++)
+                M4_TRY(launch_vec<double>(n, EpiGmOrth<1>{w, Vj(j - 1), Vj(j), G, j, k, c, 0.0}, part, tk, s, tm));
+            M4_TRY(launch_vec<double>(n, EpiGmOrth<2>{w, Vj(k), Vj(0), G, 0, k, c, 0.0}, part, tk, s, tm));
+            for (int j = 1; j <= k; j++)
+                M4_TRY(launch_vec<double>(n, EpiGmOrth<3>{w, Vj(j - 1), Vj(j), G, j, k, c, 0.0}, part, tk, s, tm));
+            M4_TRY(launch_vec<double>(n, EpiGmOrth<4>{w, Vj(k), nullptr, G, 0, k, c, 0.0}, part, tk, s, tm));
+            M4_TRY(launch_vec<double>(n, EpiGmScale{w, G, k, c, 0.0, 0}, part, tk, s));
+            hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, G, k, c);
+            M4_HIP(hipGetLastError());
+        }
+        // least squares, x += solve(R, V*y), r = b - A*x, terminate(r)  (:98-118)
+        hipLaunchKernelGGL(k_gm_lsq, dim3(1), dim3(256), 0, s, G, kmax, it->atol, c);
+        M4_HIP(hipGetLastError());
+        M4_TRY(launch_vec<double>(n, EpiGmComb{Vj(0), n, rv, G, c, 0}, part, tk, s));
+        const double* t = rv;
+        if (R) {
+            M4_TRY(pc_apply<double>(R, rv, tmp, y0, &c->done, W.pc_counters.p, s));
+            t = tmp;
+        }
+        M4_TRY(launch_vec<double>(n, EpiGmX{x, t, c}, part, tk, s));
+        M4_TRY(launch_csr<double>(A, val, x, EpiGmRes<true>{b, r0, c, W.hist.p}, part, tk, s, tm));
+        M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        M4_HIP(hipStreamSynchronize(s));
+        if (W.h_ctl->done || restart == 0) break;
+        i_now = W.h_ctl->i;
+    }
+    if (L) M4_TRY(pc_check_error(L, s));
+    if (R) M4_TRY(pc_check_error(R, s));
+    M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    M4_HIP(hipStreamSynchronize(s));
+    const Ctl& hc = *W.h_ctl;
+    if (restart > 0 && !hc.done) {
+        set_error("gmres did not stop after %lld cycles", (long long)max_cycles);
+        return MTL4G_ERR_INVALID;
+    }
+    it->i = hc.i;
+    it->error = hc.error;
+    it->resid = hc.resid;
+    if (h_hist && hist_cap > 0) {
+        int32_t nh = std::min(hc.hist_n, hist_cap);
+        if (nh > 0) M4_HIP(hipMemcpy(h_hist, W.hist.p, (size_t)nh * sizeof(double), hipMemcpyDeviceToHost));
+        for (int32_t k = nh; k < hist_cap; k++) h_hist[k] = NAN;
+    }
+    if (hc.status != 0) {
+        set_error("non-finite residual");
+        return hc.status;
+    }
+    if (hc.error == 2) set_error("GMRES did not find any direction to correct x");
+    return hc.error;
+}
+
 template int krylov_run<double>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<double>&,
                                 double*, const double*, mtl4g_iteration*, double*, int32_t, hipStream_t);
 template int krylov_run<float>(const mtl4g_matrix_s*, const mtl4g_pc_s*, int, int, Workspace<float>&,
@@ -2108,6 +2599,12 @@ int mtl4g_bicgstab_ell(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* x, const 
                        mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream) {
     Workspace<double> W;
     return bicgstab_ell_run(A, L, R, l, W, x, b, it, h_hist, hist_cap, S(stream));
+}
+
+int mtl4g_gmres(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* x, const double* b, int32_t restart,
+                mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream) {
+    Workspace<double> W;
+    return gmres_run(A, L, R, restart, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,

@@ -65,6 +65,12 @@ int bicgstab_ell_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_p
                      Workspace<double>& W, double* x, const double* b, mtl4g_iteration* it, double* h_hist,
                      int32_t hist_cap, hipStream_t s);
 
+// itl::gmres(A, x, b, L, R, iter, restart) (restart > 0) or itl::gmres_full
+// (restart == 0) on the device, at most 256 Krylov vectors per cycle.
+int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R, int32_t restart,
+              Workspace<double>& W, double* x, const double* b, mtl4g_iteration* it, double* h_hist,
+              int32_t hist_cap, hipStream_t s);
+
 // Mixed precision with reliable updates: one FP32 recurrence (values in
 // value_format), solution accumulated in FP64, the true residual b - A*x
 // recomputed in FP64 whenever the FP32 residual fell by delta since the last

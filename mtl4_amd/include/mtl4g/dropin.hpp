@@ -1071,9 +1071,12 @@ class cyclic_iteration : public basic_iteration<Real> {
         for (std::size_t k = 0; k < hist.size(); k++) {
             if (method == MTL4G_TFQMR        ? k > 1        // finished(rt) before the loop, then tau
                 : method == MTL4G_BICGSTAB_2 ? k % 3 == 1   // loop head, then r_hat[0] and r_hat[1]
+                : method >= 1000             ? false        // gmres: set per cycle below
                 : method >= 100              ? k % (method - 100 + 1) == 1   // BiCGSTAB(l), l = method - 100
                                              : (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)))   // cg, cgs: one per iteration
                 i++;
+            if (method >= 1000 && k % 2 == 1)   // end of a cycle of `restart` Arnoldi steps
+                i = std::min(i + (method > 1000 ? method - 1000 : this->max_iter), this->max_iter);
             this->i = i;
             this->resid_ = hist[k];
             print_resid();
@@ -1255,7 +1258,7 @@ namespace detail {
 template <typename PC>
 inline mtl4g_pc pc_handle(const PC& P) { return P.handle_(); }
 
-// method: MTL4G_* or 100 + l for BiCGSTAB(l)
+// method: MTL4G_* or 100 + l for BiCGSTAB(l), 1000 + restart for GMRES
 template <typename Matrix, typename VX, typename VB, typename Iter>
 inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter& iter,
                mtl4g_pc R = nullptr) {
@@ -1264,7 +1267,7 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
     mtl4g_iteration s = iteration_access::state(iter);
     const int i0 = s.i;
     const bool want = iteration_access::wants_history(iter);
-    const int per = method >= 100 ? method - 100 + 1 : 3;   // finished() calls per iteration, at most
+    const int per = method >= 1000 ? 2 : method >= 100 ? method - 100 + 1 : 3;   // finished() calls per iteration, at most
     const int cap = want ? per * std::max(s.max_iter - s.i, 0) + 4 : 0;
     std::vector<double> hist(cap);
     mtl4g_matrix h = A.device_();
@@ -1275,6 +1278,7 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
              : method == MTL4G_CGS   ? mtl4g_cgs(h, P, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_TFQMR ? mtl4g_tfqmr(h, P, R, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_BICGSTAB_2 ? mtl4g_bicgstab_2(h, P, dx, db, &s, hp, cap, nullptr)
+             : method >= 1000 ? mtl4g_gmres(h, P, R, dx, db, method - 1000, &s, hp, cap, nullptr)
              : method >= 100 ? mtl4g_bicgstab_ell(h, P, R, dx, db, method - 100, &s, hp, cap, nullptr)
                                      : mtl4g_bicgstab(h, P, dx, db, &s, hp, cap, nullptr);
     std::vector<double> used;
@@ -1358,8 +1362,31 @@ int bicgstab_ell(const LinearOperator& A, Vector& x, const Vector& b, const Left
     return iter;
 }
 
+// itl::gmres_full (krylov/gmres.hpp:37-119) and itl::gmres with restart
+// (:122-137); at most 256 Krylov vectors per cycle here
+template <typename Matrix, typename Vector, typename LeftPreconditioner, typename RightPreconditioner,
+          typename Iteration>
+int gmres_full(const Matrix& A, Vector& x, const Vector& b, LeftPreconditioner& L, RightPreconditioner& R,
+               Iteration& iter) {
+    if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
+    detail::run(1000, A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
+                pc::is_identity(R) ? nullptr : detail::pc_handle(R));
+    return iter;
+}
+template <typename Matrix, typename Vector, typename LeftPreconditioner, typename RightPreconditioner,
+          typename Iteration>
+int gmres(const Matrix& A, Vector& x, const Vector& b, LeftPreconditioner& L, RightPreconditioner& R,
+          Iteration& iter, std::size_t restart) {
+    if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
+    MTL_THROW_IF(restart < 1, mtl::logic_error("gmres: restart must be positive"));
+    detail::run(1000 + int(restart), A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
+                pc::is_identity(R) ? nullptr : detail::pc_handle(R));
+    return iter;
+}
+
 // itl::base_solver, cg_solver, bicgstab_solver, cgs_solver, tfqmr_solver,
-// bicgstab_2_solver, bicgstab_ell_solver (bicgstab_ell.hpp:137-167) (krylov/base_solver.hpp:24-84, cg.hpp:122-156,
+// bicgstab_2_solver, bicgstab_ell_solver (bicgstab_ell.hpp:137-167),
+// gmres_solver (gmres.hpp:139-176) (krylov/base_solver.hpp:24-84, cg.hpp:122-156,
 // bicgstab.hpp:80-112, cgs.hpp:64-95, tfqmr.hpp:98-127, bicgstab_2.hpp:132-165)
 template <typename Solver, typename LinearOperator>
 struct base_solver {
@@ -1498,6 +1525,29 @@ class bicgstab_ell_solver
 
   private:
     std::size_t l;
+    Preconditioner L;
+    RightPreconditioner R;
+};
+
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class gmres_solver
+    : public base_solver<gmres_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<gmres_solver, LinearOperator> base;
+
+  public:
+    explicit gmres_solver(const LinearOperator& A, std::size_t restart = 8) : base(A), restart(restart), L(A), R(A) {}
+    gmres_solver(const LinearOperator& A, std::size_t restart, const Preconditioner& L)
+        : base(A), restart(restart), L(L), R(A) {}
+    gmres_solver(const LinearOperator& A, std::size_t restart, const Preconditioner& L, const RightPreconditioner& R)
+        : base(A), restart(restart), L(L), R(R) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const {
+        return gmres(this->A, x, b, L, R, iter, restart);
+    }
+
+  private:
+    std::size_t restart;
     Preconditioner L;
     RightPreconditioner R;
 };

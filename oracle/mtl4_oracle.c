@@ -955,3 +955,179 @@ out:
     free(r);
     return st != ORC_OK ? st : it->error;
 }
+
+/* ------------------------------------------------------------------ */
+/* itl::gmres_full / itl::gmres, itl/krylov/gmres.hpp:37-137.           */
+
+/* mtl::mat::givens(H, a, b).trafo(i), operation/givens.hpp: the rotation
+ * G = [c s; -s c] of set_rotation(a, b), then rows i, i+1 of H (all
+ * columns) <- trans(G) * rows and columns i, i+1 (all rows) <- cols * G.
+ * Each 2-term product sums from zero (0 + p0) + p1 as the naive dense
+ * product does.  H is (rows x ld), row-major.                           */
+static void gm_givens_trafo(double* H, int32_t rows, int32_t ld, int32_t i, double a, double b)
+{
+    double c, s, t;
+    if (b == 0.0) { c = 1.0; s = 0.0; }
+    else if (fabs(b) > fabs(a)) { t = -a / b; s = 1.0 / sqrt(1.0 + t * t); c = s * t; }
+    else { t = -b / a; c = 1.0 / sqrt(1.0 + t * t); s = c * t; }
+    const double ms = -s;
+    for (int32_t j = 0; j < ld; j++) {
+        double b0 = H[(size_t)i * ld + j], b1 = H[(size_t)(i + 1) * ld + j];
+        H[(size_t)i * ld + j] = (0.0 + c * b0) + ms * b1;
+        H[(size_t)(i + 1) * ld + j] = (0.0 + s * b0) + c * b1;
+    }
+    for (int32_t r = 0; r < rows; r++) {
+        double m0 = H[(size_t)r * ld + i], m1 = H[(size_t)r * ld + i + 1];
+        H[(size_t)r * ld + i] = (0.0 + m0 * c) + m1 * ms;
+        H[(size_t)r * ld + i + 1] = (0.0 + m0 * s) + m1 * c;
+    }
+}
+
+/* lu_solve(H[range][range], g[range]), operation/lu.hpp:62-89,117-139, on
+ * the m x m leading block of H, IN PLACE: the sub-matrix is a view and
+ * the copy LU(A) of a view shares its memory (contiguous_memory_block.hpp:
+ * 295-302).  Partial pivoting (first row of the largest magnitude, rows
+ * swapped within the block), singular when a pivot other than the last
+ * has |p| <= 0.  permute(P, b) scatters: b'[P[i]] = b[i] (permute.hpp:
+ * 25-35); then the unit lower and the upper sweep (upper divides by the
+ * diagonal).  Returns 0, or 1 when singular (H keeps the partial work). */
+static int gm_lu_solve(double* H, int32_t ld, int32_t m, const double* g, double* y, int32_t* P, double* z)
+{
+#define HH(r, c) H[(size_t)(r) * ld + (c)]
+    for (int32_t i = 0; i < m; i++) P[i] = i;
+    for (int32_t i = 0; i + 1 < m; i++) {
+        int32_t rmax = i;
+        double best = fabs(HH(i, i));
+        for (int32_t r = i + 1; r < m; r++)
+            if (fabs(HH(r, i)) > best) { best = fabs(HH(r, i)); rmax = r; }
+        if (rmax != i) {
+            for (int32_t c = 0; c < m; c++) { double t = HH(i, c); HH(i, c) = HH(rmax, c); HH(rmax, c) = t; }
+            int32_t t = P[i]; P[i] = P[rmax]; P[rmax] = t;
+        }
+        if (fabs(HH(i, i)) <= 0.0) return 1;
+        for (int32_t r = i + 1; r < m; r++) HH(r, i) /= HH(i, i);
+        for (int32_t r = i + 1; r < m; r++)
+            for (int32_t c = i + 1; c < m; c++) HH(r, c) -= HH(r, i) * HH(i, c);
+    }
+    for (int32_t i = 0; i < m; i++) y[P[i]] = g[i];       /* y holds the permuted rhs */
+    for (int32_t r = 0; r < m; r++) {
+        double rr = y[r];
+        for (int32_t c = 0; c < r; c++) rr -= HH(r, c) * z[c];
+        z[r] = rr;
+    }
+    for (int32_t r = m; r-- > 0;) {
+        double rr = z[r];
+        for (int32_t c = r + 1; c < m; c++) rr -= HH(r, c) * y[c];
+        y[r] = rr / HH(r, r);
+    }
+    return 0;
+#undef HH
+}
+
+/* One gmres_full call (gmres.hpp:37-119) on the iteration state `in`. */
+static int gm_full(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
+                   orc_iter* in, double* hist, int32_t cap, int32_t* nh)
+{
+    int32_t n = A->n;
+    int64_t rem = (int64_t)in->max_iter - in->i;
+    int32_t kmax = (rem < 0 || rem > n) ? n : (int32_t)rem;   /* min(size(x), Size(max - i)) */
+    size_t sn = (size_t)n, K1 = (size_t)kmax + 1;
+    double* wk = (double*)calloc(sn * (K1 + 4), sizeof(double));
+    double* sc = (double*)calloc(K1 * (size_t)(kmax > 0 ? kmax : 1) + 5 * K1, sizeof(double));
+    int32_t* P = (int32_t*)malloc(K1 * sizeof(int32_t));
+    if (!wk || !sc || !P) { free(wk); free(sc); free(P); return ORC_OOM; }
+    double *r0 = wk, *r = wk + sn, *tmp = wk + 2 * sn, *va0 = wk + 3 * sn, *V = wk + 4 * sn;
+    double *H = sc, *g = sc + K1 * (size_t)(kmax > 0 ? kmax : 1), *cc = g + K1, *ss = cc + K1, *y = ss + K1,
+           *z = y + K1;
+    int st = ORC_OK, k;
+#define GM_TRY(e) do { st = (e); if (st != ORC_OK) goto out; } while (0)
+#define VV(j) (V + (size_t)(j) * sn)
+#define HK(i, j) H[(size_t)(i) * kmax + (j)]
+    residual(A, x, b, r0, tmp);                           /* r0(b - A*x), r(solve(L, r0)) */
+    GM_TRY(tf_solve(L, r0, r, n));
+    double rho = g[0] = orc_two_norm(n, r);
+    if (finished(in, rho, hist, cap, nh)) goto out;
+    for (int32_t i = 0; i < n; i++) VV(0)[i] = r[i] / rho;
+    for (k = 0; k < kmax; ++k, ++in->i) {
+        GM_TRY(tf_solve(R, VV(k), tmp, n));
+        orc_spmv(A, tmp, va0, ORC_ASSIGN);
+        GM_TRY(tf_solve(L, va0, VV(k + 1), n));
+        double* w = VV(k + 1);
+        for (int32_t j = 0; j <= k; j++) {                /* modified Gram-Schmidt */
+            HK(j, k) = orc_dot(n, VV(j), w, 4);
+            for (int32_t i = 0; i < n; i++) w[i] -= HK(j, k) * VV(j)[i];
+        }
+        HK(k + 1, k) = orc_two_norm(n, w);
+        for (int32_t j = 0; j <= k; j++) {                /* reorthogonalize */
+            double hr = orc_dot(n, w, VV(j), 4);
+            HK(j, k) += hr;
+            for (int32_t i = 0; i < n; i++) w[i] -= hr * VV(j)[i];
+        }
+        HK(k + 1, k) = orc_two_norm(n, w);
+        if (HK(k + 1, k) != 0.0) {
+            double f = 1. / HK(k + 1, k);
+            for (int32_t i = 0; i < n; i++) w[i] *= f;
+        }
+        for (int32_t i = 0; i < k; i++) gm_givens_trafo(H, kmax + 1, kmax, i, HK(i, k - 1), HK(i + 1, k - 1));
+        double nu = sqrt(HK(k, k) * HK(k, k) + HK(k + 1, k) * HK(k + 1, k));
+        if (nu != 0.0) {
+            cc[k] = HK(k, k) / nu;
+            ss[k] = -HK(k + 1, k) / nu;
+            HK(k, k) = cc[k] * HK(k, k) - ss[k] * HK(k + 1, k);
+            HK(k + 1, k) = 0;
+            double w1 = cc[k] * g[k] - ss[k] * g[k + 1], w2 = ss[k] * g[k] + cc[k] * g[k + 1];
+            g[k] = w1;
+            g[k + 1] = w2;
+        }
+    }
+    while (k > 0 && fabs(g[k - 1]) <= in->atol) k--;
+    int32_t m = k;
+    for (; m > 0; --m)
+        if (gm_lu_solve(H, kmax, m, g, y, P, z) == 0) break;
+    if (m == 0) { in->error = 2; goto out; }             /* "GMRES did not find any direction" */
+    for (int32_t i = 0; i < n; i++) {                     /* Vector(V.vector(range) * y[range]) */
+        double t = 0.0;
+        for (int32_t j = 0; j < m; j++) t += VV(j)[i] * y[j];
+        r0[i] = t;
+    }
+    GM_TRY(tf_solve(R, r0, tmp, n));
+    for (int32_t i = 0; i < n; i++) x[i] += tmp[i];
+    residual(A, x, b, r, va0);                            /* r = b - A*x; terminate(r) */
+    finished(in, orc_two_norm(n, r), hist, cap, nh);
+out:
+#undef GM_TRY
+#undef VV
+#undef HK
+    free(wk);
+    free(sc);
+    free(P);
+    return st != ORC_OK ? st : in->error;
+}
+
+/* restart > 0: itl::gmres (gmres.hpp:122-137): cycles of gmres_full on a
+ * copy of iter limited to restart more iterations, each followed by
+ * iter.update_progress (basic_iteration.hpp:145-155).  restart == 0:
+ * gmres_full alone.  hist: the residuals gmres_full hands to finished(). */
+int orc_gmres(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
+              int32_t restart, orc_iter* it, double* hist, int32_t hist_cap)
+{
+    int32_t nh = 0;
+    if (A->n != A->ncols) return ORC_NOT_SQUARE;
+    if (A->n == 0 || restart < 0) return ORC_INVALID;     /* "empty rhs vector" */
+    it->error = 0;
+    if (restart == 0) return gm_full(A, L, R, x, b, it, hist, hist_cap, &nh);
+    for (;;) {
+        orc_iter inner = *it;
+        inner.max_iter = it->i + restart < it->max_iter ? it->i + restart : it->max_iter;
+        int st = gm_full(A, L, R, x, b, &inner, hist, hist_cap, &nh);
+        if (st < 0) return st;
+        it->i = inner.i;
+        it->resid = inner.resid;
+        if (inner.error > 1) { it->error = inner.error; break; }
+        double r = it->resid;
+        int conv = it->norm_r0 == 0 ? (r <= it->atol) : (r <= it->rtol * it->norm_r0 || r <= it->atol);
+        if (conv) break;
+        if (it->i >= it->max_iter) { it->error = 1; break; }
+    }
+    return it->error;
+}

@@ -92,6 +92,9 @@ int orc_bicgstab_2(const orc_csr* A, const orc_pc* L, double* x, const double* b
                    orc_iter* it, double* hist, int32_t hist_cap);
 int orc_bicgstab_ell(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
                      int32_t l, orc_iter* it, double* hist, int32_t hist_cap);
+/* itl::gmres (restart > 0) / itl::gmres_full (restart == 0), gmres.hpp:37-137 */
+int orc_gmres(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
+              int32_t restart, orc_iter* it, double* hist, int32_t hist_cap);
 
 #ifdef __cplusplus
 }

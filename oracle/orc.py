@@ -91,6 +91,8 @@ def lib():
         L.orc_bicgstab_2.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
         L.orc_bicgstab_ell.restype = C.c_int
         L.orc_bicgstab_ell.argtypes = [P, P, P, P, P, I32, C.POINTER(Iter), P, I32]
+        L.orc_gmres.restype = C.c_int
+        L.orc_gmres.argtypes = [P, P, P, P, P, I32, C.POINTER(Iter), P, I32]
         _lib = L
     return _lib
 
@@ -286,5 +288,18 @@ def bicgstab_ell(A, b, L=None, R=None, l=2, x0=None, it=None, hist_cap=0, **kw):
     hist = np.full(max(hist_cap, 1), np.nan)
     st = lib().orc_bicgstab_ell(A.ref(), L.h if L is not None else None, R.h if R is not None else None,
                                 _p(x), _p(b), int(l), C.byref(it), _p(hist), hist_cap)
+    nh = int(np.argmax(np.isnan(hist))) if np.isnan(hist).any() else len(hist)
+    return x, st, it, hist[:min(nh, hist_cap)]
+
+
+def gmres(A, b, L=None, R=None, restart=30, x0=None, it=None, hist_cap=0, **kw):
+    """itl::gmres(A, x, b, L, R, iter, restart) (restart > 0) or itl::gmres_full
+    (restart == 0), itl/krylov/gmres.hpp:37-137."""
+    it = iteration(b, **kw) if it is None else it
+    b = _f64(b)
+    x = np.zeros(A.n) if x0 is None else _f64(x0).copy()
+    hist = np.full(max(hist_cap, 1), np.nan)
+    st = lib().orc_gmres(A.ref(), L.h if L is not None else None, R.h if R is not None else None,
+                         _p(x), _p(b), int(restart), C.byref(it), _p(hist), hist_cap)
     nh = int(np.argmax(np.isnan(hist))) if np.isnan(hist).any() else len(hist)
     return x, st, it, hist[:min(nh, hist_cap)]

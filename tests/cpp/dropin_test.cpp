@@ -233,6 +233,37 @@ int main() {
         CHECK(S.solve(y, b, it2) == 0);
     });
 
+    // libs/numeric/itl/test/gmres_test.cpp: laplacian_setup(2, 2), b = 1;
+    // the cases it asserts: gmres(1) from x = 5 in one iteration, gmres(8),
+    // (16), (32) from x = 2.5: |A x - b| <= 5e-5
+    run("gmres_test", [] {
+        const int N = 2;
+        matrix_type A(N * N, N * N);
+        laplacian_setup(A, N, N);
+        vector_type b(N * N, 1.0), x(N * N), r(N * N);
+        itl::pc::identity<matrix_type> Ident(A);
+        x = 5.0;
+        itl::basic_iteration<double> iter_1(b, 1, 1.e-8);
+        gmres(A, x, b, Ident, Ident, iter_1, 1);
+        r = b;
+        r -= A * x;
+        CHECK(two_norm(r) <= 0.00005);
+        const int restarts[3] = {8, 16, 32}, iters[3] = {8, 32, 32};
+        for (int t = 0; t < 3; t++) {
+            x = 2.5;
+            itl::basic_iteration<double> it(b, iters[t], 1.e-8);
+            gmres(A, x, b, Ident, Ident, it, restarts[t]);
+            r = b;
+            r -= A * x;
+            CHECK(two_norm(r) <= 0.00005);
+        }
+        itl::pc::ilu_0<matrix_type> P(A);
+        itl::gmres_solver<matrix_type, itl::pc::ilu_0<matrix_type>> S(A, 4, P);
+        vector_type y(N * N, 2.0);
+        itl::basic_iteration<double> it2(b, 100, 1.e-8);
+        CHECK(S.solve(y, b, it2) == 0);
+    });
+
     // libs/numeric/itl/test/tfqmr_test.cpp: 10x10 laplacian, b = 1, the
     // same preconditioner left and right
     run("tfqmr_test", [] {

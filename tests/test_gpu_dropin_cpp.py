@@ -22,4 +22,4 @@ def test_dropin_cpp_program():
     passed = [l for l in p.stdout.splitlines() if l.startswith("PASS ")]
     failed = [l for l in p.stdout.splitlines() if l.startswith("FAIL ")]
     assert p.returncode == 0 and not failed, p.stdout[-4000:] + p.stderr[-2000:]
-    assert len(passed) >= 15
+    assert len(passed) >= 16

@@ -237,6 +237,40 @@ def test_bicgstab_ell_bitwise(m, mtl4_order, fam, l, lr):
     assert np.array_equal(x.numpy(), xo)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000"])
+@pytest.mark.parametrize("restart", [1, 2, 5, 16, 0])
+@pytest.mark.parametrize("lr", [("identity", "identity"), ("ilu_0", "identity"), ("identity", "ilu_0"),
+                                ("diagonal", "ilu_0")])
+def test_gmres_bitwise(m, mtl4_order, fam, restart, lr):
+    """itl::gmres (restart > 0) and itl::gmres_full (restart 0: one cycle of
+    min(n, max_iter) steps) in MTL4's reduction order, nonzero start:
+    status, iteration count, the residuals at each cycle's start and end,
+    and the solution bit for bit with the oracle -- converged or not."""
+    rp, ci, va = family(fam, seed=13)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    rng = np.random.default_rng(113)
+    b = orc.spmv(O, rng.uniform(-1, 1, n))
+    x0 = rng.uniform(-1, 1, n)
+    OL, OR = [None if k == "identity" else orc.Preconditioner(k, O) for k in lr]
+    max_iter = 24 if restart == 0 else 60
+    xo, st, ito, ho = orc.gmres(O, b, OL, OR, restart, x0=x0, max_iter=max_iter, rtol=1e-10, hist_cap=400)
+    L, R = [getattr(m.pc, k)(A) for k in lr]
+    x = dv(m, x0)
+    it = m.cyclic_iteration(dv(m, b), max_iter, 1e-10, 0.0, 10**9)
+    it.set_quite(True)
+    it.suppress_resume(True)
+    if restart == 0:
+        err = m.gmres_full(A, x, dv(m, b), L, R, it)
+    else:
+        err = m.gmres(A, x, dv(m, b), L, R, it, restart)
+    assert err == st
+    assert it.iterations() == ito.i
+    assert it.resid() == ito.resid
+    assert np.array_equal(np.asarray(it.history), ho)
+    assert np.array_equal(x.numpy(), xo)
+
+
 def test_bicgstab_2_omega_breakdown(m, mtl4_order):
     """omega == 0 is iter.fail(3) (bicgstab_2.hpp:107): status 3 after one
     iteration and x returned without the shift back x += x0, bit for bit

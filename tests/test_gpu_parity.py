@@ -830,6 +830,63 @@ def test_bicgstab_ell_reference_case(m):
         m.bicgstab_ell(E, m.dense_vector(0), m.dense_vector(0), None, None, m.basic_iteration(0.0, 3, 1e-8), 2)
 
 
+@pytest.mark.parametrize("fam", ["convdiff_40", "lap2d_64", "random_3000", "diff3d_16"])
+@pytest.mark.parametrize("restart", [1, 4, 30])
+@pytest.mark.parametrize("lr", [("ilu_0", "identity"), ("identity", "ilu_0")])
+def test_gmres_families(m, fam, restart, lr):
+    """itl::gmres in tree order against the oracle on a fixed budget of
+    iterations: same status and iteration count, the true residual of the
+    device's x within 1e-6 relative of the oracle's (or both below 1e-9 |b|)."""
+    rp, ci, va = family(fam, seed=4)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    xt = np.random.default_rng(3).uniform(-1, 1, n)
+    b = orc.spmv(O, xt)
+    OL, OR = [None if k == "identity" else orc.Preconditioner(k, O) for k in lr]
+    xo, st, ito, _ = orc.gmres(O, b, OL, OR, restart, max_iter=120, rtol=1e-10)
+    L, R = [getattr(m.pc, k)(A) for k in lr]
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 120, 1e-10)
+    err = m.gmres(A, x, dv(m, b), L, R, it, restart)
+    res = np.linalg.norm(b - orc.spmv(O, x.numpy()))
+    ores = np.linalg.norm(b - orc.spmv(O, xo))
+    if st == 1:   # not converged within the budget: the same budget is used up
+        assert err == 1 and it.iterations() == ito.i
+        assert abs(res - ores) <= 1e-6 * ores + 1e-9 * np.linalg.norm(b)
+    else:
+        assert err == st == 0
+        assert abs(it.iterations() - ito.i) <= max(restart, 2 + 0.15 * ito.i)
+        assert res <= max(1e-9 * np.linalg.norm(b), 10 * ores)
+
+
+def test_gmres_reference_cases(m):
+    """gmres_test.cpp's asserted cases on laplacian_setup(2, 2): gmres(1)
+    from x = 5 in one iteration and gmres(8/16/32) from x = 2.5 reach |A x -
+    b| <= 5e-5; GMRES(1) on a rotation finds no direction (iter.fail(2));
+    more than 256 Krylov vectors per cycle, a restart below 1 and an empty
+    system are errors."""
+    A = m.laplacian_setup(2, 2)
+    b = m.dense_vector(np.ones(4))
+    for restart, x0, iters in [(1, 5.0, 1), (8, 2.5, 8), (16, 2.5, 32), (32, 2.5, 32)]:
+        x = m.dense_vector(np.full(4, x0))
+        m.gmres(A, x, b, None, None, m.basic_iteration(b, iters, 1e-8), restart)
+        assert np.linalg.norm((A * x).numpy() - 1.0) <= 5e-5
+    Rot = m.compressed2D(2, 2, np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.array([1.0, -1.0]))
+    x = m.dense_vector(2)
+    it = m.basic_iteration(m.dense_vector(np.array([1.0, 0.0])), 10, 1e-12)
+    assert m.gmres(Rot, x, m.dense_vector(np.array([1.0, 0.0])), None, None, it, 1) == 2
+    assert it.iterations() == 1 and np.array_equal(x.numpy(), np.zeros(2))
+    big = m.laplacian_setup(30, 30)
+    with pytest.raises(m.MtlError):
+        m.gmres(big, m.dense_vector(900), m.dense_vector(np.ones(900)), None, None,
+                m.basic_iteration(30.0, 1000, 1e-8), 300)
+    with pytest.raises(m.MtlError):
+        m.gmres(A, m.dense_vector(4), b, None, None, m.basic_iteration(b, 10, 1e-8), 0)
+    with pytest.raises(m.MtlError):
+        E = m.compressed2D(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        m.gmres(E, m.dense_vector(0), m.dense_vector(0), None, None, m.basic_iteration(0.0, 3, 1e-8), 4)
+
+
 def test_bicgstab_2_solver_handle(m):
     """A solver handle for BiCGSTAB(2) keeps its workspace: two solves give
     the same bits as each other and as the one-shot call."""

@@ -796,3 +796,102 @@ def test_bicgstab_ell_breakdown_and_bad_l():
     E = orc.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
     _, st, _, _ = orc.bicgstab_ell(E, np.zeros(0), None, None, 2, max_iter=3, rtol=1e-8)
     assert st == orc.INVALID
+
+
+# ---------------------------------------------------------------- GMRES
+def test_gmres_reference_cases():
+    """gmres_test.cpp, the cases it asserts: laplacian_setup(2, 2), b = 1;
+    gmres(1) from x = 5 within one iteration, gmres(8), (16), (32) from x =
+    2.5: |A x - b| <= 5e-5.  And gmres_preconditioned_test.cpp's GMRES(4)
+    with left ILU(0) from x = (2, 3, 4, 8), which converges."""
+    A = orc.laplacian_setup(2, 2)
+    b = np.ones(4)
+    x, st, it, _ = orc.gmres(A, b, None, None, 1, x0=np.full(4, 5.0), max_iter=1, rtol=1e-8)
+    assert np.linalg.norm(orc.spmv(A, x) - b) <= 5e-5
+    for restart, iters in [(8, 8), (16, 32), (32, 32)]:
+        x, st, it, _ = orc.gmres(A, b, None, None, restart, x0=np.full(4, 2.5), max_iter=iters, rtol=1e-8)
+        assert np.linalg.norm(orc.spmv(A, x) - b) <= 5e-5
+    P = orc.Preconditioner("ilu_0", A)
+    x, st, it, _ = orc.gmres(A, b, P, None, 4, x0=np.array([2.0, 3.0, 4.0, 8.0]), max_iter=100, rtol=1e-8)
+    assert st == 0 and np.linalg.norm(orc.spmv(A, x) - b) <= 1e-5
+
+
+@pytest.mark.parametrize("side", ["identity", "left", "right"])
+def test_gmres_1_cycles_are_minimal_residual(side):
+    """GMRES(1): one Krylov vector per cycle, no earlier rotation to apply,
+    so every cycle is textbook minimal residual: x_new minimizes the
+    (preconditioned) residual over x + R^-1 span{r} -- computed here by
+    dense least squares.  The residuals handed to finished() are, per cycle,
+    the preconditioned |r| at its start and |b - A x_new| at its end."""
+    n = 40
+    D, rng = _nonsym(n, 41)
+    A = from_dense(D)
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    Dinv = np.diag(1.0 / np.diag(D))
+    P = orc.Preconditioner("diagonal", A)
+    L, R = {"identity": (None, None), "left": (P, None), "right": (None, P)}[side]
+    Lm = Dinv if side == "left" else np.eye(n)
+    Rm = Dinv if side == "right" else np.eye(n)
+    cycles = 4
+    x, st, it, h = orc.gmres(A, b, L, R, 1, x0=x0, max_iter=cycles, rtol=1e-14, hist_cap=20)
+    assert st == 1 and it.i == cycles and len(h) == 2 * cycles
+    xs = x0.copy()
+    for cyc in range(cycles):
+        r = Lm @ (b - D @ xs)
+        Mr = Lm @ D @ Rm @ r
+        xs = xs + Rm @ r * (Mr @ r) / (Mr @ Mr)
+        assert h[2 * cyc] == pytest.approx(np.linalg.norm(r), rel=1e-9)
+        assert h[2 * cyc + 1] == pytest.approx(np.linalg.norm(b - D @ xs), rel=1e-9)
+    assert np.allclose(x, xs, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("restart", [2, 5])
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_gmres_update_in_krylov_space(restart, side):
+    """Whatever coefficients a cycle's small solve yields, its update of x is
+    R^-1 V y with V the Arnoldi basis of K_m(M, r): x_new - x_old must lie in
+    R^-1 K_m (projection residual at rounding level), and the end-of-cycle
+    residual is |b - A x_new|.  (With m >= 2 the coefficients follow
+    gmres.hpp:84-94, which rotates with column k-1 of H; no textbook identity
+    fixes them.)"""
+    n = 40
+    D, rng = _nonsym(n, 43)
+    A = from_dense(D)
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    Dinv = np.diag(1.0 / np.diag(D))
+    P = orc.Preconditioner("diagonal", A)
+    L, R = (P, None) if side == "left" else (None, P)
+    Lm = Dinv if side == "left" else np.eye(n)
+    Rm = Dinv if side == "right" else np.eye(n)
+    x, st, it, h = orc.gmres(A, b, L, R, restart, x0=x0, max_iter=restart, rtol=1e-14, hist_cap=10)
+    assert it.i == restart and len(h) == 2
+    r = Lm @ (b - D @ x0)
+    assert h[0] == pytest.approx(np.linalg.norm(r), rel=1e-12)
+    assert h[1] == pytest.approx(np.linalg.norm(b - D @ x), rel=1e-9)
+    M = Lm @ D @ Rm
+    K = np.column_stack([np.linalg.matrix_power(M, p) @ r for p in range(restart)])
+    Q = np.linalg.qr(K)[0]
+    u = np.linalg.solve(Rm, x - x0)
+    assert np.linalg.norm(u - Q @ (Q.T @ u)) <= 1e-9 * np.linalg.norm(u)
+    assert np.linalg.norm(u) > 1e-3
+
+
+def test_gmres_no_direction():
+    """GMRES(1) on a rotation: A r is orthogonal to r, the one-dimensional
+    least-squares correction is zero, so g[0] = 0 after the rotation and
+    the cycle finds no direction: iter.fail(2), x unchanged (gmres.hpp:
+    99, 113-114)."""
+    D = np.array([[0.0, 1.0], [-1.0, 0.0]])
+    b = np.array([1.0, 0.0])
+    x, st, it, h = orc.gmres(from_dense(D), b, None, None, 1, max_iter=10, rtol=1e-12, hist_cap=10)
+    assert st == 2 and it.i == 1 and np.array_equal(x, np.zeros(2)) and len(h) == 1
+
+
+def test_gmres_bad_arguments():
+    E = orc.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    _, st, _, _ = orc.gmres(E, np.zeros(0), None, None, 4, max_iter=3, rtol=1e-8)
+    assert st == orc.INVALID
+    _, st, _, _ = orc.gmres(orc.laplacian_setup(2, 2), np.ones(4), None, None, -1, max_iter=3, rtol=1e-8)
+    assert st == orc.INVALID
