@@ -61,6 +61,20 @@ def dia_bytes(rowptr, col, vb=8):
     return ndiag * padded_rows * vb + ndiag * 4 + n * vb + n * vb
 
 
+def dia_sym_bytes(rowptr, col, vb=8):
+    """Algorithmic bytes of y = A*x on the symmetric diagonal layout: one
+    value per row and stored diagonal (offsets >= 0, absent entries
+    included), the offsets and their map, x once, y once; the entries below
+    the diagonal are the stored ones of other rows, read again."""
+    lens = np.diff(np.asarray(rowptr, dtype=np.int64))
+    rows = np.repeat(np.arange(len(lens), dtype=np.int64), lens)
+    offs = np.unique(np.asarray(col, dtype=np.int64) - rows)
+    nst = int((offs >= 0).sum())
+    n = len(lens)
+    padded_rows = (n + 63) // 64 * 64
+    return nst * padded_rows * vb + 2 * len(offs) * 4 + n * vb + n * vb
+
+
 def timed_spmv(m, torch, A, x, y, reps):
     """Average duration of the SpMV kernel, HIP events on the launch stream."""
     for _ in range(3):
@@ -116,6 +130,8 @@ def traffic_key(m, kern, grid=0):
         return "sliced/%d/%d" % (rows, group)
     if kind == m._lib.SPMV_DIA:
         return "dia/%d/%d" % (rows, group)
+    if kind == m._lib.SPMV_DIA_SYM:
+        return "dia_sym/%d/%d" % (rows, group)
     return None
 
 
@@ -402,11 +418,17 @@ def main():
         layout, by_layout = "sliced_ell", sliced_bytes(rp)
     elif kern[0] == m._lib.SPMV_DIA:
         layout, by_layout = "diagonal", dia_bytes(rp, ci)
+    elif kern[0] == m._lib.SPMV_DIA_SYM:
+        layout, by_layout = "symmetric_diagonal", dia_sym_bytes(rp, ci)
     spmv_gbs = by / t_spmv / 1e9
     grid = A.grid()
     traffic = load_traffic(traffic_key(m, kern, grid))
     if kern[0] == m._lib.SPMV_DIA:
         kname = "k_dia (diagonal layout, one lane per row, %d diagonals per chunk, %d workgroups/CU)" % (kern[2], kern[1])
+    elif kern[0] == m._lib.SPMV_DIA_SYM:
+        kname = ("k_dia_sym (symmetric diagonal layout: offsets >= 0 stored, entries below the diagonal read "
+                 "from their mirror images; one lane per row, %d diagonals per chunk, %d workgroups/CU)"
+                 % (kern[2], kern[1]))
     elif kern[0] == m._lib.SPMV_SLICED:
         kname = "k_sell (sliced ELL, one lane per row, %d entries per chunk, %d workgroups/CU)" % (kern[2], kern[1])
     elif kern[0] == m._lib.SPMV_VECTOR:
@@ -427,6 +449,10 @@ def main():
                 "layout_GBs": round(by_layout / t_spmv / 1e9, 1),
                 "layout_frac": round(by_layout / t_spmv / 1e9 / HBM_PEAK_GBS, 4),
                 "avg_launch_us": round(t_spmv * 1e6, 2)}
+    if by_layout < by:
+        roofline["note"] = ("achieved counts the CSR bytes of y = A*x; the %s layout moves fewer "
+                            "(layout_bytes_per_launch), so frac can exceed 1; layout_frac is that "
+                            "layout's own fraction of peak" % layout)
     if args.only_spmv:   # the roofline kernel alone (profiling runs)
         if args.copy_calibration:   # a copy of known size for the PMC byte counters
             src = torch.rand(64 << 20, dtype=torch.float64, device="cuda")

@@ -165,6 +165,10 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
  * row and diagonal, no column indices -- for matrices whose entries lie on
  * at most 32 diagonals filling at least 4/5 of their slots (stencils);
  * `group` and `rows_per_block` as for MTL4G_SPMV_SLICED; refused otherwise.
+ * MTL4G_SPMV_DIA_SYM: the same for a square matrix equal to its transpose
+ * bit for bit; only the diagonals with offset >= 0 are stored, the entries
+ * below the diagonal are read from their mirror images (the value stream
+ * is about half); refused for other matrices.
  * All sum every row in column order.  0 = library default. */
 #define MTL4G_SPMV_TILE   0
 #define MTL4G_SPMV_VECTOR 1
@@ -173,6 +177,7 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
 #define MTL4G_SPMV_STREAM2 4
 #define MTL4G_SPMV_SLICED 5
 #define MTL4G_SPMV_DIA 6
+#define MTL4G_SPMV_DIA_SYM 7
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);

@@ -38,7 +38,7 @@ class IRStats(C.Structure):
     _fields_ = [("outer", C.c_int32), ("inner_total", C.c_int32), ("relres", C.c_double)]
 
 
-SPMV_TILE, SPMV_VECTOR, SPMV_STREAM, SPMV_SCALAR, SPMV_STREAM2, SPMV_SLICED, SPMV_DIA = 0, 1, 2, 3, 4, 5, 6
+SPMV_TILE, SPMV_VECTOR, SPMV_STREAM, SPMV_SCALAR, SPMV_STREAM2, SPMV_SLICED, SPMV_DIA, SPMV_DIA_SYM = 0, 1, 2, 3, 4, 5, 6, 7
 ORDER_TREE, ORDER_MTL4 = 0, 1
 
 

@@ -298,6 +298,14 @@ struct mtl4g_matrix_s {
     mtl4g::DevBuf<float> dia_val32;
     mtl4g::DevBuf<uint16_t> dia_valbf;
     int ndiag = 0;
+    // symmetric diagonal layout (MTL4G_SPMV_DIA_SYM): the diagonals with offset >= 0 of a
+    // matrix equal to its transpose; dsym_off: all offsets ascending, dsym_map: stored index of |offset|
+    std::vector<int32_t> h_dsym_off;
+    mtl4g::DevBuf<int32_t> dsym_off, dsym_map;
+    mtl4g::DevBuf<double> dsym_val;
+    mtl4g::DevBuf<float> dsym_val32;
+    mtl4g::DevBuf<uint16_t> dsym_valbf;
+    int ndsym = 0, nsym = 0;
     int sl_unroll = 8;                      // sliced / diagonal: entries per lane per chunk (4 or 8)
     int sl_unroll_lowp[2] = {8, 8};         // the same for FP32 / BF16 values
     int32_t max_row = 0;

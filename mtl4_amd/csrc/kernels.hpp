@@ -693,6 +693,118 @@ __global__ __launch_bounds__(BS) void k_dia(const int32_t* __restrict__ doff,
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
 }
 
+// Symmetric diagonal layout (MTL4G_SPMV_DIA_SYM): for a matrix equal to its
+// transpose bit for bit, only the diagonals with offset >= 0 are stored
+// (slice-major as above, nst of them); the entry of row r on diagonal -o is
+// the stored entry of row r - o on diagonal +o.  The walk and the summation
+// order are k_dia's over the full offset list (ascending columns), so the
+// result is the same bit for bit; the stored diagonals are read with
+// caching loads, and the mirrored reads of rows a few slices back mostly
+// hit the cache, so the value stream from memory is nst of the ndiag
+// diagonals.  dmap[d]: the stored diagonal of full diagonal d.
+template <typename MT>
+__device__ __forceinline__ MT dia_absent_value();
+template <>
+__device__ __forceinline__ double dia_absent_value<double>() { return __longlong_as_double((long long)kDiaAbsent64); }
+template <>
+__device__ __forceinline__ float dia_absent_value<float>() { return __uint_as_float(kDiaAbsent32); }
+template <>
+__device__ __forceinline__ uint16_t dia_absent_value<uint16_t>() { return kDiaAbsent16; }
+
+template <int BS, int UNR, typename AT, typename MT, typename XT, typename Epi>
+__global__ __launch_bounds__(BS) void k_dia_sym(const int32_t* __restrict__ doff, const int32_t* __restrict__ dmap,
+                                                const MT* __restrict__ val, const XT* __restrict__ x, int nrows,
+                                                int ncols, int ndiag, int nst, Epi epi, AT* partials,
+                                                unsigned* ticket) {
+    if (epi.skip()) return;
+    epi.prepare();
+    constexpr int NP = Epi::NP > 0 ? Epi::NP : 1;
+    constexpr int WPB = BS / kWave;
+    __shared__ AT sh[WPB];
+    __shared__ int offs[kDiaMax], maps[kDiaMax];
+    if ((int)threadIdx.x < ndiag) {
+        offs[threadIdx.x] = doff[threadIdx.x];
+        maps[threadIdx.x] = dmap[threadIdx.x];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int xcd = blockIdx.x % kXcd;
+    const int wstep = (int)(gridDim.x / kXcd) * WPB;
+    const int nsl = (nrows + kWave - 1) / kWave;
+    const int per = (nsl + kXcd - 1) / kXcd;
+    const int hi = min(nsl, (xcd + 1) * per);
+    AT part[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[k] = AT(0);
+
+    struct Chunk {
+        MT v[UNR];
+        int off[UNR];   // wavefront-uniform
+        typename Epi::Pre pre;
+    };
+    int s = xcd * per + (int)(blockIdx.x / kXcd) * WPB + wv;
+    int k0 = 0;
+    auto fetch = [&](Chunk& ch) {
+        const int r = s * kWave + lane;
+        ch.pre = epi.preload(min(r, nrows - 1));
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int d = min(k0 + u, ndiag - 1);
+            const int o = __builtin_amdgcn_readfirstlane(offs[d]);
+            const int mi = __builtin_amdgcn_readfirstlane(maps[d]);
+            ch.off[u] = o;
+            if (o >= 0) {
+                ch.v[u] = val[((int64_t)s * nst + mi) * kWave + lane];
+            } else {   // row r - |o| on the mirrored diagonal; none above row 0
+                const int r2 = r + o;
+                const int rc = max(r2, 0);
+                const MT v = val[((int64_t)(rc / kWave) * nst + mi) * kWave + rc % kWave];
+                ch.v[u] = r2 >= 0 ? v : dia_absent_value<MT>();
+            }
+        }
+    };
+    AT acc = AT(0);
+    auto step = [&](const Chunk& cur, Chunk& nxt) -> bool {
+        const int kc = k0, row = s * kWave + lane;
+        AT g[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) g[u] = static_cast<AT>(x[min(max(row + cur.off[u], 0), ncols - 1)]);
+        const bool last = __builtin_amdgcn_readfirstlane(kc + UNR >= ndiag) != 0;
+        const int sn = __builtin_amdgcn_readfirstlane(last ? s + wstep : s);
+        auto sum = [&]() {
+#pragma unroll
+            for (int u = 0; u < UNR; u++)
+                if (kc + u < ndiag && !dia_absent(cur.v[u])) acc += dia_value<AT>(cur.v[u]) * g[u];
+            if (last) {
+                if (row < nrows) epi(row, acc, cur.pre, part);
+                acc = AT(0);
+            }
+        };
+        if (__builtin_amdgcn_readfirstlane(sn >= hi)) {
+            sum();
+            return false;
+        }
+        s = sn;
+        k0 = last ? 0 : kc + UNR;
+        fetch(nxt);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) asm volatile("" : "+v"(g[u])::"memory");
+        sum();
+        return true;
+    };
+    if (s < hi) {
+        Chunk A, B;
+        fetch(A);
+        for (;;) {
+            if (!step(A, B)) break;
+            if (!step(B, A)) break;
+        }
+    }
+    if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
+}
+
 // Element epilogues may split operator()(i, part) into In load(i) (all
 // reads of element i) and store(i, const In&, part) (the arithmetic, writes
 // and partial sums); k_vec then batches the loads of several elements.
@@ -970,6 +1082,35 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
         if (unr == 4) M4_LAUNCHD(4);
         else M4_LAUNCHD(8);
 #undef M4_LAUNCHD
+        M4_HIP(hipGetLastError());
+        return MTL4G_OK;
+    }
+    if (A->kernel == MTL4G_SPMV_DIA_SYM) {
+        // the symmetric diagonal copy of the value array the caller passed
+        const MT* dv = nullptr;
+        if constexpr (std::is_same<MT, double>::value) dv = val == A->val.p ? A->dsym_val.p : nullptr;
+        else if constexpr (std::is_same<MT, float>::value) dv = val == A->val32.p ? A->dsym_val32.p : nullptr;
+        else dv = val == A->valbf.p ? A->dsym_valbf.p : nullptr;
+        if (!dv || !A->dsym_off.p) {
+            set_error("symmetric diagonal SpMV: no symmetric diagonal copy of these values");
+            return MTL4G_ERR_INVALID;
+        }
+        int pcu = A->per_cu, unr = A->sl_unroll;
+        const int li = sizeof(MT) == 4 ? 0 : 1;
+        if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) {
+            pcu = A->per_cu_lowp[li];
+            unr = A->sl_unroll_lowp[li];
+        }
+        const int units = ((A->n + kWave - 1) / kWave + 3) / 4;
+#define M4_LAUNCHY(U)                                                                                      \
+    do {                                                                                                   \
+        const int g = persistent_grid<k_dia_sym<256, U, AT, MT, XT, Epi>>(A->cus, pcu, units);             \
+        hipLaunchKernelGGL((k_dia_sym<256, U, AT, MT, XT, Epi>), dim3(g), dim3(256), 0, s, A->dsym_off.p,  \
+                           A->dsym_map.p, dv, x, A->n, A->m, A->ndsym, A->nsym, epi, partials, ticket);    \
+    } while (0)
+        if (unr == 4) M4_LAUNCHY(4);
+        else M4_LAUNCHY(8);
+#undef M4_LAUNCHY
         M4_HIP(hipGetLastError());
         return MTL4G_OK;
     }

@@ -308,10 +308,111 @@ static int build_dia(mtl4g_matrix_s* A) {
     return MTL4G_OK;
 }
 
+// ---- symmetric diagonal copy (MTL4G_SPMV_DIA_SYM)
+// Diagonals with offset >= 0 of a square matrix whose every entry (r, c)
+// has a mirror (c, r) with the same bits; slice-major as dia_fill, rows'
+// entries with column >= row only.
+template <typename T, typename F>
+static void dsym_fill(const mtl4g_matrix_s* A, const std::vector<int32_t>& up, T absent, F conv,
+                      std::vector<T>& out) {
+    const int nd = (int)up.size();
+    const size_t nsl = ((size_t)A->n + kWave - 1) / kWave;
+    out.assign(nsl * nd * kWave, absent);
+    for (int32_t r = 0; r < A->n; r++) {
+        int d = 0;
+        const size_t base = (size_t)(r / kWave) * nd * kWave + r % kWave;
+        for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++) {
+            const int32_t o = A->h_col[j] - r;
+            if (o < 0) continue;
+            while (up[d] != o) d++;
+            out[base + (size_t)d * kWave] = conv(A->h_val[j]);
+        }
+    }
+}
+
+static std::vector<int32_t> dsym_stored(const mtl4g_matrix_s* A) {
+    std::vector<int32_t> up;
+    for (int32_t o : A->h_dsym_off)
+        if (o >= 0) up.push_back(o);
+    return up;
+}
+
+static void free_dsym(mtl4g_matrix_s* A) {
+    A->h_dsym_off.clear();
+    A->dsym_off.free();
+    A->dsym_map.free();
+    A->dsym_val.free();
+    A->dsym_val32.free();
+    A->dsym_valbf.free();
+    A->ndsym = A->nsym = 0;
+}
+
+static int dsym_values(mtl4g_matrix_s* A, int fmt) {
+    if (A->h_dsym_off.empty()) return MTL4G_OK;
+    const std::vector<int32_t> up = dsym_stored(A);
+    if (fmt == MTL4G_F64 && !A->dsym_val.p) {
+        std::vector<double> v;
+        dsym_fill(A, up, absent64(), [](double d) { return d; }, v);
+        M4_TRY(upload(A->dsym_val, v));
+    }
+    if (fmt == MTL4G_F32 && !A->dsym_val32.p) {
+        std::vector<float> v;
+        dsym_fill(A, up, absent32(), [](double d) { return (float)d; }, v);
+        M4_TRY(upload(A->dsym_val32, v));
+    }
+    if (fmt == MTL4G_BF16 && !A->dsym_valbf.p) {
+        std::vector<uint16_t> v;
+        dsym_fill(A, up, kDiaAbsent16, [](double d) { return f32_to_bf16((float)d); }, v);
+        M4_TRY(upload(A->dsym_valbf, v));
+    }
+    return MTL4G_OK;
+}
+
+// The diagonal layout's conditions, plus a square matrix equal to its
+// transpose bit for bit (every entry has its mirror with the same value).
+static int build_dsym(mtl4g_matrix_s* A) {
+    if (!A->h_dsym_off.empty()) return MTL4G_OK;
+    std::vector<int32_t> off = dia_offsets(A);
+    if (A->n == 0 || A->n != A->m || off.empty() || (double)off.size() * A->n * 0.8 > (double)A->nnz) {
+        set_error("symmetric diagonal layout: not a square matrix on at most %d well-filled diagonals", kDiaMax);
+        return MTL4G_ERR_UNSUPPORTED;
+    }
+    for (int32_t r = 0; r < A->n; r++)
+        for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++) {
+            const int32_t c = A->h_col[j];
+            const double v = A->h_val[j];
+            if (bits_equal(v, absent64())) {
+                set_error("symmetric diagonal layout: a value has the reserved absent-entry bits");
+                return MTL4G_ERR_UNSUPPORTED;
+            }
+            const auto b = A->h_col.begin() + A->h_rowptr[c], e = A->h_col.begin() + A->h_rowptr[c + 1];
+            const auto it = std::lower_bound(b, e, r);
+            if (it == e || *it != r || !bits_equal(A->h_val[it - A->h_col.begin()], v)) {
+                set_error("symmetric diagonal layout: the matrix is not equal to its transpose");
+                return MTL4G_ERR_UNSUPPORTED;
+            }
+        }
+    A->h_dsym_off = off;
+    const std::vector<int32_t> up = dsym_stored(A);
+    std::vector<int32_t> map(off.size());
+    for (size_t d = 0; d < off.size(); d++)
+        map[d] = (int32_t)(std::lower_bound(up.begin(), up.end(), std::abs(off[d])) - up.begin());
+    A->ndsym = (int)off.size();
+    A->nsym = (int)up.size();
+    int st;
+    if ((st = upload(A->dsym_off, off)) || (st = upload(A->dsym_map, map)) || (st = dsym_values(A, MTL4G_F64)) ||
+        (A->val32.p && (st = dsym_values(A, MTL4G_F32))) || (A->valbf.p && (st = dsym_values(A, MTL4G_BF16)))) {
+        free_dsym(A);
+        return st;
+    }
+    return MTL4G_OK;
+}
+
 int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     if (fmt == MTL4G_F32 || fmt == MTL4G_BF16) {
         M4_TRY(sliced_lowp_values(A, fmt));
         M4_TRY(dia_values(A, fmt));
+        M4_TRY(dsym_values(A, fmt));
     }
     if (fmt == MTL4G_F32 && !A->val32.p) {
         std::vector<float> v(A->h_val.begin(), A->h_val.end());
@@ -392,7 +493,7 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     A->per_cu_lowp[fmt - 1] = 0;
     A->lowp_wide[fmt - 1] = false;
     if ((A->kernel != MTL4G_SPMV_STREAM && A->kernel != MTL4G_SPMV_STREAM2 && A->kernel != MTL4G_SPMV_SLICED &&
-         A->kernel != MTL4G_SPMV_DIA) ||
+         A->kernel != MTL4G_SPMV_DIA && A->kernel != MTL4G_SPMV_DIA_SYM) ||
         A->n == 0 || A->m == 0)
         return MTL4G_OK;
     if (A->kernel == MTL4G_SPMV_STREAM && !A->desc_wide.p) {   // wide partition for the candidates below
@@ -419,7 +520,7 @@ static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     };
     std::vector<Cand> cands;
     int win_unr = A->sl_unroll;
-    if (A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA) {
+    if (A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA || A->kernel == MTL4G_SPMV_DIA_SYM) {
         for (int u : {4, 8})
             for (int f : {2, 4, 6, 8}) cands.push_back({f, false, u});
     } else {
@@ -479,6 +580,9 @@ static int assign_grid(const mtl4g_matrix_s* A) {
     case MTL4G_SPMV_DIA:
         return unr == 4 ? persistent_grid<k_dia<256, 4, AT, MT, AT, Epi>>(A->cus, pcu, units)
                         : persistent_grid<k_dia<256, 8, AT, MT, AT, Epi>>(A->cus, pcu, units);
+    case MTL4G_SPMV_DIA_SYM:
+        return unr == 4 ? persistent_grid<k_dia_sym<256, 4, AT, MT, AT, Epi>>(A->cus, pcu, units)
+                        : persistent_grid<k_dia_sym<256, 8, AT, MT, AT, Epi>>(A->cus, pcu, units);
     case MTL4G_SPMV_STREAM:
         if (sizeof(MT) < 8 && A->lowp_wide[li] && A->nb_wide > 0)
             return persistent_grid<k_csrp<256, 4096, AT, MT, AT, Epi, 2>>(A->cus, pcu, A->nb_wide);
@@ -801,6 +905,18 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where) {
         sliced_fill(A, A->h_val.data(), 0.0, [](double d) { return d; }, v);
         M4_TRY(upload(A->sl_val, v));
     }
+    if (!A->h_dsym_off.empty()) {
+        const bool sym = A->kernel == MTL4G_SPMV_DIA_SYM;
+        free_dsym(A);
+        if (sym && build_dsym(A)) {   // no longer symmetric: the plain diagonal layout, else the stream kernel
+            A->kernel = MTL4G_SPMV_DIA;
+            if (!A->h_dia_off.empty()) free_dia(A);
+            if (build_dia(A)) {
+                A->kernel = MTL4G_SPMV_STREAM;
+                if (!A->desc.p) M4_TRY(build_row_blocks(A, 0, 0));
+            }
+        }
+    }
     if (!A->h_dia_off.empty()) {
         const bool dia = A->kernel == MTL4G_SPMV_DIA;
         free_dia(A);
@@ -842,14 +958,14 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
         A->lowp_wide[0] = A->lowp_wide[1] = false;
         return MTL4G_OK;
     }
-    if (kernel == MTL4G_SPMV_SLICED || kernel == MTL4G_SPMV_DIA) {
+    if (kernel == MTL4G_SPMV_SLICED || kernel == MTL4G_SPMV_DIA || kernel == MTL4G_SPMV_DIA_SYM) {
         if (group == 0) group = A->per_cu;
         if (rows_per_block == 0) rows_per_block = 8;
         if (group < 1 || group > 8 || (rows_per_block != 4 && rows_per_block != 8)) {
             set_error("sliced / diagonal kernel: 1 to 8 workgroups per compute unit, 4 or 8 entries per lane and chunk");
             return MTL4G_ERR_INVALID;
         }
-        M4_TRY(kernel == MTL4G_SPMV_SLICED ? build_sliced(A) : build_dia(A));
+        M4_TRY(kernel == MTL4G_SPMV_SLICED ? build_sliced(A) : kernel == MTL4G_SPMV_DIA ? build_dia(A) : build_dsym(A));
         A->kernel = kernel;
         A->per_cu = group;
         A->sl_unroll = rows_per_block;
@@ -897,13 +1013,16 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
     if (group)
         *group = A->kernel == MTL4G_SPMV_VECTOR   ? A->gs
                  : A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2 ||
-                           A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA
+                           A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA ||
+                           A->kernel == MTL4G_SPMV_DIA_SYM
                      ? A->per_cu
                  : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
                                                   : 0;
     if (rows_per_block)
         *rows_per_block = A->kernel == MTL4G_SPMV_VECTOR   ? A->vrows
-                          : A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA ? A->sl_unroll
+                          : A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA ||
+                                    A->kernel == MTL4G_SPMV_DIA_SYM
+                              ? A->sl_unroll
                                                            : A->rows_per_block;
     return MTL4G_OK;
 }
@@ -934,6 +1053,11 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     if (had_dia || build_dia(A) == MTL4G_OK)
         for (int u : {4, 8})
             for (int f : {2, 4, 6, 8}) cands.push_back({MTL4G_SPMV_DIA, f, u, 0, 0});
+    // its symmetric half where the matrix equals its transpose
+    const bool had_dsym = !A->h_dsym_off.empty();
+    if (had_dsym || build_dsym(A) == MTL4G_OK)
+        for (int u : {4, 8})
+            for (int f : {2, 4, 6, 8}) cands.push_back({MTL4G_SPMV_DIA_SYM, f, u, 0, 0});
     // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
     const double avg = A->n ? (double)A->nnz / A->n : 0.0;
     for (int g : {4, 8, 16, 32, 64}) {
@@ -976,6 +1100,7 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     (void)hipFreeAsync(y, s);
     if (win.kernel != MTL4G_SPMV_SLICED && !had_sliced) free_sliced(A);
     if (win.kernel != MTL4G_SPMV_DIA && !had_dia) free_dia(A);
+    if (win.kernel != MTL4G_SPMV_DIA_SYM && !had_dsym) free_dsym(A);
     if (win.kernel == MTL4G_SPMV_STREAM || win.kernel == MTL4G_SPMV_STREAM2)
         M4_TRY(mtl4g_csr_tune(A, win.rows, win.nnz));
     M4_TRY(win.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, win.a, win.b)

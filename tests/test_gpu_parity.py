@@ -254,6 +254,104 @@ def test_spmv_dia_bitwise(m, fam, shape):
         assert np.linalg.norm(b - orc.spmv(O, x32.numpy())) <= 1.01e-10 * np.linalg.norm(b)
 
 
+def _symmetric_bits(rp, ci, va):
+    import scipy.sparse as sp
+    n = len(rp) - 1
+    A = sp.csr_matrix((np.asarray(va, np.float64), np.asarray(ci), np.asarray(rp)), shape=(n, n))
+    T = A.T.tocsr()
+    A.sort_indices()
+    T.sort_indices()
+    return (np.array_equal(A.indptr, T.indptr) and np.array_equal(A.indices, T.indices)
+            and np.array_equal(A.data.view(np.uint64), T.data.view(np.uint64)))
+
+
+@pytest.mark.parametrize("shape", [(1, 8), (4, 8), (2, 4), (6, 4)])
+@pytest.mark.parametrize("fam", FAMILIES)
+def test_spmv_dia_sym_bitwise(m, fam, shape):
+    """Symmetric diagonal layout (only offsets >= 0 stored, the lower entries
+    read from their mirror images): same bits as MTL4's row sums for =, +=,
+    -=; FP32 and BF16 values give the stream kernel's bits; refused for a
+    matrix that is not its own transpose bit for bit; CG and mixed
+    precision on it within their bars."""
+    per_cu, unroll = shape
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    n = A.num_rows()
+    x = np.random.default_rng(8).uniform(-1, 1, A.num_cols())
+    A.set_kernel(m._lib.SPMV_STREAM, 4, 0)
+    ref32 = {f: _spmv32(m, A, f, x) for f in (m.F32, m.BF16)}
+    if not (_dia_eligible(rp, ci) and A.num_rows() == A.num_cols() and _symmetric_bits(rp, ci, va)):
+        with pytest.raises(m.MtlError):
+            A.set_kernel(m._lib.SPMV_DIA_SYM, per_cu, unroll)
+        return
+    A.set_kernel(m._lib.SPMV_DIA_SYM, per_cu, unroll)
+    assert A.get_kernel() == (m._lib.SPMV_DIA_SYM, per_cu, unroll)
+    y0 = np.random.default_rng(9).uniform(-1, 1, n)
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+    for mode in (m.PLUS, m.MINUS):
+        w = dv(m, y0)
+        A.mult(dv(m, x), w, mode)
+        yr = y0.copy()
+        orc.spmv(O, x, yr, mode)
+        assert np.array_equal(w.numpy(), yr)
+    for f in (m.F32, m.BF16):
+        assert np.array_equal(_spmv32(m, A, f, x), ref32[f])
+    if fam in ("lap2d_64", "diff3d_16"):
+        b = orc.spmv(O, x)
+        xo, st, ito, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=3000, rtol=1e-10)
+        xs = m.dense_vector(n)
+        it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+        assert m.cg(A, xs, dv(m, b), m.pc.diagonal(A), it) == 0
+        assert abs(it.iterations() - ito.i) <= 2
+        assert np.linalg.norm(xs.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+        x32 = m.dense_vector(n)
+        it = m.basic_iteration(dv(m, b), 20000, 1e-10)
+        err, _ = m.ir_solve(A, x32, dv(m, b), it, m.pc.diagonal(A, m.F32), value_format=m.F32)
+        assert err == 0
+        assert np.linalg.norm(b - orc.spmv(O, x32.numpy())) <= 1.01e-10 * np.linalg.norm(b)
+
+
+def test_spmv_dia_sym_holes_signed_zeros_and_set_values(m):
+    """A symmetric band with symmetric holes, -0.0 entries and rows of the
+    first slices whose mirrored entries lie above row 0: bits as MTL4's row
+    sums.  New symmetric values (D A D) keep the layout; values that break
+    the symmetry move the matrix to the plain diagonal layout, same bits."""
+    n = 700
+    rng = np.random.default_rng(12)
+    dense_rows = {r: {} for r in range(n)}
+    for r in range(n):
+        dense_rows[r][r] = rng.uniform(2, 3)
+        for o in (1, 65, 130):
+            c = r + o
+            if c < n and (r * 7 + o) % 13 != 4:
+                v = -0.0 if (r + o) % 17 == 0 else rng.uniform(-1, 1)
+                dense_rows[r][c] = v
+                dense_rows[c][r] = v
+    rp, ci, va = [0], [], []
+    for r in range(n):
+        for c in sorted(dense_rows[r]):
+            ci.append(c)
+            va.append(dense_rows[r][c])
+        rp.append(len(ci))
+    rp, ci, va = np.asarray(rp, np.int32), np.asarray(ci, np.int32), np.asarray(va)
+    O, A = pair(rp, ci, va)
+    A.set_kernel(m._lib.SPMV_DIA_SYM, 2, 4)
+    x = rng.uniform(-1, 1, n)
+    got, ref = (A * dv(m, x)).numpy(), orc.spmv(O, x)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    d = rng.uniform(0.5, 1.5, n)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    va2 = va * (d[rows] * d[ci])   # d_r d_c = d_c d_r: the mirror keeps its bits
+    A.set_values(va2)
+    assert A.get_kernel()[0] == m._lib.SPMV_DIA_SYM
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(pair(rp, ci, va2)[0], x))
+    va3 = va2.copy()
+    va3[3] += 1.0   # (0, c) no longer equals (c, 0)
+    A.set_values(va3)
+    assert A.get_kernel()[0] == m._lib.SPMV_DIA
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(pair(rp, ci, va3)[0], x))
+
+
 @pytest.mark.parametrize("fam", FAMILIES)
 def test_default_kernel_by_structure(m, fam):
     """Without the autotuner the kernel is chosen from the structure alone:
