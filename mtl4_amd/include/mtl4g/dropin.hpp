@@ -1006,7 +1006,8 @@ class basic_iteration {
         return resid_ <= rtol_ * my_norm_r0 || resid_ <= atol_;
     }
     // residuals of the device solve, one per finished() call
-    virtual void replay_(const std::vector<double>&, int /*method*/, int /*i0*/) {}
+    // arg: l of BiCGSTAB(l), restart of GMRES (0: gmres_full), unused otherwise
+    virtual void replay_(const std::vector<double>&, int /*method*/, int /*arg*/, int /*i0*/) {}
     virtual bool wants_history_() const { return false; }
 
     int error, i;
@@ -1066,17 +1067,17 @@ class cyclic_iteration : public basic_iteration<Real> {
 
   protected:
     // the device ran the loop; print what finished() would have printed
-    void replay_(const std::vector<double>& hist, int method, int i0) override {
+    void replay_(const std::vector<double>& hist, int method, int arg, int i0) override {
         int i = i0;
         for (std::size_t k = 0; k < hist.size(); k++) {
             if (method == MTL4G_TFQMR        ? k > 1        // finished(rt) before the loop, then tau
                 : method == MTL4G_BICGSTAB_2 ? k % 3 == 1   // loop head, then r_hat[0] and r_hat[1]
-                : method >= 1000             ? false        // gmres: set per cycle below
-                : method >= 100              ? k % (method - 100 + 1) == 1   // BiCGSTAB(l), l = method - 100
+                : method == MTL4G_GMRES      ? false        // set per cycle below
+                : method == MTL4G_BICGSTAB_ELL ? k % (arg + 1) == 1   // loop head, then r_hat[0..l-1]
                                              : (k > 0 && (method != MTL4G_BICGSTAB || k % 2 == 1)))   // cg, cgs: one per iteration
                 i++;
-            if (method >= 1000 && k % 2 == 1)   // end of a cycle of `restart` Arnoldi steps
-                i = std::min(i + (method > 1000 ? method - 1000 : this->max_iter), this->max_iter);
+            if (method == MTL4G_GMRES && k % 2 == 1)   // end of a cycle of `restart` Arnoldi steps
+                i = std::min(i + (arg > 0 ? arg : this->max_iter), this->max_iter);
             this->i = i;
             this->resid_ = hist[k];
             print_resid();
@@ -1133,8 +1134,8 @@ struct iteration_access {
     static bool wants_history(const basic_iteration<Real>& it) { return it.wants_history_(); }
     template <class Real>
     static void update(basic_iteration<Real>& it, const mtl4g_iteration& s, const std::vector<double>& hist,
-                       int method, int i0) {
-        if (!hist.empty()) it.replay_(hist, method, i0);
+                       int method, int arg, int i0) {
+        if (!hist.empty()) it.replay_(hist, method, arg, i0);
         it.i = s.i;
         it.resid_ = s.resid;
         it.error = s.error;
@@ -1258,16 +1259,17 @@ namespace detail {
 template <typename PC>
 inline mtl4g_pc pc_handle(const PC& P) { return P.handle_(); }
 
-// method: MTL4G_* or 100 + l for BiCGSTAB(l), 1000 + restart for GMRES
+// method: MTL4G_*; arg: l of BiCGSTAB(l), restart of GMRES (0: gmres_full)
 template <typename Matrix, typename VX, typename VB, typename Iter>
 inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter& iter,
-               mtl4g_pc R = nullptr) {
+               mtl4g_pc R = nullptr, int arg = 0) {
     MTL_THROW_IF(A.num_rows() != A.num_cols(), mtl::matrix_not_square());
     MTL_THROW_IF(x.size() != A.num_cols() || b.size() != A.num_rows(), mtl::incompatible_size());
     mtl4g_iteration s = iteration_access::state(iter);
     const int i0 = s.i;
     const bool want = iteration_access::wants_history(iter);
-    const int per = method >= 1000 ? 2 : method >= 100 ? method - 100 + 1 : 3;   // finished() calls per iteration, at most
+    // finished() calls per iteration, at most
+    const int per = method == MTL4G_GMRES ? 2 : method == MTL4G_BICGSTAB_ELL ? arg + 1 : 3;
     const int cap = want ? per * std::max(s.max_iter - s.i, 0) + 4 : 0;
     std::vector<double> hist(cap);
     mtl4g_matrix h = A.device_();
@@ -1278,13 +1280,13 @@ inline int run(int method, const Matrix& A, VX& x, const VB& b, mtl4g_pc P, Iter
              : method == MTL4G_CGS   ? mtl4g_cgs(h, P, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_TFQMR ? mtl4g_tfqmr(h, P, R, dx, db, &s, hp, cap, nullptr)
              : method == MTL4G_BICGSTAB_2 ? mtl4g_bicgstab_2(h, P, dx, db, &s, hp, cap, nullptr)
-             : method >= 1000 ? mtl4g_gmres(h, P, R, dx, db, method - 1000, &s, hp, cap, nullptr)
-             : method >= 100 ? mtl4g_bicgstab_ell(h, P, R, dx, db, method - 100, &s, hp, cap, nullptr)
+             : method == MTL4G_GMRES ? mtl4g_gmres(h, P, R, dx, db, arg, &s, hp, cap, nullptr)
+             : method == MTL4G_BICGSTAB_ELL ? mtl4g_bicgstab_ell(h, P, R, dx, db, arg, &s, hp, cap, nullptr)
                                      : mtl4g_bicgstab(h, P, dx, db, &s, hp, cap, nullptr);
     std::vector<double> used;
     for (double v : hist)
         if (v == v) used.push_back(v);
-    iteration_access::update(iter, s, used, method, i0);
+    iteration_access::update(iter, s, used, method, arg, i0);
     if (st == MTL4G_ERR_BREAKDOWN) throw itl::unexpected_orthogonality();
     mtl::detail::check(st);
     return s.error;
@@ -1357,8 +1359,8 @@ int bicgstab_ell(const LinearOperator& A, Vector& x, const Vector& b, const Left
                  const RightPreconditioner& R, Iteration& iter, std::size_t l) {
     if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
     MTL_THROW_IF(l < 1 || l > 8, mtl::logic_error("bicgstab_ell: l must be in 1..8"));
-    detail::run(100 + int(l), A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
-                pc::is_identity(R) ? nullptr : detail::pc_handle(R));
+    detail::run(MTL4G_BICGSTAB_ELL, A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
+                pc::is_identity(R) ? nullptr : detail::pc_handle(R), int(l));
     return iter;
 }
 
@@ -1369,8 +1371,8 @@ template <typename Matrix, typename Vector, typename LeftPreconditioner, typenam
 int gmres_full(const Matrix& A, Vector& x, const Vector& b, LeftPreconditioner& L, RightPreconditioner& R,
                Iteration& iter) {
     if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
-    detail::run(1000, A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
-                pc::is_identity(R) ? nullptr : detail::pc_handle(R));
+    detail::run(MTL4G_GMRES, A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
+                pc::is_identity(R) ? nullptr : detail::pc_handle(R), 0);
     return iter;
 }
 template <typename Matrix, typename Vector, typename LeftPreconditioner, typename RightPreconditioner,
@@ -1379,8 +1381,8 @@ int gmres(const Matrix& A, Vector& x, const Vector& b, LeftPreconditioner& L, Ri
           Iteration& iter, std::size_t restart) {
     if (b.size() == 0) throw mtl::logic_error("empty rhs vector");
     MTL_THROW_IF(restart < 1, mtl::logic_error("gmres: restart must be positive"));
-    detail::run(1000 + int(restart), A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
-                pc::is_identity(R) ? nullptr : detail::pc_handle(R));
+    detail::run(MTL4G_GMRES, A, x, b, pc::is_identity(L) ? nullptr : detail::pc_handle(L), iter,
+                pc::is_identity(R) ? nullptr : detail::pc_handle(R), int(restart));
     return iter;
 }
 
