@@ -257,6 +257,51 @@ def c3_section(m, torch, gen):
     return out
 
 
+def solvers_section(m, torch, gen):
+    """Every Krylov method of the library on one workload: diffusion3d 128^3
+    (2,097,152 rows), b = A*x_true, diagonal preconditioner (left for the
+    methods that take two), FP64, rtol 1e-10, default reduction order.
+    Per method: iterations, best of two solves, time per iteration and per
+    product A*v.  GMRES runs MTL4's algorithm, whose cycles of more than one
+    Krylov vector follow no textbook identity: it gets a fixed budget of 300
+    iterations and reports its residual instead of a convergence time."""
+    rp, ci, va = gen.diffusion3d(128, seed=0)
+    n = len(rp) - 1
+    A = m.compressed2D(n, n, rp, ci, va)
+    A.autotune()
+    b = A * m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
+    bnorm = m.two_norm(b)
+    P = m.pc.diagonal(A)
+    runs = [("cg", 1, lambda x, it: m.cg(A, x, b, P, it)),
+            ("bicgstab", 2, lambda x, it: m.bicgstab(A, x, b, P, it)),
+            ("cgs", 2, lambda x, it: m.cgs(A, x, b, P, it)),
+            ("tfqmr", 2, lambda x, it: m.tfqmr(A, x, b, P, None, it)),
+            ("bicgstab_2", 4, lambda x, it: m.bicgstab_2(A, x, b, P, it)),
+            ("bicgstab_ell_4", 8, lambda x, it: m.bicgstab_ell(A, x, b, P, None, it, 4)),
+            ("gmres_30", 1, lambda x, it: m.gmres(A, x, b, P, None, it, 30))]
+    out = {"workload": "diffusion3d 128^3, pc::diagonal, FP64, rtol 1e-10", "products_per_iteration": {}}
+    for name, prods, fn in runs:
+        def one():
+            best, it_n, err, rel = 1e30, 0, 0, 0.0
+            for _ in range(2):
+                x = m.dense_vector(n)
+                it = m.basic_iteration(bnorm, 300 if name.startswith("gmres") else 5000, 1e-10)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                err = fn(x, it)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+                it_n = it.iterations()
+                rel = relres_of(m, A, x, b, bnorm)
+            return {"iterations": it_n, "error_code": err, "relres": rel, "solve_s": round(best, 4),
+                    "ms_per_iteration": round(1000 * best / max(it_n, 1), 4),
+                    "ms_per_product": round(1000 * best / max(it_n * prods, 1), 4)}
+        log("solvers: %s" % name)
+        out[name] = guarded(one)
+        out["products_per_iteration"][name] = prods
+    return out
+
+
 def batch_systems(gen, count, seed=0):
     """Uneven SPD systems: N log-uniform in [256, 65536], families 2D 5-point,
     3D 7-point and random diagonally dominant (about 12 non-zeros per row)."""
@@ -380,7 +425,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="with --only-spmv: also copy 512 MiB on the device (PMC calibration)")
     ap.add_argument("--sections", default="all",
-                    help="extra measurements: all, none, or a list of ic0,c3,batched,serving")
+                    help="extra measurements: all, none, or a list of ic0,c3,solvers,batched,serving")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
@@ -551,13 +596,16 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(rp, ci, va, b.numpy(), iters, args.cpu_iters)
 
-    sections = set(args.sections.split(",")) if args.sections != "all" else {"ic0", "c3", "batched", "serving"}
+    sections = (set(args.sections.split(",")) if args.sections != "all"
+                else {"ic0", "c3", "solvers", "batched", "serving"})
     extra = {}
     if "ic0" in sections:
         extra["cg_ic0_160"] = guarded(lambda: pair_section(m, torch, A, b, bnorm, "ic_0", m.CG, x64))
     del S, P, P32
     if "c3" in sections:
         extra["c3"] = guarded(lambda: c3_section(m, torch, gen))
+    if "solvers" in sections:
+        extra["solvers"] = guarded(lambda: solvers_section(m, torch, gen))
     if "batched" in sections:
         extra["batched"] = guarded(lambda: batched_section(m, torch, gen, args.batch_systems))
     if "serving" in sections:
