@@ -635,59 +635,65 @@ __global__ __launch_bounds__(BS) void k_dia(const int32_t* __restrict__ doff,
 #pragma unroll
     for (int k = 0; k < NP; k++) part[k] = AT(0);
 
+    // Three chunks rotate: two are loading while the third is summed.  A
+    // chunk is UNR diagonals of one slice of kWave rows, with its cursor.
     struct Chunk {
         MT v[UNR];
+        AT g[UNR];      // x at the chunk's columns
         int off[UNR];   // wavefront-uniform
+        int s, k0;      // slice and first diagonal (wavefront-uniform)
         typename Epi::Pre pre;
     };
-    int s = xcd * per + (int)(blockIdx.x / kXcd) * WPB + wv;
-    int k0 = 0;
+    int fs = xcd * per + (int)(blockIdx.x / kXcd) * WPB + wv;   // fetch cursor
+    int fk = 0;
     auto fetch = [&](Chunk& ch) {
-        const int r = s * kWave + lane;
-        ch.pre = epi.preload(min(r, nrows - 1));
+        ch.s = fs;
+        ch.k0 = fk;
+        if (__builtin_amdgcn_readfirstlane(fs < hi)) {
+            const int r = fs * kWave + lane;
+            ch.pre = epi.preload(min(r, nrows - 1));
 #pragma unroll
-        for (int u = 0; u < UNR; u++) {
-            const int d = min(k0 + u, ndiag - 1);
-            ch.off[u] = __builtin_amdgcn_readfirstlane(offs[d]);
-            ch.v[u] = __builtin_nontemporal_load(val + ((int64_t)s * ndiag + d) * kWave + lane);
+            for (int u = 0; u < UNR; u++) {
+                const int d = min(fk + u, ndiag - 1);
+                const int o = __builtin_amdgcn_readfirstlane(offs[d]);
+                ch.off[u] = o;
+                ch.v[u] = __builtin_nontemporal_load(val + ((int64_t)fs * ndiag + d) * kWave + lane);
+                ch.g[u] = static_cast<AT>(x[min(max(r + o, 0), ncols - 1)]);
+            }
+        }
+        if (fk + UNR >= ndiag) {
+            fk = 0;
+            fs += wstep;
+        } else {
+            fk += UNR;
         }
     };
     AT acc = AT(0);
+    // Sums chunk cur (false when it lies past the wave's last slice) after
+    // starting the load of nxt, two chunks ahead.
     auto step = [&](const Chunk& cur, Chunk& nxt) -> bool {
-        const int kc = k0, row = s * kWave + lane;
-        AT g[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; u++) g[u] = static_cast<AT>(x[min(max(row + cur.off[u], 0), ncols - 1)]);
-        const bool last = __builtin_amdgcn_readfirstlane(kc + UNR >= ndiag) != 0;
-        const int sn = __builtin_amdgcn_readfirstlane(last ? s + wstep : s);
-        auto sum = [&]() {
-#pragma unroll
-            for (int u = 0; u < UNR; u++)
-                if (kc + u < ndiag && !dia_absent(cur.v[u])) acc += dia_value<AT>(cur.v[u]) * g[u];
-            if (last) {
-                if (row < nrows) epi(row, acc, cur.pre, part);
-                acc = AT(0);
-            }
-        };
-        if (__builtin_amdgcn_readfirstlane(sn >= hi)) {
-            sum();
-            return false;
-        }
-        s = sn;
-        k0 = last ? 0 : kc + UNR;
+        const int cs = __builtin_amdgcn_readfirstlane(cur.s);
+        if (cs >= hi) return false;
         fetch(nxt);
         __builtin_amdgcn_sched_barrier(0);
+        const int kc = __builtin_amdgcn_readfirstlane(cur.k0), row = cs * kWave + lane;
 #pragma unroll
-        for (int u = 0; u < UNR; u++) asm volatile("" : "+v"(g[u])::"memory");
-        sum();
+        for (int u = 0; u < UNR; u++)
+            if (kc + u < ndiag && !dia_absent(cur.v[u])) acc += dia_value<AT>(cur.v[u]) * cur.g[u];
+        if (kc + UNR >= ndiag) {
+            if (row < nrows) epi(row, acc, cur.pre, part);
+            acc = AT(0);
+        }
         return true;
     };
-    if (s < hi) {
-        Chunk A, B;
+    {
+        Chunk A, B, C;
         fetch(A);
+        fetch(B);
         for (;;) {
-            if (!step(A, B)) break;
+            if (!step(A, C)) break;
             if (!step(B, A)) break;
+            if (!step(C, B)) break;
         }
     }
     if constexpr (Epi::NP > 0) combine_partials<BS>(part, partials, ticket, blockIdx.x, (int)gridDim.x, epi, sh);
