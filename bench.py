@@ -451,6 +451,10 @@ def main():
                     help="with --only-spmv: also copy 512 MiB on the device (PMC calibration)")
     ap.add_argument("--sections", default="all",
                     help="extra measurements: all, none, or a list of ic0,c3,solvers,batched,serving")
+    ap.add_argument("--kernel", default="",
+                    help="SpMV configuration instead of the autotuner's choice, as "
+                         "dia_sym|dia|sliced/<entries per chunk 4|8>/<workgroups per CU> "
+                         "(profiling runs, where the profiler distorts the autotuner's timings)")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
@@ -469,7 +473,12 @@ def main():
     rp, ci, va = gen.diffusion3d(g, seed=0)
     n, nnz = len(rp) - 1, len(ci)
     A = m.compressed2D(n, n, rp, ci, va)
-    A.autotune()                          # CSR kernel and shape for this matrix
+    if args.kernel:
+        kinds = {"dia_sym": m._lib.SPMV_DIA_SYM, "dia": m._lib.SPMV_DIA, "sliced": m._lib.SPMV_SLICED}
+        kind, unr, pcu = args.kernel.split("/")
+        A.set_kernel(kinds[kind], int(pcu), int(unr))
+    else:
+        A.autotune()                      # CSR kernel and shape for this matrix
     xt = m.dense_vector(np.random.default_rng(1).uniform(-1.0, 1.0, n))
     b = A * xt
     bnorm = m.two_norm(b)
