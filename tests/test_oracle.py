@@ -466,6 +466,163 @@ def test_bicgstab_breakdown():
     assert st == orc.BREAKDOWN
 
 
+# ---------------------------------------------------------------- BiCGSTAB pinned
+# Three checks fix orc_bicgstab (bicgstab.hpp:27-78) independently of its
+# recurrences, each run on the oracle and on the NumPy restatement in
+# tests/golden/bicgstab_numpy.py; that restatement's four plausible mistakes
+# (beta = 0, p = r + beta*p, omega = t.s/s.s, no x += alpha*phat at the
+# half-step exit) must each fail at least one of them.  A solver here is
+# f(rowptr, col, val, b, pc, x0, max_iter, rtol) -> (x, status, iterations,
+# residuals handed to finished()).
+def _orc_bicgstab(rp, ci, va, b, pc, x0, max_iter, rtol):
+    A = orc.Matrix(rp, ci, va)
+    P = None if pc == "identity" else orc.Preconditioner(pc, A)
+    x, st, it, h = orc.bicgstab(A, b, P, x0=x0, max_iter=max_iter, rtol=rtol, hist_cap=4 * max_iter + 8)
+    return x, st, it.i, h
+
+
+def _numpy_bicgstab(mutant=None):
+    import bicgstab_numpy as bn
+
+    def solve(rp, ci, va, b, pc, x0, max_iter, rtol):
+        A = bn.Csr(rp, ci, np.asarray(va, dtype=float))
+        inv = None if pc == "identity" else 1.0 / A.diag
+        x, err, it = bn.bicgstab(A, np.asarray(b, dtype=float), inv, x0, max_iter, rtol, mutant=mutant)
+        return x, err, it.i, np.array(it.history)
+    return solve
+
+
+def _csr(D):
+    rp, ci, va = [0], [], []
+    for r in range(D.shape[0]):
+        nz = np.nonzero(D[r])[0]
+        ci.extend(int(c) for c in nz)
+        va.extend(float(D[r, c]) for c in nz)
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(va)
+
+
+def _check_bicgstab_first_residuals(solve, pc):
+    """The six residuals of the first 2 1/2 iterations, from Krylov
+    characterizations rather than the recurrences.  bicgstab preconditions
+    from the right: phat = M^-1 p, v = A phat, so with K = A M^-1, r0 = b
+    and the shadow residual rtilde = r0:
+      |r0|; |s1| with s1 the Petrov-Galerkin BiCG residual of step 1
+      (s1 in r0 + span{K r0}, orthogonal to rtilde); |r1| = min over w of
+      |s1 - w K s1|; |s2| = |(I - w1 K) phi2| with phi2 the BiCG residual of
+      step 2 (orthogonal to rtilde and K^T rtilde); |r2| likewise; |s3| =
+      |(I - w2 K)(I - w1 K) phi3|.  max_iter = 3 stops at the half-step
+      check of iteration 3 with x += alpha*phat, so b - A x = s3."""
+    n = 40
+    D, rng = _nonsym(n, 21)
+    b = rng.standard_normal(n)
+    Minv = np.diag(1.0 / np.diag(D)) if pc == "diagonal" else np.eye(n)
+    K = D @ Minv
+    x, st, its, h = solve(*_csr(D), b, pc, None, 3, 1e-14)
+    assert st == 1 and its == 3 and len(h) == 6, (st, its, len(h))
+    r0 = b.copy()
+    want, Q = [np.linalg.norm(r0)], np.eye(n)
+    for k in (1, 2, 3):
+        s = Q @ _pg_residual(K, r0, r0, k)
+        want.append(np.linalg.norm(s))
+        if k < 3:
+            t = K @ s
+            w = (t @ s) / (t @ t)
+            want.append(np.linalg.norm(s - w * t))
+            Q = (np.eye(n) - w * K) @ Q
+    assert np.allclose(h, want, rtol=1e-9, atol=0), (h, want)
+    assert np.linalg.norm(b - D @ x) == pytest.approx(want[5], rel=1e-8)
+
+
+def _check_bicgstab_half_step_exit(solve, pc):
+    """A circulant tridiagonal (-1, 4, -1) has the constant vector as an
+    eigenvector (eigenvalue 2, and 1/2 after the diagonal preconditioner):
+    for b and x0 constant, s1 = r0 - alpha*A*phat is exactly zero, the
+    half-step check stops the first iteration, and x = x0 + alpha*phat =
+    1/2 exactly (every intermediate is a power of two times an integer)."""
+    n = 16
+    D = 4.0 * np.eye(n)
+    for i in range(n):
+        D[i, (i + 1) % n] = D[i, (i - 1) % n] = -1.0
+    for x0 in (None, np.full(n, 0.25)):
+        x, st, its, h = solve(*_csr(D), np.ones(n), pc, x0, 10, 1e-12)
+        assert st == 0 and its == 1 and len(h) == 2 and h[1] == 0.0, (st, its, h)
+        assert np.array_equal(x, np.full(n, 0.5)), x
+
+
+def _check_bicgstab_finite_termination(solve, k):
+    """With k distinct eigenvalues BiCG's residual polynomial of degree k
+    annihilates r0: s_k = 0 in exact arithmetic, so the half-step check of
+    iteration k stops the solve, after 2k checks, and x then solves A x = b
+    (only if the half-step exit adds alpha*phat)."""
+    vals = np.repeat([1.0, 2.0, 5.0, 3.0, 7.0][:k], 6)
+    n = len(vals)
+    b = np.random.default_rng(k).uniform(0.5, 1.5, n)
+    x, st, its, h = solve(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), vals, b,
+                          "identity", None, 20, 1e-11)
+    assert st == 0 and its == k and len(h) == 2 * k, (st, its, len(h))
+    assert min(h[:-1]) > 1e-6 and h[-1] <= 1e-11 * np.linalg.norm(b)
+    assert np.allclose(x, b / vals, rtol=1e-10, atol=0)
+
+
+_BICGSTAB_CHECKS = [lambda s: _check_bicgstab_first_residuals(s, "identity"),
+                    lambda s: _check_bicgstab_first_residuals(s, "diagonal"),
+                    lambda s: _check_bicgstab_half_step_exit(s, "identity"),
+                    lambda s: _check_bicgstab_half_step_exit(s, "diagonal"),
+                    lambda s: _check_bicgstab_finite_termination(s, 1),
+                    lambda s: _check_bicgstab_finite_termination(s, 3)]
+
+
+@pytest.mark.parametrize("pc", ["identity", "diagonal"])
+def test_bicgstab_checked_residuals(pc):
+    _check_bicgstab_first_residuals(_orc_bicgstab, pc)
+
+
+@pytest.mark.parametrize("pc", ["identity", "diagonal"])
+def test_bicgstab_half_step_exit(pc):
+    _check_bicgstab_half_step_exit(_orc_bicgstab, pc)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5])
+def test_bicgstab_finite_termination(k):
+    _check_bicgstab_finite_termination(_orc_bicgstab, k)
+
+
+def test_bicgstab_checks_pass_numpy_restatement():
+    for check in _BICGSTAB_CHECKS:
+        check(_numpy_bicgstab())
+
+
+@pytest.mark.parametrize("mutant", ["beta0", "no_omega_v", "omega_ss", "no_half_step_x"])
+def test_bicgstab_checks_reject_mutants(mutant):
+    failed = 0
+    for check in _BICGSTAB_CHECKS:
+        try:
+            check(_numpy_bicgstab(mutant))
+        except AssertionError:
+            failed += 1
+    assert failed >= 1, mutant
+
+
+_BICGSTAB_FIXTURE = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "bicgstab_fixture.json")))
+
+
+@pytest.mark.parametrize("case", _BICGSTAB_FIXTURE["cases"], ids=lambda c: c["name"])
+def test_bicgstab_fixture(case):
+    """The oracle against tests/golden/bicgstab_fixture.json (written by the
+    NumPy restatement with MTL4's reduction orders): status, iteration
+    count, every checked residual and x, bit for bit."""
+    def f(hs):
+        return np.array([float.fromhex(v) for v in hs])
+    sy = _BICGSTAB_FIXTURE["systems"][case["system"]]
+    x0 = None if sy["x0"] is None else f(sy["x0"])
+    x, st, its, h = _orc_bicgstab(np.array(sy["rowptr"], np.int32), np.array(sy["col"], np.int32), f(sy["val"]),
+                                  f(sy["b"]), case["pc"], x0, case["max_iter"], case["rtol"])
+    assert (st, its) == (case["error"], case["iterations"])
+    assert np.array_equal(h, f(case["history"]))
+    assert np.array_equal(x, f(case["x"]))
+
+
 # ---------------------------------------------------------------- CGS
 def test_cgs_ic0_reference_case():
     """The reference's ic_0_cgs_test.cpp case: 10x10 laplacian_setup, ic_0,
