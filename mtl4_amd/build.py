@@ -32,8 +32,20 @@ def sources():
     return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
-def _compile(src):
+def _stale(src, obj):
+    """True when obj is missing or older than its source or any header."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps.append(os.path.join(ROOT, "include", "mtl4g.h"))
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(BUILD, src + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC] + CFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l)
@@ -42,11 +54,13 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False, jobs=4):
+def build(verbose=False, jobs=6, incremental=False):
+    """Compiles every source (incremental=True: only those whose object is
+    older than the source or a header) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda f: _compile(f, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     p = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,5 +73,5 @@ def build(verbose=False, jobs=4):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)
