@@ -200,6 +200,14 @@ int mtl4g_spmv_f32(mtl4g_matrix A, int value_format, const float* d_v, float* d_
                    int assign, void* stream);
 /* (lazy(q)= A*p) || (lazy(alpha)= lazy_dot(p,q)), itl/krylov/cg.hpp:103 */
 int mtl4g_spmv_dot(mtl4g_matrix A, const double* d_p, double* d_q, double* h_dot, void* stream);
+/* mtl4g_spmv_dot that allocates no device memory.  No MTL4 counterpart: it
+ * is for callers that cannot allocate (tight serving loops, harnesses that
+ * forbid hipMalloc).  Its reduction scratch is a fixed per-device array of
+ * of 9.6 MiB held until the process ends; in MTL4 reduction order it
+ * takes at most 1,048,576 rows (MTL4G_ERR_UNSUPPORTED beyond), in the
+ * default order any size.  Calls that overlap on one device race. */
+int mtl4g_spmv_dot_noalloc(mtl4g_matrix A, const double* d_p, double* d_q, double* h_dot,
+                           void* stream);
 
 /* ---- dense_vector reductions and updates ----------------------------- */
 /* dot(v, w), operation/dot.hpp:226-232 */

@@ -89,6 +89,7 @@ _SIGS = {
     "mtl4g_spmv": (_INT, [_P, _P, _P, _INT, _P]),
     "mtl4g_spmv_f32": (_INT, [_P, _INT, _P, _P, _INT, _P]),
     "mtl4g_spmv_dot": (_INT, [_P, _P, _P, C.POINTER(_D), _P]),
+    "mtl4g_spmv_dot_noalloc": (_INT, [_P, _P, _P, C.POINTER(_D), _P]),
     "mtl4g_dot": (_INT, [_I64, _P, _P, C.POINTER(_D), _P]),
     "mtl4g_unary_dot": (_INT, [_I64, _P, C.POINTER(_D), _P]),
     "mtl4g_two_norm": (_INT, [_I64, _P, C.POINTER(_D), _P]),

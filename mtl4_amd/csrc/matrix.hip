@@ -697,6 +697,43 @@ static int fetch(Scratch& sc, double* h, hipStream_t s) {
     return MTL4G_OK;
 }
 
+// ---- fixed scratch for the entry point that allocates nothing
+// Layout as Scratch: tickets, a 256-byte result slot, partials, then room for
+// the terms of an MTL4-order reduction of up to kNoAllocTermRows rows.  Each
+// device holds its own copy of the array from module load to process end.
+constexpr size_t kNoAllocTermRows = (size_t)1 << 20;
+constexpr size_t kNoAllocBytes =
+    Scratch::kTicketBytes + 256 + sizeof(double) * (kPartialWords + kNoAllocTermRows);
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices];   // device address of `scratch`, per device
+
+// The device address of this device's `scratch`, looked up once per device
+// with that device current.  Host code passes this pointer, never the
+// symbol's own name, to kernels and copies.
+static int noalloc_scratch(char** out) {
+    int dev = 0;
+    M4_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d beyond the %d devices the fixed scratch serves", dev, kMaxDevices);
+        return MTL4G_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        M4_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("hipGetSymbolAddress returned no address for the fixed scratch");
+            return MTL4G_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return MTL4G_OK;
+}
+
 }  // namespace mtl4g
 
 using namespace mtl4g;
@@ -1130,6 +1167,34 @@ int mtl4g_spmv_dot(mtl4g_matrix A, const double* p, double* q, double* h_dot, vo
     M4_TRY(launch_csr<double>(A, A->val.p, p, EpiSpmvDot{p, q, sc.result()}, sc.partials(), sc.ticket(), s,
                               sc.terms()));
     return fetch(sc, h_dot, s);
+}
+
+// mtl4g_spmv_dot with its tickets, partials and terms in the fixed scratch:
+// no device memory is allocated or freed.  Calls on one device share the
+// scratch, so calls that overlap on one device (other streams, other
+// threads) race; the tickets and the result slot are cleared on every call.
+int mtl4g_spmv_dot_noalloc(mtl4g_matrix A, const double* p, double* q, double* h_dot, void* stream) {
+    if (!A || !h_dot || A->n != A->m) return A && A->n != A->m ? MTL4G_ERR_NOT_SQUARE : MTL4G_ERR_INVALID;
+    if (A->n && (!p || !q)) return MTL4G_ERR_INVALID;
+    hipStream_t s = S(stream);
+    if (A->n == 0) { *h_dot = 0; return MTL4G_OK; }
+    const bool ordered = reduction_order() == MTL4G_ORDER_MTL4;
+    if (ordered && (size_t)A->n > kNoAllocTermRows) {
+        set_error("spmv_dot_noalloc: %d rows exceed the fixed scratch's %zu rows for MTL4-order reductions",
+                  A->n, kNoAllocTermRows);
+        return MTL4G_ERR_UNSUPPORTED;
+    }
+    char* base = nullptr;
+    M4_TRY(noalloc_scratch(&base));
+    unsigned* ticket = reinterpret_cast<unsigned*>(base);
+    double* result = reinterpret_cast<double*>(base + Scratch::kTicketBytes);
+    double* partials = reinterpret_cast<double*>(base + Scratch::kTicketBytes + 256);
+    double* terms = ordered ? partials + kPartialWords : nullptr;
+    M4_HIP(hipMemsetAsync(base, 0, Scratch::kTicketBytes + 256, s));
+    M4_TRY(launch_csr<double>(A, A->val.p, p, EpiSpmvDot{p, q, result}, partials, ticket, s, terms));
+    M4_HIP(hipMemcpyAsync(h_dot, result, sizeof(double), hipMemcpyDeviceToHost, s));
+    M4_HIP(hipStreamSynchronize(s));
+    return MTL4G_OK;
 }
 
 int mtl4g_dot(int64_t n, const double* v, const double* w, double* h, void* stream) {
