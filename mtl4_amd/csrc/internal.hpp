@@ -78,23 +78,39 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(static_cast<uint32_t>(h) << 16);
 }
 
+// float -> bfloat16 bits, round to nearest even; NaN stays NaN
+__host__ __device__ inline uint16_t bf16_bits(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+// the value of bfloat16 bits, on the host
+inline double bf16_value(uint16_t h) {
+    const uint32_t u = static_cast<uint32_t>(h) << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
 template <typename MT>
 __device__ __forceinline__ double val_f64(MT v) { return static_cast<double>(v); }
 
-template <typename AT, typename MT>
-__device__ __forceinline__ AT load_val(const MT* p, int64_t j) { return static_cast<AT>(p[j]); }
+// A stored value (double, float, or bfloat16 bits) in the arithmetic type T.
+template <typename T, typename MT>
+__device__ __forceinline__ T to_arith(MT v) { return static_cast<T>(v); }
 template <>
-__device__ __forceinline__ float load_val<float, uint16_t>(const uint16_t* p, int64_t j) {
-    return bf16_to_f32(p[j]);
-}
+__device__ __forceinline__ float to_arith<float, uint16_t>(uint16_t v) { return bf16_to_f32(v); }
+template <>
+__device__ __forceinline__ double to_arith<double, uint16_t>(uint16_t v) { return (double)bf16_to_f32(v); }
+
+template <typename AT, typename MT>
+__device__ __forceinline__ AT load_val(const MT* p, int64_t j) { return to_arith<AT>(p[j]); }
 
 template <typename AT, typename MT>
 __device__ __forceinline__ AT load_val_nt(const MT* p, int64_t j) {
-    return static_cast<AT>(__builtin_nontemporal_load(p + j));
-}
-template <>
-__device__ __forceinline__ float load_val_nt<float, uint16_t>(const uint16_t* p, int64_t j) {
-    return bf16_to_f32(__builtin_nontemporal_load(p + j));
+    return to_arith<AT>(__builtin_nontemporal_load(p + j));
 }
 
 // Agent-scope loads/stores for values exchanged between workgroups

@@ -56,12 +56,21 @@ __device__ __forceinline__ bool conv_test(const Ctl* c, double resid) {
     return c->norm_r0 == 0 ? (resid <= c->atol) : (resid <= c->rtol * c->norm_r0 || resid <= c->atol);
 }
 
-// Mixed precision, loop head on the FP32 recurrence residual: it only
-// estimates convergence, so an estimated stop (or the iteration limit)
-// pauses the recurrence and asks for an FP64 refresh, which decides.
+// A recurrence residual this far above the last FP64 residual is refreshed
+// at once instead of iterating on.
+constexpr double kGrowth = 10.0;
+
+// Mixed precision, loop head on the recurrence residual: it only estimates
+// convergence, so an estimated stop (or the iteration limit) pauses the
+// recurrence and asks for an FP64 refresh, which decides.  A non-finite
+// residual also pauses it and discards the low-precision correction; a
+// residual that has grown by kGrowth since the last refresh is refreshed
+// before the recurrence goes on.
 __device__ __forceinline__ void mixed_check(Ctl* c, double resid32) {
-    if (!isfinite(resid32) || conv_test(c, resid32)) { c->want_ru = 1; c->pause = 1; return; }
+    if (!isfinite(resid32)) { c->want_ru = 1; c->pause = 1; c->drop = 1; return; }
+    if (conv_test(c, resid32)) { c->want_ru = 1; c->pause = 1; return; }
     if (c->i >= c->max_iter) { c->want_ru = 1; c->pause = 1; c->limit = 1; return; }
+    if (resid32 > kGrowth * c->rmax) { c->want_ru = 1; c->pause = 1; return; }
     c->i++;
     c->first = c->i <= 1;
     if (resid32 <= c->delta * c->rmax) c->want_ru = 1;
@@ -91,9 +100,10 @@ __device__ __forceinline__ void ru_finish(Ctl* c, double rr64, double rho_new, b
         c->pause = 0;
         c->init = 0;
     }
-    if (c->restart) {   // BiCGSTAB half step that FP64 did not confirm: restart p
-        c->first = 1;
+    if (c->restart || c->drop) {   // BiCGSTAB half step that FP64 did not confirm, or a
+        c->first = 1;              // discarded non-finite correction: restart p from r
         c->restart = 0;
+        c->drop = 0;
         c->half = 0;
     }
     c->rmax = resid;
@@ -1591,10 +1601,11 @@ struct EpiRu1 {
     double* x;
     float* x32;
     Ctl* c;
+    int drop;
     __device__ bool skip() const { return c->done != 0 || c->want_ru == 0; }
-    __device__ void prepare() {}
+    __device__ void prepare() { drop = c->drop; }
     __device__ void operator()(int64_t i, double*) const {
-        x[i] = x[i] + (double)x32[i];
+        if (!drop) x[i] = x[i] + (double)x32[i];
         x32[i] = 0.0f;
     }
     __device__ void finish(const double*) const {}
@@ -1630,6 +1641,30 @@ struct EpiRu2 {
     __device__ void finish(const double* tot) const {
         ru_finish(c, tot[0], tot[1], cg && PCM != PCM_GENERAL);
     }
+};
+
+// FP64 vectors on low-precision values (BiCGSTAB): r = b - A*x with the FP64
+// values replaces the recurrence's r; the first refresh also sets
+// rtilde = r (bicgstab.hpp:37-38); ||r||^2 decides in ru_finish
+struct EpiRu64 {
+    static constexpr int NP = 1;
+    struct Pre {
+        double b;
+    };
+    const double* b;
+    double *r, *rt;
+    Ctl* c;
+    int init;
+    __device__ bool skip() const { return c->done != 0 || c->want_ru == 0; }
+    __device__ void prepare() { init = c->init; }
+    __device__ Pre preload(int row) const { return Pre{b[row]}; }
+    __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
+        const double v = pre.b - s;
+        r[row] = v;
+        if (init) rt[row] = v;
+        part[0] += v * v;
+    }
+    __device__ void finish(const double* tot) const { ru_finish(c, tot[0], 0.0, false); }
 };
 
 __global__ void k_ctl_init(Ctl* c, Ctl init) { *c = init; }
@@ -1675,7 +1710,9 @@ static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_p
     if (!W.hist.p) M4_TRY(W.hist.alloc(1));
     if (sizeof(VT) == 8 && reduction_order() == MTL4G_ORDER_MTL4 && W.terms.n < kMaxNP * (size_t)n)
         M4_TRY(W.terms.alloc(kMaxNP * (size_t)n));
-    size_t words = P && (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) ? pc_counter_words(P) : 1;
+    const bool sweeps = P && (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0);
+    if (sweeps && sizeof(VT) == 4 && W.pc64.n < 2 * (size_t)n) M4_TRY(W.pc64.alloc(2 * (size_t)std::max(n, 1)));
+    size_t words = sweeps ? pc_counter_words(P) : 1;
     if (W.pc_counters.n < words) {
         M4_TRY(W.pc_counters.alloc(words));
         M4_TRY(zero_on(W.pc_counters.p, words * sizeof(unsigned), s));
@@ -1698,7 +1735,7 @@ static int cg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s
     VT* tm = reinterpret_cast<VT*>(W.terms.p);
     unsigned* tk = W.tickets.p;
     if (PCM == PCM_GENERAL) {
-        M4_TRY(pc_apply<VT>(P, r, z, y0, &c->done, W.pc_counters.p, s));
+        M4_TRY(pc_apply<VT>(P, r, z, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
         M4_TRY(launch_vec<VT>(n, EpiCgRho<VT>{r, z, c}, part, tk, s, tm));
     }
     M4_TRY(launch_vec<VT>(n, EpiCgP<VT, DT, PCM>{r, d, z, p, c, VT(0), 0}, part, tk, s));
@@ -1748,11 +1785,11 @@ static int bicg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc
     unsigned* tk = W.tickets.p;
     M4_TRY(launch_vec<VT>(n, EpiBiRho<VT>{rt, r, c}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiBiP<VT>{r, v, p, c, VT(0), VT(0), 0}, part, tk, s));
-    if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
     M4_TRY(launch_csr<VT>(A, val, phat, EpiBiV<VT>{rt, v, c}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiBiS<VT>{r, v, sv, c, W.hist.p, VT(0)}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiBiHalf<VT>{phat, x, c, VT(0)}, part, tk, s));
-    if (!ident) M4_TRY(pc_apply<VT>(P, sv, shat, y0, &c->done, W.pc_counters.p, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, sv, shat, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
     M4_TRY(launch_csr<VT>(A, val, shat, EpiBiT<VT>{sv, t, c}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiBiXR<VT>{x, r, shat, phat, sv, t, c, W.hist.p, VT(0), VT(0)}, part, tk, s, tm));
     return MTL4G_OK;
@@ -1776,10 +1813,10 @@ static int cgs_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_
     unsigned* tk = W.tickets.p;
     M4_TRY(launch_vec<VT>(n, EpiCgsRho<VT>{rt, r, c}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiCgsUP<VT>{r, q, u, p, c, VT(0), 0}, part, tk, s));
-    if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, p, phat, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
     M4_TRY(launch_csr<VT>(A, val, phat, EpiCgsV<VT>{rt, vhat, c}, part, tk, s, tm));
     M4_TRY(launch_vec<VT>(n, EpiCgsQU<VT>{vhat, q, u, c, VT(0)}, part, tk, s));
-    if (!ident) M4_TRY(pc_apply<VT>(P, u, uhat, y0, &c->done, W.pc_counters.p, s));
+    if (!ident) M4_TRY(pc_apply<VT>(P, u, uhat, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
     M4_TRY(launch_csr<VT>(A, val, uhat, EpiCgsXR<VT>{x, r, uhat, c, W.hist.p, VT(0)}, part, tk, s, tm));
     return MTL4G_OK;
 }
@@ -1787,9 +1824,10 @@ static int cgs_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_
 template <typename VT, typename MT>
 static int run_typed(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P, int method,
                      Workspace<VT>& W, VT* x, const VT* b, hipStream_t s, int32_t budget) {
-    const int pcm = (!P || P->kind == MTL4G_PC_IDENTITY) ? PCM_NONE
-                    : P->kind == MTL4G_PC_DIAGONAL          ? PCM_DIAG
-                                                             : PCM_GENERAL;
+    // a bfloat16 inverse diagonal is applied by the preconditioner kernel
+    const int pcm = (!P || P->kind == MTL4G_PC_IDENTITY)                     ? PCM_NONE
+                    : P->kind == MTL4G_PC_DIAGONAL && P->fmt != MTL4G_BF16 ? PCM_DIAG
+                                                                          : PCM_GENERAL;
     // inverse diagonal in the vector type of the iteration
     const VT* d = nullptr;
     if (pcm == PCM_DIAG) {
@@ -1948,7 +1986,7 @@ int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
     auto a_r = [&](const VT* in, VT* y) -> int {
         const VT* src = in;
         if (R) {
-            M4_TRY(pc_apply<VT>(R, in, tmp, y0, &c->done, W.pc_counters.p, s));
+            M4_TRY(pc_apply<VT>(R, in, tmp, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
             src = tmp;
         }
         return launch_csr<VT>(A, val, src, EpiTfAv<VT>{y, c}, part, tk, s);
@@ -1963,11 +2001,11 @@ int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
     // rt = b - A*solve(R, x); finished(rt); r = solve(L, rt)  (:44-49)
     const VT* xs = x;
     if (R) {
-        M4_TRY(pc_apply<VT>(R, x, tmp, y0, nullptr, W.pc_counters.p, s));
+        M4_TRY(pc_apply<VT>(R, x, tmp, y0, nullptr, W.pc_counters.p, s, W.pc64.p));
         xs = tmp;
     }
     M4_TRY(launch_csr<VT>(A, val, xs, EpiTfInit<VT>{b, rt, c, W.hist.p}, part, tk, s, tm));
-    if (L) M4_TRY(pc_apply<VT>(L, rt, r, y0, &c->done, W.pc_counters.p, s));
+    if (L) M4_TRY(pc_apply<VT>(L, rt, r, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
     else M4_TRY(launch_vec<VT>(n, EpiTfCopy<VT>{rt, r, nullptr, c}, part, tk, s));
     // y1 = w = r; u1 = v = solve(L, A*solve(R, y1)); tau, rho; loop head  (:50-57)
     M4_TRY(launch_vec<VT>(n, EpiTfCopy<VT>{r, y1, w, c}, part, tk, s));
@@ -2002,7 +2040,7 @@ int tfqmr_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
     const Ctl& hc = *W.h_ctl;
     // shift back: x = solve(R, x)  (:95), not after the early return at :48
     if (R && hc.looped) {
-        M4_TRY(pc_apply<VT>(R, x, tmp, y0, nullptr, W.pc_counters.p, s));
+        M4_TRY(pc_apply<VT>(R, x, tmp, y0, nullptr, W.pc_counters.p, s, W.pc64.p));
         M4_HIP(hipMemcpyAsync(x, tmp, (size_t)n * sizeof(VT), hipMemcpyDeviceToDevice, s));
     }
     if (L) M4_TRY(pc_check_error(L, s));
@@ -2071,7 +2109,7 @@ int bicgstab2_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, Workspace<double
     M4_TRY(launch_csr<double>(A, val, x, EpiB2Init{b, x, r0, x0, c, 0}, part, tk, s, tm));
     M4_TRY(launch_vec<double>(n, EpiB2Start{x, rt, u0, r0, c, 0, 0.0}, part, tk, s));
     if (L) {   // y = solve(L, r_hat[0]); r_hat[0] = y: the two buffers swap roles
-        M4_TRY(pc_apply<double>(L, r0, y, y0, nullptr, W.pc_counters.p, s));
+        M4_TRY(pc_apply<double>(L, r0, y, y0, nullptr, W.pc_counters.p, s, W.pc64.p));
         std::swap(r0, y);
     }
     M4_TRY(launch_vec<double>(n, EpiB2Head<false>{x, r0, u0, r1, r2, u1, u2, rt, c, W.hist.p}, part, tk, s, tm));
@@ -2081,7 +2119,7 @@ int bicgstab2_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, Workspace<double
         constexpr bool XUP = decltype(xup)::value;
         if (!L) return launch_csr<double>(A, val, in, EpiB2Mv<MODE, XUP>{a, out, x, u0, c, 0.0}, part, tk, s, tm);
         M4_TRY((launch_csr<double>(A, val, in, EpiB2Mv<B2_NONE, XUP>{a, y, x, u0, c, 0.0}, part, tk, s)));
-        M4_TRY(pc_apply<double>(L, y, out, y0, &c->done, W.pc_counters.p, s));
+        M4_TRY(pc_apply<double>(L, y, out, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
         return launch_vec<double>(n, EpiB2Dot<MODE>{a, out, c}, part, tk, s, tm);
     };
     using Gamma = std::integral_constant<int, B2_GAMMA>;
@@ -2201,7 +2239,7 @@ int bicgstab_ell_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_p
     M4_TRY(launch_csr<double>(A, val, x, EpiB2Init{b, x, V.r[0], x0, c, 0}, part, tk, s, tm));
     M4_TRY(launch_vec<double>(n, EpiB2Start{x, rt, V.u[0], V.r[0], c, 0, 0.0}, part, tk, s));
     if (L) {   // y = solve(L, r_hat[0]); r_hat[0] = y: the two buffers swap roles
-        M4_TRY(pc_apply<double>(L, V.r[0], y, y0, nullptr, W.pc_counters.p, s));
+        M4_TRY(pc_apply<double>(L, V.r[0], y, y0, nullptr, W.pc_counters.p, s, W.pc64.p));
         std::swap(V.r[0], y);
     }
     M4_TRY(launch_vec<double>(n, EpiEllHead<false>{V, x, rt, E, c, W.hist.p}, part, tk, s, tm));
@@ -2211,12 +2249,12 @@ int bicgstab_ell_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_p
         constexpr int MODE = decltype(mode)::value;
         constexpr bool XUP = decltype(xup)::value;
         if (R) {
-            M4_TRY(pc_apply<double>(R, in, tmp, y0, &c->done, W.pc_counters.p, s));
+            M4_TRY(pc_apply<double>(R, in, tmp, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
             in = tmp;
         }
         if (!L) return launch_csr<double>(A, val, in, EpiB2Mv<MODE, XUP>{rt, out, x, V.u[0], c, 0.0}, part, tk, s, tm);
         M4_TRY((launch_csr<double>(A, val, in, EpiB2Mv<B2_NONE, XUP>{rt, y, x, V.u[0], c, 0.0}, part, tk, s)));
-        M4_TRY(pc_apply<double>(L, y, out, y0, &c->done, W.pc_counters.p, s));
+        M4_TRY(pc_apply<double>(L, y, out, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
         if constexpr (MODE == B2_NONE) return MTL4G_OK;
         else return launch_vec<double>(n, EpiB2Dot<MODE>{rt, out, c}, part, tk, s, tm);
     };
@@ -2256,7 +2294,7 @@ int bicgstab_ell_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_p
     if (hc.error != 3) {   // x = solve(R, x); x += x0
         const double* src = x;
         if (R) {
-            M4_TRY(pc_apply<double>(R, x, tmp, y0, nullptr, W.pc_counters.p, s));
+            M4_TRY(pc_apply<double>(R, x, tmp, y0, nullptr, W.pc_counters.p, s, W.pc64.p));
             src = tmp;
         }
         M4_TRY(launch_vec<double>(n, EpiEllShift{x, src, x0}, part, tk, s));
@@ -2352,7 +2390,7 @@ int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
         // r = solve(L, r0); rho = g[0] = two_norm(r); finished(rho); V_0 = r / rho  (:50-59)
         const double* r = r0;
         if (L) {
-            M4_TRY(pc_apply<double>(L, r0, rv, y0, &c->done, W.pc_counters.p, s));
+            M4_TRY(pc_apply<double>(L, r0, rv, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
             r = rv;
         }
         M4_TRY(launch_vec<double>(n, EpiGmRho{r, G, c, W.hist.p}, part, tk, s, tm));
@@ -2360,7 +2398,7 @@ int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
         for (int k = 0; k < kmax; k++) {   // (:63-96)
             const double* in = Vj(k);
             if (R) {
-                M4_TRY(pc_apply<double>(R, in, tmp, y0, &c->done, W.pc_counters.p, s));
+                M4_TRY(pc_apply<double>(R, in, tmp, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
                 in = tmp;
             }
             double* w = Vj(k + 1);
@@ -2368,7 +2406,7 @@ int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
                 M4_TRY((launch_csr<double>(A, val, in, EpiGmMv<true>{Vj(0), w, G, k, c}, part, tk, s, tm)));
             } else {
                 M4_TRY((launch_csr<double>(A, val, in, EpiGmMv<false>{Vj(0), rv, G, k, c}, part, tk, s)));
-                M4_TRY(pc_apply<double>(L, rv, w, y0, &c->done, W.pc_counters.p, s));
+                M4_TRY(pc_apply<double>(L, rv, w, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
                 M4_TRY(launch_vec<double>(n, EpiGmOrth<0>{w, nullptr, Vj(0), G, 0, k, c, 0.0}, part, tk, s, tm));
             }
             for (int j = 1; j <= k; j++)
@@ -2387,7 +2425,7 @@ int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
         M4_TRY(launch_vec<double>(n, EpiGmComb{Vj(0), n, rv, G, c, 0}, part, tk, s));
         const double* t = rv;
         if (R) {
-            M4_TRY(pc_apply<double>(R, rv, tmp, y0, &c->done, W.pc_counters.p, s));
+            M4_TRY(pc_apply<double>(R, rv, tmp, y0, &c->done, W.pc_counters.p, s, W.pc64.p));
             t = tmp;
         }
         M4_TRY(launch_vec<double>(n, EpiGmX{x, t, c}, part, tk, s));
@@ -2433,21 +2471,23 @@ static int mixed_typed(mtl4g_matrix_s* A, const MT* val32, const mtl4g_pc_s* P, 
                        Workspace<float>& W, float* x32, double* x, const double* b, int32_t budget,
                        hipStream_t s) {
     const int64_t n = A->n;
-    const int pcm = (!P || P->kind == MTL4G_PC_IDENTITY) ? PCM_NONE
-                    : P->kind == MTL4G_PC_DIAGONAL          ? PCM_DIAG
-                                                             : PCM_GENERAL;
-    if (pcm == PCM_DIAG && P->fmt != MTL4G_F32) {
-        set_error("mixed precision needs a preconditioner stored in FP32");
+    // a bfloat16 inverse diagonal is applied by the preconditioner kernel
+    const int pcm = (!P || P->kind == MTL4G_PC_IDENTITY)                     ? PCM_NONE
+                    : P->kind == MTL4G_PC_DIAGONAL && P->fmt == MTL4G_F32  ? PCM_DIAG
+                                                                          : PCM_GENERAL;
+    if (P && P->kind == MTL4G_PC_DIAGONAL && P->fmt == MTL4G_F64) {
+        set_error("mixed precision needs a preconditioner stored in FP32 or BF16");
         return MTL4G_ERR_INVALID;
     }
+    (void)method;
     const float* d = pcm == PCM_DIAG ? P->inv_diag32.p : nullptr;
     Ctl* c = W.ctl.p;
     double* part = W.partials.p;
     unsigned* tk = W.tickets.p;
-    float* r32 = W.vecs.p + (method == MTL4G_CG ? 2 : 6) * n;
-    const int cg = method == MTL4G_CG;
+    float* r32 = W.vecs.p + 2 * n;
+    const int cg = 1;
     auto refresh = [&]() -> int {
-        M4_TRY(launch_vec<double>(n, EpiRu1{x, x32, c}, part, tk, s));
+        M4_TRY(launch_vec<double>(n, EpiRu1{x, x32, c, 0}, part, tk, s));
         if (pcm == PCM_DIAG)
             return launch_csr<double>(A, A->val.p, x, EpiRu2<float, PCM_DIAG>{b, r32, d, c, cg}, part, tk, s);
         if (pcm == PCM_NONE)
@@ -2455,16 +2495,11 @@ static int mixed_typed(mtl4g_matrix_s* A, const MT* val32, const mtl4g_pc_s* P, 
         return launch_csr<double>(A, A->val.p, x, EpiRu2<float, PCM_GENERAL>{b, r32, d, c, cg}, part, tk, s);
     };
     auto iteration = [&]() -> int {
-        if (method == MTL4G_BICGSTAB) return bicg_iteration<float, MT>(A, val32, P, W, x32, s);
         if (pcm == PCM_NONE) return cg_iteration<float, MT, float, PCM_NONE>(A, val32, P, d, W, x32, s);
         if (pcm == PCM_DIAG) return cg_iteration<float, MT, float, PCM_DIAG>(A, val32, P, d, W, x32, s);
         return cg_iteration<float, MT, float, PCM_GENERAL>(A, val32, P, d, W, x32, s);
     };
     M4_TRY(refresh());   // initial FP64 residual (Ctl::init set)
-    if (method == MTL4G_BICGSTAB) {   // rtilde = r
-        float* rt = W.vecs.p + 7 * n;
-        M4_HIP(hipMemcpyAsync(rt, r32, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    }
     int32_t launched = 0, chunk = 2;
     for (;;) {
         M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -2482,9 +2517,45 @@ static int mixed_typed(mtl4g_matrix_s* A, const MT* val32, const mtl4g_pc_s* P, 
     return MTL4G_OK;
 }
 
+// BiCGSTAB keeps its vectors and arithmetic in FP64 and reads the matrix
+// and preconditioner values in low precision: an FP32 BiCGSTAB recurrence
+// loses the shadow residual's biorthogonality and diverges once the
+// residual is a few units of FP32 rounding below the start (seen on the
+// 1024^2 convection-diffusion problem below a relative residual of 1e-8).
+// The values the products read are still the FP32 (or BF16) copies, and the
+// FP64 refreshes with the FP64 values correct for them.
+template <typename MT>
+static int mixed_bicg_typed(mtl4g_matrix_s* A, const MT* lowp, const mtl4g_pc_s* P, Workspace<double>& W,
+                            double* x, const double* b, int32_t budget, hipStream_t s) {
+    const int64_t n = A->n;
+    Ctl* c = W.ctl.p;
+    double* r = W.vecs.p + 6 * n;
+    double* rt = W.vecs.p + 7 * n;
+    auto refresh = [&]() -> int {
+        return launch_csr<double>(A, A->val.p, x, EpiRu64{b, r, rt, c, 0}, W.partials.p, W.tickets.p, s,
+                                  (double*)nullptr);
+    };
+    M4_TRY(refresh());   // initial FP64 residual (Ctl::init set), rtilde = r
+    int32_t launched = 0, chunk = 2;
+    for (;;) {
+        M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        M4_HIP(hipStreamSynchronize(s));
+        if (W.h_ctl->done) break;
+        if (launched > budget) {
+            set_error("mixed solver did not stop after %d launched iterations", launched);
+            return MTL4G_ERR_INVALID;
+        }
+        for (int k = 0; k < chunk; k++) M4_TRY((bicg_iteration<double, MT>(A, lowp, P, W, x, s)));
+        M4_TRY(refresh());
+        launched += chunk;
+        chunk = std::min(chunk * 2, 16);
+    }
+    return MTL4G_OK;
+}
+
 int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, double delta,
-              int32_t max_refresh, Workspace<float>& W, float* x32, double* x, const double* b,
-              mtl4g_iteration* it, int32_t* refreshes, hipStream_t s) {
+              int32_t max_refresh, Workspace<float>& W, Workspace<double>& W64, float* x32, double* x,
+              const double* b, mtl4g_iteration* it, int32_t* refreshes, hipStream_t s) {
     (void)max_refresh;
     if (method != MTL4G_CG && method != MTL4G_BICGSTAB) {
         set_error("mixed precision: cg and bicgstab only");
@@ -2497,10 +2568,13 @@ int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, doubl
         it->error = 0;
         return 0;
     }
-    const int nvec = method == MTL4G_CG ? 5 : 9;
-    M4_TRY(ensure_workspace(W, A->n, nvec, P, 0, s));
+    const bool bicg = method == MTL4G_BICGSTAB;
+    if (bicg) M4_TRY(ensure_workspace(W64, A->n, 9, P, 0, s));
+    else M4_TRY(ensure_workspace(W, A->n, 5, P, 0, s));
+    Ctl* ctl = bicg ? W64.ctl.p : W.ctl.p;
+    Ctl* h_ctl = bicg ? W64.h_ctl : W.h_ctl;
     M4_TRY(ensure_lowp_values(A, fmt, s));
-    M4_HIP(hipMemsetAsync(x32, 0, (size_t)A->n * sizeof(float), s));
+    if (!bicg) M4_HIP(hipMemsetAsync(x32, 0, (size_t)A->n * sizeof(float), s));
     Ctl init;
     std::memset(&init, 0, sizeof init);
     init.norm_r0 = it->norm_r0;
@@ -2512,20 +2586,25 @@ int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, doubl
     init.init = 1;
     init.want_ru = 1;
     init.delta = delta;
-    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, W.ctl.p, init);
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, init);
     M4_HIP(hipGetLastError());
     // a safety net against a device loop that never stops: every counted
     // iteration can leave at most one chunk (16) of launches skipped behind a
     // pause for an FP64 refresh
     const int32_t budget = (std::max(0, it->max_iter - it->i) + 8) * 16 + 64;
     int st;
-    if (fmt == MTL4G_BF16) st = mixed_typed<uint16_t>(A, A->valbf.p, P, method, W, x32, x, b, budget, s);
-    else st = mixed_typed<float>(A, A->val32.p, P, method, W, x32, x, b, budget, s);
+    if (bicg) {
+        if (fmt == MTL4G_BF16) st = mixed_bicg_typed<uint16_t>(A, A->valbf.p, P, W64, x, b, budget, s);
+        else st = mixed_bicg_typed<float>(A, A->val32.p, P, W64, x, b, budget, s);
+    } else {
+        if (fmt == MTL4G_BF16) st = mixed_typed<uint16_t>(A, A->valbf.p, P, method, W, x32, x, b, budget, s);
+        else st = mixed_typed<float>(A, A->val32.p, P, method, W, x32, x, b, budget, s);
+    }
     M4_TRY(st);
     if (P) M4_TRY(pc_check_error(P, s));
-    M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    M4_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
     M4_HIP(hipStreamSynchronize(s));
-    const Ctl& c = *W.h_ctl;
+    const Ctl& c = *h_ctl;
     it->i = c.i;
     it->error = c.error;
     it->resid = c.resid;

@@ -21,6 +21,7 @@ struct Ctl {
     // mixed precision (reliable updates): FP32 recurrence, FP64 refreshes
     int32_t mixed, pause, want_ru, ru_count, limit, init, restart;
     int32_t stall;           // consecutive FP64 refreshes that did not halve the residual
+    int32_t drop;            // the FP32 recurrence went non-finite: discard its correction
     unsigned ticket[4];
 };
 
@@ -36,6 +37,7 @@ struct Workspace {
     DevBuf<unsigned> pc_counters;
     DevBuf<double> terms;     // 3n, reductions in MTL4's order (FP64 only)
     DevBuf<double> small;     // bicgstab_ell: tau, gamma, gamma_a, gamma_aa
+    DevBuf<double> pc64;      // float vectors with ilu_0/ic_0: the sweeps' FP64 chains (2n)
     Ctl* h_ctl = nullptr;     // pinned mirror for polling
     ~Workspace() {
         if (h_ctl) (void)hipHostFree(h_ctl);
@@ -71,13 +73,16 @@ int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
               Workspace<double>& W, double* x, const double* b, mtl4g_iteration* it, double* h_hist,
               int32_t hist_cap, hipStream_t s);
 
-// Mixed precision with reliable updates: one FP32 recurrence (values in
-// value_format), solution accumulated in FP64, the true residual b - A*x
-// recomputed in FP64 whenever the FP32 residual fell by delta since the last
-// refresh, and the stop decision taken on the FP64 residual.
+// Mixed precision with reliable updates: products read the matrix values in
+// value_format (FP32 or BF16) and the preconditioner's values as P32 stores
+// them; the true residual b - A*x is recomputed with the FP64 values
+// whenever the recurrence residual fell by delta (or grew tenfold) since the
+// last refresh, and the stop decision is taken on that FP64 residual.
+// CG: an FP32 recurrence in W, its correction x32 folded into x at each
+// refresh.  BiCGSTAB: FP64 vectors in W64, x updated in place (x32 unused).
 int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P32, int method, int value_format,
-              double delta, int32_t max_refresh, Workspace<float>& W, float* x32,
-              double* x, const double* b, mtl4g_iteration* it, int32_t* refreshes,
+              double delta, int32_t max_refresh, Workspace<float>& W, Workspace<double>& W64,
+              float* x32, double* x, const double* b, mtl4g_iteration* it, int32_t* refreshes,
               hipStream_t s);
 
 }  // namespace mtl4g

@@ -103,13 +103,7 @@ int build_row_blocks(mtl4g_matrix_s* A, int rows_per_block, int nnz_per_block) {
     return MTL4G_OK;
 }
 
-static uint16_t f32_to_bf16(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);   // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                                           // nearest even
-    return (uint16_t)(u >> 16);
-}
+static uint16_t f32_to_bf16(float f) { return bf16_bits(f); }
 
 static int tune_lowp(mtl4g_matrix_s* A, int fmt, hipStream_t s);
 

@@ -82,12 +82,13 @@ extern "C" int mtl4g_ir_solve(mtl4g_matrix A, mtl4g_pc P32, double* x, const dou
     const int32_t n = A->n;
     if (prm->mode == MTL4G_IR_RELIABLE) {
         Workspace<float> W32;
+        Workspace<double> W64;
         DevBuf<float> x32;
-        M4_TRY(x32.alloc(n));
+        if (prm->method == MTL4G_CG) M4_TRY(x32.alloc(n));
         int32_t refreshes = 0;
         it->error = 0;
         int st = mixed_run(A, P32, prm->method, prm->value_format, prm->inner_rtol,
-                           prm->max_outer, W32, x32.p, x, b, it, &refreshes, s);
+                           prm->max_outer, W32, W64, x32.p, x, b, it, &refreshes, s);
         if (stats) {
             stats->outer = refreshes;
             stats->inner_total = it->i;

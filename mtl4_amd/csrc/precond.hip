@@ -69,9 +69,12 @@ __global__ void k_fill_pending(int64_t n, VT* w, const int* skip) {
 // values.  Every row depended on belongs to a workgroup that took its
 // ticket earlier and is running, so every wait ends -- or gives up after
 // kSpinLimit polls and raises the abort flag.
-template <int BS, typename MT, typename VT>
-__global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __restrict__ v,
-                                              VT* w, unsigned* counters, int* err,
+// v (input, type IT) and w are distinct arrays; w holds the results in VT,
+// the type the row arithmetic runs in, and w32 (may be null) receives each
+// result rounded to float as well.
+template <int BS, typename MT, typename IT, typename VT>
+__global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const IT* __restrict__ v,
+                                              VT* w, float* w32, unsigned* counters, int* err,
                                               const int* skip) {
     if (skip && *skip) return;
     __shared__ int s_b, s_ok, s_final;
@@ -93,13 +96,13 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
         r = S.perm[k];
         a = S.rowptr[k];
         e = S.rowptr[k + 1];
-        rr = v[r];
-        if (!S.unit) dk = static_cast<VT>(S.dia[k]);
+        rr = static_cast<VT>(v[r]);
+        if (!S.unit) dk = to_arith<VT>(S.dia[k]);
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const bool in = a + u < e;
             c[u] = in ? S.col[a + u] : 0;
-            m[u] = in ? static_cast<VT>(S.val[a + u]) : VT(0);
+            m[u] = in ? to_arith<VT>(S.val[a + u]) : VT(0);
         }
     }
     if (threadIdx.x == 0) {
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
                 for (int u = 0; u < 8; u++) {
                     const bool in = j0 + u < e;
                     c[u] = in ? S.col[j0 + u] : 0;
-                    m[u] = in ? static_cast<VT>(S.val[j0 + u]) : VT(0);
+                    m[u] = in ? to_arith<VT>(S.val[j0 + u]) : VT(0);
                 }
             }
             VT wv[8];
@@ -159,7 +162,9 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const VT* __r
                 if (j0 + u < e) rr -= m[u] * wv[u];
         }
         if (!S.unit) rr = rr * dk;
-        st_agent(w + r, quiet_if_pending(rr));   // the L1 writes through; wait_memory() below waits for it
+        rr = quiet_if_pending(rr);
+        st_agent(w + r, rr);   // the L1 writes through; wait_memory() below waits for it
+        if (w32) w32[r] = (float)rr;
     }
     wait_memory();
     __syncthreads();
@@ -182,7 +187,7 @@ __global__ void k_diag_apply(int64_t n, const MT* __restrict__ d, const VT* __re
     if (skip && *skip) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        z[i] = static_cast<VT>(d[i]) * r[i];
+        z[i] = to_arith<VT>(d[i]) * r[i];
 }
 
 template <typename VT>
@@ -202,6 +207,20 @@ static int64_t find_entry(const std::vector<int32_t>& rp, const std::vector<int3
 }
 
 static float round_f32(double v) { return (float)v; }
+
+// A factor value as a sweep stores it: double, float, or bfloat16 bits
+// rounded from the float (as the matrix's bfloat16 copy is).
+template <typename MT>
+__host__ __device__ inline MT stored_as(double v) { return static_cast<MT>(v); }
+template <>
+__host__ __device__ inline uint16_t stored_as<uint16_t>(double v) { return bf16_bits((float)v); }
+// the value a stored factor entry has, as a double (host)
+static double stored_value(int fmt, double v) {
+    if (fmt == MTL4G_F32) return (double)(float)v;
+    if (fmt == MTL4G_BF16) return bf16_value(bf16_bits((float)v));
+    return v;
+}
+
 
 int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
     const int32_t n = R.n;
@@ -275,19 +294,28 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
         if (fmt == MTL4G_F64) {
             M4_TRY(S->val64.alloc(std::max<int64_t>(nnz, 1)));
             M4_TRY(S->dia64.alloc(std::max(n, 1)));
-        } else {
+        } else if (fmt == MTL4G_F32) {
             M4_TRY(S->val32.alloc(std::max<int64_t>(nnz, 1)));
             M4_TRY(S->dia32.alloc(std::max(n, 1)));
+        } else {
+            M4_TRY(S->val16.alloc(std::max<int64_t>(nnz, 1)));
+            M4_TRY(S->dia16.alloc(std::max(n, 1)));
         }
     } else if (fmt == MTL4G_F64) {
         M4_TRY(up(S->val64, val));
         M4_TRY(up(S->dia64, dia));
-    } else {
+    } else if (fmt == MTL4G_F32) {
         std::vector<float> v32(val.size()), d32(dia.size());
         for (size_t i = 0; i < val.size(); i++) v32[i] = round_f32(val[i]);
         for (size_t i = 0; i < dia.size(); i++) d32[i] = round_f32(dia[i]);
         M4_TRY(up(S->val32, v32));
         M4_TRY(up(S->dia32, d32));
+    } else {
+        std::vector<uint16_t> v16(val.size()), d16(dia.size());
+        for (size_t i = 0; i < val.size(); i++) v16[i] = stored_as<uint16_t>(val[i]);
+        for (size_t i = 0; i < dia.size(); i++) d16[i] = stored_as<uint16_t>(dia[i]);
+        M4_TRY(up(S->val16, v16));
+        M4_TRY(up(S->dia16, d16));
     }
     M4_TRY(S->counters.alloc(sweep_counter_words(nlev)));
     M4_HIP(hipMemset(S->counters.p, 0, sweep_counter_words(nlev) * sizeof(unsigned)));
@@ -428,9 +456,9 @@ __global__ void k_fill_sweep(int64_t nnz, int32_t n, const int32_t* __restrict__
                              MT* __restrict__ val, MT* __restrict__ dia) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += stride)
-        val[i] = static_cast<MT>(fv[src[i]]);
+        val[i] = stored_as<MT>(fv[src[i]]);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
-        dia[i] = dsrc[i] < 0 ? MT(1) : static_cast<MT>(fv[dsrc[i]]);
+        dia[i] = dsrc[i] < 0 ? stored_as<MT>(1.0) : stored_as<MT>(fv[dsrc[i]]);
 }
 
 static int grid_for(int64_t n) {
@@ -438,8 +466,8 @@ static int grid_for(int64_t n) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
 }
 
-template <typename VT>
-int launch_sweep(const Sweep& S, int fmt, const VT* v, VT* w, const int* skip, int* err,
+template <typename IT, typename VT>
+int launch_sweep(const Sweep& S, int fmt, const IT* v, VT* w, float* w32, const int* skip, int* err,
                  unsigned* counters, hipStream_t s) {
     if (S.n == 0) return MTL4G_OK;
     if (!counters) counters = S.counters.p;
@@ -447,20 +475,27 @@ int launch_sweep(const Sweep& S, int fmt, const VT* v, VT* w, const int* skip, i
     if (fmt == MTL4G_F64) {
         SweepView<double, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
                                 S.blk_level.p, S.val64.p, S.dia64.p, S.nlev, S.unit ? 1 : 0};
-        hipLaunchKernelGGL((k_sweep<kSweepBS, double, VT>), dim3(S.nblocks), dim3(kSweepBS), 0, s,
-                           V, v, w, counters, err, skip);
-    } else {
+        hipLaunchKernelGGL((k_sweep<kSweepBS, double, IT, VT>), dim3(S.nblocks), dim3(kSweepBS), 0, s,
+                           V, v, w, w32, counters, err, skip);
+    } else if (fmt == MTL4G_F32) {
         SweepView<float, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
                                S.blk_level.p, S.val32.p, S.dia32.p, S.nlev, S.unit ? 1 : 0};
-        hipLaunchKernelGGL((k_sweep<kSweepBS, float, VT>), dim3(S.nblocks), dim3(kSweepBS), 0, s,
-                           V, v, w, counters, err, skip);
+        hipLaunchKernelGGL((k_sweep<kSweepBS, float, IT, VT>), dim3(S.nblocks), dim3(kSweepBS), 0, s,
+                           V, v, w, w32, counters, err, skip);
+    } else {
+        SweepView<uint16_t, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
+                                  S.blk_level.p, S.val16.p, S.dia16.p, S.nlev, S.unit ? 1 : 0};
+        hipLaunchKernelGGL((k_sweep<kSweepBS, uint16_t, IT, VT>), dim3(S.nblocks), dim3(kSweepBS), 0, s,
+                           V, v, w, w32, counters, err, skip);
     }
     M4_HIP(hipGetLastError());
     return MTL4G_OK;
 }
 
-template int launch_sweep<double>(const Sweep&, int, const double*, double*, const int*, int*, unsigned*, hipStream_t);
-template int launch_sweep<float>(const Sweep&, int, const float*, float*, const int*, int*, unsigned*, hipStream_t);
+template int launch_sweep<double, double>(const Sweep&, int, const double*, double*, float*, const int*, int*,
+                                          unsigned*, hipStream_t);
+template int launch_sweep<float, double>(const Sweep&, int, const float*, double*, float*, const int*, int*,
+                                         unsigned*, hipStream_t);
 
 
 size_t pc_counter_words(const mtl4g_pc_s* P) {
@@ -469,7 +504,7 @@ size_t pc_counter_words(const mtl4g_pc_s* P) {
 
 template <typename VT>
 int pc_apply(const mtl4g_pc_s* P, const VT* r, VT* z, VT* y0, const int* skip,
-             unsigned* counters, hipStream_t s) {
+             unsigned* counters, hipStream_t s, double* w64) {
     const int64_t n = P->n;
     if (n == 0) return MTL4G_OK;
     switch (P->kind) {
@@ -480,22 +515,42 @@ int pc_apply(const mtl4g_pc_s* P, const VT* r, VT* z, VT* y0, const int* skip,
         if (P->fmt == MTL4G_F64)
             hipLaunchKernelGGL((k_diag_apply<VT, double>), dim3(grid_for(n)), dim3(256), 0, s, n,
                                P->inv_diag64.p, r, z, skip);
-        else
+        else if (P->fmt == MTL4G_F32)
             hipLaunchKernelGGL((k_diag_apply<VT, float>), dim3(grid_for(n)), dim3(256), 0, s, n,
                                P->inv_diag32.p, r, z, skip);
+        else
+            hipLaunchKernelGGL((k_diag_apply<VT, uint16_t>), dim3(grid_for(n)), dim3(256), 0, s, n,
+                               P->inv_diag16.p, r, z, skip);
         break;
-    default:
-        M4_TRY(launch_sweep<VT>(P->lower, P->fmt, r, y0, skip, P->err.p, counters, s));
-        M4_TRY(launch_sweep<VT>(P->upper, P->fmt, y0, z, skip, P->err.p,
-                                counters ? counters + sweep_counter_words(P->lower.nlev) : nullptr, s));
+    default: {
+        unsigned* cu = counters ? counters + sweep_counter_words(P->lower.nlev) : nullptr;
+        if constexpr (sizeof(VT) == 8) {
+            M4_TRY((launch_sweep<double, double>(P->lower, P->fmt, r, y0, nullptr, skip, P->err.p, counters, s)));
+            M4_TRY((launch_sweep<double, double>(P->upper, P->fmt, y0, z, nullptr, skip, P->err.p, cu, s)));
+        } else {
+            // float vectors: each sweep's chain runs in FP64 (w64: 2n doubles),
+            // only the result is rounded to float -- a rounding error made
+            // inside a chain would be amplified along it, and a preconditioner
+            // that is not one fixed linear map stalls the Krylov recurrence
+            if (!w64) {
+                set_error("float preconditioner solve without its FP64 sweep buffer");
+                return MTL4G_ERR_INVALID;
+            }
+            (void)y0;
+            M4_TRY((launch_sweep<float, double>(P->lower, P->fmt, r, w64, nullptr, skip, P->err.p, counters, s)));
+            M4_TRY((launch_sweep<double, double>(P->upper, P->fmt, w64, w64 + n, z, skip, P->err.p, cu, s)));
+        }
         return MTL4G_OK;
+    }
     }
     M4_HIP(hipGetLastError());
     return MTL4G_OK;
 }
 
-template int pc_apply<double>(const mtl4g_pc_s*, const double*, double*, double*, const int*, unsigned*, hipStream_t);
-template int pc_apply<float>(const mtl4g_pc_s*, const float*, float*, float*, const int*, unsigned*, hipStream_t);
+template int pc_apply<double>(const mtl4g_pc_s*, const double*, double*, double*, const int*, unsigned*, hipStream_t,
+                              double*);
+template int pc_apply<float>(const mtl4g_pc_s*, const float*, float*, float*, const int*, unsigned*, hipStream_t,
+                             double*);
 
 int pc_check_error(const mtl4g_pc_s* P, hipStream_t s) {
     if (!P || !P->err.p) return MTL4G_OK;
@@ -520,9 +575,12 @@ int fill_sweep(Sweep& S, int fmt, const double* fval, hipStream_t s) {
     if (fmt == MTL4G_F64)
         hipLaunchKernelGGL(k_fill_sweep<double>, dim3(grid), dim3(256), 0, s, S.nnz, S.n, S.src.p, S.dsrc.p, fval,
                            S.val64.p, S.dia64.p);
-    else
+    else if (fmt == MTL4G_F32)
         hipLaunchKernelGGL(k_fill_sweep<float>, dim3(grid), dim3(256), 0, s, S.nnz, S.n, S.src.p, S.dsrc.p, fval,
                            S.val32.p, S.dia32.p);
+    else
+        hipLaunchKernelGGL(k_fill_sweep<uint16_t>, dim3(grid), dim3(256), 0, s, S.nnz, S.n, S.src.p, S.dsrc.p, fval,
+                           S.val16.p, S.dia16.p);
     M4_HIP(hipGetLastError());
     return MTL4G_OK;
 }
@@ -702,8 +760,7 @@ static int ensure_host_values(mtl4g_pc_s* P) {
     std::vector<double> fv(P->fval.n);
     M4_HIP(hipDeviceSynchronize());
     if (!fv.empty()) M4_HIP(hipMemcpy(fv.data(), P->fval.p, fv.size() * 8, hipMemcpyDeviceToHost));
-    const bool f32 = P->fmt != MTL4G_F64;
-    auto store = [&](double v) { return f32 ? (double)(float)v : v; };
+    auto store = [&](double v) { return stored_value(P->fmt, v); };
     P->L_val.resize(P->L_src.size());
     P->U_val.resize(P->U_src.size());
     for (size_t j = 0; j < P->L_src.size(); j++) P->L_val[j] = store(fv[P->L_src[j]]);
@@ -775,13 +832,35 @@ static int build_adjoint(mtl4g_pc_s* P) {
 
 }  // namespace mtl4g
 
+namespace mtl4g {
+// FP32 / BF16 inverse diagonal of a diagonal preconditioner: rounded from
+// h_inv_diag, which then keeps the stored values.
+static int upload_inv_diag(mtl4g_pc_s* P, int32_t n) {
+    if (P->fmt == MTL4G_F32) {
+        std::vector<float> f(P->h_inv_diag.begin(), P->h_inv_diag.end());
+        for (size_t i = 0; i < f.size(); i++) P->h_inv_diag[i] = f[i];
+        if (!P->inv_diag32.p) M4_TRY(P->inv_diag32.alloc(std::max(n, 1)));
+        if (n) M4_HIP(hipMemcpy(P->inv_diag32.p, f.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    } else {
+        std::vector<uint16_t> h(n);
+        for (int32_t i = 0; i < n; i++) {
+            h[i] = stored_as<uint16_t>(P->h_inv_diag[i]);
+            P->h_inv_diag[i] = bf16_value(h[i]);
+        }
+        if (!P->inv_diag16.p) M4_TRY(P->inv_diag16.alloc(std::max(n, 1)));
+        if (n) M4_HIP(hipMemcpy(P->inv_diag16.p, h.data(), (size_t)n * 2, hipMemcpyHostToDevice));
+    }
+    return MTL4G_OK;
+}
+}  // namespace mtl4g
+
 using namespace mtl4g;
 
 extern "C" {
 
 int mtl4g_pc_create(mtl4g_pc* out, mtl4g_matrix A, int kind, int fmt) {
     if (!out || !A || kind < MTL4G_PC_IDENTITY || kind > MTL4G_PC_IC0 ||
-        (fmt != MTL4G_F64 && fmt != MTL4G_F32)) {
+        (fmt != MTL4G_F64 && fmt != MTL4G_F32 && fmt != MTL4G_BF16)) {
         set_error("mtl4g_pc_create: bad arguments");
         return MTL4G_ERR_INVALID;
     }
@@ -811,11 +890,7 @@ int mtl4g_pc_create(mtl4g_pc* out, mtl4g_matrix A, int kind, int fmt) {
             if (!st && A->n)
                 st = hipMemcpy(P->inv_diag64.p, P->h_inv_diag.data(), A->n * 8, hipMemcpyHostToDevice) ? MTL4G_ERR_HIP : 0;
         } else {
-            std::vector<float> f(P->h_inv_diag.begin(), P->h_inv_diag.end());
-            for (size_t i = 0; i < f.size(); i++) P->h_inv_diag[i] = f[i];
-            st = P->inv_diag32.alloc(A->n);
-            if (!st && A->n)
-                st = hipMemcpy(P->inv_diag32.p, f.data(), A->n * 4, hipMemcpyHostToDevice) ? MTL4G_ERR_HIP : 0;
+            st = upload_inv_diag(P, A->n);
         }
     } else if (!st && (kind == MTL4G_PC_ILU0 || kind == MTL4G_PC_IC0)) {
         st = setup_factor(P);
@@ -843,8 +918,8 @@ static int pc_solve_impl(mtl4g_pc P, const double* x, double* y, void* stream, b
         M4_HIP(hipMallocAsync((void**)&y0, (size_t)P->n * sizeof(double), s));
         int st;
         if (adjoint && P->kind == MTL4G_PC_ILU0) {
-            st = launch_sweep<double>(P->adj_lower, P->fmt, x, y0, nullptr, P->err.p, nullptr, s);
-            if (!st) st = launch_sweep<double>(P->adj_upper, P->fmt, y0, y, nullptr, P->err.p, nullptr, s);
+            st = launch_sweep<double, double>(P->adj_lower, P->fmt, x, y0, nullptr, nullptr, P->err.p, nullptr, s);
+            if (!st) st = launch_sweep<double, double>(P->adj_upper, P->fmt, y0, y, nullptr, nullptr, P->err.p, nullptr, s);
         } else {
             st = pc_apply<double>(P, x, y, y0, nullptr, nullptr, s);
         }
@@ -905,9 +980,7 @@ int mtl4g_pc_refactor(mtl4g_pc P, void* stream) {
         if (P->fmt == MTL4G_F64) {
             if (A->n) M4_HIP(hipMemcpy(P->inv_diag64.p, P->h_inv_diag.data(), (size_t)A->n * 8, hipMemcpyHostToDevice));
         } else {
-            std::vector<float> f(P->h_inv_diag.begin(), P->h_inv_diag.end());
-            for (size_t i = 0; i < f.size(); i++) P->h_inv_diag[i] = f[i];
-            if (A->n) M4_HIP(hipMemcpy(P->inv_diag32.p, f.data(), (size_t)A->n * 4, hipMemcpyHostToDevice));
+            M4_TRY(upload_inv_diag(P, A->n));
         }
     }
     return MTL4G_OK;

@@ -27,6 +27,7 @@ struct Sweep {
     DevBuf<int32_t> src, dsrc;   // values gathered from a device factor: entry / diagonal sources
     DevBuf<double> val64, dia64;
     DevBuf<float> val32, dia32;
+    DevBuf<uint16_t> val16, dia16;   // bfloat16 storage (MTL4G_BF16)
     DevBuf<unsigned> counters;   // default counters (sweep_counter_words)
     std::vector<int32_t> h_lev_ptr;
 };
@@ -51,8 +52,10 @@ int fill_sweep(Sweep& S, int value_format, const double* fval, hipStream_t s);
 
 // counters: sweep_counter_words(nlev) zeroed words owned by the caller (NULL =
 // the sweep's own); each launch leaves them zeroed again.
-template <typename VT>
-int launch_sweep(const Sweep& S, int value_format, const VT* v, VT* w, const int* skip, int* err,
+// The row arithmetic runs in VT; v is read as IT; w32 (may be null) also
+// receives the results rounded to float.
+template <typename IT, typename VT>
+int launch_sweep(const Sweep& S, int value_format, const IT* v, VT* w, float* w32, const int* skip, int* err,
                  unsigned* counters, hipStream_t s);
 
 }  // namespace mtl4g
@@ -64,6 +67,7 @@ struct mtl4g_pc_s {
     const mtl4g_matrix_s* A = nullptr;
     mtl4g::DevBuf<double> inv_diag64;
     mtl4g::DevBuf<float> inv_diag32;
+    mtl4g::DevBuf<uint16_t> inv_diag16;   // bfloat16 storage
     mtl4g::Sweep lower, upper;            // solve: lower then upper
     mtl4g::Sweep adj_lower, adj_upper;    // adjoint solve (built on demand)
     bool has_adjoint = false;
@@ -86,11 +90,13 @@ struct mtl4g_pc_s {
 
 namespace mtl4g {
 // z = P^{-1} r on the device.  y0 is caller scratch of n values (ilu/ic
-// only); skip (may be null) makes every kernel a no-op when *skip != 0;
-// counters (may be null) are pc_counter_words(P) zeroed words of the caller.
+// with double vectors); w64 is caller scratch of 2n doubles (ilu/ic with
+// float vectors, whose sweeps run in FP64); skip (may be null) makes every
+// kernel a no-op when *skip != 0; counters (may be null) are
+// pc_counter_words(P) zeroed words of the caller.
 template <typename VT>
 int pc_apply(const mtl4g_pc_s* P, const VT* r, VT* z, VT* y0, const int* skip,
-             unsigned* counters, hipStream_t s);
+             unsigned* counters, hipStream_t s, double* w64 = nullptr);
 size_t pc_counter_words(const mtl4g_pc_s* P);
 int pc_check_error(const mtl4g_pc_s* P, hipStream_t s);
 }  // namespace mtl4g

@@ -315,8 +315,8 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
                 } else {
                     if (x32.n < (size_t)q.n && x32.alloc(q.n)) { worker_err = MTL4G_ERR_OOM; break; }
                     int32_t rf = 0;
-                    st = mixed_run(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, prm.precision, 0.1, 0, W32, x32.p, xx,
-                                   bb, &it, &rf, bs);
+                    st = mixed_run(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, prm.precision, 0.1, 0, W32, W64, x32.p,
+                                   xx, bb, &it, &rf, bs);
                 }
                 if (st != 0) big_errors++;
                 big_solves++;
