@@ -404,21 +404,27 @@ int mtl4g_batch_solve_items(mtl4g_batch B, int pc_kind, int precision,
 
 /* ---- serving loop: continuous batching (no MTL4 counterpart) --------- */
 /* Matrices stay resident in HBM (one mtl4g_batch).  Requests (matrix, number
- * of right-hand sides) are admitted in arrival order while their device
- * footprint (b, x and workspace of every right-hand side) fits the cap; the
- * admitted requests are packed into persistent batched launches on `streams`
- * concurrent streams as soon as a stream is free, so launches overlap on the
- * device.  Systems with more than big_rows rows are solved by the
- * whole-GPU engine on a stream of their own.  Every solve is itl::cg with
- * the identity or diagonal preconditioner at rtol; precision as in
- * mtl4g_batch_solve. */
+ * of right-hand sides, solver) are admitted in arrival order while their
+ * device footprint (b, x and workspace of every right-hand side) fits the
+ * cap; the admitted requests are packed into persistent batched launches on
+ * `streams` concurrent streams as soon as a stream is free (one solver per
+ * launch), so launches overlap on the device.  Systems with more than
+ * big_rows rows are solved by the whole-GPU engine, `streams` of them at a
+ * time on streams of their own.  A solve is itl::cg or itl::bicgstab with
+ * the identity, diagonal or ilu_0 preconditioner at rtol; precision as in
+ * mtl4g_batch_solve (F32 / BF16: mixed precision refined to rtol).  The
+ * device arena holding the requests' vectors is allocated by
+ * mtl4g_server_run: the smaller of cap_bytes and the trace's total
+ * footprint. */
+#define MTL4G_SERVER_SOLVER(method, pc_kind) (((method) << 8) | (pc_kind))
 typedef struct mtl4g_server_s* mtl4g_server;
 
 typedef struct {
     int32_t streams;        /* concurrent batched launches (e.g. 4) */
     int32_t max_items;      /* right-hand sides per launch (e.g. 4096) */
     int32_t precision;      /* MTL4G_F64, MTL4G_F32 or MTL4G_BF16 */
-    int32_t pc_kind;        /* MTL4G_PC_IDENTITY or MTL4G_PC_DIAGONAL */
+    int32_t pc_kind;        /* preconditioner of requests without a solver code:
+                               MTL4G_PC_IDENTITY, _DIAGONAL or _ILU0 (cg) */
     int32_t max_iter;
     int32_t big_rows;       /* larger systems use the whole-GPU engine */
     double rtol;
@@ -441,13 +447,16 @@ int mtl4g_server_create(mtl4g_server* S, mtl4g_batch B, const mtl4g_server_param
 int mtl4g_server_destroy(mtl4g_server S);
 /* Replays a request trace in real time: request r arrives arrival_s[r]
  * seconds after the start for matrix mat[r] with nrhs[r] right-hand sides
- * generated on the device from seed[r] (values uniform in [-1,1]).
+ * generated on the device from seed[r] (values uniform in [-1,1]), solved
+ * by solver[r] = MTL4G_SERVER_SOLVER(MTL4G_CG or MTL4G_BICGSTAB, pc kind)
+ * (solver NULL: itl::cg with params.pc_kind for every request).
  * latency_s[r] = completion - arrival.  Optional outputs: iters (one per
  * right-hand side, requests in order), x_out + x_off[r] (the solutions of
  * request r, nrhs[r] vectors of the matrix size one after the other). */
 int mtl4g_server_run(mtl4g_server S, int32_t nreq, const int32_t* mat, const int32_t* nrhs,
-                     const double* arrival_s, const uint64_t* seed, double* latency_s,
-                     int32_t* iters, double* x_out, const int64_t* x_off, mtl4g_server_stats* st);
+                     const int32_t* solver, const double* arrival_s, const uint64_t* seed,
+                     double* latency_s, int32_t* iters, double* x_out, const int64_t* x_off,
+                     mtl4g_server_stats* st);
 
 #ifdef __cplusplus
 }

@@ -672,12 +672,20 @@ class Server:
             _lib._lib.mtl4g_server_destroy(h)
             self.h = None
 
-    def run(self, mat, nrhs, arrival_s, seed, want_x=False):
-        """Replay a trace; returns (latency_s, iterations per rhs, x list or None, stats)."""
+    @staticmethod
+    def solver(method, pc_kind):
+        """Solver code of a request: MTL4G_SERVER_SOLVER(method, pc_kind)."""
+        return (int(method) << 8) | int(pc_kind)
+
+    def run(self, mat, nrhs, arrival_s, seed, want_x=False, solver=None):
+        """Replay a trace; returns (latency_s, iterations per rhs, x list or None, stats).
+        solver: per-request codes (Server.solver(method, pc_kind)); None: cg
+        with the server's pc_kind."""
         mat = np.ascontiguousarray(mat, np.int32)
         nrhs = np.ascontiguousarray(nrhs, np.int32)
         arr = np.ascontiguousarray(arrival_s, np.float64)
         sd = np.ascontiguousarray(seed, np.uint64)
+        sv = None if solver is None else np.ascontiguousarray(solver, np.int32)
         n = len(mat)
         lat = np.zeros(n)
         its = np.zeros(int(nrhs.sum()), np.int32)
@@ -687,7 +695,7 @@ class Server:
         xo = np.zeros(int(xoff[-1])) if want_x else None
         st = _lib.ServerStats()
         p = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
-        check(_L().mtl4g_server_run(self.h, n, p(mat), p(nrhs), p(arr), p(sd), p(lat), p(its),
+        check(_L().mtl4g_server_run(self.h, n, p(mat), p(nrhs), p(sv), p(arr), p(sd), p(lat), p(its),
                                     p(xo), p(xoff) if want_x else None, C.byref(st)), "server_run")
         xs = None
         if want_x:

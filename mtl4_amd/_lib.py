@@ -135,7 +135,7 @@ _SIGS = {
     "mtl4g_batch_solve_items": (_INT, [_P, _INT, _INT, _P, _I32, _P, _P, _P, _P, _P, _P]),
     "mtl4g_server_create": (_INT, [_PP, _P, C.POINTER(ServerParams)]),
     "mtl4g_server_destroy": (_INT, [_P]),
-    "mtl4g_server_run": (_INT, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(ServerStats)]),
+    "mtl4g_server_run": (_INT, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(ServerStats)]),
 }
 
 _lib = None

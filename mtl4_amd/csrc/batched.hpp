@@ -5,6 +5,17 @@
 
 namespace mtl4g {
 
+// One triangular sweep of every system's ILU(0) factor, level scheduled
+// inside the system's workgroup.  Position k (the positions of system s are
+// [sys_row[s], sys_row[s+1])) handles row perm[k]; its off-diagonal entries
+// rowptr[k]..rowptr[k+1] follow in the order MTL4 subtracts them.  Levels of
+// system s: positions lev[lev_off[s] + l] .. lev[lev_off[s] + l + 1].
+struct BatchSweepDev {
+    const int32_t *perm, *rowptr, *col, *lev, *lev_off;
+    const double *val, *dia;       // FP64 factor values; dia: stored inverse diagonal (upper)
+    const float *val32, *dia32;    // the same rounded to float
+};
+
 struct BatchDev {
     const int32_t* rowptr;    // block-diagonal CSR over all systems
     const int32_t* col;       // absolute column indices
@@ -14,6 +25,17 @@ struct BatchDev {
     const double* dinv;       // 1 / A[i][i]
     const float* dinv32;
     const int32_t* sys_row;   // first row of each system, nsys+1
+    BatchSweepDev lo, up;     // ILU(0): unit lower sweep, upper sweep (inverse diagonal)
+};
+
+// Host side of one batched sweep.
+struct BatchSweep {
+    DevBuf<int32_t> perm, rowptr, col, lev, lev_off;
+    DevBuf<double> val, dia;
+    DevBuf<float> val32, dia32;
+    BatchSweepDev view() const {
+        return BatchSweepDev{perm.p, rowptr.p, col.p, lev.p, lev_off.p, val.p, dia.p, val32.p, dia32.p};
+    }
 };
 
 // Launch shape picked by the autotuner.
@@ -45,6 +67,9 @@ struct mtl4g_batch_s {
     mtl4g::DevBuf<double> val, dinv;
     mtl4g::DevBuf<float> val32, dinv32;
     mtl4g::DevBuf<uint16_t> valbf;
+    // ILU(0) of every system (built on demand by batch_ensure_ilu0)
+    mtl4g::BatchSweep ilu_lo, ilu_up;
+    bool has_ilu = false;
     // solve-all scratch (items in cost order, workspace, queue)
     mtl4g::DevBuf<mtl4g_batch_item> items;
     mtl4g::DevBuf<mtl4g_batch_result> results;
@@ -70,11 +95,19 @@ struct mtl4g_batch_s {
 };
 
 namespace mtl4g {
-// Workspace elements (doubles) one item of system size n needs.
+// Workspace elements (doubles) one itl::cg item of system size n needs with
+// the identity or diagonal preconditioner.
 int64_t batch_ws_elems(int32_t n, int precision);
-// Launch the persistent kernel over `nitems` items (queue is reset inside).
-int batch_launch(mtl4g_batch_s* B, int pc_kind, int precision, const BatchShape& sh,
+// The same for a method (MTL4G_CG / MTL4G_BICGSTAB) and preconditioner.
+int64_t batch_ws_elems(int32_t n, int precision, int method, int pc_kind);
+// Launch the persistent kernel over `nitems` items (queue is reset inside):
+// method MTL4G_CG or MTL4G_BICGSTAB, preconditioner identity, diagonal or
+// ilu_0 (batch_ensure_ilu0 first).
+int batch_launch(mtl4g_batch_s* B, int method, int pc_kind, int precision, const BatchShape& sh,
                  const mtl4g_batch_item* d_items, int nitems, const double* d_b, double* d_x,
                  double* d_work, mtl4g_batch_result* d_out, unsigned* d_queue, hipStream_t s);
 int batch_ensure_values(mtl4g_batch_s* B, int precision);
+// ILU(0) factors of every system (the device factorization of
+// itl::pc::ilu_0, one system at a time) laid out for the batched sweeps.
+int batch_ensure_ilu0(mtl4g_batch_s* B);
 }  // namespace mtl4g

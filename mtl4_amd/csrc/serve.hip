@@ -15,7 +15,11 @@
 // Systems above big_rows rows go to `streams` worker threads that run the
 // whole-GPU Krylov engine (matrix autotuned at set-up), each on a stream of
 // its own, overlapping with each other and with the batched launches.
-// Right-hand sides are generated on the device from a per-request seed.
+// Every request names its solver (itl::cg or itl::bicgstab with the
+// identity, diagonal or ilu_0 preconditioner); a batched launch serves one
+// solver.  Right-hand sides are generated on the device from a per-request
+// seed.  The arena of request vectors is allocated when a trace is run: the
+// smaller of the admission cap and the trace's total footprint.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -109,10 +113,12 @@ struct Arena {
 
 struct Req {
     int32_t id, mat, nrhs, n;
+    int32_t method = MTL4G_CG, pc = MTL4G_PC_IDENTITY;
     double arrival;
     uint64_t seed;
-    int64_t off = 0, elems = 0;
+    int64_t off = 0, elems = 0, stride = 0;   // stride: b, x (and workspace) of one right-hand side
     bool big = false;
+    int32_t solver() const { return MTL4G_SERVER_SOLVER(method, pc); }
 };
 
 struct Slot {
@@ -137,9 +143,9 @@ struct mtl4g_server_s {
     mtl4g_server_params prm{};
     mtl4g::DevBuf<double> arena;
     std::vector<std::unique_ptr<mtl4g::Slot>> slots;
-    // whole-GPU engine for large systems
+    // whole-GPU engine for large systems: matrix, diagonal and ilu_0
     std::vector<mtl4g_matrix_s*> big_A;
-    std::vector<mtl4g_pc_s*> big_P;
+    std::vector<mtl4g_pc_s*> big_Pd, big_Pi;
     std::vector<hipStream_t> big_s;   // one per big-system worker
     ~mtl4g_server_s() {
         for (auto& slp : slots) {
@@ -150,7 +156,8 @@ struct mtl4g_server_s {
             if (sl.h_gen) (void)hipHostFree(sl.h_gen);
             if (sl.h_res) (void)hipHostFree(sl.h_res);
         }
-        for (auto* p : big_P) delete p;
+        for (auto* p : big_Pd) delete p;
+        for (auto* p : big_Pi) delete p;
         for (auto* a : big_A) delete a;
         for (hipStream_t s : big_s) (void)hipStreamDestroy(s);
     }
@@ -170,7 +177,7 @@ extern "C" {
 
 int mtl4g_server_create(mtl4g_server* out, mtl4g_batch B, const mtl4g_server_params* prm) {
     if (!out || !B || !prm || prm->streams < 1 || prm->streams > 16 || prm->max_items < 1 || prm->cap_bytes <= 0 ||
-        (prm->pc_kind != MTL4G_PC_IDENTITY && prm->pc_kind != MTL4G_PC_DIAGONAL))
+        (prm->pc_kind != MTL4G_PC_IDENTITY && prm->pc_kind != MTL4G_PC_DIAGONAL && prm->pc_kind != MTL4G_PC_ILU0))
         return MTL4G_ERR_INVALID;
     *out = nullptr;
     mtl4g_server_s* S_ = new (std::nothrow) mtl4g_server_s;
@@ -180,8 +187,7 @@ int mtl4g_server_create(mtl4g_server* out, mtl4g_batch B, const mtl4g_server_par
     S_->prm = *prm;
     int st = batch_ensure_values(B, prm->precision);
     if (st) return fail(st);
-    const int64_t cap_elems = (int64_t)(prm->cap_bytes / 8.0);
-    if ((st = S_->arena.alloc((size_t)cap_elems))) return fail(st);
+    if ((st = batch_ensure_ilu0(B))) return fail(st);   // ilu_0 requests on the batched kernel
     for (int k = 0; k < prm->streams; k++) S_->slots.emplace_back(new Slot);
     for (auto& slp : S_->slots) {
         Slot& sl = *slp;
@@ -196,7 +202,8 @@ int mtl4g_server_create(mtl4g_server* out, mtl4g_batch B, const mtl4g_server_par
     }
     // large systems: their own CSR handle and preconditioner for the whole-GPU engine
     S_->big_A.assign(B->nsys, nullptr);
-    S_->big_P.assign(B->nsys, nullptr);
+    S_->big_Pd.assign(B->nsys, nullptr);
+    S_->big_Pi.assign(B->nsys, nullptr);
     for (int32_t k = 0; k < B->nsys; k++) {
         const int32_t a = B->h_sys_row[k], e = B->h_sys_row[k + 1], n = e - a;
         if (n <= prm->big_rows) continue;
@@ -212,8 +219,10 @@ int mtl4g_server_create(mtl4g_server* out, mtl4g_batch B, const mtl4g_server_par
         if (prm->precision != MTL4G_F64 && (st = ensure_lowp_values(A, prm->precision, nullptr))) return fail(st);
         mtl4g_pc P = nullptr;
         const int fmt = prm->precision == MTL4G_F64 ? MTL4G_F64 : MTL4G_F32;
-        if ((st = mtl4g_pc_create(&P, A, prm->pc_kind, fmt))) return fail(st);
-        S_->big_P[k] = P;
+        if ((st = mtl4g_pc_create(&P, A, MTL4G_PC_DIAGONAL, fmt))) return fail(st);
+        S_->big_Pd[k] = P;
+        if ((st = mtl4g_pc_create(&P, A, MTL4G_PC_ILU0, fmt))) return fail(st);
+        S_->big_Pi[k] = P;
     }
     S_->big_s.assign(prm->streams, nullptr);
     for (hipStream_t& bs : S_->big_s)
@@ -228,14 +237,12 @@ int mtl4g_server_destroy(mtl4g_server S_) {
 }
 
 int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const int32_t* nrhs,
-                     const double* arrival_s, const uint64_t* seed, double* latency_s, int32_t* iters,
-                     double* x_out, const int64_t* x_off, mtl4g_server_stats* stats) {
+                     const int32_t* solver, const double* arrival_s, const uint64_t* seed, double* latency_s,
+                     int32_t* iters, double* x_out, const int64_t* x_off, mtl4g_server_stats* stats) {
     if (!S_ || nreq < 0 || (nreq && (!mat || !nrhs || !arrival_s || !seed || !latency_s))) return MTL4G_ERR_INVALID;
     mtl4g_batch_s* B = S_->B;
     const mtl4g_server_params& prm = S_->prm;
-    Arena arena;
-    arena.init((int64_t)S_->arena.n);
-    double* base = S_->arena.p;
+    const int64_t cap_elems = (int64_t)(prm.cap_bytes / 8.0);
     std::vector<Req> reqs(nreq);
     std::vector<int64_t> iter_off(nreq + 1, 0);
     for (int32_t r = 0; r < nreq; r++) {
@@ -251,14 +258,42 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
         q.arrival = arrival_s[r];
         q.seed = seed[r];
         q.big = S_->big_A[q.mat] != nullptr;
-        const int64_t per = Arena::round(q.n) * 2 + (q.big ? 0 : Arena::round(batch_ws_elems(q.n, MTL4G_F64)));
-        q.elems = per * q.nrhs;
-        if (q.elems > arena.cap) {
+        if (solver) {
+            q.method = solver[r] >> 8;
+            q.pc = solver[r] & 0xff;
+        } else {
+            q.method = MTL4G_CG;
+            q.pc = prm.pc_kind;
+        }
+        if ((q.method != MTL4G_CG && q.method != MTL4G_BICGSTAB) ||
+            (q.pc != MTL4G_PC_IDENTITY && q.pc != MTL4G_PC_DIAGONAL && q.pc != MTL4G_PC_ILU0)) {
+            set_error("request %d: solver code %d (method %d, preconditioner %d) not served", r,
+                      solver ? solver[r] : -1, q.method, q.pc);
+            return MTL4G_ERR_INVALID;
+        }
+        q.stride = Arena::round(q.n) * 2 +
+                   (q.big ? 0 : Arena::round(batch_ws_elems(q.n, MTL4G_F64, q.method, q.pc)));
+        q.elems = q.stride * q.nrhs;
+        if (q.elems > cap_elems) {
             set_error("request %d needs %lld bytes, above the cap", r, (long long)(q.elems * 8));
             return MTL4G_ERR_INVALID;
         }
         iter_off[r + 1] = iter_off[r] + q.nrhs;
     }
+    // the arena: admission never holds more than the cap, nor more than the
+    // whole trace at once
+    {
+        int64_t total = 0;
+        for (const Req& q : reqs) total += Arena::round(q.elems);
+        const int64_t want = std::max<int64_t>(64, std::min(cap_elems, total));
+        if ((int64_t)S_->arena.n < want) {
+            S_->arena.free();
+            M4_TRY(S_->arena.alloc((size_t)want));
+        }
+    }
+    Arena arena;
+    arena.init(std::min<int64_t>((int64_t)S_->arena.n, cap_elems));
+    double* base = S_->arena.p;
     std::vector<int32_t> order(nreq);
     for (int32_t r = 0; r < nreq; r++) order[r] = r;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return reqs[a].arrival < reqs[b].arrival; });
@@ -292,8 +327,11 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
             }
             const Req& q = reqs[id];
             const int64_t pn = Arena::round(q.n);
+            mtl4g_pc_s* P = q.pc == MTL4G_PC_DIAGONAL ? S_->big_Pd[q.mat]
+                            : q.pc == MTL4G_PC_ILU0    ? S_->big_Pi[q.mat]
+                                                       : nullptr;
             for (int32_t k = 0; k < q.nrhs; k++) {
-                GenItem g{q.off + k * 2 * pn, q.off + k * 2 * pn + pn, q.n, 0, q.seed * 131 + (uint64_t)k};
+                GenItem g{q.off + k * q.stride, q.off + k * q.stride + pn, q.n, 0, q.seed * 131 + (uint64_t)k};
                 if (hipMemcpyAsync(dg.p, &g, sizeof g, hipMemcpyHostToDevice, bs)) {
                     worker_err = MTL4G_ERR_HIP;
                     break;
@@ -310,13 +348,15 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
                 it.norm_r0 = nb;
                 int st;
                 if (prm.precision == MTL4G_F64) {
-                    st = krylov_run<double>(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, MTL4G_F64, W64, xx, bb, &it,
-                                            nullptr, 0, bs);
+                    st = krylov_run<double>(S_->big_A[q.mat], P, q.method, MTL4G_F64, W64, xx, bb, &it, nullptr, 0, bs);
                 } else {
-                    if (x32.n < (size_t)q.n && x32.alloc(q.n)) { worker_err = MTL4G_ERR_OOM; break; }
+                    if (q.method == MTL4G_CG && x32.n < (size_t)q.n && x32.alloc(q.n)) {
+                        worker_err = MTL4G_ERR_OOM;
+                        break;
+                    }
                     int32_t rf = 0;
-                    st = mixed_run(S_->big_A[q.mat], S_->big_P[q.mat], MTL4G_CG, prm.precision, 0.1, 0, W32, W64, x32.p,
-                                   xx, bb, &it, &rf, bs);
+                    st = mixed_run(S_->big_A[q.mat], P, q.method, prm.precision, 0.1, 0, W32, W64, x32.p, xx, bb, &it,
+                                   &rf, bs);
                 }
                 if (st != 0) big_errors++;
                 big_solves++;
@@ -344,9 +384,8 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
         if (x_out && x_off) {
             const Req& q = reqs[id];
             const int64_t pn = Arena::round(q.n);
-            const int64_t stride = q.big ? 2 * pn : 2 * pn + Arena::round(batch_ws_elems(q.n, MTL4G_F64));
             for (int32_t k = 0; k < q.nrhs; k++)
-                (void)hipMemcpy(x_out + x_off[id] + (int64_t)k * q.n, base + q.off + k * stride + pn,
+                (void)hipMemcpy(x_out + x_off[id] + (int64_t)k * q.n, base + q.off + k * q.stride + pn,
                                 (size_t)q.n * 8, hipMemcpyDeviceToHost);
         }
         arena.release(reqs[id].off, reqs[id].elems);
@@ -421,19 +460,23 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
         for (auto& slp : S_->slots) {
             Slot& sl = *slp;
             if (sl.busy || admitted.empty()) continue;
+            // one solver per launch: the oldest admitted request's, then every
+            // admitted request of that solver in order (the others keep theirs)
+            const int32_t solv = reqs[admitted.front()].solver();
+            const int method = solv >> 8, pc_kind = solv & 0xff;
             int nit = 0;
-            while (!admitted.empty() && nit + reqs[admitted.front()].nrhs <= prm.max_items &&
-                   !(prm.per_request && nit > 0)) {
-                const Req& q = reqs[admitted.front()];
+            for (auto ita = admitted.begin(); ita != admitted.end();) {
+                const Req& q = reqs[*ita];
+                if (q.solver() != solv) { ++ita; continue; }
+                if (nit + q.nrhs > prm.max_items || (prm.per_request && nit > 0)) break;
                 const int64_t pn = Arena::round(q.n);
-                const int64_t stride = 2 * pn + Arena::round(batch_ws_elems(q.n, MTL4G_F64));
                 for (int32_t k = 0; k < q.nrhs; k++, nit++) {
-                    const int64_t o = q.off + k * stride;
+                    const int64_t o = q.off + k * q.stride;
                     sl.h_gen[nit] = GenItem{o, o + pn, q.n, 0, q.seed * 131 + (uint64_t)k};
                     sl.h_items[nit] = mtl4g_batch_item{q.mat, prm.max_iter, o, o + pn, o + 2 * pn, prm.rtol, 0.0};
                 }
                 sl.reqs.push_back(q.id);
-                admitted.pop_front();
+                ita = admitted.erase(ita);
             }
             sl.nitems = nit;
             hipStream_t s = sl.s;
@@ -443,7 +486,7 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
                 st = MTL4G_ERR_HIP;
             if (!st) {
                 hipLaunchKernelGGL(k_gen_items, dim3(std::min(nit, 4096)), dim3(256), 0, s, sl.d_gen.p, nit, base);
-                st = batch_launch(B, prm.pc_kind, prm.precision, B->shape[prm.precision == MTL4G_F64 ? 0 : prm.precision == MTL4G_F32 ? 1 : 2],
+                st = batch_launch(B, method, pc_kind, prm.precision, B->shape[prm.precision == MTL4G_F64 ? 0 : prm.precision == MTL4G_F32 ? 1 : 2],
                                   sl.d_items.p, nit, base, base, base, sl.d_res.p, sl.d_queue.p, s);
             }
             if (!st && (hipMemcpyAsync(sl.h_res, sl.d_res.p, nit * sizeof(mtl4g_batch_result), hipMemcpyDeviceToHost, s) ||

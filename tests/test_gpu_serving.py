@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from mtl4_amd import generators as gen
+import orc
 from test_gpu_batched import make_systems
 
 pytestmark = pytest.mark.gpu
@@ -128,3 +129,76 @@ def test_serving_concurrent_big_workers(m, prec):
     for r in (0, 13, 27, 39):
         _, _, x1, _ = S.run(mat[r:r + 1], nrhs[r:r + 1], np.zeros(1), seeds[r:r + 1], want_x=True)
         assert np.array_equal(x1[0], xs[r]), r
+
+
+SOLVERS = [("cg", "identity"), ("cg", "diagonal"), ("cg", "ilu_0"),
+           ("bicgstab", "identity"), ("bicgstab", "diagonal"), ("bicgstab", "ilu_0")]
+
+
+def _codes(m, names):
+    meth = {"cg": m.CG, "bicgstab": m.BICGSTAB}
+    pcs = {"identity": m._lib.PC_IDENTITY, "diagonal": m._lib.PC_DIAGONAL, "ilu_0": m._lib.PC_ILU0}
+    return np.array([m.Server.solver(meth[a], pcs[b]) for a, b in names], np.int32)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_serving_solver_mix(m, setup, prec):
+    """Requests of every solver (cg / bicgstab with identity, diagonal or
+    ilu_0) arrive mixed: each is solved to the FP64 tolerance, on the
+    batched kernel or (the big system) the whole-GPU engine, and a request's
+    result is the same bits alone and in the mix."""
+    systems, B = setup
+    S = m.Server(B, precision=m.F32 if prec == "f32" else m.F64, rtol=1e-10, big_rows=5000,
+                 streams=4, max_items=128, cap_bytes=2e9)
+    mat, nrhs, arr, seeds = trace(B.nsys, 90, 6000.0, 4)
+    rng = np.random.default_rng(5)
+    names = [SOLVERS[i] for i in rng.integers(0, len(SOLVERS), len(mat))]
+    mat[3] = B.nsys - 1                      # big system with bicgstab + ilu_0
+    names[3] = ("bicgstab", "ilu_0")
+    mat[8] = B.nsys - 1                      # and with cg + ilu_0
+    names[8] = ("cg", "ilu_0")
+    codes = _codes(m, names)
+    print("serving solver mix %s: %d requests" % (prec, len(mat)), flush=True)
+    lat, its, xs, st = S.run(mat, nrhs, arr, seeds, want_x=True, solver=codes)
+    assert st["errors"] == 0 and st["solves"] == nrhs.sum(), st
+    for r in range(len(mat)):
+        rp, ci, va = systems[mat[r]]
+        for k in range(nrhs[r]):
+            b = splitmix_uniform(int(seeds[r]) * 131 + k, len(rp) - 1)
+            res = np.linalg.norm(b - gen.csr_matvec(rp, ci, va, xs[r][k])) / np.linalg.norm(b)
+            assert res <= 1.0e-10 * 1.01, (r, k, names[r], res)
+    picks = [3, 8] + [r for r in range(len(mat)) if names[r] == ("bicgstab", "ilu_0")][:3] + \
+            [r for r in range(len(mat)) if names[r] == ("cg", "ilu_0")][:2] + [0]
+    for r in picks:
+        _, _, x1, _ = S.run(mat[r:r + 1], nrhs[r:r + 1], np.zeros(1), seeds[r:r + 1], want_x=True,
+                            solver=codes[r:r + 1])
+        assert np.array_equal(x1[0], xs[r]), (r, names[r])
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_serving_fp64_ilu0_matches_oracle(m, setup, method):
+    """FP64 requests with ilu_0 on the batched kernel (one workgroup per
+    system: MTL4's ILU(0) factors from the device factorization, level-
+    scheduled sweeps inside the workgroup) agree with the oracle's solve of
+    the same system and right-hand side: iteration count within 2, solution
+    within what the residual tolerance allows."""
+    systems, B = setup
+    S = m.Server(B, precision=m.F64, rtol=1e-10, big_rows=10 ** 9, streams=2, max_items=64,
+                 cap_bytes=2e9)
+    rng = np.random.default_rng(6)
+    idx = rng.choice(B.nsys - 1, 8, replace=False)
+    n = len(idx)
+    seeds = rng.integers(1, 2 ** 62, n, dtype=np.uint64)
+    code = _codes(m, [(method, "ilu_0")])[0]
+    lat, its, xs, st = S.run(idx, np.ones(n, np.int32), np.zeros(n), seeds, want_x=True,
+                             solver=np.full(n, code, np.int32))
+    assert st["errors"] == 0
+    for k, r in enumerate(idx):
+        rp, ci, va = systems[r]
+        O = orc.Matrix(rp, ci, va)
+        b = splitmix_uniform(int(seeds[k]) * 131, len(rp) - 1)
+        fn = orc.cg if method == "cg" else orc.bicgstab
+        xo, sto, ito, _ = fn(O, b, orc.Preconditioner("ilu_0", O), max_iter=20000, rtol=1e-10)
+        assert sto == 0
+        assert abs(int(its[k]) - ito.i) <= 2, (r, its[k], ito.i)
+        assert np.linalg.norm(xs[k][0] - xo) <= 1e-6 * np.linalg.norm(xo), r
