@@ -174,20 +174,59 @@ def load_traffic(key):
         return None
 
 
-def cpu_baseline(rp, ci, va, b, iters_gpu, cpu_iters):
-    """The CPU oracle's CG (single thread), timed on cpu_iters iterations of
-    the same solve and scaled to the GPU's iteration count."""
+def cpu_model():
+    """The host CPU's model name and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, avail
+
+
+def cpu_baseline(rp, ci, va, b, iters_gpu, cpu_iters, max_solve_s=25.0):
+    """The CPU oracle's cg + diagonal on the same system: MTL4's OpenMP build
+    restated (mat_vec_mult.hpp:648-674, dot.hpp:112-134, reduction.hpp,
+    vec_vec_aop_expr.hpp) on the host cores OpenMP is given (OMP_NUM_THREADS),
+    full solves after a 20-iteration warm-up (the median of up to 5 while
+    they fit max_solve_s in total); and MTL4's default single-thread build,
+    timed on cpu_iters iterations and scaled to the GPU's iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     O = orc.Matrix(rp, ci, va)
+    model, avail = cpu_model()
+    threads = orc.omp_threads()
+    orc.cg_diag_omp(O, b, threads=threads, max_iter=20, rtol=1e-10)   # warm-up (page-in, thread team)
+    times, its = [], []
+    t_all = time.perf_counter()
+    while len(times) < 5:
+        t0 = time.perf_counter()
+        _, st, it = orc.cg_diag_omp(O, b, threads=threads, max_iter=4 * iters_gpu, rtol=1e-10)
+        times.append(time.perf_counter() - t0)
+        its.append(it.i)
+        if time.perf_counter() - t_all + times[-1] > max_solve_s:
+            break
+    t_omp = float(np.median(times))
+    # single thread, MTL4's default build
     P = orc.Preconditioner("diagonal", O)
     t0 = time.perf_counter()
     orc.cg(O, b, P, max_iter=cpu_iters, rtol=1e-10)
     dt = time.perf_counter() - t0
     per_iter = dt / cpu_iters
-    return {"value": 1.0 / (per_iter * iters_gpu), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "%d CG+diagonal iterations of the 160^3 solve (%.1f s, %.1f ms/iteration), "
-                      "scaled to the GPU's %d iterations" % (cpu_iters, dt, per_iter * 1e3, iters_gpu)}
+    return {"value": round(1.0 / t_omp, 5), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": "%d full CG+diagonal solves of the 160^3 system (%d iterations, median %.2f s) with MTL4's "
+                      "OpenMP passes on %d threads" % (len(times), its[-1], t_omp, threads),
+            "cpu_model": model, "cores_available": avail,
+            "single_thread": {"value": round(1.0 / (per_iter * iters_gpu), 5), "unit": "solves/s", "cores": 1,
+                              "sample": "%d CG+diagonal iterations (%.1f s, %.1f ms/iteration), scaled to the "
+                                        "GPU's %d iterations" % (cpu_iters, dt, per_iter * 1e3, iters_gpu)}}
 
 
 T_START = time.perf_counter()
@@ -241,7 +280,7 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
     t64 = time.perf_counter() - t0
     log("  FP64 solve: %d iterations, %.3f s" % (it.iterations(), t64))
     xm = m.dense_vector(n)
-    # the FP32 recurrence may stagnate or diverge: at most 2x the FP64 iteration count
+    # a recurrence that stagnates stops at 2x the FP64 iteration count
     itm = m.basic_iteration(bnorm, 2 * max(it.iterations(), 10), 1e-10)
     t0 = time.perf_counter()
     failure = None
@@ -251,14 +290,36 @@ def pair_section(m, torch, A, b, bnorm, pc, method, x_ref=None):
         errm, st, failure = -1, {"inner_total": itm.iterations(), "outer": -1}, "%s: %s" % (type(e).__name__, e)
     torch.cuda.synchronize()
     tm = time.perf_counter() - t0
-    log("  mixed solve: %d FP32 iterations, error %d, %.3f s" % (st["inner_total"], errm, tm))
+    log("  mixed solve: %d iterations, error %d, %.3f s" % (st["inner_total"], errm, tm))
     ref = x.t if x_ref is None else x_ref
     rel_err = float(torch.linalg.vector_norm(xm.t - ref) / torch.linalg.vector_norm(ref))
+    # BF16-stored preconditioner values (FP32 matrix values), same refinement
+    P16 = getattr(m.pc, pc)(A, m.BF16)
+    xb = m.dense_vector(n)
+    itb = m.basic_iteration(bnorm, 2 * max(it.iterations(), 10), 1e-10)
+    t0 = time.perf_counter()
+    fail16 = None
+    try:
+        errb, stb = m.ir_solve(A, xb, b, itb, P16, method=method, value_format=m.F32, mode="reliable")
+    except m.MtlError as e:
+        errb, stb, fail16 = -1, {"inner_total": itb.iterations(), "outer": -1}, "%s: %s" % (type(e).__name__, e)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    log("  mixed solve, BF16 preconditioner: %d iterations, error %d, %.3f s" % (stb["inner_total"], errb, tb))
+    rel_err_b = float(torch.linalg.vector_norm(xb.t - ref) / torch.linalg.vector_norm(ref))
+    rec = ("FP32 recurrence on FP32 values" if method == m.CG else
+           "FP64 vectors on FP32 values") + ", FP32 preconditioner values, FP64 refresh at 0.1x"
     return {"fp64": {"solves_per_s": round(1 / t64, 4), "iterations": it.iterations(), "error_code": err,
                      "relres": relres_of(m, A, x, b, bnorm)},
-            "mixed_f32": {"solves_per_s": round(1 / tm, 4), "fp32_iterations": st["inner_total"],
+            "mixed_f32": {"solves_per_s": round(1 / tm, 4), "iterations": st["inner_total"],
                           "fp64_refreshes": st["outer"], "error_code": errm, "failure": failure,
-                          "relres": relres_of(m, A, xm, b, bnorm), "rel_err_vs_fp64": rel_err},
+                          "relres": relres_of(m, A, xm, b, bnorm), "rel_err_vs_fp64": rel_err,
+                          "recurrence": rec},
+            "mixed_bf16_pc": {"solves_per_s": round(1 / tb, 4), "iterations": stb["inner_total"],
+                              "fp64_refreshes": stb["outer"], "error_code": errb, "failure": fail16,
+                              "relres": relres_of(m, A, xb, b, bnorm), "rel_err_vs_fp64": rel_err_b,
+                              "speedup": round(t64 / tb, 3),
+                              "recurrence": rec.replace("FP32 preconditioner", "BF16 preconditioner")},
             "speedup": round(t64 / tm, 3), "steps": 1,
             "pc_setup_s": round(t_setup, 4), "pc_refactor_s": round(t_refactor, 5)}
 
@@ -385,13 +446,18 @@ def batched_section(m, torch, gen, count):
     return out
 
 
-def serving_section(m, torch, gen, nreq):
+def serving_section(m, torch, gen, nreq, sweep_requests=400):
     """Continuous batching over 512 resident matrices (the batched families
     plus 3D stencils up to 96^3), Poisson arrivals, 1-8 right-hand sides per
-    request, mixed precision to 1e-10 (systems above 16384 rows on 8
-    whole-GPU workers, the rest continuously batched on 8 streams), against
-    plain FP64 per-request launches on one stream (on the first quarter of
-    each trace)."""
+    request, each request itl::cg or itl::bicgstab with pc::diagonal or
+    pc::ilu_0 (uniformly), mixed precision refined to 1e-10: systems above
+    65536 rows (the larger 3D stencils) on 8 whole-GPU workers, the rest
+    continuously batched on 8 streams.  Against plain FP64 per-request
+    launches on one stream (on the first quarter of each trace).  The fixed
+    rates 2k, 8k and 20k req/s are above this capacity (their latencies are
+    queueing time); a sweep of lower rates finds the highest one whose
+    queue drains as requests arrive (sustained: the last completion within
+    max(0.25 s, 10 %) of the trace's duration after the last arrival)."""
     log("serving: generating 512 matrices")
     systems = batch_systems(gen, 480, seed=1)
     rng = np.random.default_rng(3)
@@ -399,41 +465,101 @@ def serving_section(m, torch, gen, nreq):
         a = int(round(np.exp(rng.uniform(np.log(32), np.log(96)))))
         systems.append(gen.laplacian3d(a, a, a))
     B = m.Batch(systems)
-    out = {"matrices": len(systems), "requests_per_rate": nreq, "cap_bytes": 200e9}
-    rates = (2000.0, 8000.0, 20000.0)
-    traces = {}
-    for rate in rates:
+    big_rows = 65536
+    sizes = np.diff(B.sys_row)
+    out = {"matrices": len(systems), "requests_per_rate": nreq, "cap_bytes": 200e9,
+           "solvers": "cg | bicgstab  x  diagonal | ilu_0, uniform per request",
+           "big_rows": big_rows, "whole_gpu_matrices": int((sizes > big_rows).sum())}
+    codes = [m.Server.solver(meth, pc) for meth in (m.CG, m.BICGSTAB)
+             for pc in (m._lib.PC_DIAGONAL, m._lib.PC_ILU0)]
+
+    def make_trace(rate, count):
         rg = np.random.default_rng(int(rate))
-        traces[rate] = (rg.integers(0, len(systems), nreq), rg.integers(1, 9, nreq),
-                        np.cumsum(rg.exponential(1.0 / rate, nreq)),
-                        rg.integers(1, 2 ** 62, nreq, dtype=np.uint64))
+        return (rg.integers(0, len(systems), count), rg.integers(1, 9, count),
+                np.cumsum(rg.exponential(1.0 / rate, count)),
+                rg.integers(1, 2 ** 62, count, dtype=np.uint64),
+                np.array(codes, np.int32)[rg.integers(0, len(codes), count)])
+
+    def record(lat, arr, st):
+        drain = st["wall_s"] - float(arr[-1])
+        return {"p50_ms": round(1000 * float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(1000 * float(np.percentile(lat, 99)), 3),
+                "solves_per_s": round(st["solves"] / st["wall_s"], 1),
+                "wall_s": round(st["wall_s"], 3), "last_arrival_s": round(float(arr[-1]), 3),
+                "drain_s": round(drain, 3),
+                "sustained": bool(drain <= max(0.25, 0.1 * float(arr[-1]))),
+                "launches": st["launches"], "max_inflight": st["max_inflight"], "errors": st["errors"],
+                "big_solves": st["big_solves"], "peak_bytes": st["peak_bytes"]}
+
+    rates = (2000.0, 8000.0, 20000.0)
+    traces = {rate: make_trace(rate, nreq) for rate in rates}
+    for rate in rates:
         out["%dk_req_s" % int(rate / 1000)] = {}
-    # the plain baseline replays the first quarter of each trace (it is
-    # saturated at every rate; a shorter queue only flatters its latencies)
     nbase = max(1, nreq // 4)
     out["baseline_requests_per_rate"] = nbase
     for name, kw in (("mixed", dict(precision=m.F32, streams=8, max_items=1024)),
                      ("fp64_per_request", dict(precision=m.F64, streams=1, max_items=8,
                                                 per_request=True))):
-        Sv = m.Server(B, rtol=1e-10, big_rows=16384, cap_bytes=200e9, **kw)
+        log("serving: %s server" % name)
+        Sv = m.Server(B, rtol=1e-10, big_rows=big_rows, cap_bytes=200e9, **kw)
         for rate in rates:
             log("serving: %s at %d req/s" % (name, rate))
-            mat, nrhs, arr, seeds = traces[rate]
+            mat, nrhs, arr, seeds, sv = traces[rate]
             if name == "fp64_per_request":
-                mat, nrhs, arr, seeds = mat[:nbase], nrhs[:nbase], arr[:nbase], seeds[:nbase]
-            lat, its, _, st = Sv.run(mat, nrhs, arr, seeds)
-            out["%dk_req_s" % int(rate / 1000)][name] = {
-                "p50_ms": round(1000 * float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(1000 * float(np.percentile(lat, 99)), 3),
-                "solves_per_s": round(st["solves"] / st["wall_s"], 1),
-                "wall_s": round(st["wall_s"], 3), "launches": st["launches"],
-                "max_inflight": st["max_inflight"], "errors": st["errors"],
-                "big_solves": st["big_solves"], "peak_bytes": st["peak_bytes"]}
+                mat, nrhs, arr, seeds, sv = mat[:nbase], nrhs[:nbase], arr[:nbase], seeds[:nbase], sv[:nbase]
+            lat, its, _, st = Sv.run(mat, nrhs, arr, seeds, solver=sv)
+            out["%dk_req_s" % int(rate / 1000)][name] = record(lat, arr, st)
+        if name == "mixed":
+            sweep = {}
+            best = None
+            for rate in (125.0, 250.0, 500.0, 1000.0):
+                log("serving: mixed sweep at %d req/s" % rate)
+                mat, nrhs, arr, seeds, sv = make_trace(rate, sweep_requests)
+                lat, its, _, st = Sv.run(mat, nrhs, arr, seeds, solver=sv)
+                sweep["%d_req_s" % int(rate)] = rec = record(lat, arr, st)
+                if rec["sustained"] and rec["errors"] == 0:
+                    best = rate
+            out["mixed_rate_sweep"] = sweep
+            out["mixed_sustained_req_s"] = best
         del Sv
     for rate in rates:
         row = out["%dk_req_s" % int(rate / 1000)]
         row["speedup_solves_per_s"] = round(row["mixed"]["solves_per_s"] /
                                             row["fp64_per_request"]["solves_per_s"], 3)
+    return out
+
+
+def sweeps_only(m, torch, gen, steps):
+    """solve(P, x) of IC(0) on diffusion3d 160^3 and of ILU(0) on the 1024^2
+    convection-diffusion matrix, `steps` times each after one warm-up: time
+    per solve (two sweeps), per level, and the sweeps' algorithmic bytes
+    (per sweep nnz_tri*(8+4) + (n+1)*4 + 3*n*8, section 8(d) of the plan)."""
+    out = {}
+    for name, mk, kind in (("ic0_diffusion3d_160", lambda: gen.diffusion3d(160, seed=0), "ic_0"),
+                           ("ilu0_convdiff_1024", lambda: gen.convection_diffusion2d(1024, peclet=10.0, seed=0),
+                            "ilu_0")):
+        log("sweeps: %s" % name)
+        rp, ci, va = mk()
+        n = len(rp) - 1
+        A = m.compressed2D(n, n, rp, ci, va)
+        P = getattr(m.pc, kind)(A)
+        lo, up = P.levels()
+        nnz_l = len(P.get_L()[1])
+        nnz_u = len(P.get_U()[1])
+        x = m.dense_vector(np.random.default_rng(0).uniform(-1.0, 1.0, n))
+        y = m.dense_vector(n)
+        P.solve(x, y)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            P.solve(x, y)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        by = (nnz_l + nnz_u) * 12 + 2 * ((n + 1) * 4 + 3 * n * 8)
+        out[name] = {"n": n, "levels": [lo, up], "ms_per_solve": round(1e3 * dt, 4),
+                     "us_per_level": round(1e6 * dt / (lo + up), 3), "bytes_per_solve": by,
+                     "GBs": round(by / dt / 1e9, 1)}
+        del P, A
     return out
 
 
@@ -445,10 +571,15 @@ def main():
     ap.add_argument("--grid", type=int, default=160)
     ap.add_argument("--cpu-iters", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0,
+                    help="time budget of the all-core CPU baseline's full solves")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--only-spmv", action="store_true", help="SpMV timing only (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="with --only-spmv: also copy 512 MiB on the device (PMC calibration)")
+    ap.add_argument("--only-sweeps", action="store_true",
+                    help="triangular sweep timing only (profiling runs): IC(0) of diffusion3d 160^3 and "
+                         "ILU(0) of convection-diffusion 1024^2, --steps solves each")
     ap.add_argument("--sections", default="all",
                     help="extra measurements: all, none, or a list of ic0,c3,solvers,batched,serving")
     ap.add_argument("--kernel", default="",
@@ -456,7 +587,7 @@ def main():
                          "dia_sym|dia|sliced/<entries per chunk 4|8>/<workgroups per CU> "
                          "(profiling runs, where the profiler distorts the autotuner's timings)")
     ap.add_argument("--batch-systems", type=int, default=4096)
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -467,6 +598,9 @@ def main():
     import mtl4_amd as m
     from mtl4_amd import generators as gen
     torch.cuda.set_device(0)
+    if args.only_sweeps:
+        print(json.dumps(sweeps_only(m, torch, gen, args.steps)))
+        return
 
     g = args.grid
     log("diffusion3d %d^3: generating" % g)
@@ -638,7 +772,7 @@ def main():
     cpu = None
     log("CPU baseline")
     if not args.no_cpu:
-        cpu = cpu_baseline(rp, ci, va, b.numpy(), iters, args.cpu_iters)
+        cpu = cpu_baseline(rp, ci, va, b.numpy(), iters, args.cpu_iters, args.cpu_seconds)
 
     sections = (set(args.sections.split(",")) if args.sections != "all"
                 else {"ic0", "c3", "solvers", "batched", "serving"})

@@ -127,6 +127,9 @@ __device__ __forceinline__ void bilu(const BatchDev& M, int sys, int r0, const d
 
 // A recurrence residual this far above the last FP64 residual is refreshed.
 constexpr double kBatchGrowth = 10.0;
+// BiCGSTAB restarts at each refresh: it refreshes after its residual fell by
+// this factor (CG: delta).
+constexpr double kBatchBicgDelta = 1e-5;
 
 // ---------------------------------------------------------------- FP64 vectors
 // itl::cg (cg.hpp:74-111) with FP64 vectors.  MT = double: the FP64 solve.
@@ -249,14 +252,16 @@ __global__ __launch_bounds__(BS) void k_batch_cg64(BatchDev M, const MT* __restr
 }
 
 // itl::bicgstab (bicgstab.hpp:27-78) with FP64 vectors, including the
-// half-step exit and the breakdown tests; MT as in k_batch_cg64 (a refresh
-// after a half-step exit restarts p).
+// half-step exit and the breakdown tests; MT as in k_batch_cg64, except that
+// every refresh restarts the recurrence (rtilde = r, p = r) and is taken
+// after a fall of the residual by kBatchBicgDelta.
 template <int BS, int PCM, typename MT>
 __global__ __launch_bounds__(BS) void k_batch_bicg64(BatchDev M, const MT* __restrict__ aval,
                                                      const mtl4g_batch_item* __restrict__ items, int nitems,
                                                      unsigned* queue, const double* __restrict__ B, double* X,
                                                      double* W, mtl4g_batch_result* out, double delta) {
     constexpr bool LOW = !std::is_same<MT, double>::value;
+    (void)delta;   // BiCGSTAB refreshes at kBatchBicgDelta
     __shared__ double sh[2 * (BS / kWave)];
     __shared__ int s_k;
     for (;;) {
@@ -310,13 +315,15 @@ __global__ __launch_bounds__(BS) void k_batch_bicg64(BatchDev M, const MT* __res
                 const bool est_stop = !isfinite(resid) || bconv(resid, norm_r0, it.rtol, it.atol) ||
                                       iters >= it.max_iter;
                 const bool grown = resid > kBatchGrowth * rmax;
-                if (est_stop || grown || resid <= delta * rmax) {   // FP64 refresh
-                    restart = restart || !isfinite(resid) || grown;
+                if (est_stop || grown || resid <= kBatchBicgDelta * rmax) {
+                    // FP64 refresh; the recurrence restarts from it (rtilde = r, p = r)
+                    restart = true;
                     rr = 0;
                     double tmp = 0;
                     for (int i = threadIdx.x; i < n; i += BS) {
                         const double vv = b[i] - brow<double>(M, M.val, r0 + i, r0, x);
                         r[i] = vv;
+                        rt[i] = vv;
                         rr += vv * vv;
                     }
                     block_sum2<BS>(rr, tmp, sh);

@@ -59,6 +59,8 @@ __device__ __forceinline__ bool conv_test(const Ctl* c, double resid) {
 // A recurrence residual this far above the last FP64 residual is refreshed
 // at once instead of iterating on.
 constexpr double kGrowth = 10.0;
+// Mixed-precision BiCGSTAB refreshes when its residual fell by this factor.
+constexpr double kBicgDelta = 1e-5;
 
 // Mixed precision, loop head on the recurrence residual: it only estimates
 // convergence, so an estimated stop (or the iteration limit) pauses the
@@ -1644,8 +1646,11 @@ struct EpiRu2 {
 };
 
 // FP64 vectors on low-precision values (BiCGSTAB): r = b - A*x with the FP64
-// values replaces the recurrence's r; the first refresh also sets
-// rtilde = r (bicgstab.hpp:37-38); ||r||^2 decides in ru_finish
+// values replaces the recurrence's r and the recurrence restarts from it,
+// rtilde = r and p = r (bicgstab.hpp:37-38, 48-49): a replaced residual
+// that the directions p, v no longer match makes BiCGSTAB erratic (a
+// diverging solve was seen on a 3D Laplacian with exactly representable
+// values); ||r||^2 decides in ru_finish
 struct EpiRu64 {
     static constexpr int NP = 1;
     struct Pre {
@@ -1654,17 +1659,19 @@ struct EpiRu64 {
     const double* b;
     double *r, *rt;
     Ctl* c;
-    int init;
     __device__ bool skip() const { return c->done != 0 || c->want_ru == 0; }
-    __device__ void prepare() { init = c->init; }
+    __device__ void prepare() {}
     __device__ Pre preload(int row) const { return Pre{b[row]}; }
     __device__ void operator()(int row, double s, const Pre& pre, double* part) const {
         const double v = pre.b - s;
         r[row] = v;
-        if (init) rt[row] = v;
+        rt[row] = v;
         part[0] += v * v;
     }
-    __device__ void finish(const double* tot) const { ru_finish(c, tot[0], 0.0, false); }
+    __device__ void finish(const double* tot) const {
+        ru_finish(c, tot[0], 0.0, false);
+        if (!c->done) c->first = 1;
+    }
 };
 
 __global__ void k_ctl_init(Ctl* c, Ctl init) { *c = init; }
@@ -2532,7 +2539,7 @@ static int mixed_bicg_typed(mtl4g_matrix_s* A, const MT* lowp, const mtl4g_pc_s*
     double* r = W.vecs.p + 6 * n;
     double* rt = W.vecs.p + 7 * n;
     auto refresh = [&]() -> int {
-        return launch_csr<double>(A, A->val.p, x, EpiRu64{b, r, rt, c, 0}, W.partials.p, W.tickets.p, s,
+        return launch_csr<double>(A, A->val.p, x, EpiRu64{b, r, rt, c}, W.partials.p, W.tickets.p, s,
                                   (double*)nullptr);
     };
     M4_TRY(refresh());   // initial FP64 residual (Ctl::init set), rtilde = r
@@ -2585,7 +2592,9 @@ int mixed_run(mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt, doubl
     init.mixed = 1;
     init.init = 1;
     init.want_ru = 1;
-    init.delta = delta;
+    // BiCGSTAB restarts at each refresh, so it refreshes only after a large
+    // fall of its residual (or an estimated stop, growth, the limit)
+    init.delta = bicg ? kBicgDelta : delta;
     hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, init);
     M4_HIP(hipGetLastError());
     // a safety net against a device loop that never stops: every counted

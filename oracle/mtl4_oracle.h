@@ -96,6 +96,12 @@ int orc_bicgstab_ell(const orc_csr* A, const orc_pc* L, const orc_pc* R, double*
 int orc_gmres(const orc_csr* A, const orc_pc* L, const orc_pc* R, double* x, const double* b,
               int32_t restart, orc_iter* it, double* hist, int32_t hist_cap);
 
+/* itl::cg + pc::diagonal as MTL4 runs it with MTL_WITH_OPENMP (all-core
+ * CPU baseline; mtl4_oracle_omp.c).  threads > 0 sets the team size. */
+int orc_cg_diag_omp(const orc_csr* A, const double* inv_diag, double* x, const double* b,
+                    orc_iter* it, int threads);
+int orc_omp_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

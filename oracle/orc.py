@@ -81,6 +81,10 @@ def lib():
         L.orc_upper_invdiag_first.argtypes = [P, P, P]
         L.orc_cg.restype = C.c_int
         L.orc_cg.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
+        L.orc_cg_diag_omp.restype = C.c_int
+        L.orc_cg_diag_omp.argtypes = [P, P, P, P, C.POINTER(Iter), C.c_int]
+        L.orc_omp_threads.restype = C.c_int
+        L.orc_omp_threads.argtypes = []
         L.orc_bicgstab.restype = C.c_int
         L.orc_bicgstab.argtypes = [P, P, P, P, C.POINTER(Iter), P, I32]
         L.orc_cgs.restype = C.c_int
@@ -245,6 +249,22 @@ def cg(A, b, P=None, x0=None, it=None, hist_cap=0, **kw):
     it = iteration(b, **kw) if it is None else it
     x, st, hist = _solve(lib().orc_cg, A, b, P, x0, it, hist_cap)
     return x, st, it, hist
+
+
+def cg_diag_omp(A, b, threads=0, x0=None, it=None, **kw):
+    """itl::cg + pc::diagonal as MTL4 runs it built with MTL_WITH_OPENMP
+    (mtl4_oracle_omp.c); threads > 0 sets the OpenMP team size.
+    Returns (x, status, iter)."""
+    it = iteration(b, **kw) if it is None else it
+    b = _f64(b)
+    x = np.zeros(A.n) if x0 is None else _f64(x0).copy()
+    inv = Preconditioner("diagonal", A).factor("inv_diag")
+    st = lib().orc_cg_diag_omp(A.ref(), _p(inv), _p(x), _p(b), C.byref(it), threads)
+    return x, st, it
+
+
+def omp_threads():
+    return lib().orc_omp_threads()
 
 
 def bicgstab(A, b, P=None, x0=None, it=None, hist_cap=0, **kw):

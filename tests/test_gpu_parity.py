@@ -1153,8 +1153,9 @@ def test_mixed_bf16_reports_stagnation(m):
 def test_mixed_bicgstab_ilu_large(m, m_grid, vf):
     """BiCGSTAB + ILU(0) on the upwind convection-diffusion problem at the
     size where an FP32 BiCGSTAB recurrence diverges (1024^2): the mixed solve
-    reaches the FP64 tolerance, with at most a few more iterations than the
-    FP64 solve, and agrees with it to the accuracy the residual allows.  The
+    (FP64 vectors on FP32 values, restarted at each FP64 refresh) reaches the
+    FP64 tolerance within twice the FP64 iteration count and agrees with the
+    FP64 solution to the accuracy the residual allows.  The
     BF16 variant stores the ILU(0) factors in bfloat16 (the matrix values the
     products read stay FP32)."""
     from mtl4_amd import generators as gen
@@ -1177,7 +1178,8 @@ def test_mixed_bicgstab_ilu_large(m, m_grid, vf):
     xg = x.numpy()
     res = np.linalg.norm(b - gen.csr_matvec(rp, ci, va, xg)) / np.linalg.norm(b)
     assert res <= 1.0e-10 * 1.01, res
-    assert it.iterations() <= 1.25 * it64.iterations() + 10, (it.iterations(), it64.iterations())
+    # each FP64 refresh restarts the recurrence: up to twice the FP64 count
+    assert it.iterations() <= 2 * it64.iterations() + 20, (it.iterations(), it64.iterations())
     x64h = x64.numpy()
     assert np.linalg.norm(xg - x64h) <= 1e-4 * np.linalg.norm(x64h)
 

@@ -202,3 +202,29 @@ def test_serving_fp64_ilu0_matches_oracle(m, setup, method):
         assert sto == 0
         assert abs(int(its[k]) - ito.i) <= 2, (r, its[k], ito.i)
         assert np.linalg.norm(xs[k][0] - xo) <= 1e-6 * np.linalg.norm(xo), r
+
+
+@pytest.mark.parametrize("big", [False, True])
+def test_serving_mixed_bicgstab_laplacian(m, big):
+    """Mixed-precision BiCGSTAB + diagonal on a 233 x 234 Laplacian (54,522
+    rows, values exact in FP32): a refresh that kept the old directions made
+    this solve diverge (20,000 iterations, residual 1e21); with the
+    recurrence restarted at every refresh it converges, on the batched
+    kernel and on the whole-GPU engine."""
+    systems = [gen.laplacian_setup(233, 234), gen.laplacian_setup(40, 41)]
+    B = m.Batch(systems)
+    S = m.Server(B, precision=m.F32, rtol=1e-10, big_rows=1000 if big else 10 ** 9, streams=4,
+                 max_items=64, cap_bytes=2e9)
+    n = 6
+    mat = np.zeros(n, np.int32)
+    seeds = (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(7919))
+    code = _codes(m, [("bicgstab", "diagonal")])[0]
+    lat, its, xs, st = S.run(mat, np.ones(n, np.int32), np.zeros(n), seeds, want_x=True,
+                             solver=np.full(n, code, np.int32))
+    assert st["errors"] == 0, (st, its)
+    rp, ci, va = systems[0]
+    for k in range(n):
+        b = splitmix_uniform(int(seeds[k]) * 131, len(rp) - 1)
+        res = np.linalg.norm(b - gen.csr_matvec(rp, ci, va, xs[k][0])) / np.linalg.norm(b)
+        assert res <= 1.0e-10 * 1.01, (k, res, its[k])
+        assert its[k] < 3000, its[k]

@@ -1052,3 +1052,21 @@ def test_gmres_bad_arguments():
     assert st == orc.INVALID
     _, st, _, _ = orc.gmres(orc.laplacian_setup(2, 2), np.ones(4), None, None, -1, max_iter=3, rtol=1e-8)
     assert st == orc.INVALID
+
+
+def test_cg_diag_omp_matches_sequential_cg():
+    """The OpenMP restatement of MTL4's cg + diagonal (the all-core CPU
+    baseline) solves config 1 like the sequential oracle: iteration count
+    within 2, solution within 1e-12 relative, with 1 and with 4 threads."""
+    O = orc.laplacian_setup(64, 64)
+    xt = np.random.default_rng(0).uniform(-1, 1, O.n)
+    b = orc.spmv(O, xt)
+    xs, sts, its, _ = orc.cg(O, b, orc.Preconditioner("diagonal", O), max_iter=2000, rtol=1e-10)
+    assert sts == 0
+    for th in (1, 4):
+        xo, sto, ito = orc.cg_diag_omp(O, b, threads=th, max_iter=2000, rtol=1e-10)
+        assert sto == 0 and abs(ito.i - its.i) <= 2, (th, ito.i, its.i)
+        assert np.linalg.norm(xo - xs) <= 1e-12 * np.linalg.norm(xs)
+    # the iteration limit is error 1, as in basic_iteration
+    _, st, it = orc.cg_diag_omp(O, b, threads=2, max_iter=5, rtol=1e-10)
+    assert st == 1 and it.i == 5
