@@ -531,9 +531,12 @@ def serving_section(m, torch, gen, nreq, sweep_requests=400):
 
 def sweeps_only(m, torch, gen, steps):
     """solve(P, x) of IC(0) on diffusion3d 160^3 and of ILU(0) on the 1024^2
-    convection-diffusion matrix, `steps` times each after one warm-up: time
-    per solve (two sweeps), per level, and the sweeps' algorithmic bytes
-    (per sweep nnz_tri*(8+4) + (n+1)*4 + 3*n*8, section 8(d) of the plan)."""
+    convection-diffusion matrix, `steps` times each after one warm-up, with
+    the chain pipeline (the default for these grid factors) and with the
+    level schedule: time per solve (two sweeps) and the sweeps' algorithmic
+    bytes (per sweep nnz_tri*(8+4) + (n+1)*4 + 3*n*8)."""
+    import ctypes as C
+    L = m._lib.lib()
     out = {}
     for name, mk, kind in (("ic0_diffusion3d_160", lambda: gen.diffusion3d(160, seed=0), "ic_0"),
                            ("ilu0_convdiff_1024", lambda: gen.convection_diffusion2d(1024, peclet=10.0, seed=0),
@@ -542,24 +545,39 @@ def sweeps_only(m, torch, gen, steps):
         rp, ci, va = mk()
         n = len(rp) - 1
         A = m.compressed2D(n, n, rp, ci, va)
-        P = getattr(m.pc, kind)(A)
-        lo, up = P.levels()
-        nnz_l = len(P.get_L()[1])
-        nnz_u = len(P.get_U()[1])
         x = m.dense_vector(np.random.default_rng(0).uniform(-1.0, 1.0, n))
-        y = m.dense_vector(n)
-        P.solve(x, y)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
+        row = {"n": n}
+        ys = {}
+        for sched_name, sched in (("chain", m._lib.SWEEP_AUTO), ("levels", m._lib.SWEEP_LEVELS)):
+            m._lib.check(L.mtl4g_set_sweep_schedule(sched))
+            try:
+                P = getattr(m.pc, kind)(A)
+            finally:
+                m._lib.check(L.mtl4g_set_sweep_schedule(m._lib.SWEEP_AUTO))
+            lo, up = C.c_int(), C.c_int()
+            m._lib.check(L.mtl4g_pc_sweeps(P.h, C.byref(lo), C.byref(up)))
+            nlo, nup = P.levels()
+            nnz_l = len(P.get_L()[1])
+            nnz_u = len(P.get_U()[1])
+            y = m.dense_vector(n)
             P.solve(x, y)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / steps
-        by = (nnz_l + nnz_u) * 12 + 2 * ((n + 1) * 4 + 3 * n * 8)
-        out[name] = {"n": n, "levels": [lo, up], "ms_per_solve": round(1e3 * dt, 4),
-                     "us_per_level": round(1e6 * dt / (lo + up), 3), "bytes_per_solve": by,
-                     "GBs": round(by / dt / 1e9, 1)}
-        del P, A
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                P.solve(x, y)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+            by = (nnz_l + nnz_u) * 12 + 2 * ((n + 1) * 4 + 3 * n * 8)
+            ys[sched_name] = y.numpy().copy()
+            row[sched_name] = {"schedules": [lo.value, up.value], "levels": [nlo, nup],
+                               "ms_per_solve": round(1e3 * dt, 4), "bytes_per_solve": by,
+                               "GBs": round(by / dt / 1e9, 1)}
+            del P
+        row["same_bits"] = bool(np.array_equal(ys["chain"].view(np.uint64), ys["levels"].view(np.uint64)))
+        row["speedup"] = round(row["levels"]["ms_per_solve"] / row["chain"]["ms_per_solve"], 3)
+        out[name] = row
+        log("sweeps: %s %s" % (name, json.dumps(row)))
+        del A
     return out
 
 

@@ -243,6 +243,19 @@ int mtl4g_pc_factor(mtl4g_pc P, int which, int32_t* n_out, int64_t* nnz_out,
                     int32_t* h_rowptr, int32_t* h_col, double* h_val);
 /* Number of dependency levels of the lower / upper sweep. */
 int mtl4g_pc_levels(mtl4g_pc P, int32_t* lower_levels, int32_t* upper_levels);
+/* Schedule of the ILU(0) / IC(0) triangular sweeps of preconditioners
+ * created afterwards (no MTL4 counterpart; MTL4 sweeps rows in order):
+ * MTL4G_SWEEP_AUTO (default): a factor of a 5- or 7-point grid structure
+ * (every row depending only on the rows 1, m and m*m before it, never
+ * across a grid edge) runs as a pipeline of row chains, any other factor
+ * level by level; MTL4G_SWEEP_LEVELS: every factor level by level.  Both
+ * compute every row in MTL4's subtraction order: the results are the same
+ * bits.  mtl4g_pc_sweeps reports the schedule of a preconditioner's two
+ * sweeps. */
+#define MTL4G_SWEEP_AUTO   0
+#define MTL4G_SWEEP_LEVELS 1
+int mtl4g_set_sweep_schedule(int schedule);
+int mtl4g_pc_sweeps(mtl4g_pc P, int* lower, int* upper);
 /* Factor again from A's current values (after mtl4g_csr_set_values), same
  * pattern: ILU(0)/IC(0) rerun the device factorization on `stream` with
  * the plan made at creation (levels, positions), nothing on the host; the

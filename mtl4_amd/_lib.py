@@ -40,6 +40,7 @@ class IRStats(C.Structure):
 
 SPMV_TILE, SPMV_VECTOR, SPMV_STREAM, SPMV_SCALAR, SPMV_STREAM2, SPMV_SLICED, SPMV_DIA, SPMV_DIA_SYM = 0, 1, 2, 3, 4, 5, 6, 7
 ORDER_TREE, ORDER_MTL4 = 0, 1
+SWEEP_AUTO, SWEEP_LEVELS = 0, 1
 
 
 class BatchItem(C.Structure):
@@ -101,6 +102,8 @@ _SIGS = {
     "mtl4g_pc_adjoint_solve": (_INT, [_P, _P, _P, _P]),
     "mtl4g_pc_factor": (_INT, [_P, _INT, C.POINTER(_I32), C.POINTER(_I64), _P, _P, _P]),
     "mtl4g_pc_levels": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32)]),
+    "mtl4g_set_sweep_schedule": (_INT, [_INT]),
+    "mtl4g_pc_sweeps": (_INT, [_P, C.POINTER(_INT), C.POINTER(_INT)]),
     "mtl4g_pc_refactor": (_INT, [_P, _P]),
     "mtl4g_cg": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_bicgstab": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),

@@ -12,6 +12,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cstdarg>
@@ -339,4 +340,16 @@ int launch_spmv(const mtl4g_matrix_s* A, const double* x, double* y, int assign,
 int launch_spmv_f32(const mtl4g_matrix_s* A, int fmt, const float* x, float* y, int assign,
                     hipStream_t s);
 int blocks_for(int64_t n);   // grid size of the vector kernels
+
+// A roctx range over one C-ABI call, named as MTL4's vampir_trace scope of
+// the same operation (interface/vpt.cpp: "crs_cvec_mult" :201, "cg" :353,
+// "bicgstab" :355, the other solvers :356-360, the preconditioners
+// :114, :313-329).  rocprofv3 --marker-trace records it; without a tool
+// attached a push and a pop cost a few tens of nanoseconds.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 }  // namespace mtl4g

@@ -2386,7 +2386,11 @@ int gmres_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* L, const mtl4g_pc_s* R,
     // r0 = b - A*x of the first cycle; later cycles reuse terminate's r
     M4_TRY(launch_csr<double>(A, val, x, EpiGmRes<false>{b, r0, c, W.hist.p}, part, tk, s));
     int32_t i_now = it->i;
-    const int64_t max_cycles = (restart > 0 ? (std::max(0, it->max_iter - it->i) / restart + 2) : 1);
+    // every cycle that does not finish the solve advances at least one
+    // iteration (a cycle is shorter than restart when n < restart or when the
+    // Givens estimate ends it early), so max_iter - i + 2 cycles always reach
+    // the solve's own stop: convergence, or error 1 at max_iter (gmres.hpp:125-133)
+    const int64_t max_cycles = (restart > 0 ? (int64_t)std::max(0, it->max_iter - it->i) + 2 : 1);
     for (int64_t cyc = 0; cyc < max_cycles; cyc++) {
         // <= kvec whenever the cycle runs (a cycle that starts at or past
         // max_iter stops at its first finished() and launches nothing that runs)
@@ -2653,6 +2657,8 @@ int mtl4g_solver_destroy(mtl4g_solver S_) {
 int mtl4g_solver_solve(mtl4g_solver S_, double* x, const double* b, mtl4g_iteration* it,
                        double* h_hist, int32_t hist_cap, void* stream) {
     if (!S_) return MTL4G_ERR_INVALID;
+    TraceRange tr(S_->method == MTL4G_CG ? "cg" : S_->method == MTL4G_BICGSTAB ? "bicgstab"
+                  : S_->method == MTL4G_CGS ? "cgs" : "bicgstab_2");
     if (S_->method == MTL4G_BICGSTAB_2)
         return bicgstab2_run(S_->A, S_->P, S_->W, x, b, it, h_hist, hist_cap, S(stream));
     return krylov_run<double>(S_->A, S_->P, S_->method, MTL4G_F64, S_->W, x, b, it, h_hist,
@@ -2661,42 +2667,49 @@ int mtl4g_solver_solve(mtl4g_solver S_, double* x, const double* b, mtl4g_iterat
 
 int mtl4g_cg(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,
              double* h_hist, int32_t hist_cap, void* stream) {
+    TraceRange tr("cg");
     Workspace<double> W;
     return krylov_run<double>(A, P, MTL4G_CG, MTL4G_F64, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_tfqmr(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* x, const double* b, mtl4g_iteration* it,
                 double* h_hist, int32_t hist_cap, void* stream) {
+    TraceRange tr("tfqmr");
     Workspace<double> W;
     return tfqmr_run(A, L, R, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_cgs(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,
               double* h_hist, int32_t hist_cap, void* stream) {
+    TraceRange tr("cgs");
     Workspace<double> W;
     return krylov_run<double>(A, P, MTL4G_CGS, MTL4G_F64, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_bicgstab_2(mtl4g_matrix A, mtl4g_pc L, double* x, const double* b, mtl4g_iteration* it,
                      double* h_hist, int32_t hist_cap, void* stream) {
+    TraceRange tr("bicgstab_2");
     Workspace<double> W;
     return bicgstab2_run(A, L, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_bicgstab_ell(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* x, const double* b, int32_t l,
                        mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream) {
+    TraceRange tr("bicgstab_ell");
     Workspace<double> W;
     return bicgstab_ell_run(A, L, R, l, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_gmres(mtl4g_matrix A, mtl4g_pc L, mtl4g_pc R, double* x, const double* b, int32_t restart,
                 mtl4g_iteration* it, double* h_hist, int32_t hist_cap, void* stream) {
+    TraceRange tr("gmres");
     Workspace<double> W;
     return gmres_run(A, L, R, restart, W, x, b, it, h_hist, hist_cap, S(stream));
 }
 
 int mtl4g_bicgstab(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,
                    double* h_hist, int32_t hist_cap, void* stream) {
+    TraceRange tr("bicgstab");
     Workspace<double> W;
     return krylov_run<double>(A, P, MTL4G_BICGSTAB, MTL4G_F64, W, x, b, it, h_hist, hist_cap,
                               S(stream));

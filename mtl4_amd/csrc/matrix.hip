@@ -1142,18 +1142,21 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
 
 int mtl4g_spmv(mtl4g_matrix A, const double* v, double* w, int assign, void* stream) {
     if (!A || (A->n && !w) || (A->m && !v)) return MTL4G_ERR_INVALID;
+    TraceRange tr("crs_cvec_mult");
     return launch_spmv(A, v, w, assign, S(stream));
 }
 
 int mtl4g_spmv_f32(mtl4g_matrix A, int fmt, const float* v, float* w, int assign, void* stream) {
     if (!A || (A->n && !w) || (A->m && !v) || (fmt != MTL4G_F32 && fmt != MTL4G_BF16))
         return MTL4G_ERR_INVALID;
+    TraceRange tr("crs_cvec_mult");
     M4_TRY(ensure_lowp_values(A, fmt, S(stream)));
     return launch_spmv_f32(A, fmt, v, w, assign, S(stream));
 }
 
 int mtl4g_spmv_dot(mtl4g_matrix A, const double* p, double* q, double* h_dot, void* stream) {
     if (!A || !h_dot || A->n != A->m) return A && A->n != A->m ? MTL4G_ERR_NOT_SQUARE : MTL4G_ERR_INVALID;
+    TraceRange tr("crs_cvec_mult");
     hipStream_t s = S(stream);
     if (A->n == 0) { *h_dot = 0; return MTL4G_OK; }
     Scratch sc;
@@ -1170,6 +1173,7 @@ int mtl4g_spmv_dot(mtl4g_matrix A, const double* p, double* q, double* h_dot, vo
 int mtl4g_spmv_dot_noalloc(mtl4g_matrix A, const double* p, double* q, double* h_dot, void* stream) {
     if (!A || !h_dot || A->n != A->m) return A && A->n != A->m ? MTL4G_ERR_NOT_SQUARE : MTL4G_ERR_INVALID;
     if (A->n && (!p || !q)) return MTL4G_ERR_INVALID;
+    TraceRange tr("crs_cvec_mult");
     hipStream_t s = S(stream);
     if (A->n == 0) { *h_dot = 0; return MTL4G_OK; }
     const bool ordered = reduction_order() == MTL4G_ORDER_MTL4;

@@ -78,6 +78,7 @@ extern "C" int mtl4g_ir_solve(mtl4g_matrix A, mtl4g_pc P32, double* x, const dou
     }
     if (P32 && P32->n != A->n) return MTL4G_ERR_SIZE;
     if (prm->method != MTL4G_CG && prm->method != MTL4G_BICGSTAB) return MTL4G_ERR_INVALID;
+    TraceRange tr(prm->method == MTL4G_CG ? "cg" : "bicgstab");
     hipStream_t s = S(stream);
     const int32_t n = A->n;
     if (prm->mode == MTL4G_IR_RELIABLE) {

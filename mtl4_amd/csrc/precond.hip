@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -222,6 +223,205 @@ static double stored_value(int fmt, double v) {
 }
 
 
+}  // namespace mtl4g
+
+#include "chain.hpp"
+
+namespace mtl4g {
+
+Sweep::Sweep() = default;
+Sweep::~Sweep() = default;
+
+static std::atomic<int> g_sweep_schedule{MTL4G_SWEEP_AUTO};
+
+// A sweep whose rows depend only on the rows 1, Lc, Lc*Cx away (towards the
+// start for a lower, the end for an upper sweep), never across the edge of a
+// chain of Lc rows or of a plane of Cx chains, with every row's entries in
+// ascending column order: the chain pipeline's layout.  Anything else keeps
+// the level schedule.
+static bool chain_layout(const SweepRows& R, int32_t& Lc, int32_t& Cx, int32_t& Cy, int& ndep) {
+    const int32_t n = R.n;
+    if (n < 2 || g_sweep_schedule.load() != MTL4G_SWEEP_AUTO) return false;
+    // every byte offset the kernel forms (3 FP64 values per row at most)
+    // stays below its out-of-range marker, 0x7ffffff0
+    if ((int64_t)n * 3 * 8 >= 0x7ffffff0) return false;
+    int64_t offs[3];
+    int no = 0;
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t j = R.rowptr[r]; j < R.rowptr[r + 1]; j++) {
+            const int64_t d = R.lower ? (int64_t)r - R.col[j] : (int64_t)R.col[j] - r;
+            if (d <= 0) return false;
+            bool seen = false;
+            for (int k = 0; k < no; k++) seen |= offs[k] == d;
+            if (!seen) {
+                if (no == 3) return false;
+                offs[no++] = d;
+            }
+        }
+    if (no == 0) return false;
+    std::sort(offs, offs + no);
+    if (offs[0] != 1) return false;
+    if (no == 1) {
+        Lc = n, Cx = 1, Cy = 1;
+    } else {
+        if (offs[1] > n || n % offs[1]) return false;
+        Lc = (int32_t)offs[1];
+        if (no == 3) {
+            if (offs[2] % offs[1] || n % offs[2]) return false;
+            Cx = (int32_t)(offs[2] / offs[1]);
+            Cy = (int32_t)(n / offs[2]);
+        } else {
+            Cx = n / Lc;
+            Cy = 1;
+        }
+    }
+    ndep = no;
+    const int64_t plane = (int64_t)Lc * Cx;
+    for (int32_t r = 0; r < n; r++) {
+        const int64_t rm = R.lower ? r : (int64_t)n - 1 - r;   // index in the sweep's own direction
+        const int64_t e = rm % Lc, cx = (rm / Lc) % Cx, cy = rm / plane;
+        int64_t last = 0;
+        for (int32_t j = R.rowptr[r]; j < R.rowptr[r + 1]; j++) {
+            const int64_t d = R.lower ? (int64_t)r - R.col[j] : (int64_t)R.col[j] - r;
+            if ((d == 1 && e < 1) || (d == Lc && no >= 2 && cx < 1) || (d == plane && no >= 3 && cy < 1))
+                return false;
+            // subtraction order of the stored row: ascending columns
+            const int64_t key = R.lower ? -d : d;
+            if (j > R.rowptr[r] && key <= last) return false;
+            last = key;
+        }
+    }
+    return true;
+}
+
+static int build_chain(const SweepRows& R, int fmt, Sweep* S) {
+    int32_t Lc = 0, Cx = 0, Cy = 0;
+    int ndep = 0;
+    S->chain.reset();
+    if (!chain_layout(R, Lc, Cx, Cy, ndep)) return MTL4G_OK;
+    std::unique_ptr<ChainSweep> C(new ChainSweep);
+    const int32_t n = R.n;
+    C->lower = R.lower;
+    C->unit = R.unit;
+    C->n = n;
+    C->Lc = Lc;
+    C->Cx = Cx;
+    C->Cy = Cy;
+    C->ndep = ndep;
+    // tile: up to four waves of 64 chains along x, up to eight waves in all
+    C->txw = (int)std::min<int64_t>(4, ((int64_t)Cx + kWave - 1) / kWave);
+    C->ty = (int)std::min<int64_t>(std::max(1, kChainMaxWaves / C->txw), Cy);
+    C->tiles_x = (int)(((int64_t)Cx + C->txw * kWave - 1) / (C->txw * kWave));
+    C->tiles_y = (int)(((int64_t)Cy + C->ty - 1) / C->ty);
+    const int64_t plane = (int64_t)Lc * Cx;
+    auto slot = [&](int64_t d) { return d == 1 ? 0 : d == Lc ? 1 : 2; };
+    const bool gathered = !R.dsrc.empty();
+    std::vector<int32_t> src(gathered ? (size_t)ndep * n : 0, -1);
+    std::vector<double> val(gathered ? 0 : (size_t)ndep * n, 0.0);
+    std::vector<char> present(gathered ? 0 : (size_t)ndep * n, 0);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t j = R.rowptr[r]; j < R.rowptr[r + 1]; j++) {
+            const int64_t d = R.lower ? (int64_t)r - R.col[j] : (int64_t)R.col[j] - r;
+            const size_t at = (size_t)slot(d) * n + r;
+            if (gathered) src[at] = R.src[j];
+            else {
+                val[at] = R.val[j];
+                present[at] = 1;
+            }
+        }
+    (void)plane;
+    auto up = [](auto& buf, const auto& v) -> int {
+        using T = typename std::remove_reference<decltype(v)>::type::value_type;
+        M4_TRY(buf.alloc(std::max<size_t>(v.size(), 1)));
+        if (!v.empty()) M4_HIP(hipMemcpy(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return MTL4G_OK;
+    };
+    const size_t nv = (size_t)ndep * n;
+    if (gathered) {   // values come from the device factor (fill_sweep)
+        M4_TRY(up(C->src, src));
+        M4_TRY(up(C->dsrc, R.dsrc));
+        if (fmt == MTL4G_F64) {
+            M4_TRY(C->val64.alloc(nv));
+            M4_TRY(C->dia64.alloc(n));
+        } else if (fmt == MTL4G_F32) {
+            M4_TRY(C->val32.alloc(nv));
+            M4_TRY(C->dia32.alloc(n));
+        } else {
+            M4_TRY(C->val16.alloc(nv));
+            M4_TRY(C->dia16.alloc(n));
+        }
+    } else {
+        auto fill = [&](auto& vbuf, auto& dbuf, auto absent, auto conv) -> int {
+            using T = decltype(absent);
+            std::vector<T> v(nv), dg(n);
+            for (size_t i = 0; i < nv; i++) v[i] = present[i] ? conv(val[i]) : absent;
+            for (int32_t r = 0; r < n; r++) dg[r] = conv(R.dia.empty() ? 1.0 : R.dia[r]);
+            M4_TRY(up(vbuf, v));
+            return up(dbuf, dg);
+        };
+        const uint64_t a64 = kDiaAbsent64;
+        const uint32_t a32 = kDiaAbsent32;
+        double ad;
+        float af;
+        std::memcpy(&ad, &a64, 8);
+        std::memcpy(&af, &a32, 4);
+        if (fmt == MTL4G_F64) M4_TRY(fill(C->val64, C->dia64, ad, [](double x) { return x; }));
+        else if (fmt == MTL4G_F32) M4_TRY(fill(C->val32, C->dia32, af, [](double x) { return round_f32(x); }));
+        else M4_TRY(fill(C->val16, C->dia16, kDiaAbsent16, [](double x) { return stored_as<uint16_t>(x); }));
+    }
+    S->chain = std::move(C);
+    return MTL4G_OK;
+}
+
+int fill_chain(ChainSweep& C, int fmt, const double* fval, hipStream_t s) {
+    const int64_t n = C.n;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n * C.ndep + 1023) / 1024, 2048));
+    const int32_t* dsrc = C.unit ? nullptr : C.dsrc.p;
+    if (fmt == MTL4G_F64)
+        hipLaunchKernelGGL(k_fill_chain<double>, dim3(grid), dim3(256), 0, s, n, C.ndep, C.src.p, dsrc, fval,
+                           C.val64.p, C.dia64.p);
+    else if (fmt == MTL4G_F32)
+        hipLaunchKernelGGL(k_fill_chain<float>, dim3(grid), dim3(256), 0, s, n, C.ndep, C.src.p, dsrc, fval,
+                           C.val32.p, C.dia32.p);
+    else
+        hipLaunchKernelGGL(k_fill_chain<uint16_t>, dim3(grid), dim3(256), 0, s, n, C.ndep, C.src.p, dsrc, fval,
+                           C.val16.p, C.dia16.p);
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
+template <typename MT, typename IT, bool LOWER, bool UNIT>
+static void launch_chain_t(const ChainSweep& C, const MT* val, const MT* dia, const IT* v, double* w, float* w32,
+                           unsigned* counters, int* err, const int* skip, hipStream_t s) {
+    ChainView<MT> V{val, dia, C.n, C.Lc, C.Cx, C.Cy, C.txw, C.ty, C.tiles_x};
+    const dim3 grid(C.tiles_x * C.tiles_y), block(C.txw * C.ty * kWave);
+    if (C.ndep == 1)
+        hipLaunchKernelGGL((k_chain<MT, IT, LOWER, UNIT, 1>), grid, block, 0, s, V, v, w, w32, counters, err, skip);
+    else if (C.ndep == 2)
+        hipLaunchKernelGGL((k_chain<MT, IT, LOWER, UNIT, 2>), grid, block, 0, s, V, v, w, w32, counters, err, skip);
+    else
+        hipLaunchKernelGGL((k_chain<MT, IT, LOWER, UNIT, 3>), grid, block, 0, s, V, v, w, w32, counters, err, skip);
+}
+
+template <typename MT, typename IT>
+static void launch_chain_m(const ChainSweep& C, const MT* val, const MT* dia, const IT* v, double* w, float* w32,
+                           unsigned* counters, int* err, const int* skip, hipStream_t s) {
+    if (C.lower && C.unit) launch_chain_t<MT, IT, true, true>(C, val, dia, v, w, w32, counters, err, skip, s);
+    else if (C.lower) launch_chain_t<MT, IT, true, false>(C, val, dia, v, w, w32, counters, err, skip, s);
+    else if (C.unit) launch_chain_t<MT, IT, false, true>(C, val, dia, v, w, w32, counters, err, skip, s);
+    else launch_chain_t<MT, IT, false, false>(C, val, dia, v, w, w32, counters, err, skip, s);
+}
+
+template <typename IT>
+static int launch_chain(const ChainSweep& C, int fmt, const IT* v, double* w, float* w32, unsigned* counters,
+                        int* err, const int* skip, hipStream_t s) {
+    if (fmt == MTL4G_F64) launch_chain_m<double, IT>(C, C.val64.p, C.dia64.p, v, w, w32, counters, err, skip, s);
+    else if (fmt == MTL4G_F32) launch_chain_m<float, IT>(C, C.val32.p, C.dia32.p, v, w, w32, counters, err, skip, s);
+    else launch_chain_m<uint16_t, IT>(C, C.val16.p, C.dia16.p, v, w, w32, counters, err, skip, s);
+    M4_HIP(hipGetLastError());
+    return MTL4G_OK;
+}
+
 int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
     const int32_t n = R.n;
     // dependency levels
@@ -317,6 +517,7 @@ int upload_sweep(const SweepRows& R, int fmt, Sweep* S) {
         M4_TRY(up(S->val16, v16));
         M4_TRY(up(S->dia16, d16));
     }
+    M4_TRY(build_chain(R, fmt, S));
     M4_TRY(S->counters.alloc(sweep_counter_words(nlev)));
     M4_HIP(hipMemset(S->counters.p, 0, sweep_counter_words(nlev) * sizeof(unsigned)));
     // complete before any stream (including non-blocking ones) uses the counters
@@ -472,6 +673,10 @@ int launch_sweep(const Sweep& S, int fmt, const IT* v, VT* w, float* w32, const 
     if (S.n == 0) return MTL4G_OK;
     if (!counters) counters = S.counters.p;
     hipLaunchKernelGGL(k_fill_pending<VT>, dim3(grid_for(S.n)), dim3(256), 0, s, (int64_t)S.n, w, skip);
+    if (S.chain) {
+        static_assert(std::is_same<VT, double>::value, "the chain pipeline keeps its results in FP64");
+        return launch_chain<IT>(*S.chain, fmt, v, w, w32, counters, err, skip, s);
+    }
     if (fmt == MTL4G_F64) {
         SweepView<double, VT> V{S.perm.p, S.rowptr.p, S.col.p, S.lev_ptr.p, S.lev_blk.p,
                                 S.blk_level.p, S.val64.p, S.dia64.p, S.nlev, S.unit ? 1 : 0};
@@ -571,6 +776,7 @@ int pc_check_error(const mtl4g_pc_s* P, hipStream_t s) {
 
 int fill_sweep(Sweep& S, int fmt, const double* fval, hipStream_t s) {
     if (S.n == 0) return MTL4G_OK;
+    if (S.chain) M4_TRY(fill_chain(*S.chain, fmt, fval, s));
     const int grid = grid_for(std::max<int64_t>(S.nnz, S.n));
     if (fmt == MTL4G_F64)
         hipLaunchKernelGGL(k_fill_sweep<double>, dim3(grid), dim3(256), 0, s, S.nnz, S.n, S.src.p, S.dsrc.p, fval,
@@ -869,6 +1075,8 @@ int mtl4g_pc_create(mtl4g_pc* out, mtl4g_matrix A, int kind, int fmt) {
         set_error("preconditioner needs a square matrix");
         return MTL4G_ERR_NOT_SQUARE;
     }
+    TraceRange tr(kind == MTL4G_PC_ILU0 ? "ilu_0::factorize" : kind == MTL4G_PC_IC0 ? "ic_0::factorize"
+                  : kind == MTL4G_PC_DIAGONAL ? "diagonal::setup" : "identity::setup");
     mtl4g_pc_s* P = new (std::nothrow) mtl4g_pc_s;
     if (!P) return MTL4G_ERR_OOM;
     P->kind = kind;
@@ -910,6 +1118,11 @@ int mtl4g_pc_destroy(mtl4g_pc P) {
 
 static int pc_solve_impl(mtl4g_pc P, const double* x, double* y, void* stream, bool adjoint) {
     if (!P || (P->n && (!x || !y))) return MTL4G_ERR_INVALID;
+    static const char* const kSolve[4][2] = {{"identity::solve", "identity::adjoint_solve"},
+                                             {"diagonal::solve", "diagonal::adjoint_solve"},
+                                             {"ilu_0::solve", "ilu_0::adjoint_solve"},
+                                             {"ic_0::solve", "ic_0::adjoint_solve"}};
+    TraceRange tr(kSolve[std::min(std::max(P->kind, 0), 3)][adjoint ? 1 : 0]);
     hipStream_t s = S(stream);
     if (P->n == 0) return MTL4G_OK;
     if (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) {
@@ -965,6 +1178,8 @@ int mtl4g_pc_factor(mtl4g_pc P, int which, int32_t* n_out, int64_t* nnz_out, int
 
 int mtl4g_pc_refactor(mtl4g_pc P, void* stream) {
     if (!P || !P->A || P->A->n != P->n) return MTL4G_ERR_INVALID;
+    TraceRange tr(P->kind == MTL4G_PC_ILU0 ? "ilu_0::factorize" : P->kind == MTL4G_PC_IC0 ? "ic_0::factorize"
+                  : P->kind == MTL4G_PC_DIAGONAL ? "diagonal::setup" : "identity::setup");
     hipStream_t s = S(stream);
     if (P->kind == MTL4G_PC_ILU0 || P->kind == MTL4G_PC_IC0) {
         M4_TRY(factor_device(P, s));
@@ -983,6 +1198,19 @@ int mtl4g_pc_refactor(mtl4g_pc P, void* stream) {
             M4_TRY(upload_inv_diag(P, A->n));
         }
     }
+    return MTL4G_OK;
+}
+
+int mtl4g_set_sweep_schedule(int schedule) {
+    if (schedule != MTL4G_SWEEP_AUTO && schedule != MTL4G_SWEEP_LEVELS) return MTL4G_ERR_INVALID;
+    g_sweep_schedule.store(schedule);
+    return MTL4G_OK;
+}
+
+int mtl4g_pc_sweeps(mtl4g_pc P, int* lower, int* upper) {
+    if (!P) return MTL4G_ERR_INVALID;
+    if (lower) *lower = P->lower.chain ? MTL4G_SWEEP_AUTO : MTL4G_SWEEP_LEVELS;
+    if (upper) *upper = P->upper.chain ? MTL4G_SWEEP_AUTO : MTL4G_SWEEP_LEVELS;
     return MTL4G_OK;
 }
 

@@ -1,6 +1,8 @@
 // precond.hpp -- preconditioner handle and the level-scheduled sweep.
 #pragma once
 
+#include <memory>
+
 #include "internal.hpp"
 
 namespace mtl4g {
@@ -19,6 +21,8 @@ inline size_t sweep_counter_words(int32_t nlev) { return ((size_t)nlev + 3) * kC
 // One triangular sweep  w[r] = (v[r] - sum_j T[r][c_j] * w[c_j]) * dia[r]
 // stored in level order: position k of the sweep handles row perm[k]; the
 // off-diagonal entries of that row follow in the order MTL4 subtracts them.
+struct ChainSweep;
+
 struct Sweep {
     int32_t n = 0, nlev = 0, nblocks = 0;
     int64_t nnz = 0;
@@ -30,6 +34,9 @@ struct Sweep {
     DevBuf<uint16_t> val16, dia16;   // bfloat16 storage (MTL4G_BF16)
     DevBuf<unsigned> counters;   // default counters (sweep_counter_words)
     std::vector<int32_t> h_lev_ptr;
+    std::unique_ptr<ChainSweep> chain;   // grid-structured factor: the chain pipeline runs instead
+    Sweep();
+    ~Sweep();
 };
 
 // Host description of a sweep before upload: rows in natural order.

@@ -5,6 +5,7 @@ updates, triangular sweeps, factor values); 1e-13 relative for the tree
 reductions (dot, norms), whose summation order differs from MTL4's unrolled
 partial sums; 1e-12 relative on the solution and +-2 iterations for CG with
 the diagonal preconditioner at config 1, as the MTL4 path requires."""
+import ctypes as C
 import json
 import math
 import os
@@ -1044,6 +1045,30 @@ def test_gmres_reference_cases(m):
         m.gmres(E, m.dense_vector(0), m.dense_vector(0), None, None, m.basic_iteration(0.0, 3, 1e-8), 4)
 
 
+@pytest.mark.parametrize("restart", [16, 30])
+def test_gmres_short_cycles_run_to_max_iter(m, restart):
+    """n = 12 < restart: every cycle is shorter than restart.  With rtol 0
+    the solve cannot converge, so it runs to max_iter and reports error 1
+    (gmres.hpp:125-133), as the oracle does, instead of stopping early on a
+    cycle count."""
+    rng = np.random.default_rng(1)
+    n = 12
+    Ad = rng.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    rp = np.arange(0, n * n + 1, n).astype(np.int32)
+    ci = np.tile(np.arange(n), n).astype(np.int32)
+    O, A = pair(rp, ci, Ad.ravel().copy())
+    b = rng.uniform(-1, 1, n)
+    xo, st, ito, _ = orc.gmres(O, b, None, None, restart, max_iter=50, rtol=0.0)
+    assert st == 1 and ito.i == 50
+    x = m.dense_vector(n)
+    it = m.basic_iteration(dv(m, b), 50, 0.0)
+    assert m.gmres(A, x, dv(m, b), None, None, it, restart) == 1
+    assert it.iterations() == 50
+    res = np.linalg.norm(b - orc.spmv(O, x.numpy()))
+    ores = np.linalg.norm(b - orc.spmv(O, xo))
+    assert abs(res - ores) <= 1e-6 * ores + 1e-12
+
+
 def test_bicgstab_2_solver_handle(m):
     """A solver handle for BiCGSTAB(2) keeps its workspace: two solves give
     the same bits as each other and as the one-shot call."""
@@ -1192,3 +1217,100 @@ def test_mixed_iteration_limit(m):
     it = m.basic_iteration(dv(m, b), 5, 1e-10)
     err, st = m.ir_solve(A, x, dv(m, b), it, m.pc.diagonal(A, m.F32))
     assert err == 1 and it.iterations() == 5
+
+
+# ---------------------------------------------------------------- chain-pipeline sweeps
+def _grid_spd(nx, ny, nz, seed):
+    """7-point pattern (idx = i + nx*(j + ny*k)) with random symmetric
+    off-diagonal values and a dominant diagonal: SPD, values not exact in FP32."""
+    from mtl4_amd import generators as gen
+    rp, ci, va = gen.laplacian3d(nx, ny, nz)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    lo, hi = np.minimum(rows, ci), np.maximum(rows, ci)
+    h = (lo * 2654435761 + hi * 40503 + seed) % 1000003
+    off = -(0.1 + (h % 997) / 997.0)
+    va = np.where(rows == ci, 0.0, off)
+    diag = np.zeros(len(rp) - 1)
+    np.add.at(diag, rows, -va)
+    va = np.where(rows == ci, diag[rows] + 1.0 + (rows % 7) * 0.01, va)
+    return rp, ci, va
+
+
+def _convdiff(mg):
+    from mtl4_amd import generators as gen
+    return gen.convection_diffusion2d(mg, 10.0, 3)
+
+
+CHAIN_CASES = {
+    "grid_20x300x3": lambda: _grid_spd(20, 300, 3, 1),     # two tiles along x and along y
+    "grid_40x40x40": lambda: _grid_spd(40, 40, 40, 2),     # five tiles along y
+    "lap2d_64": lambda: family("lap2d_64"),
+    "convdiff_300": lambda: _convdiff(300),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CHAIN_CASES))
+@pytest.mark.parametrize("kind", ["ilu_0", "ic_0"])
+@pytest.mark.parametrize("fmt", ["f64", "f32", "bf16"])
+def test_pc_chain_schedule_bitwise(m, case, kind, fmt):
+    """Grid-structured ILU(0)/IC(0) factors run their sweeps as a pipeline of
+    row chains; the solve is bit for bit the level-scheduled one (and, in
+    FP64, the oracle's), across tile edges in x and y."""
+    if kind == "ic_0" and case.startswith("convdiff"):
+        pytest.skip("ic_0 needs a symmetric matrix")
+    rp, ci, va = CHAIN_CASES[case]()
+    O, A = pair(rp, ci, va)
+    vf = {"f64": m.F64, "f32": m.F32, "bf16": m.BF16}[fmt]
+    L = m._lib.lib()
+    cls = getattr(m.pc, kind)
+    Pc = cls(A, vf)
+    lo, up = C.c_int(-1), C.c_int(-1)
+    m._lib.check(L.mtl4g_pc_sweeps(Pc.h, C.byref(lo), C.byref(up)))
+    assert (lo.value, up.value) == (m._lib.SWEEP_AUTO, m._lib.SWEEP_AUTO)
+    m._lib.check(L.mtl4g_set_sweep_schedule(m._lib.SWEEP_LEVELS))
+    try:
+        Pl = cls(A, vf)
+    finally:
+        m._lib.check(L.mtl4g_set_sweep_schedule(m._lib.SWEEP_AUTO))
+    m._lib.check(L.mtl4g_pc_sweeps(Pl.h, C.byref(lo), C.byref(up)))
+    assert (lo.value, up.value) == (m._lib.SWEEP_LEVELS, m._lib.SWEEP_LEVELS)
+    x = np.random.default_rng(3).uniform(-1, 1, A.num_rows())
+    for _ in range(2):   # the counters are left ready for the next launch
+        got = Pc.solve(dv(m, x)).numpy()
+        ref = Pl.solve(dv(m, x)).numpy()
+        assert np.array_equal(got, ref), np.abs(got - ref).max()
+    if fmt != "bf16":
+        assert np.array_equal(got, orc.Preconditioner(kind, O, 8 if fmt == "f64" else 4).solve(x))
+
+
+def test_pc_chain_not_for_general_patterns(m):
+    rp, ci, va = family("spd_2000", seed=1)
+    _, A = pair(rp, ci, va)
+    P = m.pc.ilu_0(A)
+    lo, up = C.c_int(-1), C.c_int(-1)
+    m._lib.check(m._lib.lib().mtl4g_pc_sweeps(P.h, C.byref(lo), C.byref(up)))
+    assert (lo.value, up.value) == (m._lib.SWEEP_LEVELS, m._lib.SWEEP_LEVELS)
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_mixed_solve_same_bits_on_both_sweep_schedules(m, method):
+    """A mixed-precision solve (float vectors: each sweep's chain in FP64,
+    results rounded to float) gives the same bits with the chain pipeline
+    as with the level schedule."""
+    rp, ci, va = _grid_spd(24, 20, 18, 4)
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.random.default_rng(5).uniform(-1, 1, A.num_rows()))
+    L = m._lib.lib()
+    xs = []
+    for sched in (m._lib.SWEEP_AUTO, m._lib.SWEEP_LEVELS):
+        m._lib.check(L.mtl4g_set_sweep_schedule(sched))
+        try:
+            P = (m.pc.ic_0 if method == "cg" else m.pc.ilu_0)(A, m.F32)
+        finally:
+            m._lib.check(L.mtl4g_set_sweep_schedule(m._lib.SWEEP_AUTO))
+        x = m.dense_vector(A.num_rows())
+        it = m.basic_iteration(dv(m, b), 2000, 1e-10)
+        err, st = m.ir_solve(A, x, dv(m, b), it, P, method=m.CG if method == "cg" else m.BICGSTAB)
+        assert err == 0
+        xs.append(x.numpy())
+    assert np.array_equal(xs[0], xs[1])
