@@ -256,6 +256,12 @@ int mtl4g_pc_levels(mtl4g_pc P, int32_t* lower_levels, int32_t* upper_levels);
 #define MTL4G_SWEEP_LEVELS 1
 int mtl4g_set_sweep_schedule(int schedule);
 int mtl4g_pc_sweeps(mtl4g_pc P, int* lower, int* upper);
+/* Tile of the chain pipeline (no MTL4 counterpart): Tx x Ty row chains per
+ * workgroup (one wave: Tx*Ty = 64) for factors created afterwards; (0, 0)
+ * restores the choice per grid.  mtl4g_pc_chain_tile reports a
+ * preconditioner's tiles (0 for a sweep that runs level by level). */
+int mtl4g_set_chain_tile(int32_t tx, int32_t ty);
+int mtl4g_pc_chain_tile(mtl4g_pc P, int32_t* lower_tx, int32_t* lower_ty, int32_t* upper_tx, int32_t* upper_ty);
 /* Factor again from A's current values (after mtl4g_csr_set_values), same
  * pattern: ILU(0)/IC(0) rerun the device factorization on `stream` with
  * the plan made at creation (levels, positions), nothing on the host; the

@@ -104,6 +104,8 @@ _SIGS = {
     "mtl4g_pc_levels": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32)]),
     "mtl4g_set_sweep_schedule": (_INT, [_INT]),
     "mtl4g_pc_sweeps": (_INT, [_P, C.POINTER(_INT), C.POINTER(_INT)]),
+    "mtl4g_set_chain_tile": (_INT, [_I32, _I32]),
+    "mtl4g_pc_chain_tile": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "mtl4g_pc_refactor": (_INT, [_P, _P]),
     "mtl4g_cg": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),
     "mtl4g_bicgstab": (_INT, [_P, _P, _P, _P, C.POINTER(Iteration), _P, _I32, _P]),

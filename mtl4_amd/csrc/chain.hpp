@@ -7,15 +7,22 @@
 // ILU(0) factors of 5- and 7-point stencils in natural order -- is swept
 // chain by chain: chain (cx, cy) (rows (cy*Cx + cx)*Lc .. +Lc) handles its
 // element e at step e + cx + cy.  A lane owns one chain and keeps its
-// previous element in a register; the chains cx-1 and cy-1 hand over their
-// element of the previous step through LDS inside a workgroup (a tile of
-// 64*TXW chains along x by TY along y, one barrier per step) and through
-// memory between workgroups (agent-scope stores, polled reads of the
-// reserved pending pattern, issued U steps ahead).  A workgroup therefore
-// pays a barrier per step, not a device-wide round trip per level.  Every
-// row is computed by one lane with MTL4's subtraction order (ascending
-// columns, lower_trisolve.hpp:174-209, upper_trisolve.hpp:161-176), so the
-// results are the level-scheduled sweep's bit for bit.
+// previous element in a register.  A workgroup is one wave and owns a tile
+// of Tx x Ty = 64 chains; inside it the chains cx-1 and cy-1 hand over their
+// element of the previous step lane to lane (ds_bpermute: no LDS store, no
+// barrier), between tiles through memory (agent-scope stores of the tile's
+// edge chains, reads of the reserved pending pattern issued U steps ahead,
+// polled when still pending).
+//
+// Memory access is coalesced although the chains' rows lie Lc apart: the
+// factor's values are stored tile by tile in step-major order (slot =
+// tile*stile + (step - first step of the tile)*64 + lane), so a step's loads
+// read consecutive slots at a scalar offset; the input and output vectors keep their natural
+// order and move through LDS in blocks of U steps (8 consecutive rows of a
+// chain per 8 lanes).  Every row is computed by one lane with MTL4's
+// subtraction order (ascending columns, lower_trisolve.hpp:174-209,
+// upper_trisolve.hpp:161-176), so the results are the level-scheduled
+// sweep's bit for bit.
 #pragma once
 
 #include <vector>
@@ -24,27 +31,37 @@
 
 namespace mtl4g {
 
-constexpr int kChainU = 8;         // steps per unrolled block: loads are issued this far ahead
-constexpr int kChainMaxWaves = 8;  // waves per workgroup
+constexpr int kChainU = 8;     // steps per block: v and w staged per block, values issued this far ahead
+constexpr int kChainPad = 8;   // LDS row padding of the staging arrays (bank spread)
 
 struct ChainSweep {
-    bool on = false;
     bool lower = true, unit = false;
     int32_t n = 0, Lc = 0, Cx = 0, Cy = 0;
     int ndep = 0;                   // 1..3 offsets: 1, Lc, Lc*Cx
-    int txw = 1, ty = 1, tiles_x = 1, tiles_y = 1;
+    int Tx = 64, Ty = 1, tiles_x = 1, tiles_y = 1;
+    int64_t stile = 0, nslots = 0;  // slots per tile, slots in all
     DevBuf<int32_t> src, dsrc;      // gathered values: positions in the device factor (-1: absent / unit)
-    DevBuf<double> val64, dia64;    // val: ndep arrays of n (entry of offset k of row r), absent pattern
+    DevBuf<double> val64, dia64;    // val: ndep arrays of nslots (entry of offset k), absent pattern
     DevBuf<float> val32, dia32;
     DevBuf<uint16_t> val16, dia16;
 };
 
 template <typename MT>
 struct ChainView {
-    const MT* val;   // ndep * n
-    const MT* dia;   // n (unused for a unit sweep)
-    int32_t n, Lc, Cx, Cy, txw, ty, tiles_x;
+    const MT* val;   // ndep * nslots, tile-local step-major
+    const MT* dia;   // nslots (unused for a unit sweep)
+    int32_t n, Lc, Cx, Cy, Tx, Ty, tiles_x, stile, nslots;
 };
+
+// Slot of row r of the sweep's own index (rm = r for a lower sweep,
+// n-1-r for an upper one) in the tile-local step-major layout.
+__host__ __device__ inline int64_t chain_slot(int64_t rm, int64_t Lc, int64_t Cx, int64_t Tx, int64_t Ty,
+                                              int64_t tiles_x, int64_t stile) {
+    const int64_t e = rm % Lc, q = rm / Lc, cx = q % Cx, cy = q / Cx;
+    const int64_t tx = cx / Tx, ty = cy / Ty;
+    const int64_t l = (cy - ty * Ty) * Tx + (cx - tx * Tx);
+    return (ty * tiles_x + tx) * stile + (e + cx + cy - tx * Tx - ty * Ty) * (Tx * Ty) + l;
+}
 
 template <typename T>
 __device__ __forceinline__ bool chain_pending(T v);
@@ -71,23 +88,49 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t chain_rsrc(const void* p, int3
                                              __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
 }
 template <int AUX>
-__device__ __forceinline__ double chain_ld64(__amdgpu_buffer_rsrc_t r, int32_t off) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
+__device__ __forceinline__ double chain_ld64(__amdgpu_buffer_rsrc_t r, int32_t off, int32_t soff = 0) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, AUX));
 }
-__device__ __forceinline__ float chain_ld32(__amdgpu_buffer_rsrc_t r, int32_t off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+__device__ __forceinline__ float chain_ld32(__amdgpu_buffer_rsrc_t r, int32_t off, int32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
 }
-__device__ __forceinline__ uint16_t chain_ld16(__amdgpu_buffer_rsrc_t r, int32_t off) {
-    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+__device__ __forceinline__ uint16_t chain_ld16(__amdgpu_buffer_rsrc_t r, int32_t off, int32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
 }
+// off: per-lane byte offset (range-checked against the buffer's size);
+// soff: a wave-uniform byte offset added to it
 template <typename T>
-__device__ __forceinline__ T chain_ld(__amdgpu_buffer_rsrc_t r, int32_t off);
+__device__ __forceinline__ T chain_ld(__amdgpu_buffer_rsrc_t r, int32_t off, int32_t soff = 0);
 template <>
-__device__ __forceinline__ double chain_ld<double>(__amdgpu_buffer_rsrc_t r, int32_t off) { return chain_ld64<0>(r, off); }
+__device__ __forceinline__ double chain_ld<double>(__amdgpu_buffer_rsrc_t r, int32_t off, int32_t soff) {
+    return chain_ld64<0>(r, off, soff);
+}
 template <>
-__device__ __forceinline__ float chain_ld<float>(__amdgpu_buffer_rsrc_t r, int32_t off) { return chain_ld32(r, off); }
+__device__ __forceinline__ float chain_ld<float>(__amdgpu_buffer_rsrc_t r, int32_t off, int32_t soff) {
+    return chain_ld32(r, off, soff);
+}
 template <>
-__device__ __forceinline__ uint16_t chain_ld<uint16_t>(__amdgpu_buffer_rsrc_t r, int32_t off) { return chain_ld16(r, off); }
+__device__ __forceinline__ uint16_t chain_ld<uint16_t>(__amdgpu_buffer_rsrc_t r, int32_t off, int32_t soff) {
+    return chain_ld16(r, off, soff);
+}
+
+// The value v of lane l-1 (lane 0: 0), for every lane: a DPP shift of the
+// whole wave by one lane, no LDS.
+__device__ __forceinline__ double chain_from_prev_lane(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)b, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The value v of lane src (src4 = 4*src), for every lane: two ds_bpermute
+// (the LDS crossbar, no LDS storage).
+__device__ __forceinline__ double chain_from(double v, int src4) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(src4, (int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(src4, (int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // Waits (bounded) until the values another workgroup stores at w[ix]
 // (when wx) and w[iy] (when wy) are no longer the pending marker; returns
@@ -112,104 +155,146 @@ __device__ __noinline__ double2 chain_poll(const double* w, int32_t ix, int32_t 
     return o;
 }
 
-// The pipeline kernel.  LOWER: rows depend on smaller rows (else on larger
-// ones: the sweep runs in the mirrored index r' = n-1-r).  NDEP offsets;
-// UNIT: no diagonal factor.  v is read as IT, every result is kept in w
-// (double) and, when w32 is given, rounded to float there too.  The step
-// body is straight-line for every lane: operands of absent entries are
-// computed and dropped by a select (the skipped subtraction leaves acc's
-// bits as they are), rows outside the chain read and write out of range.
+// The pipeline kernel, one wave per tile.  LOWER: rows depend on smaller
+// rows (else on larger ones: the sweep runs in the mirrored index
+// r' = n-1-r).  NDEP offsets; UNIT: no diagonal factor.  v is read as IT,
+// every result is kept in w (double) and, when w32 is given, rounded to
+// float there too.  The step body is straight-line for every lane: operands
+// of absent entries are computed and dropped by a select (the skipped
+// subtraction leaves acc's bits as they are); lanes without a row this step
+// compute on padding.
 template <typename MT, typename IT, bool LOWER, bool UNIT, int NDEP>
-__global__ __launch_bounds__(kChainMaxWaves * kWave) void k_chain(ChainView<MT> C, const IT* __restrict__ v,
-                                                                  double* w, float* w32, unsigned* counters,
-                                                                  int* err, const int* skip) {
+__global__ __launch_bounds__(kWave) void k_chain(ChainView<MT> C, const IT* __restrict__ v, double* w, float* w32,
+                                                 unsigned* counters, int* err, const int* skip) {
     constexpr int U = kChainU;
+    constexpr int RW = kWave + kChainPad;   // staging row: one step of the block, all chains
     if (skip && *skip) return;
-    __shared__ double xch[2][kChainMaxWaves * kWave];   // each lane's element of the last step
-    __shared__ int s_tile, s_final;
-    if (threadIdx.x == 0)
-        s_tile = (int)__hip_atomic_fetch_add(counters, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int tile = s_tile;   // tiles in row-major order: producers come first
+    __shared__ double vst[U * RW];          // v of the next block: [step][chain]
+    __shared__ double wst[U * RW];          // results of the block: [step][chain]
+    const int l = threadIdx.x;
+    unsigned tk = 0;
+    if (l == 0) tk = __hip_atomic_fetch_add(counters, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int tile = (int)__builtin_amdgcn_readfirstlane(tk);   // tiles in row-major order: producers first
     const int tx = tile % C.tiles_x, ty = tile / C.tiles_x;
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int wx = wave % C.txw, wy = wave / C.txw;
-    const int tw = C.txw * kWave;                       // tile width in chains
-    const int gx = wx * kWave + lane;                   // position in the tile's x extent
-    const int x0 = tx * tw;                             // first chain of the tile
-    const int cx = x0 + gx, cy = ty * C.ty + wy;
+    const int x0 = tx * C.Tx, y0 = ty * C.Ty;
+    const int gx = l % C.Tx, gy = l / C.Tx;
+    const int cx = x0 + gx, cy = y0 + gy;
     const bool chain = cx < C.Cx && cy < C.Cy;
-    const int32_t Lc = C.Lc, plane = C.Lc * C.Cx;
-    const int32_t base = (cy * C.Cx + cx) * Lc;         // first row of the chain (mirrored index)
-    const int xl = min(C.Cx, x0 + tw) - 1, yl = min(C.Cy, (ty + 1) * C.ty) - 1;
-    const int s0 = x0 + ty * C.ty, s1 = (Lc - 1) + xl + yl;
-    const bool ext_x = cx == x0 && tx > 0 && NDEP >= 2;      // the chain cx-1 is another tile's
-    const bool ext_y = wy == 0 && ty > 0 && NDEP >= 3;       // the chain cy-1 is another tile's
-    const int32_t n = C.n;
-    constexpr int32_t kOut = 0x7ffffff0;                 // an offset past every buffer
+    const int32_t Lc = C.Lc, plane = C.Lc * C.Cx, n = C.n;
+    const int32_t base = chain ? (cy * C.Cx + cx) * Lc : 0;   // first row of the chain (mirrored index)
+    const int e0 = cx + cy;                                   // step of the chain's first element
+    const int xl = min(C.Cx, x0 + C.Tx) - 1, yl = min(C.Cy, y0 + C.Ty) - 1;
+    const int s0 = x0 + y0, s1 = (Lc - 1) + xl + yl;
+    const bool ext_x = NDEP >= 2 && chain && gx == 0 && tx > 0;   // the chain cx-1 is another tile's
+    const bool ext_y = NDEP >= 3 && chain && gy == 0 && ty > 0;   // the chain cy-1 is another tile's
+    const int srcy = max(l - C.Tx, 0) * 4;
+    constexpr int32_t kOut = 0x7ffffff0;                          // an offset past every buffer
     const __amdgpu_buffer_rsrc_t rv_ = chain_rsrc(v, n * (int32_t)sizeof(IT));
-    const __amdgpu_buffer_rsrc_t rval = chain_rsrc(C.val, NDEP * n * (int32_t)sizeof(MT));
-    const __amdgpu_buffer_rsrc_t rdia = chain_rsrc(C.dia, n * (int32_t)sizeof(MT));
+    const __amdgpu_buffer_rsrc_t rval = chain_rsrc(C.val, NDEP * C.nslots * (int32_t)sizeof(MT));
+    const __amdgpu_buffer_rsrc_t rdia = chain_rsrc(C.dia, C.nslots * (int32_t)sizeof(MT));
     const __amdgpu_buffer_rsrc_t rw = chain_rsrc(w, n * 8);
     const __amdgpu_buffer_rsrc_t rw32 = chain_rsrc(w32, n * 4);
     const unsigned* abort_flag = counters + kCounterStride;
     auto row_of = [=](int32_t rm) -> int32_t { return LOWER ? rm : n - 1 - rm; };
-    auto dep_row = [=](int32_t r, int32_t d) -> int32_t { return LOWER ? r - d : r + d; };
+    // byte offsets of this lane's slot at the tile's first step; a step adds
+    // the wave-uniform (step - s0) * 64 * sizeof(MT), clamped to the tile
+    const int32_t vo = (tile * C.stile + l) * (int32_t)sizeof(MT);
+    // rows of the other tiles' elements: mirrored index of element e of the
+    // chain cx-1 (cy-1) is hx + st (hy + st) at step st
+    const int32_t hx = base - Lc - e0, hy = base - plane - e0;
 
-    // register rings: the operands of the steps s..s+U-1 (slot j: step s+j);
-    // the descriptors are passed by value so that they stay scalar
-    IT rv[U];
+    // Block staging: lane l moves element jj = l % 8 of the chains
+    // c = l/8 + 8i (i = 0..7) between the natural-order vectors and LDS.
+    const int jj = l & 7;
+    auto stage_row = [=](int S, int i, bool& ok) -> int32_t {
+        const int c = (l >> 3) + i * 8;
+        const int ccx = x0 + c % C.Tx, ccy = y0 + c / C.Tx;
+        const int e = S + jj - ccx - ccy;
+        ok = ccx < C.Cx && ccy < C.Cy && e >= 0 && e < Lc;
+        return row_of(ok ? (ccy * C.Cx + ccx) * Lc + e : 0);
+    };
+    // chains that another tile reads: their rows are stored at agent scope
+    // (through to memory), the others with plain stores that complete in L2
+    auto stage_shared = [=](int i) -> bool {
+        const int c = (l >> 3) + i * 8;
+        const int ccx = x0 + c % C.Tx, ccy = y0 + c / C.Tx;
+        return (NDEP >= 2 && c % C.Tx == C.Tx - 1 && ccx + 1 < C.Cx) ||
+               (NDEP >= 3 && c / C.Tx == C.Ty - 1 && ccy + 1 < C.Cy);
+    };
+    IT vin[U];       // v of the block after the next one, in flight
+    double vb[U];    // v of this block (this lane's chain)
+    double res[U];   // results of this block (this lane's chain)
+    auto stage_issue = [&](int S) {
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            bool ok;
+            const int32_t r = stage_row(S, i, ok);
+            vin[i] = chain_ld<IT>(rv_, ok ? r * (int32_t)sizeof(IT) : kOut);
+        }
+    };
+    auto stage_next = [&]() {   // vin -> LDS -> vb
+#pragma unroll
+        for (int i = 0; i < U; i++) vst[jj * RW + (l >> 3) + i * 8] = static_cast<double>(vin[i]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+#pragma unroll
+        for (int j = 0; j < U; j++) vb[j] = vst[j * RW + l];
+    };
+
+    // register rings: the values of the steps s..s+U-1 (slot j: step s+j),
+    // and the other tiles' elements they need
     MT rl[NDEP][U];
     MT rd[U];
     double ex[U], ey[U];
 #define M4_CHAIN_ISSUE(ST, J)                                                                              \
     do {                                                                                                  \
-        const int e_ = (ST) - cx - cy;                                                                    \
-        const bool in_ = chain && e_ >= 0 && e_ < Lc;                                                     \
-        const int32_t r_ = row_of(base + (in_ ? e_ : 0));                                                 \
-        rv[J] = chain_ld<IT>(rv_, in_ ? r_ * (int32_t)sizeof(IT) : kOut);                                 \
+        const int32_t so_ = (min((ST), s1) - s0) * (kWave * (int32_t)sizeof(MT));                        \
         _Pragma("unroll") for (int k_ = 0; k_ < NDEP; k_++) rl[k_][J] =                                  \
-            chain_ld<MT>(rval, in_ ? (k_ * n + r_) * (int32_t)sizeof(MT) : kOut);                         \
-        if (!UNIT) rd[J] = chain_ld<MT>(rdia, in_ ? r_ * (int32_t)sizeof(MT) : kOut);                    \
-        if (NDEP >= 2) ex[J] = chain_ld64<kBufSc1>(rw, in_ && ext_x ? dep_row(r_, Lc) * 8 : kOut);       \
-        if (NDEP >= 3) ey[J] = chain_ld64<kBufSc1>(rw, in_ && ext_y ? dep_row(r_, plane) * 8 : kOut);    \
+            chain_ld<MT>(rval, vo + k_ * C.nslots * (int32_t)sizeof(MT), so_);                            \
+        if (!UNIT) rd[J] = chain_ld<MT>(rdia, vo, so_);                                                   \
+        const bool a_ = (unsigned)((ST) - e0) < (unsigned)Lc;                                             \
+        if (NDEP >= 2) ex[J] = chain_ld64<kBufSc1>(rw, a_ && ext_x ? row_of(hx + (ST)) * 8 : kOut);      \
+        if (NDEP >= 3) ey[J] = chain_ld64<kBufSc1>(rw, a_ && ext_y ? row_of(hy + (ST)) * 8 : kOut);      \
     } while (0)
 
+    stage_issue(s0);
+    stage_next();
+    stage_issue(s0 + U);
 #pragma unroll
     for (int j = 0; j < U; j++) M4_CHAIN_ISSUE(s0 + j, j);
-    double prev = 0.0;
-    int par = 0;
-    for (int s = s0; s <= s1; s += U) {
+    double prev = 0.0;   // the chain's element of the last step it had one; the neighbours read it
+    for (int S = s0; S <= s1; S += U) {
 #pragma unroll
         for (int j = 0; j < U; j++) {
-            const int st = s + j;
-            if (st > s1) break;   // uniform over the workgroup
-            const int e = st - cx - cy;
-            const bool act = chain && e >= 0 && e < Lc;
-            const int32_t r = row_of(base + (act ? e : 0));
+            const int st = S + j;
+            if (st > s1) break;   // uniform over the wave
+            const bool act = chain && (unsigned)(st - e0) < (unsigned)Lc;
             // operands: the previous element, the chain cx-1 (ox) and the
-            // chain cy-1 (oy), each of the previous step.  Both selects read
-            // LDS unconditionally (the index is clamped into the array).
-            const double lx = xch[par ^ 1][wy * tw + max(gx - 1, 0)];
-            const double ly = xch[par ^ 1][max(wy - 1, 0) * tw + gx];
+            // chain cy-1 (oy), each of the previous step
             double ox = 0.0, oy = 0.0;
-            if (NDEP >= 2) ox = cx > x0 ? lx : ex[j];
-            if (NDEP >= 3) oy = wy > 0 ? ly : ey[j];
+            if (NDEP >= 2) {
+                const double nx = chain_from_prev_lane(prev);   // lane l-1 is the chain cx-1 when gx > 0
+                ox = gx > 0 ? nx : ex[j];
+            }
+            if (NDEP >= 3) {
+                const double ny = chain_from(prev, srcy);
+                oy = gy > 0 ? ny : ey[j];
+            }
             bool present[3];
 #pragma unroll
-            for (int k = 0; k < NDEP; k++) present[k] = act && !dia_absent(rl[k][j]);
+            for (int k = 0; k < NDEP; k++) present[k] = act & !dia_absent(rl[k][j]);
             // another tile's value not yet there: poll it (bounded)
-            const bool wait_x = NDEP >= 2 && ext_x && present[1] && chain_pending(ox);
-            const bool wait_y = NDEP >= 3 && ext_y && present[2] && chain_pending(oy);
-            if (__builtin_amdgcn_ballot_w64(wait_x || wait_y)) {
-                const double2 o = chain_poll(w, dep_row(r, Lc), NDEP >= 3 ? dep_row(r, plane) : 0, wait_x, wait_y,
+            const bool wait_x = NDEP >= 2 && (ext_x & present[1] & chain_pending(ox));
+            const bool wait_y = NDEP >= 3 && (ext_y & present[2] & chain_pending(oy));
+            if (__builtin_expect(__builtin_amdgcn_ballot_w64(wait_x | wait_y) != 0, 0)) {
+                const double2 o = chain_poll(w, row_of(hx + st), NDEP >= 3 ? row_of(hy + st) : 0, wait_x, wait_y,
                                              make_double2(ox, oy), const_cast<unsigned*>(abort_flag));
                 ox = o.x;
                 oy = o.y;
             }
             // ascending columns: lower rows subtract offset Lc*Cx, Lc, 1;
             // upper rows 1, Lc, Lc*Cx
-            double acc = static_cast<double>(rv[j]);
+            double acc = vb[j];
 #pragma unroll
             for (int t = 0; t < NDEP; t++) {
                 const int k = LOWER ? NDEP - 1 - t : t;
@@ -219,36 +304,44 @@ __global__ __launch_bounds__(kChainMaxWaves * kWave) void k_chain(ChainView<MT> 
             }
             if (!UNIT) acc = acc * to_arith<double>(rd[j]);
             acc = chain_pending(acc) ? __longlong_as_double((long long)0x7ffdc3c3c3c3c3c3ull) : acc;
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(chain_u2, acc), rw,
-                                                  act ? r * 8 : kOut, 0, kBufSc1);
-            if (w32)
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)acc), rw32,
-                                                      act ? r * 4 : kOut, 0, 0);
+            res[j] = acc;
             prev = act ? acc : prev;
-            xch[par][wy * tw + gx] = acc;
-            // a barrier that orders the LDS exchange only: the loads issued
-            // for later steps stay in flight across it
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_s_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-            par ^= 1;
             M4_CHAIN_ISSUE(st + U, j);
         }
+        // the block's results to w (8 consecutive rows of a chain per 8
+        // lanes), the next block's v from LDS, the one after it in flight
+#pragma unroll
+        for (int j = 0; j < U; j++) wst[j * RW + l] = res[j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            bool ok;
+            const int32_t r = stage_row(S, i, ok);
+            const double y = wst[jj * RW + (l >> 3) + i * 8];
+            const bool sh = stage_shared(i);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(chain_u2, y), rw, ok && !sh ? r * 8 : kOut, 0,
+                                                  0);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(chain_u2, y), rw, ok && sh ? r * 8 : kOut, 0,
+                                                  kBufSc1);
+            if (w32)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)y), rw32,
+                                                      ok ? r * 4 : kOut, 0, 0);
+        }
+        stage_next();
+        stage_issue(S + 2 * U);
     }
 #undef M4_CHAIN_ISSUE
     wait_memory();
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    if (l == 0) {
         const unsigned done =
             __hip_atomic_fetch_add(counters + 2 * kCounterStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        s_final = done == (unsigned)gridDim.x;
-    }
-    __syncthreads();
-    if (s_final && threadIdx.x == 0) {   // every other workgroup has finished: reset for the next launch
-        if (ld_agent(abort_flag) != 0u) atomicOr(err, 1);
-        st_agent(counters, 0u);
-        st_agent(const_cast<unsigned*>(abort_flag), 0u);
-        st_agent(counters + 2 * kCounterStride, 0u);
+        if (done == (unsigned)gridDim.x) {   // every other tile has finished: reset for the next launch
+            if (ld_agent(abort_flag) != 0u) atomicOr(err, 1);
+            st_agent(counters, 0u);
+            st_agent(const_cast<unsigned*>(abort_flag), 0u);
+            st_agent(counters + 2 * kCounterStride, 0u);
+        }
     }
 }
 

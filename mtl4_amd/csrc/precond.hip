@@ -233,6 +233,7 @@ Sweep::Sweep() = default;
 Sweep::~Sweep() = default;
 
 static std::atomic<int> g_sweep_schedule{MTL4G_SWEEP_AUTO};
+static int g_chain_tile[2] = {0, 0};   // chain tile (Tx, Ty) for factors created later; 0: chosen per grid
 
 // A sweep whose rows depend only on the rows 1, Lc, Lc*Cx away (towards the
 // start for a lower, the end for an upper sweep), never across the edge of a
@@ -308,54 +309,74 @@ static int build_chain(const SweepRows& R, int fmt, Sweep* S) {
     C->Cx = Cx;
     C->Cy = Cy;
     C->ndep = ndep;
-    // tile: up to four waves of 64 chains along x, up to eight waves in all
-    C->txw = (int)std::min<int64_t>(4, ((int64_t)Cx + kWave - 1) / kWave);
-    C->ty = (int)std::min<int64_t>(std::max(1, kChainMaxWaves / C->txw), Cy);
-    C->tiles_x = (int)(((int64_t)Cx + C->txw * kWave - 1) / (C->txw * kWave));
-    C->tiles_y = (int)(((int64_t)Cy + C->ty - 1) / C->ty);
-    const int64_t plane = (int64_t)Lc * Cx;
+    // tile: Tx x Ty = 64 chains, one wave.  Square tiles for 3-D grids: the
+    // hand-overs between tiles on the way from the first to the last chain
+    // number Cx/Tx + Cy/Ty.
+    {
+        int ty = 1;
+        while (ty < 8 && ty < Cy) ty *= 2;
+        C->Tx = kWave / ty;
+        C->Ty = ty;
+    }
+    if (g_chain_tile[0] > 0 && g_chain_tile[1] > 0) {   // set by mtl4g_set_chain_tile
+        C->Tx = g_chain_tile[0];
+        C->Ty = g_chain_tile[1];
+    }
+    C->tiles_x = (int)(((int64_t)Cx + C->Tx - 1) / C->Tx);
+    C->tiles_y = (int)(((int64_t)Cy + C->Ty - 1) / C->Ty);
+    const int64_t T = (int64_t)C->Tx * C->Ty;
+    C->stile = ((int64_t)Lc + C->Tx + C->Ty - 2) * T;
+    C->nslots = C->stile * C->tiles_x * C->tiles_y;
+    // every byte offset the kernel forms stays below its out-of-range
+    // marker 0x7ffffff0; else the level schedule stays
+    if ((C->nslots * ndep + 1024) * 8 >= 0x7ffffff0) return MTL4G_OK;
+    const int64_t ns = C->nslots;
     auto slot = [&](int64_t d) { return d == 1 ? 0 : d == Lc ? 1 : 2; };
     const bool gathered = !R.dsrc.empty();
-    std::vector<int32_t> src(gathered ? (size_t)ndep * n : 0, -1);
-    std::vector<double> val(gathered ? 0 : (size_t)ndep * n, 0.0);
-    std::vector<char> present(gathered ? 0 : (size_t)ndep * n, 0);
-    for (int32_t r = 0; r < n; r++)
+    std::vector<int32_t> src(gathered ? (size_t)ndep * ns : 0, -1), dsrc(gathered ? (size_t)ns : 0, -1);
+    std::vector<double> val(gathered ? 0 : (size_t)ndep * ns, 0.0), dia(gathered ? 0 : (size_t)ns, 1.0);
+    std::vector<char> present(gathered ? 0 : (size_t)ndep * ns, 0);
+    for (int32_t r = 0; r < n; r++) {
+        const int64_t rm = R.lower ? r : (int64_t)n - 1 - r;
+        const int64_t at = chain_slot(rm, Lc, Cx, C->Tx, C->Ty, C->tiles_x, C->stile);
+        if (gathered) dsrc[at] = R.dsrc[r];
+        else if (!R.dia.empty()) dia[at] = R.dia[r];
         for (int32_t j = R.rowptr[r]; j < R.rowptr[r + 1]; j++) {
             const int64_t d = R.lower ? (int64_t)r - R.col[j] : (int64_t)R.col[j] - r;
-            const size_t at = (size_t)slot(d) * n + r;
-            if (gathered) src[at] = R.src[j];
+            const size_t k = (size_t)slot(d) * ns + at;
+            if (gathered) src[k] = R.src[j];
             else {
-                val[at] = R.val[j];
-                present[at] = 1;
+                val[k] = R.val[j];
+                present[k] = 1;
             }
         }
-    (void)plane;
+    }
     auto up = [](auto& buf, const auto& v) -> int {
         using T = typename std::remove_reference<decltype(v)>::type::value_type;
         M4_TRY(buf.alloc(std::max<size_t>(v.size(), 1)));
         if (!v.empty()) M4_HIP(hipMemcpy(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         return MTL4G_OK;
     };
-    const size_t nv = (size_t)ndep * n;
+    const size_t nv = (size_t)ndep * ns;
     if (gathered) {   // values come from the device factor (fill_sweep)
         M4_TRY(up(C->src, src));
-        M4_TRY(up(C->dsrc, R.dsrc));
+        M4_TRY(up(C->dsrc, dsrc));
         if (fmt == MTL4G_F64) {
             M4_TRY(C->val64.alloc(nv));
-            M4_TRY(C->dia64.alloc(n));
+            M4_TRY(C->dia64.alloc(ns));
         } else if (fmt == MTL4G_F32) {
             M4_TRY(C->val32.alloc(nv));
-            M4_TRY(C->dia32.alloc(n));
+            M4_TRY(C->dia32.alloc(ns));
         } else {
             M4_TRY(C->val16.alloc(nv));
-            M4_TRY(C->dia16.alloc(n));
+            M4_TRY(C->dia16.alloc(ns));
         }
     } else {
         auto fill = [&](auto& vbuf, auto& dbuf, auto absent, auto conv) -> int {
             using T = decltype(absent);
-            std::vector<T> v(nv), dg(n);
+            std::vector<T> v(nv), dg(ns);
             for (size_t i = 0; i < nv; i++) v[i] = present[i] ? conv(val[i]) : absent;
-            for (int32_t r = 0; r < n; r++) dg[r] = conv(R.dia.empty() ? 1.0 : R.dia[r]);
+            for (int64_t i = 0; i < ns; i++) dg[i] = conv(dia[i]);
             M4_TRY(up(vbuf, v));
             return up(dbuf, dg);
         };
@@ -374,7 +395,7 @@ static int build_chain(const SweepRows& R, int fmt, Sweep* S) {
 }
 
 int fill_chain(ChainSweep& C, int fmt, const double* fval, hipStream_t s) {
-    const int64_t n = C.n;
+    const int64_t n = C.nslots;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n * C.ndep + 1023) / 1024, 2048));
     const int32_t* dsrc = C.unit ? nullptr : C.dsrc.p;
     if (fmt == MTL4G_F64)
@@ -393,8 +414,8 @@ int fill_chain(ChainSweep& C, int fmt, const double* fval, hipStream_t s) {
 template <typename MT, typename IT, bool LOWER, bool UNIT>
 static void launch_chain_t(const ChainSweep& C, const MT* val, const MT* dia, const IT* v, double* w, float* w32,
                            unsigned* counters, int* err, const int* skip, hipStream_t s) {
-    ChainView<MT> V{val, dia, C.n, C.Lc, C.Cx, C.Cy, C.txw, C.ty, C.tiles_x};
-    const dim3 grid(C.tiles_x * C.tiles_y), block(C.txw * C.ty * kWave);
+    ChainView<MT> V{val, dia, C.n, C.Lc, C.Cx, C.Cy, C.Tx, C.Ty, C.tiles_x, (int32_t)C.stile, (int32_t)C.nslots};
+    const dim3 grid(C.tiles_x * C.tiles_y), block(C.Tx * C.Ty);
     if (C.ndep == 1)
         hipLaunchKernelGGL((k_chain<MT, IT, LOWER, UNIT, 1>), grid, block, 0, s, V, v, w, w32, counters, err, skip);
     else if (C.ndep == 2)
@@ -1204,6 +1225,31 @@ int mtl4g_pc_refactor(mtl4g_pc P, void* stream) {
 int mtl4g_set_sweep_schedule(int schedule) {
     if (schedule != MTL4G_SWEEP_AUTO && schedule != MTL4G_SWEEP_LEVELS) return MTL4G_ERR_INVALID;
     g_sweep_schedule.store(schedule);
+    return MTL4G_OK;
+}
+
+int mtl4g_set_chain_tile(int32_t tx, int32_t ty) {
+    if (tx == 0 && ty == 0) {
+        g_chain_tile[0] = g_chain_tile[1] = 0;
+        return MTL4G_OK;
+    }
+    if (tx < 1 || ty < 1 || (int64_t)tx * ty != kWave) {
+        set_error("mtl4g_set_chain_tile: Tx*Ty must be %d (one wave)", kWave);
+        return MTL4G_ERR_INVALID;
+    }
+    g_chain_tile[0] = tx;
+    g_chain_tile[1] = ty;
+    return MTL4G_OK;
+}
+
+int mtl4g_pc_chain_tile(mtl4g_pc P, int32_t* lower_tx, int32_t* lower_ty, int32_t* upper_tx, int32_t* upper_ty) {
+    if (!P) return MTL4G_ERR_INVALID;
+    const ChainSweep* lo = P->lower.chain.get();
+    const ChainSweep* hi = P->upper.chain.get();
+    if (lower_tx) *lower_tx = lo ? lo->Tx : 0;
+    if (lower_ty) *lower_ty = lo ? lo->Ty : 0;
+    if (upper_tx) *upper_tx = hi ? hi->Tx : 0;
+    if (upper_ty) *upper_ty = hi ? hi->Ty : 0;
     return MTL4G_OK;
 }
 

@@ -50,3 +50,16 @@ def test_null_handles_rejected():
     assert lib.mtl4g_csr_info(None, None, None, None) == _lib.ERR_INVALID
     assert lib.mtl4g_pc_levels(None, None, None) == _lib.ERR_INVALID
     assert lib.mtl4g_csr_create(None, 1, 1, 0, None, None, None, 0) == _lib.ERR_INVALID
+
+
+def test_sweep_schedule_and_chain_tile_arguments():
+    """The schedule and tile setters check their arguments on the host."""
+    from mtl4_amd import _lib
+    lib = _lib.lib()
+    assert lib.mtl4g_set_sweep_schedule(7) == _lib.ERR_INVALID
+    for tx, ty in ((0, 4), (8, 4), (128, 1), (16, 32), (-64, -1)):
+        assert lib.mtl4g_set_chain_tile(tx, ty) == _lib.ERR_INVALID, (tx, ty)
+    for tx, ty in ((64, 1), (8, 8), (1, 64), (0, 0)):
+        assert lib.mtl4g_set_chain_tile(tx, ty) == 0, (tx, ty)
+    assert lib.mtl4g_pc_chain_tile(None, None, None, None, None) == _lib.ERR_INVALID
+    assert lib.mtl4g_pc_sweeps(None, None, None) == _lib.ERR_INVALID

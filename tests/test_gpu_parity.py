@@ -1283,6 +1283,34 @@ def test_pc_chain_schedule_bitwise(m, case, kind, fmt):
         assert np.array_equal(got, orc.Preconditioner(kind, O, 8 if fmt == "f64" else 4).solve(x))
 
 
+@pytest.mark.parametrize("case", ["grid_20x300x3", "grid_40x40x40"])
+def test_pc_chain_tiles_bitwise(m, case):
+    """Every tile shape of the chain pipeline (flat to tall, tiles cut
+    through x and y) gives the level schedule's bits."""
+    rp, ci, va = CHAIN_CASES[case]()
+    O, A = pair(rp, ci, va)
+    L = m._lib.lib()
+    x = np.random.default_rng(7).uniform(-1, 1, A.num_rows())
+    m._lib.check(L.mtl4g_set_sweep_schedule(m._lib.SWEEP_LEVELS))
+    try:
+        ref = {k: getattr(m.pc, k)(A).solve(dv(m, x)).numpy() for k in ("ilu_0", "ic_0")}
+    finally:
+        m._lib.check(L.mtl4g_set_sweep_schedule(m._lib.SWEEP_AUTO))
+    for tx, ty in ((64, 1), (32, 2), (16, 4), (8, 8), (4, 16), (1, 64)):
+        m._lib.check(L.mtl4g_set_chain_tile(tx, ty))
+        try:
+            for k in ("ilu_0", "ic_0"):
+                P = getattr(m.pc, k)(A)
+                t = [C.c_int32(0) for _ in range(4)]
+                m._lib.check(L.mtl4g_pc_chain_tile(P.h, *[C.byref(v) for v in t]))
+                assert [v.value for v in t] == [tx, ty, tx, ty]
+                for _ in range(2):
+                    got = P.solve(dv(m, x)).numpy()
+                    assert np.array_equal(got, ref[k]), (tx, ty, k, np.abs(got - ref[k]).max())
+        finally:
+            m._lib.check(L.mtl4g_set_chain_tile(0, 0))
+
+
 def test_pc_chain_not_for_general_patterns(m):
     rp, ci, va = family("spd_2000", seed=1)
     _, A = pair(rp, ci, va)
