@@ -529,6 +529,30 @@ def serving_section(m, torch, gen, nreq, sweep_requests=400):
     return out
 
 
+def random_spmv(m, torch, gen, reps, copy_calibration=False):
+    """y = A*x on the random N = 2M matrix (Poisson(16) row lengths, uniform
+    columns), the autotuner's kernel: time per launch and GB/s of the CSR
+    bytes."""
+    log("random SpMV: generating")
+    rr = gen.random_sparse(2_000_000, seed=0)
+    R = m.compressed2D(2_000_000, 2_000_000, *rr)
+    R.autotune()
+    xr = m.dense_vector(np.random.default_rng(2).uniform(-1.0, 1.0, 2_000_000))
+    yr = m.dense_vector(2_000_000)
+    t = timed_spmv(m, torch, R, xr, yr, reps)
+    by = spmv_bytes(2_000_000, len(rr[1]))
+    kern = R.get_kernel()
+    if copy_calibration:   # a copy of known size for the PMC byte counters
+        src = torch.rand(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        del src, dst
+    torch.cuda.synchronize()
+    return {"nnz": int(len(rr[1])), "us": round(t * 1e6, 2), "bytes": by, "GBs": round(by / t / 1e9, 1),
+            "kernel": list(kern)}
+
+
 def sweeps_only(m, torch, gen, steps):
     """solve(P, x) of IC(0) on diffusion3d 160^3 and of ILU(0) on the 1024^2
     convection-diffusion matrix, `steps` times each after one warm-up, with
@@ -604,6 +628,9 @@ def main():
                     help="SpMV configuration instead of the autotuner's choice, as "
                          "dia_sym|dia|sliced/<entries per chunk 4|8>/<workgroups per CU> "
                          "(profiling runs, where the profiler distorts the autotuner's timings)")
+    ap.add_argument("--only", default="", choices=["", "batched", "random-spmv"],
+                    help="one measurement alone (profiling runs): the batched section, or the "
+                         "random N = 2M SpMV (with --copy-calibration: plus a 512 MiB device copy)")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=1000)
     args = ap.parse_args()
@@ -618,6 +645,12 @@ def main():
     torch.cuda.set_device(0)
     if args.only_sweeps:
         print(json.dumps(sweeps_only(m, torch, gen, args.steps)))
+        return
+    if args.only == "batched":
+        print(json.dumps(batched_section(m, torch, gen, args.batch_systems)))
+        return
+    if args.only == "random-spmv":
+        print(json.dumps(random_spmv(m, torch, gen, args.spmv_reps, args.copy_calibration)))
         return
 
     g = args.grid
@@ -705,14 +738,8 @@ def main():
         print(json.dumps({"spmv_gbs": spmv_gbs, "roofline": roofline}))
         return
     roofline["hbm_context"] = hbm_context(torch)
-    rr = gen.random_sparse(2_000_000, seed=0)
-    R = m.compressed2D(2_000_000, 2_000_000, *rr)
-    R.autotune()
-    xr = m.dense_vector(np.random.default_rng(2).uniform(-1.0, 1.0, 2_000_000))
-    yr = m.dense_vector(2_000_000)
-    t_rnd = timed_spmv(m, torch, R, xr, yr, args.spmv_reps)
-    rnd_gbs = spmv_bytes(2_000_000, len(rr[1])) / t_rnd / 1e9
-    del R, xr, yr
+    rnd = random_spmv(m, torch, gen, args.spmv_reps)
+    rnd_gbs = rnd["GBs"]
 
     log("SpMV %.1f GB/s; FP64 cg + diagonal" % spmv_gbs)
     # ---- FP64 CG + diagonal: the drop-in path
