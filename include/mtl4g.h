@@ -169,6 +169,13 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
  * bit for bit; only the diagonals with offset >= 0 are stored, the entries
  * below the diagonal are read from their mirror images (the value stream
  * is about half); refused for other matrices.
+ * MTL4G_SPMV_PANEL: the columns split into panels of `rows_per_block`
+ * columns (0 = 262144: 2 MB of FP64 x, within one XCD's L2), one CSR copy
+ * of each panel's entries, the stream kernel over each panel in column
+ * order with every row's partial sum carried from panel to panel (a
+ * stream-ordered scratch vector per call), `group` workgroups per CU; for
+ * matrices whose x gathers scatter over a vector larger than the L2;
+ * refused when a row's columns are not ascending.
  * All sum every row in column order.  0 = library default. */
 #define MTL4G_SPMV_TILE   0
 #define MTL4G_SPMV_VECTOR 1
@@ -178,6 +185,7 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where);
 #define MTL4G_SPMV_SLICED 5
 #define MTL4G_SPMV_DIA 6
 #define MTL4G_SPMV_DIA_SYM 7
+#define MTL4G_SPMV_PANEL 8
 int mtl4g_csr_tune(mtl4g_matrix A, int32_t rows_per_block, int32_t nnz_per_block);
 int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows_per_block);
 int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* rows_per_block);

@@ -39,6 +39,7 @@ class IRStats(C.Structure):
 
 
 SPMV_TILE, SPMV_VECTOR, SPMV_STREAM, SPMV_SCALAR, SPMV_STREAM2, SPMV_SLICED, SPMV_DIA, SPMV_DIA_SYM = 0, 1, 2, 3, 4, 5, 6, 7
+SPMV_PANEL = 8
 ORDER_TREE, ORDER_MTL4 = 0, 1
 SWEEP_AUTO, SWEEP_LEVELS = 0, 1
 

@@ -323,6 +323,20 @@ struct mtl4g_matrix_s {
     mtl4g::DevBuf<float> dsym_val32;
     mtl4g::DevBuf<uint16_t> dsym_valbf;
     int ndsym = 0, nsym = 0;
+    // column panels (MTL4G_SPMV_PANEL): for each panel of pan_w columns, the
+    // CSR of the entries in those columns (rows in order, columns ascending),
+    // panel after panel; absolute entry indices
+    int npan = 0;
+    int32_t pan_w = 0;
+    int pan_tile = 2048;                    // non-zeros per row block (1024 or 2048)
+    std::vector<int32_t> h_pan_src;         // CSR index of each panel entry
+    std::vector<int> h_pan_nb;              // row blocks of each panel
+    mtl4g::DevBuf<int32_t> pan_rowptr;      // npan * (n + 1)
+    mtl4g::DevBuf<int32_t> pan_col;
+    mtl4g::DevBuf<double> pan_val;
+    mtl4g::DevBuf<float> pan_val32;
+    mtl4g::DevBuf<uint16_t> pan_valbf;
+    mtl4g::DevBuf<mtl4g::RowBlock> pan_desc;
     int sl_unroll = 8;                      // sliced / diagonal: entries per lane per chunk (4 or 8)
     int sl_unroll_lowp[2] = {8, 8};         // the same for FP32 / BF16 values
     int32_t max_row = 0;

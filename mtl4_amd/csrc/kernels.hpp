@@ -142,6 +142,38 @@ struct pair_of<uint16_t> {
 };
 typedef int int_pair __attribute__((ext_vector_type(2)));
 
+// A row's sum starts at epi.init(row) when the epilogue has one (a column
+// panel continuing the sums of the panels before it), else at 0.
+template <typename E, typename = void>
+struct has_init : std::false_type {};
+template <typename E>
+struct has_init<E, std::void_t<decltype(std::declval<const E&>().init(0))>> : std::true_type {};
+
+// Column panels: every panel but the last stores each row's partial sum
+// (started from the previous panel's) in t; the last one starts from t and
+// runs the caller's epilogue on the full sum.  The caller's skip() holds for
+// every panel.
+template <typename AT, typename Epi>
+struct EpiPanelCarry {
+    static constexpr int NP = 0;
+    struct Pre {};
+    const AT* tin;   // null for the first panel
+    AT* tout;
+    Epi inner;
+    __device__ bool skip() const { return inner.skip(); }
+    __device__ void prepare() {}
+    __device__ Pre preload(int) const { return Pre{}; }
+    __device__ AT init(int row) const { return tin ? tin[row] : AT(0); }
+    __device__ void operator()(int row, AT s, const Pre&, AT*) const { tout[row] = s; }
+    __device__ void finish(const AT*) const {}
+};
+template <typename AT, typename Epi>
+struct EpiPanelLast : Epi {
+    const AT* tin;
+    EpiPanelLast(const Epi& e, const AT* t) : Epi(e), tin(t) {}
+    __device__ AT init(int row) const { return tin[row]; }
+};
+
 template <int BS, int TILE, typename AT, typename MT, typename XT, typename Epi, int RPL = 1>
 __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
                                              const int32_t* __restrict__ col,
@@ -232,6 +264,9 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
                 s += load_val<AT>(val, (int64_t)dc.j0 + k) * static_cast<AT>(x[col[dc.j0 + k]]);
             s = block_sum<BS>(s, sh);
             if (bn < hi) fetch(bn);
+            if constexpr (has_init<Epi>::value) {
+                if (t == 0) s = epi.init(row) + s;
+            }
             if (t == 0) epi(row, s, pc[0], part);
         } else {
             // tile-relative index of the element lane t holds in slot u
@@ -257,6 +292,9 @@ __global__ __launch_bounds__(BS) void k_csrp(const int32_t* __restrict__ rowptr,
 #pragma unroll
             for (int q = 0; q < RPL; q++) {
                 AT s = AT(0);
+                if constexpr (has_init<Epi>::value) {
+                    if (row + q * BS < dc.r1) s = epi.init(row + q * BS);
+                }
                 for (int k = a[q]; k < e[q]; k++) s += prod[buf][k];
                 if (row + q * BS < dc.r1) epi(row + q * BS, s, pc[q], part);
             }
@@ -1124,6 +1162,53 @@ int launch_csr(const mtl4g_matrix_s* A, const MT* val, const XT* x, const Epi& e
 #undef M4_LAUNCHY
         M4_HIP(hipGetLastError());
         return MTL4G_OK;
+    }
+    if (A->kernel == MTL4G_SPMV_PANEL) {
+        // the panel copy of the value array the caller passed
+        const MT* pv = nullptr;
+        if constexpr (std::is_same<MT, double>::value) pv = val == A->val.p ? A->pan_val.p : nullptr;
+        else if constexpr (std::is_same<MT, float>::value) pv = val == A->val32.p ? A->pan_val32.p : nullptr;
+        else pv = val == A->valbf.p ? A->pan_valbf.p : nullptr;
+        if (!pv || !A->npan) {
+            set_error("panel SpMV: no panel copy of these values");
+            return MTL4G_ERR_INVALID;
+        }
+        int pcu = A->per_cu;
+        const int li = sizeof(MT) == 4 ? 0 : 1;
+        if (sizeof(MT) < 8 && A->per_cu_lowp[li] > 0) pcu = A->per_cu_lowp[li];
+        // each call's partial sums live in a stream-ordered scratch vector:
+        // calls on other streams never share it
+        AT* t = nullptr;
+        if (A->npan > 1) M4_HIP(hipMallocAsync((void**)&t, (size_t)A->n * sizeof(AT), s));
+        int64_t boff = 0;
+        int st = MTL4G_OK;
+        for (int p = 0; p < A->npan && !st; p++) {
+            const int32_t* rp = A->pan_rowptr.p + (size_t)p * (A->n + 1);
+            const RowBlock* d = A->pan_desc.p + boff;
+            const int nb = A->h_pan_nb[p];
+            boff += nb;
+            if (nb == 0) continue;
+            if (p + 1 < A->npan) {
+                using E = EpiPanelCarry<AT, Epi>;
+                const E e{p ? t : nullptr, t, epi};
+                const int g = persistent_grid<k_csrp<256, 2048, AT, MT, XT, E>>(A->cus, pcu, nb);
+                hipLaunchKernelGGL((k_csrp<256, 2048, AT, MT, XT, E>), dim3(g), dim3(256), 0, s, rp, A->pan_col.p,
+                                   pv, x, d, nb, e, (AT*)nullptr, (unsigned*)nullptr);
+            } else if (A->npan > 1) {
+                using E = EpiPanelLast<AT, Epi>;
+                const E e(epi, t);
+                const int g = persistent_grid<k_csrp<256, 2048, AT, MT, XT, E>>(A->cus, pcu, nb);
+                hipLaunchKernelGGL((k_csrp<256, 2048, AT, MT, XT, E>), dim3(g), dim3(256), 0, s, rp, A->pan_col.p,
+                                   pv, x, d, nb, e, partials, ticket);
+            } else {
+                const int g = persistent_grid<k_csrp<256, 2048, AT, MT, XT, Epi>>(A->cus, pcu, nb);
+                hipLaunchKernelGGL((k_csrp<256, 2048, AT, MT, XT, Epi>), dim3(g), dim3(256), 0, s, rp, A->pan_col.p,
+                                   pv, x, d, nb, epi, partials, ticket);
+            }
+            if (hipGetLastError() != hipSuccess) st = MTL4G_ERR_HIP;
+        }
+        if (t) (void)hipFreeAsync(t, s);
+        return st;
     }
     if (A->nb == 0) return MTL4G_OK;
     if (A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2) {

@@ -196,6 +196,132 @@ static int build_sliced(mtl4g_matrix_s* A) {
     return MTL4G_OK;
 }
 
+// ---- column panels
+static void free_panels(mtl4g_matrix_s* A) {
+    A->npan = 0;
+    A->pan_w = 0;
+    A->h_pan_src.clear();
+    A->h_pan_nb.clear();
+    A->pan_rowptr.free();
+    A->pan_col.free();
+    A->pan_val.free();
+    A->pan_val32.free();
+    A->pan_valbf.free();
+    A->pan_desc.free();
+}
+
+// the panel copy of the values: FP64 always, FP32 / BF16 for the formats in use
+static int panel_values(mtl4g_matrix_s* A) {
+    const size_t ne = A->h_pan_src.size();
+    std::vector<double> v(ne + 2, 0.0);   // padding for the aligned pair loads
+    for (size_t i = 0; i < ne; i++) v[i] = A->h_val[A->h_pan_src[i]];
+    M4_TRY(upload(A->pan_val, v));
+    A->pan_val32.free();
+    A->pan_valbf.free();
+    if (A->val32.p) {
+        std::vector<float> f(v.size());
+        for (size_t i = 0; i < v.size(); i++) f[i] = (float)v[i];
+        M4_TRY(upload(A->pan_val32, f));
+    }
+    if (A->valbf.p) {
+        std::vector<uint16_t> h(v.size());
+        for (size_t i = 0; i < v.size(); i++) h[i] = f32_to_bf16((float)v[i]);
+        M4_TRY(upload(A->pan_valbf, h));
+    }
+    return MTL4G_OK;
+}
+
+static int panel_lowp_values(mtl4g_matrix_s* A, int fmt) {
+    if (!A->npan) return MTL4G_OK;
+    const size_t ne = A->h_pan_src.size();
+    if (fmt == MTL4G_F32 && !A->pan_val32.p) {
+        std::vector<float> f(ne + 2, 0.0f);
+        for (size_t i = 0; i < ne; i++) f[i] = (float)A->h_val[A->h_pan_src[i]];
+        M4_TRY(upload(A->pan_val32, f));
+    }
+    if (fmt == MTL4G_BF16 && !A->pan_valbf.p) {
+        std::vector<uint16_t> h(ne + 2, 0);
+        for (size_t i = 0; i < ne; i++) h[i] = f32_to_bf16((float)A->h_val[A->h_pan_src[i]]);
+        M4_TRY(upload(A->pan_valbf, h));
+    }
+    return MTL4G_OK;
+}
+
+// Split the columns into panels of w columns: per panel the CSR of its
+// entries (absolute indices into the panel-major arrays) and its row blocks
+// of at most 256 rows and `tile` - 2 non-zeros.  Every row's entries must
+// be in ascending column order, so that summing panel after panel is
+// summing in column order.
+static int build_panels(mtl4g_matrix_s* A, int32_t w) {
+    if (w <= 0) w = 262144;
+    if (A->npan && A->pan_w == w) return MTL4G_OK;
+    free_panels(A);
+    const int32_t n = A->n;
+    const std::vector<int32_t>& rp = A->h_rowptr;
+    const std::vector<int32_t>& ci = A->h_col;
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t j = rp[r] + 1; j < rp[r + 1]; j++)
+            if (ci[j] <= ci[j - 1]) {
+                set_error("column panels: row %d's columns are not ascending", r);
+                return MTL4G_ERR_UNSUPPORTED;
+            }
+    const int npan = (int)std::max<int64_t>(1, ((int64_t)A->m + w - 1) / w);
+    // entries per (panel, row), then absolute row pointers panel after panel
+    std::vector<int32_t> prp((size_t)npan * (n + 1), 0);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t j = rp[r]; j < rp[r + 1]; j++) prp[(size_t)(ci[j] / w) * (n + 1) + r + 1]++;
+    int64_t off = 0;
+    for (int p = 0; p < npan; p++) {
+        int32_t* q = prp.data() + (size_t)p * (n + 1);
+        q[0] = (int32_t)off;
+        for (int32_t r = 0; r < n; r++) q[r + 1] += q[r];
+        off = q[n];
+    }
+    std::vector<int32_t> cur(prp);
+    std::vector<int32_t> pc((size_t)A->nnz + 2, 0);   // padding for the aligned pair loads
+    A->h_pan_src.assign((size_t)A->nnz, 0);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t j = rp[r]; j < rp[r + 1]; j++) {
+            const int p = ci[j] / w;
+            const int32_t at = cur[(size_t)p * (n + 1) + r]++;
+            pc[at] = ci[j];
+            A->h_pan_src[at] = j;
+        }
+    int32_t maxblk = 0;
+    for (int p = 0; p < npan; p++) {
+        const int32_t* q = prp.data() + (size_t)p * (n + 1);
+        for (int32_t r = 0; r < n; r++) maxblk = std::max(maxblk, q[r + 1] - q[r]);
+    }
+    A->pan_tile = 2048;
+    std::vector<RowBlock> desc;
+    A->h_pan_nb.assign(npan, 0);
+    for (int p = 0; p < npan; p++) {
+        const int32_t* q = prp.data() + (size_t)p * (n + 1);
+        const int cap = A->pan_tile - 2;   // headroom for the aligned pair loads
+        int32_t r = 0;
+        while (r < n) {
+            const int32_t r0 = r;
+            int64_t nz = 0;
+            while (r < n && r - r0 < 256 && nz + (q[r + 1] - q[r]) <= cap) {
+                nz += q[r + 1] - q[r];
+                r++;
+            }
+            if (r == r0) r++;
+            desc.push_back(RowBlock{r0, r, q[r0], q[r]});
+            A->h_pan_nb[p]++;
+        }
+    }
+    int st;
+    if ((st = upload(A->pan_rowptr, prp)) || (st = upload(A->pan_col, pc)) || (st = upload(A->pan_desc, desc)) ||
+        (st = panel_values(A))) {
+        free_panels(A);
+        return st;
+    }
+    A->npan = npan;
+    A->pan_w = w;
+    return MTL4G_OK;
+}
+
 // ---- diagonal copy of the matrix
 // Distinct diagonals (column - row) in ascending order, or empty when there
 // are more than kDiaMax of them.
@@ -405,6 +531,7 @@ static int build_dsym(mtl4g_matrix_s* A) {
 int ensure_lowp_values(mtl4g_matrix_s* A, int fmt, hipStream_t s) {
     if (fmt == MTL4G_F32 || fmt == MTL4G_BF16) {
         M4_TRY(sliced_lowp_values(A, fmt));
+        M4_TRY(panel_lowp_values(A, fmt));
         M4_TRY(dia_values(A, fmt));
         M4_TRY(dsym_values(A, fmt));
     }
@@ -931,6 +1058,7 @@ int mtl4g_csr_set_values(mtl4g_matrix A, const double* val, int where) {
     A->valbf.free();
     A->sl_val32.free();
     A->sl_valbf.free();
+    if (A->npan) M4_TRY(panel_values(A));
     if (!A->h_sl_off.empty()) {
         std::vector<double> v;
         sliced_fill(A, A->h_val.data(), 0.0, [](double d) { return d; }, v);
@@ -1006,6 +1134,20 @@ int mtl4g_csr_set_kernel(mtl4g_matrix A, int kernel, int32_t group, int32_t rows
         A->lowp_wide[0] = A->lowp_wide[1] = false;
         return MTL4G_OK;
     }
+    if (kernel == MTL4G_SPMV_PANEL) {
+        if (group == 0) group = A->per_cu;
+        if (group < 1 || group > 8 || rows_per_block < 0) {
+            set_error("panel kernel: 1 to 8 workgroups per compute unit, a positive panel width");
+            return MTL4G_ERR_INVALID;
+        }
+        M4_TRY(build_panels(A, rows_per_block));
+        A->kernel = kernel;
+        A->per_cu = group;
+        A->lowp_tuned[0] = A->lowp_tuned[1] = false;
+        A->per_cu_lowp[0] = A->per_cu_lowp[1] = 0;
+        A->lowp_wide[0] = A->lowp_wide[1] = false;
+        return MTL4G_OK;
+    }
     if (kernel == MTL4G_SPMV_SCALAR) {
         if (group == 0) group = A->uv;
         if (group != 4 && group != 8) {
@@ -1045,7 +1187,7 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
         *group = A->kernel == MTL4G_SPMV_VECTOR   ? A->gs
                  : A->kernel == MTL4G_SPMV_STREAM || A->kernel == MTL4G_SPMV_STREAM2 ||
                            A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA ||
-                           A->kernel == MTL4G_SPMV_DIA_SYM
+                           A->kernel == MTL4G_SPMV_DIA_SYM || A->kernel == MTL4G_SPMV_PANEL
                      ? A->per_cu
                  : A->kernel == MTL4G_SPMV_SCALAR ? A->uv
                                                   : 0;
@@ -1054,6 +1196,7 @@ int mtl4g_csr_get_kernel(mtl4g_matrix A, int* kernel, int32_t* group, int32_t* r
                           : A->kernel == MTL4G_SPMV_SLICED || A->kernel == MTL4G_SPMV_DIA ||
                                     A->kernel == MTL4G_SPMV_DIA_SYM
                               ? A->sl_unroll
+                          : A->kernel == MTL4G_SPMV_PANEL ? A->pan_w
                                                            : A->rows_per_block;
     return MTL4G_OK;
 }
@@ -1089,6 +1232,18 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     if (had_dsym || build_dsym(A) == MTL4G_OK)
         for (int u : {4, 8})
             for (int f : {2, 4, 6, 8}) cands.push_back({MTL4G_SPMV_DIA_SYM, f, u, 0, 0});
+    // column panels where x is larger than two L2s and most gathers land far
+    // from the row's own column (sampled every 61st row)
+    const bool had_panels = A->npan > 0;
+    if ((int64_t)A->m * 8 > (8 << 20) && A->n == A->m) {
+        int64_t far = 0, tot = 0;
+        for (int32_t r = 0; r < A->n; r += 61)
+            for (int32_t j = A->h_rowptr[r]; j < A->h_rowptr[r + 1]; j++, tot++)
+                far += std::abs((int64_t)A->h_col[j] - r) * 8 > (2 << 20);
+        if (had_panels || (tot > 0 && far * 2 > tot))
+            for (int w : {262144, 524288, 1048576})
+                for (int f : {2, 3, 4, 6}) cands.push_back({MTL4G_SPMV_PANEL, f, w, 0, 0});
+    }
     // wide vector groups (16+ lanes) only pay for rows of hundreds of non-zeros
     const double avg = A->n ? (double)A->nnz / A->n : 0.0;
     for (int g : {4, 8, 16, 32, 64}) {
@@ -1132,6 +1287,7 @@ int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream) {
     if (win.kernel != MTL4G_SPMV_SLICED && !had_sliced) free_sliced(A);
     if (win.kernel != MTL4G_SPMV_DIA && !had_dia) free_dia(A);
     if (win.kernel != MTL4G_SPMV_DIA_SYM && !had_dsym) free_dsym(A);
+    if (win.kernel != MTL4G_SPMV_PANEL && !had_panels) free_panels(A);
     if (win.kernel == MTL4G_SPMV_STREAM || win.kernel == MTL4G_SPMV_STREAM2)
         M4_TRY(mtl4g_csr_tune(A, win.rows, win.nnz));
     M4_TRY(win.kernel == MTL4G_SPMV_TILE ? mtl4g_csr_tune(A, win.a, win.b)

@@ -114,6 +114,64 @@ def test_spmv_stream_bitwise(m, fam, shape, depth):
         assert np.linalg.norm(sols[0][1] - sols[1][1]) <= 1e-9 * np.linalg.norm(sols[1][1])
 
 
+@pytest.mark.parametrize("width", [1, 7, 256, 100000])
+@pytest.mark.parametrize("fam", ["random_3000", "long_rows", "spd_2000", "lap2d_64", "diff3d_16"])
+def test_spmv_panel_bitwise(m, fam, width):
+    """Column panels of `width` columns (one panel per column up to a single
+    panel): every row summed panel after panel in column order gives MTL4's
+    row sums bit for bit, for =, += and -=, FP32 values give the stream
+    kernel's bits, and CG with the fused product+dot on the panels solves as
+    with the stream kernel."""
+    import torch
+    rp, ci, va = family(fam, seed=6)
+    O, A = pair(rp, ci, va)
+    A.set_kernel(m._lib.SPMV_PANEL, 2, width)
+    assert A.get_kernel() == (m._lib.SPMV_PANEL, 2, width)
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 1, A.num_cols())
+    y0 = rng.uniform(-1, 1, A.num_rows())
+    assert np.array_equal((A * dv(m, x)).numpy(), orc.spmv(O, x))
+    for mode in (m.PLUS, m.MINUS):
+        w = dv(m, y0)
+        A.mult(dv(m, x), w, mode)
+        yr = y0.copy()
+        orc.spmv(O, x, yr, mode)
+        assert np.array_equal(w.numpy(), yr)
+    # FP32 values: the same bits as the stream kernel's (same row order)
+    L = m._lib.lib()
+    x32 = torch.tensor(x, dtype=torch.float32, device="cuda")
+    outs = []
+    for k in (m._lib.SPMV_PANEL, m._lib.SPMV_STREAM):
+        if k == m._lib.SPMV_STREAM:
+            A.set_kernel(k, 2, 0)
+        y32 = torch.zeros(A.num_rows(), dtype=torch.float32, device="cuda")
+        m._lib.check(L.mtl4g_spmv_f32(A.h, m.F32, x32.data_ptr(), y32.data_ptr(), m.ASSIGN,
+                                      torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        outs.append(y32.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    if fam in ("spd_2000", "diff3d_16", "lap2d_64") and width > 1:
+        b = orc.spmv(O, x)
+        sols = []
+        for k in (m._lib.SPMV_PANEL, m._lib.SPMV_STREAM):
+            A.set_kernel(k, 2, width if k == m._lib.SPMV_PANEL else 0)
+            xs = m.dense_vector(A.num_rows())
+            it = m.basic_iteration(dv(m, b), 3000, 1e-10)
+            assert m.cg(A, xs, dv(m, b), m.pc.diagonal(A), it) == 0
+            sols.append((it.iterations(), xs.numpy()))
+        assert abs(sols[0][0] - sols[1][0]) <= 2
+        assert np.linalg.norm(sols[0][1] - sols[1][1]) <= 1e-9 * np.linalg.norm(sols[1][1])
+
+
+def test_spmv_panel_needs_ascending_rows(m):
+    """The panel walk sums in column order only for ascending rows; the
+    matrix constructor already refuses any other row."""
+    rp = np.array([0, 2, 3], np.int32)
+    ci = np.array([1, 0, 1], np.int32)
+    with pytest.raises(m.MtlError):
+        m.compressed2D(2, 2, rp, ci, np.array([1.0, 2.0, 3.0]))
+
+
 @pytest.mark.parametrize("uv", [4, 8])
 @pytest.mark.parametrize("fam", ["lap2d_5", "lap2d_64", "random_3000", "long_rows", "diff3d_16"])
 def test_spmv_scalar_bitwise(m, fam, uv):
