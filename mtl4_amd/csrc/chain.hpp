@@ -214,7 +214,10 @@ __global__ __launch_bounds__(kWave) void k_chain(ChainView<MT> C, const IT* __re
         return row_of(ok ? (ccy * C.Cx + ccx) * Lc + e : 0);
     };
     // chains that another tile reads: their rows are stored at agent scope
-    // (through to memory), the others with plain stores that complete in L2
+    // (through to memory), the others with plain stores that complete in L2.
+    // Ordering (the inter-workgroup protocol of internal.hpp): the consumer
+    // polls each value itself for the pending pattern with agent-scope
+    // loads, so no announcement is needed.
     auto stage_shared = [=](int i) -> bool {
         const int c = (l >> 3) + i * 8;
         const int ccx = x0 + c % C.Tx, ccy = y0 + c / C.Tx;

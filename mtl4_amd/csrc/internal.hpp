@@ -114,6 +114,22 @@ __device__ __forceinline__ AT load_val_nt(const MT* p, int64_t j) {
     return to_arith<AT>(__builtin_nontemporal_load(p + j));
 }
 
+// Ordering between workgroups of one kernel.  A value handed to another
+// workgroup (a partial sum, a level's or a chain's result, a factor row) is
+// written with an agent-scope atomic or agent-scope store, and read with
+// agent-scope loads: on gfx950 both are performed at the device-coherent
+// level (the sc1 cache policy), never in a per-XCD L2 line that another XCD
+// cannot see.  The write that announces such values (a ticket increment, a
+// level counter) is issued only after s_waitcnt 0 (wait_memory) has seen
+// every earlier write of the wave complete, i.e. acknowledged by that
+// level; a workgroup that reads the announcement with an agent-scope load
+// therefore reads the values too.  This is the order a release/acquire pair
+// at agent scope gives; it is built by hand because an agent-scope release
+// on gfx950 also writes back the whole L2 (buffer_wbl2), which cost six
+// times a CG iteration.  Readers that poll a value itself (the pending
+// patterns of the triangular sweeps) need no announcement: the value and
+// its validity are one 8-byte write.  Every use site names this protocol.
+//
 // Agent-scope loads/stores for values exchanged between workgroups
 // inside one kernel (partials, level flags); they bypass non-coherent
 // per-XCD cache lines.
@@ -164,13 +180,15 @@ __device__ __forceinline__ void publish(float* p, float v) {
 }
 
 // Wait until this wave's memory operations (the publishes above) have
-// completed, before the atomic that announces them.
+// completed, before the atomic that announces them (the hand-made release
+// of the protocol above).
 __device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
 
 // Last-block pattern: lane 0 of every block publishes its partials, then the
 // block that arrives last (ticket nblocks-1) reduces them in index order with
 // agent-scope loads and resets the ticket for the next launch (no memset
-// needed, graph-replay safe).
+// needed, graph-replay safe).  Ordering: the inter-workgroup protocol above
+// (publish, wait_memory, ticket; the last block reads with agent-scope loads).
 template <int BS>
 __device__ __forceinline__ bool arrive_last(unsigned* ticket, unsigned nblocks) {
     __shared__ int am_last;

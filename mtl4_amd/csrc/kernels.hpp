@@ -45,6 +45,8 @@ constexpr size_t kPartialWords = kMaxNP * (size_t)(kMaxPartials + kMaxGroups);
 // one per workgroup, and the result does not depend on arrival order.
 // partials: NP*G block values then NP*ngroups group values; tickets: words
 // [(1+g)*kTicketStride] per group and [0] on top, all left at zero.
+// Ordering: the inter-workgroup protocol of internal.hpp (publish, then
+// arrive_last's wait_memory and ticket; agent-scope reads of the partials).
 template <int BS, typename AT, typename Epi>
 __device__ __forceinline__ void combine_partials(const AT* part, AT* partials, unsigned* tickets,
                                                  int b, int G, const Epi& epi, AT* sh) {

@@ -167,6 +167,9 @@ __global__ __launch_bounds__(BS) void k_sweep(SweepView<MT, VT> S, const IT* __r
         st_agent(w + r, rr);   // the L1 writes through; wait_memory() below waits for it
         if (w32) w32[r] = (float)rr;
     }
+    // ordering (the inter-workgroup protocol of internal.hpp): the rows'
+    // agent-scope stores complete before the level counter announces them;
+    // the next level's rows also poll the values themselves
     wait_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -631,6 +634,9 @@ __global__ __launch_bounds__(BS) void k_factor(FactorView F, const int32_t* __re
         if (IC) ic0_row(q, r, F, pos, fv, finv);
         else ilu0_row(r, rp, ci, pos, fv, finv);
     }
+    // ordering (the inter-workgroup protocol of internal.hpp): the level's
+    // agent-scope row writes complete before its counter announces them to
+    // the next level, which reads the rows with agent-scope loads
     wait_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
