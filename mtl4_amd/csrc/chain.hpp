@@ -118,8 +118,9 @@ __device__ __forceinline__ uint16_t chain_ld<uint16_t>(__amdgpu_buffer_rsrc_t r,
 // whole wave by one lane, no LDS.
 __device__ __forceinline__ double chain_from_prev_lane(double v) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)b, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
-    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(b >> 32), 0x138, 0xf, 0xf, false);
+    // bound_ctrl: lane 0 (no source) reads 0, no initial value to set
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, true);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
@@ -130,29 +131,6 @@ __device__ __forceinline__ double chain_from(double v, int src4) {
     const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(src4, (int)(unsigned)b);
     const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(src4, (int)(unsigned)(b >> 32));
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Waits (bounded) until the values another workgroup stores at w[ix]
-// (when wx) and w[iy] (when wy) are no longer the pending marker; returns
-// them in x and y (the others come back unchanged).  A separate function on
-// purpose: the kernel calls it only when a lane of the wave needs it, and
-// the kernel's own step keeps no loop (nor the register copies a loop's
-// operands need) between a load and its use.
-__device__ __noinline__ double2 chain_poll(const double* w, int32_t ix, int32_t iy, bool wx, bool wy, double2 o,
-                                           unsigned* abort_flag) {
-    unsigned it = 0;
-    while (wx || wy) {
-        if (++it > (1u << 22) || ((it & 63u) == 0 && ld_agent(abort_flag) != 0u)) {
-            st_agent(abort_flag, 1u);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        if (wx) o.x = ld_agent(w + ix);
-        if (wy) o.y = ld_agent(w + iy);
-        wx = wx && chain_pending(o.x);
-        wy = wy && chain_pending(o.y);
-    }
-    return o;
 }
 
 // The pipeline kernel, one wave per tile.  LOWER: rows depend on smaller
@@ -193,7 +171,9 @@ __global__ __launch_bounds__(kWave) void k_chain(ChainView<MT> C, const IT* __re
     const __amdgpu_buffer_rsrc_t rval = chain_rsrc(C.val, NDEP * C.nslots * (int32_t)sizeof(MT));
     const __amdgpu_buffer_rsrc_t rdia = chain_rsrc(C.dia, C.nslots * (int32_t)sizeof(MT));
     const __amdgpu_buffer_rsrc_t rw = chain_rsrc(w, n * 8);
-    const __amdgpu_buffer_rsrc_t rw32 = chain_rsrc(w32, n * 4);
+    // no w32: a descriptor of size 0, every store through it is dropped (no
+    // branch around the store keeps the wait counts exact)
+    const __amdgpu_buffer_rsrc_t rw32 = chain_rsrc(w32, w32 ? n * 4 : 0);
     const unsigned* abort_flag = counters + kCounterStride;
     auto row_of = [=](int32_t rm) -> int32_t { return LOWER ? rm : n - 1 - rm; };
     // byte offsets of this lane's slot at the tile's first step; a step adds
@@ -210,19 +190,10 @@ __global__ __launch_bounds__(kWave) void k_chain(ChainView<MT> C, const IT* __re
         const int c = (l >> 3) + i * 8;
         const int ccx = x0 + c % C.Tx, ccy = y0 + c / C.Tx;
         const int e = S + jj - ccx - ccy;
-        ok = ccx < C.Cx && ccy < C.Cy && e >= 0 && e < Lc;
+        // bitwise tests, no short-circuit branches: every path issues the
+        // same memory operations, which keeps the wait counts exact
+        ok = (ccx < C.Cx) & (ccy < C.Cy) & ((unsigned)e < (unsigned)Lc);
         return row_of(ok ? (ccy * C.Cx + ccx) * Lc + e : 0);
-    };
-    // chains that another tile reads: their rows are stored at agent scope
-    // (through to memory), the others with plain stores that complete in L2.
-    // Ordering (the inter-workgroup protocol of internal.hpp): the consumer
-    // polls each value itself for the pending pattern with agent-scope
-    // loads, so no announcement is needed.
-    auto stage_shared = [=](int i) -> bool {
-        const int c = (l >> 3) + i * 8;
-        const int ccx = x0 + c % C.Tx, ccy = y0 + c / C.Tx;
-        return (NDEP >= 2 && c % C.Tx == C.Tx - 1 && ccx + 1 < C.Cx) ||
-               (NDEP >= 3 && c / C.Tx == C.Ty - 1 && ccy + 1 < C.Cy);
     };
     IT vin[U];       // v of the block after the next one, in flight
     double vb[U];    // v of this block (this lane's chain)
@@ -235,9 +206,16 @@ __global__ __launch_bounds__(kWave) void k_chain(ChainView<MT> C, const IT* __re
             vin[i] = chain_ld<IT>(rv_, ok ? r * (int32_t)sizeof(IT) : kOut);
         }
     };
-    auto stage_next = [&]() {   // vin -> LDS -> vb
+    // vin -> LDS -> vb.  An input equal to the pending pattern is quieted
+    // here: a result that is v itself (no subtraction, unit diagonal) must
+    // not read as pending, and arithmetic never yields that signalling NaN.
+    auto stage_next = [&]() {
 #pragma unroll
-        for (int i = 0; i < U; i++) vst[jj * RW + (l >> 3) + i * 8] = static_cast<double>(vin[i]);
+        for (int i = 0; i < U; i++) {
+            const double vi = static_cast<double>(vin[i]);
+            vst[jj * RW + (l >> 3) + i * 8] =
+                chain_pending(vi) ? __longlong_as_double((long long)0x7ffdc3c3c3c3c3c3ull) : vi;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 #pragma unroll
@@ -265,54 +243,90 @@ __global__ __launch_bounds__(kWave) void k_chain(ChainView<MT> C, const IT* __re
     stage_issue(s0 + U);
 #pragma unroll
     for (int j = 0; j < U; j++) M4_CHAIN_ISSUE(s0 + j, j);
-    double prev = 0.0;   // the chain's element of the last step it had one; the neighbours read it
-    for (int S = s0; S <= s1; S += U) {
+    double prev = 0.0;   // the chain's element of the last step; the neighbours read it
+    // two blocks per iteration: the loads issued in the first block for the
+    // second stay inside one iteration, whose wait counts the compiler keeps
+    // exact; only those carried into the next iteration may wait longer.
+    // Steps past s1 run on padding and store nothing.
+    for (int S2 = s0; S2 <= s1; S2 += 2 * U)
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        const int S = S2 + half * U;
 #pragma unroll
         for (int j = 0; j < U; j++) {
             const int st = S + j;
-            if (st > s1) break;   // uniform over the wave
-            const bool act = chain && (unsigned)(st - e0) < (unsigned)Lc;
+            // steps past s1 (the last block's tail) run on padding and
+            // store nothing: no exit inside the block keeps the wait counts
+            // of the loads issued ahead exact
             // operands: the previous element, the chain cx-1 (ox) and the
-            // chain cy-1 (oy), each of the previous step
-            double ox = 0.0, oy = 0.0;
-            if (NDEP >= 2) {
-                const double nx = chain_from_prev_lane(prev);   // lane l-1 is the chain cx-1 when gx > 0
-                ox = gx > 0 ? nx : ex[j];
-            }
-            if (NDEP >= 3) {
-                const double ny = chain_from(prev, srcy);
-                oy = gy > 0 ? ny : ey[j];
-            }
+            // chain cy-1 (oy), each of the previous step; the lane exchange
+            // is issued first, it is on every step's dependency chain
+            double nx, ny;
+            if (NDEP >= 2) nx = chain_from_prev_lane(prev);   // lane l-1 is the chain cx-1 when gx > 0
+            if (NDEP >= 3) ny = chain_from(prev, srcy);
             bool present[3];
 #pragma unroll
-            for (int k = 0; k < NDEP; k++) present[k] = act & !dia_absent(rl[k][j]);
-            // another tile's value not yet there: poll it (bounded)
-            const bool wait_x = NDEP >= 2 && (ext_x & present[1] & chain_pending(ox));
-            const bool wait_y = NDEP >= 3 && (ext_y & present[2] & chain_pending(oy));
-            if (__builtin_expect(__builtin_amdgcn_ballot_w64(wait_x | wait_y) != 0, 0)) {
-                const double2 o = chain_poll(w, row_of(hx + st), NDEP >= 3 ? row_of(hy + st) : 0, wait_x, wait_y,
-                                             make_double2(ox, oy), const_cast<unsigned*>(abort_flag));
-                ox = o.x;
-                oy = o.y;
-            }
+            for (int k = 0; k < NDEP; k++) present[k] = !dia_absent(rl[k][j]);   // padding slots: absent
+            // a lane at the tile's edge takes the other tile's value loaded U
+            // steps ago.  The row is computed with it at once; a value still
+            // pending (tested on the loaded value, off the exchange's chain)
+            // is polled (bounded) in a branch that computes the row again,
+            // so the only value merged after the branch is the row's result.
+            const double ex_j = NDEP >= 2 ? ex[j] : 0.0, ey_j = NDEP >= 3 ? ey[j] : 0.0;
+            const bool wait_x = NDEP >= 2 && (ext_x & present[1] & chain_pending(ex_j));
+            const bool wait_y = NDEP >= 3 && (ext_y & present[2] & chain_pending(ey_j));
             // ascending columns: lower rows subtract offset Lc*Cx, Lc, 1;
             // upper rows 1, Lc, Lc*Cx
-            double acc = vb[j];
+            auto row = [&](double qx, double qy) -> double {
+                const double ox = gx > 0 ? nx : qx;
+                const double oy = gy > 0 ? ny : qy;
+                double a = vb[j];
 #pragma unroll
-            for (int t = 0; t < NDEP; t++) {
-                const int k = LOWER ? NDEP - 1 - t : t;
-                const double o = k == 0 ? prev : k == 1 ? ox : oy;
-                const double cand = acc - to_arith<double>(rl[k][j]) * o;
-                acc = present[k] ? cand : acc;
+                for (int t = 0; t < NDEP; t++) {
+                    const int k = LOWER ? NDEP - 1 - t : t;
+                    const double o = k == 0 ? prev : k == 1 ? ox : oy;
+                    const double cand = a - to_arith<double>(rl[k][j]) * o;
+                    a = present[k] ? cand : a;
+                }
+                if (!UNIT) a = a * to_arith<double>(rd[j]);
+                return a;
+            };
+            double acc = row(ex_j, ey_j);
+            if (__builtin_expect(wait_x | wait_y, 0)) {
+                double px = ex_j, py = ey_j;
+                bool wx = wait_x, wy = wait_y;
+                unsigned it = 0;
+                while (wx || wy) {
+                    if (++it > (1u << 22) || ((it & 63u) == 0 && ld_agent(abort_flag) != 0u)) {
+                        st_agent(const_cast<unsigned*>(abort_flag), 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wx) px = ld_agent(w + row_of(hx + st));
+                    if (NDEP >= 3 && wy) py = ld_agent(w + row_of(hy + st));
+                    wx = wx && chain_pending(px);
+                    wy = wy && chain_pending(py);
+                }
+                acc = row(px, py);
             }
-            if (!UNIT) acc = acc * to_arith<double>(rd[j]);
-            acc = chain_pending(acc) ? __longlong_as_double((long long)0x7ffdc3c3c3c3c3c3ull) : acc;
             res[j] = acc;
-            prev = act ? acc : prev;
+            // a lane without a row this step computes on padding; its value
+            // is never read: the neighbours that would read it have no row
+            // this step either, and its own next row starts a chain
+            prev = acc;
             M4_CHAIN_ISSUE(st + U, j);
+            // keep the loads where they are issued: moved down to the step
+            // that uses them, they would no longer be in flight for U steps
+            __builtin_amdgcn_sched_barrier(0);
         }
         // the block's results to w (8 consecutive rows of a chain per 8
-        // lanes), the next block's v from LDS, the one after it in flight
+        // lanes), the next block's v from LDS, the one after it in flight.
+        // The rows are stored at agent scope (through to memory): the edge
+        // chains' rows are read by other tiles, which poll each value for
+        // the pending pattern with agent-scope loads (the inter-workgroup
+        // protocol of internal.hpp; no announcement needed).  One store per
+        // row keeps a block's vector-memory operations well under what the
+        // wait counter tracks, so the loads issued ahead stay in flight.
 #pragma unroll
         for (int j = 0; j < U; j++) wst[j * RW + l] = res[j];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -322,17 +336,14 @@ __global__ __launch_bounds__(kWave) void k_chain(ChainView<MT> C, const IT* __re
             bool ok;
             const int32_t r = stage_row(S, i, ok);
             const double y = wst[jj * RW + (l >> 3) + i * 8];
-            const bool sh = stage_shared(i);
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(chain_u2, y), rw, ok && !sh ? r * 8 : kOut, 0,
-                                                  0);
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(chain_u2, y), rw, ok && sh ? r * 8 : kOut, 0,
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(chain_u2, y), rw, ok ? r * 8 : kOut, 0,
                                                   kBufSc1);
-            if (w32)
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)y), rw32,
-                                                      ok ? r * 4 : kOut, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)y), rw32, ok ? r * 4 : kOut, 0,
+                                                  0);
         }
         stage_next();
         stage_issue(S + 2 * U);
+        __builtin_amdgcn_sched_barrier(0);
     }
 #undef M4_CHAIN_ISSUE
     wait_memory();
