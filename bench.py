@@ -628,9 +628,9 @@ def main():
                     help="SpMV configuration instead of the autotuner's choice, as "
                          "dia_sym|dia|sliced/<entries per chunk 4|8>/<workgroups per CU> "
                          "(profiling runs, where the profiler distorts the autotuner's timings)")
-    ap.add_argument("--only", default="", choices=["", "batched", "random-spmv"],
-                    help="one measurement alone (profiling runs): the batched section, or the "
-                         "random N = 2M SpMV (with --copy-calibration: plus a 512 MiB device copy)")
+    ap.add_argument("--only", default="", choices=["", "batched", "random-spmv", "serving"],
+                    help="one measurement alone (profiling runs): the batched or the serving section, "
+                         "or the random N = 2M SpMV (with --copy-calibration: plus a 512 MiB device copy)")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=1000)
     args = ap.parse_args()
@@ -648,6 +648,9 @@ def main():
         return
     if args.only == "batched":
         print(json.dumps(batched_section(m, torch, gen, args.batch_systems)))
+        return
+    if args.only == "serving":
+        print(json.dumps(serving_section(m, torch, gen, args.serve_requests)))
         return
     if args.only == "random-spmv":
         print(json.dumps(random_spmv(m, torch, gen, args.spmv_reps, args.copy_calibration)))
