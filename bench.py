@@ -446,7 +446,7 @@ def batched_section(m, torch, gen, count):
     return out
 
 
-def serving_section(m, torch, gen, nreq, sweep_requests=400):
+def serving_section(m, torch, gen, nreq, sweep_requests=400, big_rows=65536):
     """Continuous batching over 512 resident matrices (the batched families
     plus 3D stencils up to 96^3), Poisson arrivals, 1-8 right-hand sides per
     request, each request itl::cg or itl::bicgstab with pc::diagonal or
@@ -465,7 +465,6 @@ def serving_section(m, torch, gen, nreq, sweep_requests=400):
         a = int(round(np.exp(rng.uniform(np.log(32), np.log(96)))))
         systems.append(gen.laplacian3d(a, a, a))
     B = m.Batch(systems)
-    big_rows = 65536
     sizes = np.diff(B.sys_row)
     out = {"matrices": len(systems), "requests_per_rate": nreq, "cap_bytes": 200e9,
            "solvers": "cg | bicgstab  x  diagonal | ilu_0, uniform per request",
@@ -633,6 +632,8 @@ def main():
                          "or the random N = 2M SpMV (with --copy-calibration: plus a 512 MiB device copy)")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-big-rows", type=int, default=65536,
+                    help="serving: systems above this many rows go to the whole-GPU workers")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -650,7 +651,7 @@ def main():
         print(json.dumps(batched_section(m, torch, gen, args.batch_systems)))
         return
     if args.only == "serving":
-        print(json.dumps(serving_section(m, torch, gen, args.serve_requests)))
+        print(json.dumps(serving_section(m, torch, gen, args.serve_requests, big_rows=args.serve_big_rows)))
         return
     if args.only == "random-spmv":
         print(json.dumps(random_spmv(m, torch, gen, args.spmv_reps, args.copy_calibration)))
@@ -835,7 +836,8 @@ def main():
     if "batched" in sections:
         extra["batched"] = guarded(lambda: batched_section(m, torch, gen, args.batch_systems))
     if "serving" in sections:
-        extra["serving"] = guarded(lambda: serving_section(m, torch, gen, args.serve_requests))
+        extra["serving"] = guarded(lambda: serving_section(m, torch, gen, args.serve_requests,
+                                                           big_rows=args.serve_big_rows))
 
     v = args.steps / t_fp64
     it_bytes = by_layout + 11 * 8 * n

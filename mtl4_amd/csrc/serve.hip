@@ -301,7 +301,10 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
     // big-system workers
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<int32_t> big_todo;
+    // whole-GPU work is handed out one right-hand side at a time, so the
+    // right-hand sides of one request run on several workers at once
+    std::deque<std::pair<int32_t, int32_t>> big_todo;   // (request, right-hand side)
+    std::vector<int32_t> big_left(reqs.size(), 0);       // right-hand sides still running
     std::deque<std::pair<int32_t, double>> big_done;
     std::atomic<bool> stop{false};
     std::atomic<int> worker_err{0};
@@ -317,12 +320,13 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
             return;
         }
         for (;;) {
-            int32_t id;
+            int32_t id, k;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv.wait(lk, [&] { return stop.load() || !big_todo.empty(); });
                 if (big_todo.empty()) return;
-                id = big_todo.front();
+                id = big_todo.front().first;
+                k = big_todo.front().second;
                 big_todo.pop_front();
             }
             const Req& q = reqs[id];
@@ -330,7 +334,7 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
             mtl4g_pc_s* P = q.pc == MTL4G_PC_DIAGONAL ? S_->big_Pd[q.mat]
                             : q.pc == MTL4G_PC_ILU0    ? S_->big_Pi[q.mat]
                                                        : nullptr;
-            for (int32_t k = 0; k < q.nrhs; k++) {
+            do {
                 GenItem g{q.off + k * q.stride, q.off + k * q.stride + pn, q.n, 0, q.seed * 131 + (uint64_t)k};
                 if (hipMemcpyAsync(dg.p, &g, sizeof g, hipMemcpyHostToDevice, bs)) {
                     worker_err = MTL4G_ERR_HIP;
@@ -361,10 +365,10 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
                 if (st != 0) big_errors++;
                 big_solves++;
                 if (iters) iters[iter_off[id] + k] = it.i;
-            }
+            } while (false);
             (void)hipStreamSynchronize(bs);
             std::lock_guard<std::mutex> lk(mu);
-            big_done.emplace_back(id, now_s());
+            if (--big_left[id] == 0) big_done.emplace_back(id, now_s());
         }
     };
     std::vector<std::thread> workers;
@@ -449,8 +453,9 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
             pending.pop_front();
             if (q.big) {
                 std::lock_guard<std::mutex> lk(mu);
-                big_todo.push_back(q.id);
-                cv.notify_one();
+                big_left[q.id] = q.nrhs;
+                for (int32_t k = 0; k < q.nrhs; k++) big_todo.emplace_back(q.id, k);
+                cv.notify_all();
             } else {
                 admitted.push_back(q.id);
             }
@@ -460,14 +465,20 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
         for (auto& slp : S_->slots) {
             Slot& sl = *slp;
             if (sl.busy || admitted.empty()) continue;
-            // one solver per launch: the oldest admitted request's, then every
-            // admitted request of that solver in order (the others keep theirs)
+            // one solver and one size class per launch: the oldest admitted
+            // request's, then every admitted request of that solver and class
+            // in order (the others keep their place).  The class sets the
+            // workgroup size (one workgroup runs one system's whole loop);
+            // the workgroup's reductions depend on it, so it depends on the
+            // system alone: a request's result is the same alone and in a mix.
+            auto size_class = [](int32_t n) { return n > 8192 ? 2 : n > 2048 ? 1 : 0; };
             const int32_t solv = reqs[admitted.front()].solver();
+            const int cls = size_class(reqs[admitted.front()].n);
             const int method = solv >> 8, pc_kind = solv & 0xff;
             int nit = 0;
             for (auto ita = admitted.begin(); ita != admitted.end();) {
                 const Req& q = reqs[*ita];
-                if (q.solver() != solv) { ++ita; continue; }
+                if (q.solver() != solv || size_class(q.n) != cls) { ++ita; continue; }
                 if (nit + q.nrhs > prm.max_items || (prm.per_request && nit > 0)) break;
                 const int64_t pn = Arena::round(q.n);
                 for (int32_t k = 0; k < q.nrhs; k++, nit++) {
@@ -486,8 +497,11 @@ int mtl4g_server_run(mtl4g_server S_, int32_t nreq, const int32_t* mat, const in
                 st = MTL4G_ERR_HIP;
             if (!st) {
                 hipLaunchKernelGGL(k_gen_items, dim3(std::min(nit, 4096)), dim3(256), 0, s, sl.d_gen.p, nit, base);
-                st = batch_launch(B, method, pc_kind, prm.precision, B->shape[prm.precision == MTL4G_F64 ? 0 : prm.precision == MTL4G_F32 ? 1 : 2],
-                                  sl.d_items.p, nit, base, base, base, sl.d_res.p, sl.d_queue.p, s);
+                BatchShape sh = B->shape[prm.precision == MTL4G_F64 ? 0 : prm.precision == MTL4G_F32 ? 1 : 2];
+                if (cls == 2) sh = BatchShape{1024, prm.precision == MTL4G_F64 ? 1 : 2};
+                else if (cls == 1) sh = BatchShape{512, 2};
+                st = batch_launch(B, method, pc_kind, prm.precision, sh, sl.d_items.p, nit, base, base, base,
+                                  sl.d_res.p, sl.d_queue.p, s);
             }
             if (!st && (hipMemcpyAsync(sl.h_res, sl.d_res.p, nit * sizeof(mtl4g_batch_result), hipMemcpyDeviceToHost, s) ||
                         hipEventRecord(sl.ev, s)))
