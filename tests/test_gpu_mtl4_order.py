@@ -271,6 +271,46 @@ def test_gmres_bitwise(m, mtl4_order, fam, restart, lr):
     assert np.array_equal(x.numpy(), xo)
 
 
+def _small_gmres_system(name):
+    if name == "singular3":
+        return np.array([[0.0, -1.0, 0.0], [0.0, 0.0, 1.0], [0.0, 0.0, -1.0]]), np.array([0.0, -1.0, 1.0])
+    rng = np.random.default_rng(int(name[5:]))
+    n = int(rng.integers(4, 9))
+    D = rng.standard_normal((n, n))
+    return D, rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("name,restart", [("gauss3", 0), ("gauss3", 4), ("gauss11", 4), ("gauss17", 3),
+                                          ("singular3", 0)])
+def test_gmres_pivoting_and_singular_blocks_bitwise(m, mtl4_order, name, restart):
+    """Dense Gaussian systems, whose Hessenberg least-squares solves pivot in
+    cycles of three rows or more (MTL4's permute scatters, which differs
+    from a gather there), and a singular 3x3 whose first LU attempts meet
+    zero pivots, so the range shrinks and each attempt starts from the
+    block the previous one left: the device against the oracle bit for
+    bit (the oracle is pinned on the same systems by a NumPy restatement)."""
+    D, b = _small_gmres_system(name)
+    n = len(b)
+    rows, cols = np.nonzero(D)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    O = orc.Matrix(rp, cols.astype(np.int32), D[rows, cols].copy())
+    A = m.compressed2D(n, n, rp, cols.astype(np.int32), D[rows, cols].copy())
+    max_iter = 2 * n if name == "singular3" else 3 * n
+    xo, st, ito, ho = orc.gmres(O, b, None, None, restart, max_iter=max_iter, rtol=1e-12, hist_cap=400)
+    x = m.dense_vector(n)
+    it = m.cyclic_iteration(dv(m, b), max_iter, 1e-12, 0.0, 10**9)
+    it.set_quite(True)
+    it.suppress_resume(True)
+    if restart == 0:
+        err = m.gmres_full(A, x, dv(m, b), None, None, it)
+    else:
+        err = m.gmres(A, x, dv(m, b), None, None, it, restart)
+    assert err == st
+    assert it.iterations() == ito.i
+    assert np.array_equal(np.asarray(it.history), ho)
+    assert np.array_equal(x.numpy(), xo)
+
+
 def test_bicgstab_2_omega_breakdown(m, mtl4_order):
     """omega == 0 is iter.fail(3) (bicgstab_2.hpp:107): status 3 after one
     iteration and x returned without the shift back x += x0, bit for bit

@@ -1054,6 +1054,96 @@ def test_gmres_bad_arguments():
     assert st == orc.INVALID
 
 
+# ---------------------------------------------------------------- GMRES pinned
+# From two Krylov vectors per cycle on, the coefficients follow gmres.hpp's
+# own rotation order (rotation i rebuilt from column k-1 of H and applied to
+# the whole of H), a pivoted LU that scatters its permutation and works in
+# place on the block it factors: no textbook identity fixes them.  They are
+# pinned by a second restatement, tests/golden/gmres_numpy.py, written with
+# NumPy from the reference files: the oracle and it must agree bit for bit
+# (status, iterations, every residual handed to finished(), x), and each of
+# the restatement's four plausible mistakes must change the result on at
+# least one of the systems below, so the systems exercise the branches those
+# mistakes touch:
+#   * diagonally dominant sparse systems, restart 2, 3, 5 and full, with
+#     pc::diagonal on no side, the left, the right or both;
+#   * dense Gaussian systems, whose Hessenberg LUs pivot in cycles of three
+#     rows or more (scatter and gather permutations differ there);
+#   * a singular 3x3 system whose first LU attempts hit zero pivots, so the
+#     range shrinks and each attempt starts from the block the last one left.
+def _gauss_system(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(4, 9))
+    D = rng.standard_normal((n, n))
+    return D, rng.standard_normal(n)
+
+
+def _gmres_cases():
+    cases = []
+    for seed in (1, 2):
+        D, rng = _nonsym(30, seed, 0.3)
+        b, x0 = rng.standard_normal(30), rng.standard_normal(30)
+        for restart in (2, 3, 5, 0):
+            for side in ("identity", "left", "right", "both"):
+                cases.append((f"dominant{seed}-restart{restart}-{side}", D, b, x0, restart, side, 40))
+    for seed in (3, 11, 12, 17):
+        D, b = _gauss_system(seed)
+        for restart in (3, 4, 0):
+            cases.append((f"gauss{seed}-restart{restart}", D, b, None, restart, "identity", 3 * len(b)))
+    D = np.array([[0.0, -1.0, 0.0], [0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])
+    cases.append(("singular3-full", D, np.array([0.0, -1.0, 1.0]), None, 0, "identity", 6))
+    return cases
+
+
+_GMRES_CASES = _gmres_cases()
+
+
+def _orc_gmres_case(D, b, x0, restart, side, max_iter):
+    A = orc.Matrix(*_csr(D))
+    P = orc.Preconditioner("diagonal", A) if side != "identity" else None
+    L, R = {"identity": (None, None), "left": (P, None), "right": (None, P), "both": (P, P)}[side]
+    x, st, it, h = orc.gmres(A, b, L, R, restart, x0=x0, max_iter=max_iter, rtol=1e-12, hist_cap=4 * max_iter + 8)
+    return x, st, it.i, h
+
+
+def _numpy_gmres_case(D, b, x0, restart, side, max_iter, mutant=None):
+    import bicgstab_numpy as bn
+    import gmres_numpy as gn
+    A = bn.Csr(*_csr(D))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / A.diag
+        il, ir = {"identity": (None, None), "left": (inv, None), "right": (None, inv), "both": (inv, inv)}[side]
+        return gn.gmres(A, b, restart, il, ir, x0, max_iter, 1e-12, 0.0, mutant)
+
+
+def _same_run(u, v):
+    return (u[1], u[2]) == (v[1], v[2]) and np.array_equal(u[3], v[3], equal_nan=True) \
+        and np.array_equal(u[0], v[0], equal_nan=True)
+
+
+@pytest.mark.parametrize("case", _GMRES_CASES, ids=lambda c: c[0])
+def test_gmres_matches_numpy_restatement(case):
+    _, D, b, x0, restart, side, max_iter = case
+    o = _orc_gmres_case(D, b, x0, restart, side, max_iter)
+    p = _numpy_gmres_case(D, b, x0, restart, side, max_iter)
+    assert _same_run(o, p), (o[1:3], p[1:3], o[3], p[3])
+
+
+def test_gmres_singular_block_case():
+    """The singular 3x3 case: the oracle returns a finite x after the range
+    shrank (the restatement also agrees bit for bit above)."""
+    _, D, b, x0, restart, side, max_iter = _GMRES_CASES[-1]
+    x, st, its, h = _orc_gmres_case(D, b, x0, restart, side, max_iter)
+    assert st == 0 and its == 3 and np.isfinite(x).all() and len(h) == 2
+
+
+@pytest.mark.parametrize("mutant", ["textbook_rotations", "gather_permute", "no_reorth", "lu_on_copy"])
+def test_gmres_cases_reject_restatement_mutants(mutant):
+    changed = [c[0] for c in _GMRES_CASES
+               if not _same_run(_numpy_gmres_case(*c[1:]), _numpy_gmres_case(*c[1:], mutant=mutant))]
+    assert changed, mutant
+
+
 def test_cg_diag_omp_matches_sequential_cg():
     """The OpenMP restatement of MTL4's cg + diagonal (the all-core CPU
     baseline) solves config 1 like the sequential oracle: iteration count
