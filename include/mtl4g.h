@@ -197,6 +197,12 @@ int mtl4g_csr_grid(mtl4g_matrix A, int value_format, int32_t* workgroups);
 /* Autotuner: times y = A*x for the candidate kernels and shapes on this
  * matrix and keeps the fastest; h_best_us (may be NULL) gets its time. */
 int mtl4g_csr_autotune(mtl4g_matrix A, double* h_best_us, void* stream);
+/* Builds the MTL4G_F32 or MTL4G_BF16 copy of A's values in every device
+ * layout and tunes the product on them, which the first low-precision
+ * product or mixed-precision solve would otherwise do (no MTL4
+ * counterpart: set-up ahead of timed or latency-bound use).  MTL4G_F64 is
+ * accepted and does nothing. */
+int mtl4g_csr_prepare(mtl4g_matrix A, int value_format, void* stream);
 
 /* w (=,+=,-=) A*v: smat_cvec_mult, operation/mat_vec_mult.hpp:613-683,
  * dispatched from vector/crtp_base_vector.hpp:93-104,191-200,268-277.

@@ -189,6 +189,11 @@ class compressed2D:
         check(_L().mtl4g_csr_autotune(self.h, C.byref(us), _stream()), "autotune")
         return us.value
 
+    def prepare(self, value_format):
+        """Build the F32 or BF16 value copy and tune its product now rather
+        than in the first mixed-precision solve (F64: nothing to do)."""
+        check(_L().mtl4g_csr_prepare(self.h, value_format, _stream()), "prepare")
+
     def mult(self, v, w, assign=ASSIGN):
         """w (=,+=,-=) A*v  (operation/mat_vec_mult.hpp:613-683)."""
         if v.size() != self._m or w.size() != self._n:

@@ -129,6 +129,7 @@ _SIGS = {
     "mtl4g_csr_get_kernel": (_INT, [_P, C.POINTER(_INT), C.POINTER(_I32), C.POINTER(_I32)]),
     "mtl4g_csr_grid": (_INT, [_P, _INT, C.POINTER(_I32)]),
     "mtl4g_csr_autotune": (_INT, [_P, C.POINTER(_D), _P]),
+    "mtl4g_csr_prepare": (_INT, [_P, _INT, _P]),
     "mtl4g_batch_create": (_INT, [_PP, _I32, _P, _P, _P, _P, _INT]),
     "mtl4g_batch_destroy": (_INT, [_P]),
     "mtl4g_batch_info": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64)]),

@@ -1302,6 +1302,12 @@ int mtl4g_spmv(mtl4g_matrix A, const double* v, double* w, int assign, void* str
     return launch_spmv(A, v, w, assign, S(stream));
 }
 
+int mtl4g_csr_prepare(mtl4g_matrix A, int fmt, void* stream) {
+    if (!A || (fmt != MTL4G_F64 && fmt != MTL4G_F32 && fmt != MTL4G_BF16)) return MTL4G_ERR_INVALID;
+    if (fmt == MTL4G_F64) return MTL4G_OK;
+    return ensure_lowp_values(A, fmt, S(stream));
+}
+
 int mtl4g_spmv_f32(mtl4g_matrix A, int fmt, const float* v, float* w, int assign, void* stream) {
     if (!A || (A->n && !w) || (A->m && !v) || (fmt != MTL4G_F32 && fmt != MTL4G_BF16))
         return MTL4G_ERR_INVALID;

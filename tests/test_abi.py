@@ -63,3 +63,11 @@ def test_sweep_schedule_and_chain_tile_arguments():
         assert lib.mtl4g_set_chain_tile(tx, ty) == 0, (tx, ty)
     assert lib.mtl4g_pc_chain_tile(None, None, None, None, None) == _lib.ERR_INVALID
     assert lib.mtl4g_pc_sweeps(None, None, None) == _lib.ERR_INVALID
+
+
+def test_csr_prepare_arguments():
+    """mtl4g_csr_prepare rejects a null matrix and an unknown format on the
+    host."""
+    from mtl4_amd import _lib
+    lib = _lib.lib()
+    assert lib.mtl4g_csr_prepare(None, _lib.F32, None) == _lib.ERR_INVALID

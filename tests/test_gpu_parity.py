@@ -1400,3 +1400,25 @@ def test_mixed_solve_same_bits_on_both_sweep_schedules(m, method):
         assert err == 0
         xs.append(x.numpy())
     assert np.array_equal(xs[0], xs[1])
+
+
+def test_csr_prepare_same_results(m):
+    """compressed2D.prepare builds the FP32 and BF16 value copies and tunes
+    their products ahead of use: a mixed-precision solve on a prepared
+    matrix equals one on a matrix that builds them on first use, bit for
+    bit; F64 is accepted and does nothing."""
+    rp, ci, va = family("diff3d_16", seed=3)
+    O, A1 = pair(rp, ci, va)
+    _, A2 = pair(rp, ci, va)
+    for fmt in (m.F64, m.F32, m.BF16):
+        A1.prepare(fmt)
+    b = orc.spmv(O, np.random.default_rng(4).uniform(-1, 1, A1.num_rows()))
+    xs = []
+    for A in (A1, A2):
+        for fmt in (m.F32, m.BF16):
+            x = m.dense_vector(A.num_rows())
+            it = m.basic_iteration(dv(m, b), 300, 1e-10)
+            err, _ = m.ir_solve(A, x, dv(m, b), it, m.pc.diagonal(A, m.F32), value_format=fmt)
+            assert err == 0 or fmt == m.BF16   # BF16 values stagnate above 1e-10
+            xs.append(x.numpy())
+    assert np.array_equal(xs[0], xs[2]) and np.array_equal(xs[1], xs[3])
