@@ -221,7 +221,7 @@ struct EpiCgQ {
     __device__ void prepare() {}
     __device__ Pre preload(int row) const { return Pre{p[row]}; }
     __device__ void operator()(int row, VT s, const Pre& pre, VT* part) const {
-        q[row] = s;
+        __builtin_nontemporal_store(s, q + row);
         part[0] += pre.p * s;
     }
     __device__ void finish(const VT* tot) const {
@@ -251,10 +251,11 @@ struct EpiCgXR {
         VT x, p, r, q, d;
     };
     __device__ In load(int64_t i) const {
-        return In{x[i], p[i], r[i], q[i], PCM == PCM_DIAG ? static_cast<VT>(d[i]) : VT(0)};
+        return In{__builtin_nontemporal_load(x + i), p[i], r[i], __builtin_nontemporal_load(q + i),
+                  PCM == PCM_DIAG ? static_cast<VT>(d[i]) : VT(0)};
     }
     __device__ void store(int64_t i, const In& in, VT* part) const {
-        x[i] = in.x + alpha * in.p;
+        __builtin_nontemporal_store(in.x + alpha * in.p, x + i);
         const VT v = in.r - alpha * in.q;
         r[i] = v;
         part[0] += v * v;
