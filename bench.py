@@ -17,6 +17,7 @@ Reported beside it (same matrix, same b):
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -782,6 +783,36 @@ def main():
     torch.cuda.synchronize()
     t_fp64 = time.perf_counter() - t0
     iters = it.iterations()
+    # the roofline kernel inside the solve: one more solve with HIP events
+    # around each of its products (on the solve's stream)
+    Lb = m._lib.lib()
+    tot_us, cnt = C.c_double(), C.c_int64()
+    m._lib.check(Lb.mtl4g_set_product_timing(1))
+    m._lib.check(Lb.mtl4g_product_timing(C.byref(tot_us), C.byref(cnt)))
+    fp64_solve()
+    torch.cuda.synchronize()
+    m._lib.check(Lb.mtl4g_product_timing(C.byref(tot_us), C.byref(cnt)))
+    m._lib.check(Lb.mtl4g_set_product_timing(0))
+    q_us = tot_us.value / max(cnt.value, 1)
+    if cnt.value > 0 and kern[0] == m._lib.SPMV_DIA_SYM:
+        cold = {k: roofline.pop(k) for k in ("kernel", "timing", "achieved", "frac", "traffic", "csr_equivalent",
+                                             "avg_launch_us")}
+        cold["note"] = ("the same product with y = A*x alone, each launch after 14 vector passes of torch "
+                        "kernels (459 MB) that leave little of the matrix in the MALL")
+        q_traffic = load_traffic("cg_product_in_solve")
+        roofline.update({
+            "kernel": "%s<EpiCgQ> (q = A*p fused with p.q inside the timed FP64 CG solve, diffusion3d %d^3)"
+                      % (kname, g),
+            "timing": ("HIP events around each of the %d products of one FP64 CG solve, on the solve's "
+                       "stream (mtl4g_set_product_timing)" % cnt.value),
+            "achieved": round(by_layout / (q_us * 1e-6) / 1e9, 1),
+            "frac": round(by_layout / (q_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": q_traffic,
+            "csr_equivalent": {"bytes_per_launch": by, "achieved": round(by / (q_us * 1e-6) / 1e9, 1),
+                               "frac": round(by / (q_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                               "note": "the CSR bytes of y = A*x over the same time"},
+            "avg_launch_us": round(q_us, 2),
+            "cold_mall_context": cold})
     r = m.dense_vector(n)
     r.assign(b)
     A.mult(x, r, m.MINUS)

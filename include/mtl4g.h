@@ -121,6 +121,14 @@ int mtl4g_set_device(int device);
 #define MTL4G_ORDER_MTL4 1
 int mtl4g_set_reduction_order(int order);
 int mtl4g_get_reduction_order(void);
+/* Diagnostic (no MTL4 counterpart): with on = 1, itl::cg solves record a
+ * pair of HIP events on their stream around each product q = A*p of their
+ * iterations (at most 65536 per solve); mtl4g_product_timing returns the
+ * summed duration in microseconds and the number of products of the solves
+ * since the last call (launches past a solve's stop are not counted), and
+ * clears them.  Process-wide; for one solve at a time. */
+int mtl4g_set_product_timing(int on);
+int mtl4g_product_timing(double* h_total_us, int64_t* h_count);
 
 /* Device storage for dense_vector (vector/dense_vector.hpp:50-269 keeps its
  * elements in a contiguous_memory_block; here the block lives in HBM).

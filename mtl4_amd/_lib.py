@@ -83,6 +83,8 @@ _SIGS = {
     "mtl4g_set_device": (_INT, [_INT]),
     "mtl4g_set_reduction_order": (_INT, [_INT]),
     "mtl4g_get_reduction_order": (_INT, []),
+    "mtl4g_set_product_timing": (_INT, [_INT]),
+    "mtl4g_product_timing": (_INT, [C.POINTER(_D), C.POINTER(C.c_int64)]),
     "mtl4g_csr_create": (_INT, [_PP, _I32, _I32, _I64, _P, _P, _P, _INT]),
     "mtl4g_csr_destroy": (_INT, [_P]),
     "mtl4g_csr_info": (_INT, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64)]),

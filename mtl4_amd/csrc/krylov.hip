@@ -22,6 +22,7 @@
 #include <cstring>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "kernels.hpp"
 #include "krylov.hpp"
@@ -1728,6 +1729,45 @@ static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_p
     return MTL4G_OK;
 }
 
+// Events around the products of CG iterations (mtl4g_set_product_timing).
+struct ProductTimer {
+    static constexpr size_t kMax = 65536;
+    bool on = false;
+    std::vector<hipEvent_t> ev;   // 2 per product, created on first use
+    size_t used = 0;              // pairs recorded in the current solve
+    double total_us = 0;
+    int64_t count = 0;
+};
+static ProductTimer g_timer;
+
+static void timer_mark(hipStream_t s, bool begin) {
+    ProductTimer& T = g_timer;
+    if (!T.on || T.used >= ProductTimer::kMax) return;
+    const size_t k = 2 * T.used + (begin ? 0 : 1);
+    while (T.ev.size() <= k) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        T.ev.push_back(e);
+    }
+    (void)hipEventRecord(T.ev[k], s);
+    if (!begin) T.used++;
+}
+
+// After a solve of `iters` iterations has completed on its stream: add the
+// first iters products (later launches exited at entry).
+static void timer_collect(int32_t iters) {
+    ProductTimer& T = g_timer;
+    const size_t n = std::min(T.used, (size_t)std::max(iters, 0));
+    for (size_t k = 0; k < n; k++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, T.ev[2 * k], T.ev[2 * k + 1]) == hipSuccess) {
+            T.total_us += 1000.0 * ms;
+            T.count++;
+        }
+    }
+    T.used = 0;
+}
+
 // One iteration of CG.
 template <typename VT, typename MT, typename DT, int PCM>
 static int cg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s* P, const DT* d,
@@ -1747,7 +1787,9 @@ static int cg_iteration(const mtl4g_matrix_s* A, const MT* val, const mtl4g_pc_s
         M4_TRY(launch_vec<VT>(n, EpiCgRho<VT>{r, z, c}, part, tk, s, tm));
     }
     M4_TRY(launch_vec<VT>(n, EpiCgP<VT, DT, PCM>{r, d, z, p, c, VT(0), 0}, part, tk, s));
+    if (sizeof(VT) == 8) timer_mark(s, true);   // the FP64 solves (mtl4g_set_product_timing)
     M4_TRY(launch_csr<VT>(A, val, p, EpiCgQ<VT>{p, q, c}, part, tk, s, tm));
+    if (sizeof(VT) == 8) timer_mark(s, false);
     M4_TRY(launch_vec<VT>(n, EpiCgXR<VT, DT, PCM>{x, r, p, q, d, c, W.hist.p, VT(0)}, part, tk, s, tm));
     return MTL4G_OK;
 }
@@ -1930,6 +1972,7 @@ int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt
     M4_HIP(hipMemcpyAsync(W.h_ctl, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
     M4_HIP(hipStreamSynchronize(s));
     const Ctl& c = *W.h_ctl;
+    if (method == MTL4G_CG) timer_collect(c.i - it->i);
     it->i = c.i;
     it->error = c.error;
     it->resid = c.resid;
@@ -2664,6 +2707,25 @@ int mtl4g_solver_solve(mtl4g_solver S_, double* x, const double* b, mtl4g_iterat
         return bicgstab2_run(S_->A, S_->P, S_->W, x, b, it, h_hist, hist_cap, S(stream));
     return krylov_run<double>(S_->A, S_->P, S_->method, MTL4G_F64, S_->W, x, b, it, h_hist,
                               hist_cap, S(stream));
+}
+
+int mtl4g_set_product_timing(int on) {
+    if (on != 0 && on != 1) {
+        set_error("mtl4g_set_product_timing: on must be 0 or 1");
+        return MTL4G_ERR_INVALID;
+    }
+    g_timer.on = on != 0;
+    g_timer.used = 0;
+    return MTL4G_OK;
+}
+
+int mtl4g_product_timing(double* h_total_us, int64_t* h_count) {
+    if (!h_total_us || !h_count) return MTL4G_ERR_INVALID;
+    *h_total_us = g_timer.total_us;
+    *h_count = g_timer.count;
+    g_timer.total_us = 0;
+    g_timer.count = 0;
+    return MTL4G_OK;
 }
 
 int mtl4g_cg(mtl4g_matrix A, mtl4g_pc P, double* x, const double* b, mtl4g_iteration* it,
