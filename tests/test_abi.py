@@ -71,3 +71,12 @@ def test_csr_prepare_arguments():
     from mtl4_amd import _lib
     lib = _lib.lib()
     assert lib.mtl4g_csr_prepare(None, _lib.F32, None) == _lib.ERR_INVALID
+
+
+def test_product_timing_arguments():
+    """The product-timing diagnostic checks its arguments on the host."""
+    from mtl4_amd import _lib
+    lib = _lib.lib()
+    assert lib.mtl4g_set_product_timing(2) == _lib.ERR_INVALID
+    assert lib.mtl4g_product_timing(None, None) == _lib.ERR_INVALID
+    assert lib.mtl4g_set_product_timing(0) == 0

@@ -1422,3 +1422,34 @@ def test_csr_prepare_same_results(m):
             assert err == 0 or fmt == m.BF16   # BF16 values stagnate above 1e-10
             xs.append(x.numpy())
     assert np.array_equal(xs[0], xs[2]) and np.array_equal(xs[1], xs[3])
+
+
+def test_product_timing_counts_the_iterations(m):
+    """mtl4g_set_product_timing: one event pair per product of a CG solve's
+    iterations (launches after the stop not counted), durations positive;
+    reading clears the totals; the solve's result is unchanged."""
+    rp, ci, va = family("diff3d_16", seed=5)
+    O, A = pair(rp, ci, va)
+    b = orc.spmv(O, np.random.default_rng(6).uniform(-1, 1, A.num_rows()))
+    P = m.pc.diagonal(A)
+    L = m._lib.lib()
+    tot, cnt = C.c_double(), C.c_int64()
+    xs = []
+    for on in (0, 1):
+        m._lib.check(L.mtl4g_set_product_timing(on))
+        try:
+            m._lib.check(L.mtl4g_product_timing(C.byref(tot), C.byref(cnt)))
+            x = m.dense_vector(A.num_rows())
+            it = m.basic_iteration(dv(m, b), 500, 1e-10)
+            assert m.cg(A, x, dv(m, b), P, it) == 0
+            m._lib.check(L.mtl4g_product_timing(C.byref(tot), C.byref(cnt)))
+        finally:
+            m._lib.check(L.mtl4g_set_product_timing(0))
+        xs.append(x.numpy())
+        if on:
+            assert cnt.value == it.iterations() and tot.value > 0
+        else:
+            assert cnt.value == 0
+    assert np.array_equal(xs[0], xs[1])
+    m._lib.check(L.mtl4g_product_timing(C.byref(tot), C.byref(cnt)))
+    assert cnt.value == 0
