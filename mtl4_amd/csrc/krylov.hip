@@ -13,7 +13,10 @@
 //   P : p = z + beta*p            with z = inv_diag .* r formed on the fly
 //   Q : q = A*p, p.q              (SpMV + dot, cg.hpp:103)      -> alpha
 //   XR: x += alpha*p, r -= alpha*q, r.r, r.(inv_diag .* r)      -> stop?, beta
-// i.e. B_spmv + 11 N V bytes per iteration; z is never stored.
+// i.e. B_spmv + 11 N V bytes per iteration; z is never stored.  x (touched
+// by XR only) and q (written by Q, read once by XR) move with non-temporal
+// loads and stores, so the matrix, p, r and inv_diag stay in the MALL from
+// one iteration to the next.
 // General preconditioners add pc_apply(r -> z) and a dot for rho.
 #include <hip/hip_runtime.h>
 
