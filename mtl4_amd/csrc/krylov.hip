@@ -1732,7 +1732,8 @@ static int ensure_workspace(Workspace<VT>& W, int32_t n, int nvec, const mtl4g_p
     return MTL4G_OK;
 }
 
-// Events around the products of CG iterations (mtl4g_set_product_timing).
+// Events around the products of CG iterations (mtl4g_set_product_timing);
+// created on first use and kept for the process.
 struct ProductTimer {
     static constexpr size_t kMax = 65536;
     bool on = false;
@@ -1948,6 +1949,7 @@ int krylov_run(const mtl4g_matrix_s* A, const mtl4g_pc_s* P, int method, int fmt
         return it->error;
     }
     const int nvec = method == MTL4G_CG ? 5 : 9;
+    g_timer.used = 0;   // events of a solve that ended in an error are not counted
     M4_TRY(ensure_workspace(W, A->n, nvec, P, hist_cap, s));
     Ctl init;
     std::memset(&init, 0, sizeof init);
