@@ -799,7 +799,13 @@ def main():
                                              "avg_launch_us")}
         cold["note"] = ("the same product with y = A*x alone, each launch after 14 vector passes of torch "
                         "kernels (459 MB) that leave little of the matrix in the MALL")
-        q_traffic = load_traffic("cg_product_in_solve")
+        # the PMC record of the in-solve product holds for the configuration it was taken on
+        try:
+            rec = json.load(open(os.path.join(ROOT, "profiles", "spmv_diffusion3d_160_traffic.json")))
+            rec = rec["configs"]["cg_product_in_solve"]
+            q_traffic = rec["traffic_bytes"] if rec.get("config") == traffic_key(m, kern, grid) else None
+        except (OSError, ValueError, KeyError):
+            q_traffic = None
         roofline.update({
             "kernel": "%s<EpiCgQ> (q = A*p fused with p.q inside the timed FP64 CG solve, diffusion3d %d^3)"
                       % (kname, g),
